@@ -1,0 +1,634 @@
+// etraj_capi.cpp -- the extern "C" boundary of libetraj_hip.so (include/etraj.h).
+//
+// Validates the descriptors the way the reference's constructors assert
+// (TrajectorySimulation.jl:178-186, ADKSampler.jl:43-105), resolves them into
+// kernel parameter blocks, owns the device buffers of a context, and drives
+// the kernels on the context's stream. No exception crosses the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "etraj_engine.hpp"
+
+using namespace etraj;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            n = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t bytes = std::max<size_t>(want, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = std::max<size_t>(want, 1);
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct etraj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s;
+    DevBuf<int64_t> seg_start, cnt;
+    DevBuf<int32_t> counts, status;
+    DevBuf<Contrib> contrib;
+};
+
+namespace {
+
+int fail(etraj_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(etraj_ctx* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? ETRAJ_ENOMEM : ETRAJ_EHIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                  \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+int rows_for(int dim, int phase) {
+    return dim == 3 ? (phase == ETRAJ_PHASE_CTMC ? 8 : 9) : (phase == ETRAJ_PHASE_CTMC ? 6 : 7);
+}
+
+int check_laser(etraj_ctx* c, const etraj_laser_desc* d, LaserP& L) {
+    if (!d) return fail(c, ETRAJ_EINVAL, "laser descriptor is NULL");
+    if (make_laser(*d, L)) return fail(c, ETRAJ_EINVAL, "invalid laser descriptor (kind, wave_len>0, cyc_num>0)");
+    return ETRAJ_OK;
+}
+
+int check_target(etraj_ctx* c, const etraj_target_desc* d, TargetP& T) {
+    if (!d) return fail(c, ETRAJ_EINVAL, "target descriptor is NULL");
+    if (make_target(*d, T)) return fail(c, ETRAJ_EINVAL, "invalid target descriptor (kind, Ip>0, soft_core>=0)");
+    return ETRAJ_OK;
+}
+
+int64_t batch_max_size_of(const etraj_sampler_desc& s) {
+    if (s.sampling == ETRAJ_SAMPLING_STEP)
+        return (int64_t)s.ss_kd_num * (s.dimension == 3 ? (int64_t)s.ss_kz_num : 1);
+    return s.mc_kt_num;
+}
+
+// sp.t_samples (ADKSampler.jl:133-147): range, sorted seeded uniforms, or
+// one seeded jitter per stratum.
+int batch_times_of(etraj_ctx* c, const etraj_sampler_desc& s, std::vector<double>& t) {
+    const int n = s.sample_t_num;
+    if (n <= 0) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid time sample number");
+    t.resize(n);
+    const double lo = s.sample_t_lo, hi = s.sample_t_hi;
+    switch (s.sampling) {
+        case ETRAJ_SAMPLING_STEP:
+            for (int i = 0; i < n; ++i) t[i] = grid_point(lo, hi, n, i);
+            break;
+        case ETRAJ_SAMPLING_MONTECARLO:
+            if (!(lo < hi)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid sampling time interval");
+            for (int i = 0; i < n; ++i) {
+                U64x4 o = philox((uint64_t)i, 0, 0, kTagT0, s.seed, 0);
+                t[i] = unit_co(o.v[0]) * (hi - lo) + lo;
+            }
+            std::sort(t.begin(), t.end());
+            break;
+        case ETRAJ_SAMPLING_STRATIFIED: {
+            if (!(lo < hi)) return fail(c, ETRAJ_EINVAL, "invalid sampling time interval");
+            const double w = (hi - lo) / (double)n;
+            for (int i = 0; i < n; ++i) {
+                U64x4 o = philox((uint64_t)i, 0, 0, kTagT0, s.seed, 0);
+                t[i] = lo + ((double)i + unit_co(o.v[0])) * w;
+            }
+            break;
+        }
+        default:
+            return fail(c, ETRAJ_EINVAL, "unknown sampling mode");
+    }
+    return ETRAJ_OK;
+}
+
+// ADKSampler constructor checks (ADKSampler.jl:43-105) and the per-job
+// constants of gen_electron_batch.
+int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, const TargetP& T,
+                 const std::vector<double>& t, SamplerP& S) {
+    if (!sd) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
+    const etraj_sampler_desc& s = *sd;
+    if (s.init_method != ETRAJ_INIT_ADK)
+        return fail(c, ETRAJ_EUNSUPPORTED, "only init_cond_method = :ADK is implemented by the engine");
+    if (s.phase_method < 0 || s.phase_method > 2) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Undefined phase method");
+    if (s.dimension != 2 && s.dimension != 3) return fail(c, ETRAJ_EINVAL, "`dimension` must be either 2 or 3");
+    if (s.rate_prefix & ~(ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC | ETRAJ_PREFIX_JAC))
+        return fail(c, ETRAJ_EINVAL, "[ADKSampler] Undefined tunneling rate prefix");
+    if ((s.rate_prefix & ETRAJ_PREFIX_PRE) && (s.rate_prefix & ETRAJ_PREFIX_PRECC))
+        return fail(c, ETRAJ_EINVAL, "[ADKSampler] Rate prefixes [Pre] & [PreCC] conflict.");
+    if (!(s.cutoff_limit >= 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid cut-off limit");
+    if (s.sampling == ETRAJ_SAMPLING_STEP) {
+        if (s.dimension == 3) {
+            if (!(s.ss_kd_num > 0 && s.ss_kz_num > 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid kd,kz sample number");
+            if (!(s.ss_kd_max > 0 || s.ss_kz_max > 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid kd,kz sample boundaries");
+        } else {
+            if (!(s.ss_kd_num > 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid kd sample number");
+            if (!(s.ss_kd_max > 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid kd sample boundaries");
+        }
+    } else if (s.sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        if (!(s.mc_kt_num > 0)) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid sampling kt_num");
+        if (s.dimension == 3 ? !(s.mc_kd_max > 0 || s.mc_kz_max > 0) : !(s.mc_kd_max > 0))
+            return fail(c, ETRAJ_EINVAL, "[ADKSampler] Invalid kd,kz sample boundaries");
+    } else if (s.sampling == ETRAJ_SAMPLING_STRATIFIED) {
+        if (!(s.mc_kt_num > 0)) return fail(c, ETRAJ_EINVAL, "invalid samples per stratum");
+        if (!(s.kd_sigma > 0) || (s.dimension == 3 && !(s.kz_sigma > 0)))
+            return fail(c, ETRAJ_EINVAL, "stratified sampling needs kd_sigma > 0 (and kz_sigma > 0 in 3D)");
+    } else {
+        return fail(c, ETRAJ_EINVAL, "unknown sampling mode");
+    }
+    if (s.sampling == ETRAJ_SAMPLING_STEP && batch_max_size_of(s) > (int64_t)1 << 31)
+        return fail(c, ETRAJ_EINVAL, "batch too large");
+    S = SamplerP{};
+    S.dim = s.dimension;
+    S.phase = s.phase_method;
+    S.sampling = s.sampling;
+    int pre = s.rate_prefix;
+    if ((pre & ETRAJ_PREFIX_PRECC) && L.n != 1) {  // ADKSampler.jl:72-77
+        pre &= ~ETRAJ_PREFIX_PRECC;
+        pre |= ETRAJ_PREFIX_PRE;
+    }
+    S.prefix = pre;
+    S.t_num = s.sample_t_num;
+    S.kd_num = s.sampling == ETRAJ_SAMPLING_STEP ? s.ss_kd_num : 0;
+    S.kz_num = (s.sampling == ETRAJ_SAMPLING_STEP && s.dimension == 3) ? s.ss_kz_num : 1;
+    S.rows = rows_for(s.dimension, s.phase_method);
+    S.nk = batch_max_size_of(s);
+    S.max_bs = S.nk;
+    S.t_lo = s.sample_t_lo;
+    S.t_hi = s.sample_t_hi;
+    S.kd_max = std::fabs(s.ss_kd_max);
+    S.kz_max = std::fabs(s.ss_kz_max);
+    S.mc_kd_max = s.mc_kd_max;
+    S.mc_kz_max = s.mc_kz_max;
+    S.kd_sigma = s.kd_sigma;
+    S.kz_sigma = s.kz_sigma;
+    S.cutoff = s.cutoff_limit;
+    S.seed = s.seed;
+    const double tmax = *std::max_element(t.begin(), t.end()), tmin = *std::min_element(t.begin(), t.end());
+    const double tstep = (tmax - tmin) / (double)t.size();      // step(range) of ADKSampler.jl:267
+    if (s.sampling == ETRAJ_SAMPLING_STEP) {
+        double kd_step = s.ss_kd_num > 1 ? (2 * std::fabs(s.ss_kd_max)) / s.ss_kd_num : 1.0;
+        double kz_step = (s.dimension == 3 && s.ss_kz_num > 1) ? (2 * std::fabs(s.ss_kz_max)) / s.ss_kz_num : 1.0;
+        S.dkdt_base = tstep * kd_step * kz_step;
+    } else if (s.sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        double kd_span = s.mc_kd_max > 0 ? 2 * s.mc_kd_max : 1.0;
+        double kz_span = s.mc_kz_max > 0 ? 2 * s.mc_kz_max : 1.0;  // no dimension check (:274)
+        S.dkdt_base = tstep * kd_span * kz_span / s.mc_kt_num;
+    } else {
+        S.dkdt_base = (s.sample_t_hi - s.sample_t_lo) / s.sample_t_num / s.mc_kt_num;
+    }
+    const double kappa = std::sqrt(2 * T.Ip), nn = T.Zd / kappa;
+    S.nstar = nn;
+    S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
+    S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
+    return ETRAJ_OK;
+}
+
+int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const TargetP& T, IntegP& P) {
+    if (!rd) return fail(c, ETRAJ_EINVAL, "run descriptor is NULL");
+    const etraj_run_desc& r = *rd;
+    if (r.dimension != 2 && r.dimension != 3)
+        return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `dimension` must be either 2 or 3.");
+    if (r.phase_method < 0 || r.phase_method > 2) return fail(c, ETRAJ_EINVAL, "undefined phase method");
+    if (r.integrator == ETRAJ_INTEG_TSIT5)
+        return fail(c, ETRAJ_EUNSUPPORTED, "the adaptive Tsit5 integrator is not available in this engine build");
+    if (r.integrator != ETRAJ_INTEG_RK4) return fail(c, ETRAJ_EINVAL, "unknown integrator");
+    if (!(r.dt > 0)) return fail(c, ETRAJ_EINVAL, "RK4 needs dt > 0");
+    if (r.precision != ETRAJ_PREC_FP64 && r.precision != ETRAJ_PREC_MIXED) return fail(c, ETRAJ_EINVAL, "unknown precision");
+    if (r.precision == ETRAJ_PREC_MIXED && !(r.restep_radius >= 0)) return fail(c, ETRAJ_EINVAL, "restep_radius must be >= 0");
+    for (int d = 0; d < r.dimension; ++d)
+        if (r.final_p_num[d] < 1 || !(r.final_p_max[d] > 0))
+            return fail(c, ETRAJ_EINVAL, "final_p_num must be >= 1 and final_p_max > 0 in every dimension");
+    int64_t npix = 1;
+    for (int d = 0; d < r.dimension; ++d) npix *= r.final_p_num[d];
+    if (npix > ((int64_t)1 << 34)) return fail(c, ETRAJ_EINVAL, "momentum grid too large");
+    P = IntegP{};
+    P.L = L;
+    P.T = T;
+    P.dim = r.dimension;
+    P.phase = r.phase_method;
+    P.mixed = r.precision == ETRAJ_PREC_MIXED;
+    P.quirks = r.quirks;
+    P.rows = rows_for(r.dimension, r.phase_method);
+    for (int d = 0; d < 3; ++d) {
+        P.pnum[d] = d < r.dimension ? r.final_p_num[d] : 1;
+        P.pmax[d] = d < r.dimension ? r.final_p_max[d] : 1.0;
+    }
+    P.tf = r.traj_t_final;
+    P.dt = r.dt;
+    P.restep_radius = r.restep_radius;
+    return ETRAJ_OK;
+}
+
+int64_t npix_of(const IntegP& P) { return (int64_t)P.pnum[0] * P.pnum[1] * (P.dim == 3 ? P.pnum[2] : 1); }
+
+// Segments of equal t0 in a Julia init matrix (columns of one
+// gen_electron_batch share tr), so that each wavefront shares its field.
+void build_segments(const double* init, int64_t n, int rows, int dim, std::vector<int64_t>& start,
+                    std::vector<int32_t>& count, int& cps) {
+    start.clear();
+    count.clear();
+    int64_t maxlen = 0;
+    int64_t i = 0;
+    while (i < n) {
+        int64_t j = i + 1;
+        const double t0 = init[i * rows + 2 * dim];
+        while (j < n && init[j * rows + 2 * dim] == t0 && j - i < ((int64_t)1 << 30)) ++j;
+        start.push_back(i);
+        count.push_back((int32_t)(j - i));
+        maxlen = std::max(maxlen, j - i);
+        i = j;
+    }
+    cps = (int)((maxlen + 63) / 64);
+}
+
+int check_init(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int32_t rows) {
+    if (n < 0) return fail(c, ETRAJ_EINVAL, "n must be >= 0");
+    if (n > 0 && !init) return fail(c, ETRAJ_EINVAL, "init is NULL");
+    if (rows != P.rows)
+        return fail(c, ETRAJ_EINVAL, "init has " + std::to_string(rows) + " rows, the dimension/phase method needs " +
+                                         std::to_string(P.rows));
+    for (int64_t i = 0; i < n; ++i)
+        if (!(init[i * rows + 2 * P.dim] < P.tf))
+            return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `traj_t_final` must be greater than every t0");
+    return ETRAJ_OK;
+}
+
+// integrate n columns; out = OUT_RECORDS or OUT_CONTRIB
+int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int out, OutP& O) {
+    std::vector<int64_t> ss;
+    std::vector<int32_t> sc;
+    int cps = 0;
+    build_segments(init, n, P.rows, P.dim, ss, sc, cps);
+    HIP_TRY(c, c->init.ensure((size_t)n * P.rows));
+    HIP_TRY(c, c->seg_start.ensure(ss.size()));
+    HIP_TRY(c, c->counts.ensure(sc.size()));
+    HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * ss.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, c->seg_start.p, c->counts.p, (int64_t)ss.size(), cps));
+    return ETRAJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int etraj_abi_version(void) { return ETRAJ_ABI_VERSION; }
+
+int etraj_ctx_create(int device, etraj_ctx** out) {
+    if (!out) return ETRAJ_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ETRAJ_EHIP;
+    if (device < 0 || device >= ndev) return ETRAJ_EINVAL;
+    etraj_ctx* c = new (std::nothrow) etraj_ctx();
+    if (!c) return ETRAJ_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ETRAJ_EHIP;
+    }
+    *out = c;
+    return ETRAJ_OK;
+}
+
+void etraj_ctx_destroy(etraj_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->init.release(); c->pmd.release(); c->pmd_out.release(); c->io1.release(); c->io2.release();
+    c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
+    c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* etraj_last_error(const etraj_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int etraj_laser_field(etraj_ctx* c, const etraj_laser_desc* laser, const double* t, int64_t n, double* Fx,
+                      double* Fy, double* Ax, double* Ay) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    LaserP L;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (n < 0 || (n > 0 && !t)) return fail(c, ETRAJ_EINVAL, "bad time array");
+    if (n == 0) return ETRAJ_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->io1.ensure(n)); HIP_TRY(c, c->io2.ensure(n)); HIP_TRY(c, c->io3.ensure(n));
+    HIP_TRY(c, c->io4.ensure(n)); HIP_TRY(c, c->t0s.ensure(n));
+    HIP_TRY(c, hipMemcpyAsync(c->t0s.p, t, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_laser_field(c->stream, L, c->t0s.p, n, c->io1.p, c->io2.p, c->io3.p, c->io4.p));
+    double* outs[4] = {Fx, Fy, Ax, Ay};
+    double* devs[4] = {c->io1.p, c->io2.p, c->io3.p, c->io4.p};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k]) HIP_TRY(c, hipMemcpyAsync(outs[k], devs[k], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ETRAJ_OK;
+}
+
+int etraj_target_eval(etraj_ctx* c, const etraj_target_desc* target, const double* xyz, int64_t n, double* V,
+                      double* F) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    TargetP T;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (n < 0 || (n > 0 && !xyz)) return fail(c, ETRAJ_EINVAL, "bad point array");
+    if (n == 0) return ETRAJ_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->io1.ensure(3 * n)); HIP_TRY(c, c->io2.ensure(n)); HIP_TRY(c, c->io3.ensure(3 * n));
+    HIP_TRY(c, hipMemcpyAsync(c->io1.p, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_target_eval(c->stream, T, c->io1.p, n, c->io2.p, c->io3.p));
+    if (V) HIP_TRY(c, hipMemcpyAsync(V, c->io2.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (F) HIP_TRY(c, hipMemcpyAsync(F, c->io3.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ETRAJ_OK;
+}
+
+int etraj_batch_times(etraj_ctx* c, const etraj_sampler_desc* s, double* t0) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    if (!s || !t0) return fail(c, ETRAJ_EINVAL, "NULL argument");
+    std::vector<double> t;
+    if (int rc = batch_times_of(c, *s, t)) return rc;
+    std::memcpy(t0, t.data(), sizeof(double) * t.size());
+    return ETRAJ_OK;
+}
+
+int64_t etraj_batch_max_size(const etraj_sampler_desc* s) { return s ? batch_max_size_of(*s) : -1; }
+
+int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                             const etraj_sampler_desc* sampler, int32_t batch_id, double* out, int64_t capacity,
+                             int64_t* count) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    LaserP L;
+    TargetP T;
+    SamplerP S;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
+    if (target->l != 0)
+        return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
+    std::vector<double> t;
+    if (int rc = batch_times_of(c, *sampler, t)) return rc;
+    if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
+    if (batch_id < 1 || batch_id > S.t_num) return fail(c, ETRAJ_EINVAL, "batch_id out of range");
+    if (!count || (capacity > 0 && !out)) return fail(c, ETRAJ_EINVAL, "NULL output");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->t0s.ensure(t.size()));
+    HIP_TRY(c, c->init.ensure((size_t)S.max_bs * S.rows));
+    HIP_TRY(c, c->counts.ensure(1));
+    HIP_TRY(c, hipMemcpyAsync(c->t0s.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_adk_sample(c->stream, S, L, T, c->t0s.p, batch_id - 1, 1, c->init.p, c->counts.p));
+    int32_t cnt = 0;
+    HIP_TRY(c, hipMemcpyAsync(&cnt, c->counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *count = cnt;
+    if (cnt > capacity) return fail(c, ETRAJ_ECAPACITY, "output capacity too small for " + std::to_string(cnt) + " columns");
+    if (cnt > 0) {
+        HIP_TRY(c, hipMemcpyAsync(out, c->init.p, sizeof(double) * cnt * S.rows, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return ETRAJ_OK;
+}
+
+int etraj_integrate_records(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                            const etraj_run_desc* run, const double* init, int64_t n, int32_t rows, double* records,
+                            int32_t* status) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    LaserP L;
+    TargetP T;
+    IntegP P;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (int rc = make_integ(c, run, L, T, P)) return rc;
+    if (int rc = check_init(c, P, init, n, rows)) return rc;
+    if (n == 0) return ETRAJ_OK;
+    if (!records || !status) return fail(c, ETRAJ_EINVAL, "NULL output");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->io1.ensure((size_t)12 * n));
+    HIP_TRY(c, c->status.ensure(n));
+    OutP O{};
+    O.records = c->io1.p;
+    O.status = c->status.p;
+    if (int rc = integrate_columns(c, P, init, n, OUT_RECORDS, O)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(records, c->io1.p, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(status, c->status.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ETRAJ_OK;
+}
+
+int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                            const etraj_run_desc* run, const double* init, int64_t n, int32_t rows, void* pmd,
+                            double* ion_prob, double* ion_prob_uncollected, int64_t* n_failed) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    LaserP L;
+    TargetP T;
+    IntegP P;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (int rc = make_integ(c, run, L, T, P)) return rc;
+    if (int rc = check_init(c, P, init, n, rows)) return rc;
+    if (!pmd || !ion_prob || !ion_prob_uncollected || !n_failed) return fail(c, ETRAJ_EINVAL, "NULL output");
+    *n_failed = 0;
+    if (n == 0) return ETRAJ_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->contrib.ensure(n));
+    OutP O{};
+    O.contrib = c->contrib.p;
+    if (int rc = integrate_columns(c, P, init, n, OUT_CONTRIB, O)) return rc;
+    std::vector<Contrib> h(n);
+    HIP_TRY(c, hipMemcpyAsync(h.data(), c->contrib.p, sizeof(Contrib) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // accumulate in column order, as the reference's per-electron loop
+    // (TrajectorySimulation.jl:344-397)
+    double* out = static_cast<double*>(pmd);
+    int64_t nf = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const Contrib& k = h[i];
+        if (k.status == 1) { ++nf; continue; }
+        if (k.status == 2) continue;
+        *ion_prob += k.prob;
+        if (k.status == 3) { *ion_prob_uncollected += k.prob; continue; }
+        if (P.phase == ETRAJ_PHASE_CTMC) {
+            out[k.lin] += k.re;
+        } else {
+            out[2 * k.lin] += k.re;
+            out[2 * k.lin + 1] += k.im;
+        }
+    }
+    *n_failed = nf;
+    return ETRAJ_OK;
+}
+
+static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                        const etraj_sampler_desc* sampler, const etraj_run_desc* run, double* host_out,
+                        const double** dev_out, etraj_stats* stats) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    auto wall0 = std::chrono::steady_clock::now();
+    LaserP L;
+    TargetP T;
+    IntegP P;
+    SamplerP S;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
+    if (target->l != 0)
+        return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
+    std::vector<double> t;
+    if (int rc = batch_times_of(c, *sampler, t)) return rc;
+    if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
+    if (int rc = make_integ(c, run, L, T, P)) return rc;
+    if (sampler->dimension != run->dimension || sampler->phase_method != run->phase_method)
+        return fail(c, ETRAJ_EINVAL, "sampler and run disagree on dimension / phase method");
+    if (!(run->traj_t_final > sampler->sample_t_hi))
+        return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `traj_t_final` must be greater than the last time point of `sample_t_intv`.");
+    if (!(sampler->sample_t_lo < sampler->sample_t_hi) && sampler->sample_t_num > 1)
+        return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval.");
+    HIP_TRY(c, hipSetDevice(c->device));
+
+    const bool coh = P.phase != ETRAJ_PHASE_CTMC;
+    const int64_t npix = npix_of(P);
+    const int64_t plen = coh ? 2 * npix : npix;
+    // slabs of batches bound the init buffer to ~2^25 columns
+    const int64_t max_bs = std::max<int64_t>(S.max_bs, 1);
+    int64_t per_slab = std::max<int64_t>(1, ((int64_t)1 << 25) / max_bs);
+    per_slab = std::min<int64_t>(per_slab, S.t_num);
+    const int cps = (int)((max_bs + 63) / 64);
+
+    HIP_TRY(c, c->pmd.ensure(plen));
+    HIP_TRY(c, c->scal.ensure(2));
+    HIP_TRY(c, c->cnt.ensure(CNT_N));
+    HIP_TRY(c, c->t0s.ensure(t.size()));
+    HIP_TRY(c, c->init.ensure((size_t)per_slab * max_bs * S.rows));
+    HIP_TRY(c, c->counts.ensure(per_slab));
+    HIP_TRY(c, c->seg_start.ensure(per_slab));
+    std::vector<int64_t> ss(per_slab);
+    for (int64_t i = 0; i < per_slab; ++i) ss[i] = i * max_bs;
+    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * per_slab, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->t0s.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * plen, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N, c->stream));
+
+    OutP O{};
+    O.pmd = c->pmd.p;
+    O.scal = c->scal.p;
+    O.cnt = c->cnt.p;
+    const int nslab = (int)((S.t_num + per_slab - 1) / per_slab);
+    std::vector<hipEvent_t> ev(3 * nslab, nullptr);
+    for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
+    int64_t launched = 0;
+    std::vector<int32_t> hcounts(per_slab);
+    int rc = ETRAJ_OK;
+    for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
+        const int b0 = (int)(k * per_slab);
+        const int nb = (int)std::min<int64_t>(per_slab, S.t_num - b0);
+        hipError_t e = hipEventRecord(ev[3 * k], c->stream);
+        if (e == hipSuccess) e = launch_adk_sample(c->stream, S, L, T, c->t0s.p, b0, nb, c->init.p, c->counts.p);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], c->stream);
+        if (e == hipSuccess) e = launch_rk4(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->counts.p, nb, cps);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hcounts.data(), c->counts.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "run_job slab");
+            break;
+        }
+        for (int i = 0; i < nb; ++i) launched += hcounts[i];
+    }
+    double ms_s = 0, ms_i = 0;
+    if (rc == ETRAJ_OK) {
+        for (int k = 0; k < nslab; ++k) {
+            float a = 0, b = 0;
+            (void)hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
+            (void)hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+            ms_s += a;
+            ms_i += b;
+        }
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+
+    const double* result = c->pmd.p;
+    if (coh) {
+        HIP_TRY(c, c->pmd_out.ensure(npix));
+        HIP_TRY(c, launch_finalize_abs2(c->stream, c->pmd.p, npix, c->pmd_out.p));
+        result = c->pmd_out.p;
+    }
+    double scal[2];
+    int64_t cnt[CNT_N];
+    HIP_TRY(c, hipMemcpyAsync(scal, c->scal.p, sizeof scal, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(cnt, c->cnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (host_out) HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dev_out) *dev_out = result;
+    if (stats) {
+        stats->ion_prob = scal[0];
+        stats->ion_prob_uncollected = scal[1];
+        stats->num_effective_traj = launched;
+        stats->n_failed = cnt[CNT_FAILED];
+        stats->n_bound = cnt[CNT_BOUND];
+        stats->n_restep = cnt[CNT_RESTEP];
+        stats->n_steps = cnt[CNT_STEPS];
+        stats->ms_sample = ms_s;
+        stats->ms_integrate = ms_i;
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    }
+    return ETRAJ_OK;
+}
+
+int etraj_run_job(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                  const etraj_sampler_desc* sampler, const etraj_run_desc* run, double* pmd_out, etraj_stats* stats) {
+    if (c && !pmd_out) return fail(c, ETRAJ_EINVAL, "pmd_out is NULL");
+    return run_job_impl(c, laser, target, sampler, run, pmd_out, nullptr, stats);
+}
+
+int etraj_run_job_device(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                         const etraj_sampler_desc* sampler, const etraj_run_desc* run, const double** pmd_dev,
+                         etraj_stats* stats) {
+    if (c && !pmd_dev) return fail(c, ETRAJ_EINVAL, "pmd_dev is NULL");
+    return run_job_impl(c, laser, target, sampler, run, nullptr, pmd_dev, stats);
+}
+
+}  // extern "C"

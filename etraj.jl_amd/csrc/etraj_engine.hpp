@@ -1,0 +1,68 @@
+// etraj_engine.hpp -- kernel-side parameter blocks and launcher declarations
+// shared by the kernels (etraj_kernels.hip) and the C ABI (etraj_capi.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "etraj_physics.hpp"
+
+namespace etraj {
+
+// Resolved ADK sampler parameters for one job (ADKSampler.jl:19-150 + the
+// per-job constants of gen_electron_batch).
+struct SamplerP {
+    int dim, phase, sampling, prefix;
+    int t_num, kd_num, kz_num, rows;
+    int64_t nk;        // samples per batch
+    int64_t max_bs;    // column stride of one batch in the init buffer
+    double t_lo, t_hi;
+    double kd_max, kz_max, mc_kd_max, mc_kz_max, kd_sigma, kz_sigma;
+    double cutoff;
+    double dkdt_base;  // dk dt of one sample before any 1/pdf factor (:267-276)
+    double c_pre;      // c = 2^(n/2+1) kappa^(2n+1/2) Gamma(n/2+1) (:233)
+    double numer;      // C * D^0_00 * Y_00 (:255)
+    double nstar;      // n* = Z/kappa (:232)
+    uint64_t seed;
+};
+
+struct IntegP {
+    LaserP L;
+    TargetP T;
+    int dim, phase, mixed, quirks, rows;
+    int pnum[3];
+    double pmax[3];
+    double tf, dt, restep_radius;
+};
+
+// per-electron binning record for host-side accumulation in column order
+struct Contrib {
+    int64_t lin;   // 0-based linear PMD index, -1 if none
+    double re, im; // contribution (CTMC: re = W)
+    double prob;
+    int32_t status;  // 0 collected, 1 failed, 2 bound, 3 ionized but outside the grid
+    int32_t pad;
+};
+
+enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
+enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_N = 4 };
+
+struct OutP {
+    double* pmd;       // CTMC: npix; coherent: 2*npix (re, im)
+    double* scal;      // [ion_prob, ion_prob_uncollected]
+    int64_t* cnt;      // CNT_*
+    double* records;   // 12 per electron
+    int32_t* status;
+    Contrib* contrib;
+};
+
+hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, int64_t n, double* Fx, double* Fy,
+                              double* Ax, double* Ay);
+hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F);
+hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L, const TargetP& T, const double* t0s,
+                             int b_first, int nb, double* out, int32_t* counts);
+hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg);
+hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
+
+}  // namespace etraj

@@ -1,0 +1,537 @@
+// etraj_kernels.hip -- gfx950 kernels of the trajectory engine.
+//
+// Pipeline (one call of etraj_run_job):
+//   k_adk_sample      one workgroup per t0 batch: ADK tunnel exit, weight,
+//                     cutoff and order-preserving compaction into the init
+//                     buffer (Julia column layout, one column per electron).
+//   k_rk4             one electron per lane, 64 electrons of ONE batch per
+//                     wavefront: they share t0 and the step size, so the laser
+//                     field of 32 steps is evaluated once per wave (one time
+//                     point per lane) and broadcast with v_readlane; fixed-step
+//                     RK4 in FP64 or FP32+compensation with FP64 re-steps near
+//                     the core; FP64 Kepler asymptote; FP64 atomics into the PMD.
+//   k_finalize        |amplitude|^2 for coherent sums.
+// The work is VALU-bound ODE arithmetic (no dense contraction): no MFMA.
+#include "etraj_engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace etraj {
+
+// ---------------------------------------------------------------------------
+// lane helpers
+// ---------------------------------------------------------------------------
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ inline double readlane_f64(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline double uniform_f64(double v) {
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ inline int64_t uniform_i64(int64_t v) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// field / target evaluation (LaserFx/Fy/Ax/Ay, TargetPotential/TargetForce)
+// ---------------------------------------------------------------------------
+__global__ void k_laser_field(LaserP L, const double* __restrict__ t, int64_t n, double* Fx, double* Fy, double* Ax,
+                              double* Ay) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double fx, fy, ax, ay;
+    laser_field(L, t[i], fx, fy);
+    laser_potential(L, t[i], ax, ay);
+    if (Fx) Fx[i] = fx;
+    if (Fy) Fy[i] = fy;
+    if (Ax) Ax[i] = ax;
+    if (Ay) Ay[i] = ay;
+}
+
+__global__ void k_target_eval(TargetP T, const double* __restrict__ xyz, int64_t n, double* V, double* F) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    double r2 = x * x + y * y + z * z;
+    double v, s;
+    target_vs<double>(T, r2, true, false, v, s);
+    if (V) V[i] = v;
+    if (F) {
+        F[3 * i] = s * x;
+        F[3 * i + 1] = s * y;
+        F[3 * i + 2] = s * z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ADK sampler (ADKSampler.jl:167-396), atoms with l = 0: the prefactor's
+// Wigner-D and Y_00 factors are constants, so amp is real and positive and
+// the initial phase angle(amp) is 0.
+// ---------------------------------------------------------------------------
+struct AdkBatch {
+    double tr, F, cphi, sphi, c_cc, dkdt;
+};
+
+__device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, const TargetP& T, double tr) {
+    AdkBatch B;
+    B.tr = tr;
+    double Fx, Fy;
+    laser_field(L, tr, Fx, Fy);
+    B.F = hypot(Fx, Fy);                                            // :172
+    double phi = atan2(-Fy, -Fx);                                   // :173
+    sincos(phi, &B.sphi, &B.cphi);
+    double Ip = T.Ip;
+    double gam = 0.0;                                               // :183-187
+    if (L.n == 1) gam = L.c[0].w * sqrt(2 * Ip) / (L.c[0].F0 * mono_envelope(L.c[0], tr));
+    double kappa = sqrt(2 * Ip), nn = T.Zd / kappa;
+    const double e0 = 2.71828182845904523;
+    B.c_cc = pow(2.0, 3 * nn / 2 + 1) * pow(kappa, 5 * nn + 0.5) * pow(B.F, -nn) * pow(1 + 2 * gam / e0, -nn);  // :235
+    if (S.sampling == ETRAJ_SAMPLING_STRATIFIED)
+        B.dkdt = (S.t_hi - S.t_lo) / S.t_num / (double)S.nk;
+    else
+        B.dkdt = S.dkdt_base;
+    return B;
+}
+
+// Evaluates sample j of batch b; returns false if it is discarded.
+__device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const AdkBatch& B, int b, int64_t j, double* col) {
+    double kd0, kz0 = 0.0, w = B.dkdt;
+    if (S.sampling == ETRAJ_SAMPLING_STEP) {
+        int64_t ikd = j / S.kz_num, ikz = j - ikd * S.kz_num;      // ikd outer, ikz inner (:318-319)
+        kd0 = grid_point(-S.kd_max, S.kd_max, S.kd_num, (int)ikd);
+        if (S.dim == 3) kz0 = grid_point(-S.kz_max, S.kz_max, S.kz_num, (int)ikz);
+    } else {
+        U64x4 o = philox((uint64_t)j, (uint64_t)b, 0, kTagK, S.seed, 0);
+        if (S.sampling == ETRAJ_SAMPLING_MONTECARLO) {           // shared_methods.jl:4-12
+            kd0 = (unit_co(o.v[0]) - 0.5) * 2 * S.mc_kd_max;
+            if (S.dim == 3) {
+                kz0 = (unit_co(o.v[1]) - 0.5) * 2 * S.mc_kz_max;
+                if (S.mc_kd_max == 0) kd0 = 0.0;
+                if (S.mc_kz_max == 0) kz0 = 0.0;
+            }
+        } else {                                                  // stratified Gaussian (extension)
+            double r = sqrt(-2.0 * log(unit_oc(o.v[0])));
+            double a = 2 * kPi * unit_co(o.v[1]);
+            double sa, ca;
+            sincos(a, &sa, &ca);
+            kd0 = S.kd_sigma * (r * ca);
+            w = w / (exp(-0.5 * (kd0 / S.kd_sigma) * (kd0 / S.kd_sigma)) / (S.kd_sigma * sqrt(2 * kPi)));
+            if (S.dim == 3) {
+                kz0 = S.kz_sigma * (r * sa);
+                w = w / (exp(-0.5 * (kz0 / S.kz_sigma) * (kz0 / S.kz_sigma)) / (S.kz_sigma * sqrt(2 * kPi)));
+            }
+        }
+    }
+    if (fabs(kd0) < 1e-4) return false;                           // :312,:321
+    const double Ip = T.Ip;
+    double kx0 = kd0 * -B.sphi, ky0 = kd0 * B.cphi;                // :324-325
+    double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / B.F;          // :326
+    double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * B.F));  // :198
+    if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
+        double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
+        amp *= cpre * S.numer / pow((kx0 * kx0 + ky0 * ky0 + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
+    }
+    if (S.prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(B.F);             // :266,:283
+    double rate = amp * amp;
+    if (isnan(rate) || rate < S.cutoff) return false;              // :332
+    if (S.dim == 3) {
+        col[0] = r0 * B.cphi; col[1] = r0 * B.sphi; col[2] = 0.0;
+        col[3] = kx0; col[4] = ky0; col[5] = kz0; col[6] = B.tr; col[7] = rate;
+        if (S.phase != ETRAJ_PHASE_CTMC) col[8] = 0.0;
+    } else {
+        col[0] = r0 * B.cphi; col[1] = r0 * B.sphi;
+        col[2] = kx0; col[3] = ky0; col[4] = B.tr; col[5] = rate;
+        if (S.phase != ETRAJ_PHASE_CTMC) col[6] = 0.0;
+    }
+    return true;
+}
+
+// One workgroup per batch; survivors are compacted in sample order with a
+// wave ballot + LDS scan, so column order equals the reference's loop order.
+__global__ __launch_bounds__(256) void k_adk_sample(SamplerP S, LaserP L, TargetP T, const double* __restrict__ t0s,
+                                                    int b_first, double* __restrict__ out, int32_t* __restrict__ counts) {
+    __shared__ int wave_tot[4];
+    __shared__ int base_sh;
+    const int b = b_first + blockIdx.x;
+    const AdkBatch B = adk_batch_setup(S, L, T, t0s[b]);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double* bout = out + (int64_t)blockIdx.x * S.max_bs * S.rows;
+    if (tid == 0) base_sh = 0;
+    __syncthreads();
+    double col[9];
+    for (int64_t j0 = 0; j0 < S.nk; j0 += 256) {
+        int64_t j = j0 + tid;
+        bool keep = (j < S.nk) && adk_sample(S, T, B, b, j, col);
+        uint64_t m = __ballot(keep);
+        int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (lane == 0) wave_tot[wv] = __popcll(m);
+        __syncthreads();
+        int off = base_sh;
+        for (int w = 0; w < wv; ++w) off += wave_tot[w];
+        int tot = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        if (keep) {
+            double* c = bout + (int64_t)(off + pre) * S.rows;
+            for (int r = 0; r < S.rows; ++r) c[r] = col[r];
+        }
+        __syncthreads();
+        if (tid == 0) base_sh += tot;
+        __syncthreads();
+    }
+    if (tid == 0) counts[blockIdx.x] = base_sh;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-step RK4 integrator + Kepler asymptote + binning.
+// ---------------------------------------------------------------------------
+template <int DIM, int PHASE>
+struct Lane {
+    double t0, prob, phase0;
+    double r[DIM], p[DIM];
+};
+
+template <int DIM, int PHASE>
+__device__ inline void load_column(const double* col, Lane<DIM, PHASE>& e) {
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        e.r[i] = col[i];
+        e.p[i] = col[DIM + i];
+    }
+    e.t0 = col[2 * DIM];
+    e.prob = col[2 * DIM + 1];
+    e.phase0 = (PHASE != ETRAJ_PHASE_CTMC) ? col[2 * DIM + 2] : 0.0;
+}
+
+// Outputs of one electron after traj_t_final.
+template <int DIM, int PHASE, int OUT>
+__device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, bool active, bool failed,
+                            const double* r, const double* p, double phase, double t0, double prob) {
+    Final f = kepler_final<DIM, PHASE>(P.T, P.quirks, r, p, phase, t0);
+    if (failed) f.ionized = false;
+    bool bound = active && !failed && !f.ionized;
+    int64_t lin = -1;
+    bool inb = false;
+    double cre = 0.0, cim = 0.0;
+    if (active && f.ionized) {
+        int64_t idx[3] = {1, 1, 1};
+        inb = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            idx[d] = bin_index(f.pinf[d], P.pmax[d], P.pnum[d]);
+            if (idx[d] < 1 || idx[d] > P.pnum[d]) inb = false;
+        }
+        if (inb) {
+            lin = (idx[0] - 1) + (int64_t)P.pnum[0] * ((idx[1] - 1) + (DIM == 3 ? (int64_t)P.pnum[1] * (idx[2] - 1) : 0));
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                cre = prob;
+            } else {
+                double s = sqrt(prob), sp, cp;
+                sincos(f.phase, &sp, &cp);
+                cre = s * cp;
+                cim = s * sp;
+            }
+        }
+    }
+    if (OUT == OUT_PMD) {
+        if (inb) {
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                unsafeAtomicAdd(O.pmd + lin, cre);
+            } else {
+                unsafeAtomicAdd(O.pmd + 2 * lin, cre);
+                unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
+            }
+        }
+        double ip = wave_sum_f64((active && f.ionized) ? prob : 0.0);
+        double iu = wave_sum_f64((active && f.ionized && !inb) ? prob : 0.0);
+        int64_t nf = wave_sum_i64((active && failed) ? 1 : 0);
+        int64_t nb = wave_sum_i64(bound ? 1 : 0);
+        if (lane_id() == 0) {
+            if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
+            if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+            if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
+            if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
+        }
+    } else if (OUT == OUT_RECORDS) {
+        if (active) {
+            double* rec = O.records + 12 * col_idx;
+            rec[0] = r[0]; rec[1] = r[1]; rec[2] = DIM == 3 ? r[2] : 0.0;
+            rec[3] = p[0]; rec[4] = p[1]; rec[5] = DIM == 3 ? p[2] : 0.0;
+            rec[6] = phase; rec[7] = f.E;
+            rec[8] = f.pinf[0]; rec[9] = f.pinf[1]; rec[10] = f.pinf[2];
+            rec[11] = f.phase;
+            O.status[col_idx] = failed ? 1 : (f.ionized ? 0 : 2);
+        }
+    } else {  // OUT_CONTRIB: the caller bins on the host, in column order
+        if (active) {
+            Contrib c;
+            c.lin = lin;
+            c.re = cre;
+            c.im = cim;
+            c.prob = prob;
+            c.status = failed ? 1 : (f.ionized ? (inb ? 0 : 3) : 2);
+            c.pad = 0;
+            O.contrib[col_idx] = c;
+        }
+    }
+}
+
+// 64 electrons of one segment (equal t0) per wavefront. Segment s covers
+// columns [seg_start[s], seg_start[s] + seg_count[s]).
+template <int DIM, int PHASE, bool MIXED, int OUT>
+__global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
+                                             const int64_t* __restrict__ seg_start,
+                                             const int32_t* __restrict__ seg_count, int64_t n_seg,
+                                             int chunks_per_seg) {
+    const int lane = lane_id();
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t seg = gw / chunks_per_seg;
+    const int chunk = (int)(gw - seg * chunks_per_seg);
+    if (seg >= n_seg) return;
+    const int32_t cnt = seg_count[seg];
+    const int first = chunk * 64;
+    if (first >= cnt) return;
+    const int64_t cbase = seg_start[seg];
+    const bool active = first + lane < cnt;
+    const int64_t col_idx = cbase + first + (active ? lane : 0);
+
+    Lane<DIM, PHASE> e;
+    load_column<DIM, PHASE>(init + col_idx * P.rows, e);
+    const double t0 = uniform_f64(init[(cbase + first) * P.rows + 2 * DIM]);
+    const double span = P.tf - t0;
+    int64_t n = (int64_t)ceil(span / P.dt);
+    if (n < 1) n = 1;
+    const double h = span / (double)n;
+    const double hh = h * 0.5;
+    const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
+
+    double Fpx, Fpy;  // field at the start of the current step
+    laser_field(P.L, t0, Fpx, Fpy);
+    int64_t nre = 0;
+    bool bad = false;
+
+    double rf[DIM], pf[DIM], phase;
+    if (!MIXED) {
+        State<DIM, double> u;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) { u.r[i] = e.r[i]; u.p[i] = e.p[i]; }
+        phase = e.phase0;
+        for (int64_t base = 0; base < n; base += 32) {
+            double fx, fy;
+            laser_field(P.L, t0 + (double)(2 * base + 1 + lane) * hh, fx, fy);
+            const int nst = (int)((n - base) < 32 ? (n - base) : 32);
+            for (int s = 0; s < nst; ++s) {
+                const double Fmx = readlane_f64(fx, 2 * s), Fmy = readlane_f64(fy, 2 * s);
+                const double F1x = readlane_f64(fx, 2 * s + 1), F1y = readlane_f64(fy, 2 * s + 1);
+                State<DIM, double> d;
+                double r2m;
+                double dph = rk4_incr<DIM, PHASE, double>(P.T, saeq, u, h, Fpx, Fpy, Fmx, Fmy, F1x, F1y, d, r2m);
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    u.r[i] = u.r[i] + d.r[i];
+                    u.p[i] = u.p[i] + d.p[i];
+                }
+                phase = phase + dph;
+                Fpx = F1x;
+                Fpy = F1y;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) { rf[i] = u.r[i]; pf[i] = u.p[i]; }
+    } else {
+        // FP32 state s with Kahan compensation c (value = s - c), FP64
+        // Kahan-compensated phase, FP64 re-step when any stage has r < r_c.
+        State<DIM, float> us, uc;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            us.r[i] = (float)e.r[i];
+            us.p[i] = (float)e.p[i];
+            uc.r[i] = -(float)(e.r[i] - (double)us.r[i]);
+            uc.p[i] = -(float)(e.p[i] - (double)us.p[i]);
+        }
+        phase = e.phase0;
+        double phc = 0.0;
+        const float hf = (float)h;
+        const float rc2 = (float)(P.restep_radius * P.restep_radius);
+        float Fpxf = (float)Fpx, Fpyf = (float)Fpy;
+        for (int64_t base = 0; base < n; base += 32) {
+            double fx, fy;
+            laser_field(P.L, t0 + (double)(2 * base + 1 + lane) * hh, fx, fy);
+            const float fxf = (float)fx, fyf = (float)fy;
+            const int nst = (int)((n - base) < 32 ? (n - base) : 32);
+            for (int s = 0; s < nst; ++s) {
+                const float Fmx = readlane_f32(fxf, 2 * s), Fmy = readlane_f32(fyf, 2 * s);
+                const float F1x = readlane_f32(fxf, 2 * s + 1), F1y = readlane_f32(fyf, 2 * s + 1);
+                State<DIM, float> d;
+                float r2m;
+                float dph = rk4_incr<DIM, PHASE, float>(P.T, saeq, us, hf, Fpxf, Fpyf, Fmx, Fmy, F1x, F1y, d, r2m);
+                double dphd;
+                if (r2m < rc2) {
+                    // close encounter: redo this step in FP64 from s - c
+                    const double Dmx = readlane_f64(fx, 2 * s), Dmy = readlane_f64(fy, 2 * s);
+                    const double D1x = readlane_f64(fx, 2 * s + 1), D1y = readlane_f64(fy, 2 * s + 1);
+                    const double D0x = (s == 0) ? Fpx : readlane_f64(fx, 2 * s - 1);
+                    const double D0y = (s == 0) ? Fpy : readlane_f64(fy, 2 * s - 1);
+                    State<DIM, double> ud, dd;
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        ud.r[i] = (double)us.r[i] - (double)uc.r[i];
+                        ud.p[i] = (double)us.p[i] - (double)uc.p[i];
+                    }
+                    double r2d;
+                    dphd = rk4_incr<DIM, PHASE, double>(P.T, saeq, ud, h, D0x, D0y, Dmx, Dmy, D1x, D1y, dd, r2d);
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        double nr = ud.r[i] + dd.r[i], np = ud.p[i] + dd.p[i];
+                        us.r[i] = (float)nr;
+                        uc.r[i] = -(float)(nr - (double)us.r[i]);
+                        us.p[i] = (float)np;
+                        uc.p[i] = -(float)(np - (double)us.p[i]);
+                    }
+                    ++nre;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        float y = d.r[i] - uc.r[i];
+                        float t = us.r[i] + y;
+                        uc.r[i] = (t - us.r[i]) - y;
+                        us.r[i] = t;
+                        y = d.p[i] - uc.p[i];
+                        t = us.p[i] + y;
+                        uc.p[i] = (t - us.p[i]) - y;
+                        us.p[i] = t;
+                    }
+                    dphd = (double)dph;
+                }
+                // Kahan-compensated FP64 phase sum
+                double y = dphd - phc;
+                double t = phase + y;
+                phc = (t - phase) - y;
+                phase = t;
+                Fpxf = F1x;
+                Fpyf = F1y;
+            }
+            Fpx = readlane_f64(fx, 2 * nst - 1);
+            Fpy = readlane_f64(fy, 2 * nst - 1);
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            rf[i] = (double)us.r[i] - (double)uc.r[i];
+            pf[i] = (double)us.p[i] - (double)uc.p[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) bad = bad || isnan(rf[i]) || isnan(pf[i]);
+    if (PHASE != ETRAJ_PHASE_CTMC) bad = bad || isnan(phase);
+    emit<DIM, PHASE, OUT>(P, O, col_idx, active, bad, rf, pf, phase, t0, e.prob);
+    if (OUT == OUT_PMD) {
+        int64_t nact = wave_sum_i64(active ? 1 : 0);
+        int64_t nr = wave_sum_i64(active ? nre : 0);
+        if (lane == 0) {
+            atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)(n * nact));
+            if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nr);
+        }
+    }
+}
+
+__global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double re = amp[2 * i], im = amp[2 * i + 1];
+    out[i] = re * re + im * im;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, int64_t n, double* Fx, double* Fy,
+                              double* Ax, double* Ay) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_laser_field, dim3(grid_for(n, 256)), dim3(256), 0, st, L, t, n, Fx, Fy, Ax, Ay);
+    return hipGetLastError();
+}
+
+hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_target_eval, dim3(grid_for(n, 256)), dim3(256), 0, st, T, xyz, n, V, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L, const TargetP& T, const double* t0s,
+                             int b_first, int nb, double* out, int32_t* counts) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adk_sample, dim3(nb), dim3(256), 0, st, S, L, T, t0s, b_first, out, counts);
+    return hipGetLastError();
+}
+
+template <int DIM, int PHASE, bool MIXED, int OUT>
+static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                               const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps) {
+    int64_t waves = n_seg * (int64_t)cps;
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT>), dim3(blocks), dim3(256), 0, st, P, O, init, seg_start,
+                       seg_count, n_seg, cps);
+    return hipGetLastError();
+}
+
+template <int DIM, int PHASE, bool MIXED>
+static hipError_t launch_rk4_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
+    switch (out) {
+        case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps);
+        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps);
+        default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps);
+    }
+}
+
+template <int DIM, int PHASE>
+static hipError_t launch_rk4_m(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
+    return P.mixed ? launch_rk4_o<DIM, PHASE, true>(st, P, O, out, init, ss, sc, ns, cps)
+                   : launch_rk4_o<DIM, PHASE, false>(st, P, O, out, init, ss, sc, ns, cps);
+}
+
+template <int DIM>
+static hipError_t launch_rk4_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
+    switch (P.phase) {
+        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps);
+        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps);
+        default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps);
+    }
+}
+
+hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps) {
+    if (n_seg <= 0 || cps <= 0) return hipSuccess;
+    return P.dim == 3 ? launch_rk4_p<3>(st, P, O, out, init, seg_start, seg_count, n_seg, cps)
+                      : launch_rk4_p<2>(st, P, O, out, init, seg_start, seg_count, n_seg, cps);
+}
+
+hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize_abs2, dim3(grid_for(n, 256)), dim3(256), 0, st, amp, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace etraj

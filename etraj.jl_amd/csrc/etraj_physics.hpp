@@ -1,0 +1,409 @@
+// etraj_physics.hpp -- host+device physics of the trajectory engine.
+//
+// Everything here is `__host__ __device__` so the same arithmetic runs in the
+// gfx950 kernels and, for development checks, on the host. Field and target
+// parameters are resolved once on the host from the C-ABI descriptors into
+// the flat `LaserP` / `TargetP` structs below (passed by value as kernel
+// arguments, so they live in SGPRs / the kernarg segment).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "../../include/etraj.h"
+
+namespace etraj {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Philox4x64-10 (Salmon et al., SC'11). Counter = (index, batch, 0, tag),
+// key = (seed, 0).
+// ---------------------------------------------------------------------------
+struct U64x4 { uint64_t v[4]; };
+
+__host__ __device__ inline void mulhilo(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    hi = __umul64hi(a, b);
+    lo = a * b;
+#else
+    unsigned __int128 p = (unsigned __int128)a * b;
+    hi = (uint64_t)(p >> 64);
+    lo = (uint64_t)p;
+#endif
+}
+
+__host__ __device__ inline U64x4 philox(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0, uint64_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t h0, l0, h1, l1;
+        mulhilo(0xD2E7470EE14C6C93ULL, c0, h0, l0);
+        mulhilo(0xCA5A826395121157ULL, c2, h1, l1);
+        uint64_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+        c0 = n0; c1 = l1; c2 = n2; c3 = l0;
+        k0 += 0x9E3779B97F4A7C15ULL;
+        k1 += 0xBB67AE8584CAA73BULL;
+    }
+    U64x4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+constexpr uint64_t kTagT0 = 1, kTagK = 2;
+
+__host__ __device__ inline double unit_co(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }          // [0,1)
+__host__ __device__ inline double unit_oc(uint64_t x) { return ((double)(x >> 11) + 1.0) * 0x1.0p-53; }  // (0,1]
+
+// range(lo, hi; length=n)[i] (0-based): (lo*(n-1-i) + hi*i)/(n-1)
+__host__ __device__ inline double grid_point(double lo, double hi, int n, int i) {
+    if (n <= 1) return lo;
+    return (lo * (double)(n - 1 - i) + hi * (double)i) / (double)(n - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Lasers (src/Lasers/*Laser.jl). F = -dA/dt in closed form; the cos^n
+// envelopes carry the tanh cut-off window of Cos2Laser.jl:93.
+// ---------------------------------------------------------------------------
+struct MonoP {
+    int kind;        // ETRAJ_LASER_*
+    int azi_nz;      // azimuth != 0 selects the rotated branch (Cos2Laser.jl:99,127)
+    double F0, A0, w, N, cep, tshift, ellip, cazi, sazi;
+    double k;        // 200/Period
+    double halfw;    // N*pi/w
+    double coefN;    // 1/N (Cos2) or 2/N (Cos4)
+    double sigma2;   // Gaussian: SpreadDuration^2
+};
+
+struct LaserP {
+    int n;
+    double delay;
+    MonoP c[2];
+};
+
+inline int make_mono(const etraj_mono_laser& d, MonoP& m) {
+    if (d.kind != ETRAJ_LASER_COS2 && d.kind != ETRAJ_LASER_COS4 && d.kind != ETRAJ_LASER_GAUSSIAN) return -1;
+    if (!(d.wave_len > 0) || !(d.cyc_num > 0) || !(d.peak_int >= 0)) return -1;
+    m.kind = d.kind;
+    m.w = 45.563352525 / d.wave_len;                                   // AngFreq (Cos2Laser.jl:81)
+    double period = 2 * kPi / m.w;                                     // Period  (:82)
+    m.F0 = sqrt(d.peak_int / (1.0 + d.ellip * d.ellip) / 3.50944521e16); // LaserF0 (:85)
+    m.A0 = m.F0 / m.w;                                                 // LaserA0 (:86)
+    m.N = d.cyc_num;
+    m.cep = d.cep;
+    m.tshift = d.t_shift;
+    m.ellip = d.ellip;
+    m.azi_nz = d.azi != 0;
+    m.cazi = cos(d.azi);
+    m.sazi = sin(d.azi);
+    m.k = 200 / period;
+    m.halfw = m.N * kPi / m.w;
+    m.coefN = (d.kind == ETRAJ_LASER_COS2 ? 1.0 : 2.0) / m.N;
+    double sd = d.cyc_num * period;                                    // SpreadDuration (GaussianLaser.jl:120)
+    m.sigma2 = sd * sd;
+    return 0;
+}
+
+inline int make_laser(const etraj_laser_desc& d, LaserP& L) {
+    if (d.n_colors != 1 && d.n_colors != 2) return -1;
+    L.n = d.n_colors;
+    L.delay = d.delay21;
+    for (int i = 0; i < 2; ++i) L.c[i] = MonoP{};
+    for (int i = 0; i < L.n; ++i)
+        if (make_mono(d.color[i], L.c[i])) return -1;
+    return 0;
+}
+
+__host__ __device__ inline double cos_window(const MonoP& m, double t) {
+    return (0.5 + 0.5 * tanh(m.k * (t + m.halfw))) * (0.5 + 0.5 * tanh(-m.k * (t - m.halfw)));
+}
+
+// UnitEnvelope (Cos2Laser.jl:89-95, Cos4Laser.jl:89-95, GaussianLaser.jl:138-144)
+__host__ __device__ inline double mono_envelope(const MonoP& m, double t) {
+    t -= m.tshift;
+    if (m.kind == ETRAJ_LASER_GAUSSIAN) return exp(-t * t / 2 / m.sigma2);
+    double c = cos(m.w * t / (2 * m.N));
+    double cn = (m.kind == ETRAJ_LASER_COS2) ? c * c : (c * c) * (c * c);
+    return cn * cos_window(m, t);
+}
+
+// LaserFx/LaserFy (Cos2Laser.jl:125-152, Cos4Laser.jl:125-152, GaussianLaser.jl:176-203)
+__host__ __device__ inline void mono_field(const MonoP& m, double t, double& Fx, double& Fy) {
+    t -= m.tshift;
+    double a, b, g;
+    double sw, cw;
+    sincos(m.w * t + m.cep, &sw, &cw);
+    if (m.kind == ETRAJ_LASER_GAUSSIAN) {
+        g = m.A0 * exp(-t * t / 2 / m.sigma2);
+        a = t / m.sigma2 * cw + m.w * sw;
+        b = t / m.sigma2 * sw - m.w * cw;
+    } else {
+        double sh, ch;
+        sincos(m.w * t / (2 * m.N), &sh, &ch);
+        double env = (m.kind == ETRAJ_LASER_COS2) ? ch : ch * ch * ch;
+        g = m.F0 * env * cos_window(m, t);
+        a = ch * sw + m.coefN * sh * cw;
+        b = -ch * cw + m.coefN * sh * sw;
+    }
+    if (!m.azi_nz) {
+        Fx = g * a;
+        Fy = g * b * m.ellip;
+    } else {
+        Fx = g * (a * m.cazi + b * m.ellip * m.sazi);
+        Fy = g * (a * -m.sazi + b * m.ellip * m.cazi);
+    }
+}
+
+// LaserAx/LaserAy (Cos2Laser.jl:97-124 and siblings)
+__host__ __device__ inline void mono_potential(const MonoP& m, double t, double& Ax, double& Ay) {
+    double g = m.A0 * mono_envelope(m, t);
+    t -= m.tshift;
+    double sw, cw;
+    sincos(m.w * t + m.cep, &sw, &cw);
+    if (!m.azi_nz) {
+        Ax = g * cw;
+        Ay = g * sw * m.ellip;
+    } else {
+        Ax = g * (cw * m.cazi + sw * m.ellip * m.sazi);
+        Ay = g * (cw * -m.sazi + sw * m.ellip * m.cazi);
+    }
+}
+
+// BichromaticLaser.jl:59-85: F1(t) + F2(t - delay)
+__host__ __device__ inline void laser_field(const LaserP& L, double t, double& Fx, double& Fy) {
+    mono_field(L.c[0], t, Fx, Fy);
+    if (L.n == 2) {
+        double fx, fy;
+        mono_field(L.c[1], t - L.delay, fx, fy);
+        Fx += fx;
+        Fy += fy;
+    }
+}
+
+__host__ __device__ inline void laser_potential(const LaserP& L, double t, double& Ax, double& Ay) {
+    mono_potential(L.c[0], t, Ax, Ay);
+    if (L.n == 2) {
+        double ax, ay;
+        mono_potential(L.c[1], t - L.delay, ax, ay);
+        Ax += ax;
+        Ay += ay;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Targets (HydrogenLikeAtom.jl:75-76, SAEAtom.jl:84-96)
+// ---------------------------------------------------------------------------
+struct TargetP {
+    int kind;
+    int Z;
+    double Zd, Ip, C, sc;
+    double a1, b1, a2, b2, a3, b3;
+};
+
+inline int make_target(const etraj_target_desc& d, TargetP& T) {
+    if (d.kind != ETRAJ_TARGET_HYDROGEN_LIKE && d.kind != ETRAJ_TARGET_SAE) return -1;
+    if (!(d.Ip > 0) || !(d.soft_core >= 0)) return -1;
+    T.kind = d.kind;
+    T.Z = d.nucl_charge;
+    T.Zd = (double)d.nucl_charge;
+    T.Ip = d.Ip;
+    T.C = d.asymp_coeff;
+    T.sc = d.soft_core;
+    T.a1 = d.a1; T.b1 = d.b1; T.a2 = d.a2; T.b2 = d.b2; T.a3 = d.a3; T.b3 = d.b3;
+    return 0;
+}
+
+template <typename R>
+__host__ __device__ inline R rsqrt_t(R x);
+template <>
+__host__ __device__ inline double rsqrt_t<double>(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrt(x);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+template <>
+__host__ __device__ inline float rsqrt_t<float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrtf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
+// Potential V(r) and force scale s with F_target = s * r_vec, evaluated in
+// real type R. `sae_quirk_r2` reproduces SAEAtom.jl:136 (2D SCTS divides
+// by r^2+a instead of sqrt(r^2+a)) for the potential used in the phase.
+template <typename R>
+__host__ __device__ inline void target_vs(const TargetP& T, R r2n, bool want_v, bool sae_quirk_r2, R& V, R& s) {
+    if (T.kind == ETRAJ_TARGET_HYDROGEN_LIKE) {
+        R ir = rsqrt_t<R>(r2n + (R)T.sc);
+        R zr = (R)T.Zd * ir;
+        s = -(zr * ir * ir);
+        V = -zr;
+    } else {
+        R r = sqrt(r2n);
+        R rs2 = r2n + (R)T.sc;
+        R irs = rsqrt_t<R>(rs2);
+        R e1 = exp(-(R)T.b1 * r), e2 = exp(-(R)T.b2 * r), e3 = exp(-(R)T.b3 * r);
+        s = -((irs * irs * irs) * ((R)T.Zd + (R)T.a1 * (1 + (R)T.b1 * r) * e1 + (R)T.a3 * (1 + (R)T.b3 * r) * e3)
+              + (R)T.a2 * (R)T.b2 * irs * e2);
+        if (want_v) {
+            R num = (R)T.Zd + (R)T.a1 * e1 + (R)T.a2 * r * e2 + (R)T.a3 * e3;
+            V = sae_quirk_r2 ? -num / rs2 : -num * irs;
+        } else {
+            V = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Equations of motion (TrajectoryFunction, HydrogenLikeAtom.jl:78-159,
+// SAEAtom.jl:98-186): r' = p, p' = F_target(r) - F_laser(t) (z: target
+// only); QTMC phase' = -(Ip + p^2/2 + V); SCTS phase' = -(p^2/2 + V + r.F_target).
+// ---------------------------------------------------------------------------
+template <int DIM, typename R>
+struct State {
+    R r[DIM];
+    R p[DIM];
+};
+
+template <int DIM, int PHASE, typename R>
+__host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R Fx, R Fy,
+                                    State<DIM, R>& du, R& dphi, R& r2out) {
+    R r2n = u.r[0] * u.r[0] + u.r[1] * u.r[1];
+    if (DIM == 3) r2n += u.r[DIM - 1] * u.r[DIM - 1];
+    r2out = r2n;
+    R V, s;
+    target_vs<R>(T, r2n, PHASE != ETRAJ_PHASE_CTMC, sae_quirk, V, s);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) du.r[i] = u.p[i];
+    du.p[0] = s * u.r[0] - Fx;
+    du.p[1] = s * u.r[1] - Fy;
+    if (DIM == 3) du.p[DIM - 1] = s * u.r[DIM - 1];
+    R k2 = u.p[0] * u.p[0] + u.p[1] * u.p[1];
+    if (DIM == 3) k2 += u.p[DIM - 1] * u.p[DIM - 1];
+    if (PHASE == ETRAJ_PHASE_QTMC) {
+        dphi = -((R)T.Ip + k2 * (R)0.5 + V);
+    } else if (PHASE == ETRAJ_PHASE_SCTS) {
+        // r . F_target = s * |r|^2 (without the soft core)
+        dphi = -(k2 * (R)0.5 + V + s * r2n);
+    } else {
+        dphi = 0;
+    }
+}
+
+// One classical RK4 step of size h from state u (stage fields F0 at t,
+// Fm at t+h/2, F1 at t+h). Returns the state increment d, so that
+// u_next = u + d with d = h/6 (k1 + 2 (k2 + k3) + k4), the phase increment
+// and the minimum |r|^2 over the four stages.
+template <int DIM, int PHASE, typename R>
+__host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R h,
+                                      R F0x, R F0y, R Fmx, R Fmy, R F1x, R F1y,
+                                      State<DIM, R>& d, R& r2min) {
+    const R hh = h * (R)0.5;
+    const R h6 = h / (R)6;
+    State<DIM, R> k, y, k2;
+    R ph, ph2, phacc, r2;
+    eom<DIM, PHASE, R>(T, sae_quirk, u, F0x, F0y, k, ph, r2);
+    r2min = r2;
+    phacc = ph;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        d.r[i] = k.r[i];
+        d.p[i] = k.p[i];
+        y.r[i] = u.r[i] + hh * k.r[i];
+        y.p[i] = u.p[i] + hh * k.p[i];
+    }
+    eom<DIM, PHASE, R>(T, sae_quirk, y, Fmx, Fmy, k2, ph2, r2);
+    r2min = r2 < r2min ? r2 : r2min;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        y.r[i] = u.r[i] + hh * k2.r[i];
+        y.p[i] = u.p[i] + hh * k2.p[i];
+    }
+    eom<DIM, PHASE, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
+    r2min = r2 < r2min ? r2 : r2min;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        d.r[i] += (R)2 * (k2.r[i] + k.r[i]);
+        d.p[i] += (R)2 * (k2.p[i] + k.p[i]);
+        y.r[i] = u.r[i] + h * k.r[i];
+        y.p[i] = u.p[i] + h * k.p[i];
+    }
+    phacc += (R)2 * (ph2 + ph);
+    eom<DIM, PHASE, R>(T, sae_quirk, y, F1x, F1y, k, ph, r2);
+    r2min = r2 < r2min ? r2 : r2min;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        d.r[i] = h6 * (d.r[i] + k.r[i]);
+        d.p[i] = h6 * (d.p[i] + k.p[i]);
+    }
+    return h6 * (phacc + ph);
+}
+
+// ---------------------------------------------------------------------------
+// Post-processing after traj_t_final (TrajectorySimulation.jl:356-396,
+// :451-496): E_inf, SCTS asymptotic phase, Kepler asymptotic momentum.
+// Always FP64.
+// ---------------------------------------------------------------------------
+struct Final {
+    double E, phase, pinf[3];
+    bool ionized;
+};
+
+__host__ __device__ inline double target_potential(const TargetP& T, double x, double y, double z) {
+    double r2 = x * x + y * y + z * z;
+    double V, s;
+    target_vs<double>(T, r2, true, false, V, s);
+    return V;
+}
+
+template <int DIM, int PHASE>
+__host__ __device__ inline Final kepler_final(const TargetP& T, int quirks, const double* r, const double* p, double phase, double t0) {
+    Final f;
+    double x = r[0], y = r[1], z = DIM == 3 ? r[2] : 0.0;
+    double px = p[0], py = p[1], pz = DIM == 3 ? p[2] : 0.0;
+    double E = (px * px + py * py + pz * pz) / 2 + target_potential(T, x, y, z);
+    double Lx = y * pz - z * py, Ly = z * px - x * pz, Lz = x * py - y * px;
+    double L2 = Lx * Lx + Ly * Ly + Lz * Lz;
+    if (E > 0 && PHASE == ETRAJ_PHASE_SCTS) {
+        phase += T.Ip * t0;                                   // Ip*tr (:364)
+        double pinf = sqrt(2 * E);
+        double ib = 1.0 / pinf;                               // (2E)^(-1/2)
+        double g = sqrt(1 + 2 * E * L2);
+        double rp = (DIM == 2) ? x * px + y * py
+                               : ((quirks & ETRAJ_QUIRK_SCTS3D_XPY) ? x * py + y * py + z * pz : x * px + y * py + z * pz);
+        phase -= T.Zd * ib * (log(g) + asinh(rp / (g * ib)));  // (:368 / :463)
+    }
+    f.E = E;
+    f.phase = phase;
+    f.ionized = E >= 0;
+    if (f.ionized) {
+        double pinf = sqrt(2 * E);
+        double ir = 1.0 / sqrt(x * x + y * y + z * z);
+        double ax = (py * Lz - pz * Ly) - T.Zd * x * ir;    // a = p x L - Z r/|r| (:373)
+        double ay = (pz * Lx - px * Lz) - T.Zd * y * ir;
+        double az = (px * Ly - py * Lx) - T.Zd * z * ir;
+        double lax = Ly * az - Lz * ay, lay = Lz * ax - Lx * az, laz = Lx * ay - Ly * ax;
+        double c = pinf / (1 + pinf * pinf * L2);            // (:374)
+        f.pinf[0] = c * (pinf * lax - ax);
+        f.pinf[1] = c * (pinf * lay - ay);
+        f.pinf[2] = c * (pinf * laz - az);
+    } else {
+        f.pinf[0] = f.pinf[1] = f.pinf[2] = 0.0;
+    }
+    return f;
+}
+
+// round(Int, (p+pmax)/(pmax/num*2)) with ties to even, 1-based; 0 when the
+// value is out of the representable range (TrajectorySimulation.jl:375-384).
+__host__ __device__ inline int64_t bin_index(double p, double pmax, int num) {
+    if (num <= 1) return 1;
+    double q = (p + pmax) / (pmax / num * 2);
+    if (!(fabs(q) < 9.0e15)) return 0;
+    return (int64_t)rint(q);
+}
+
+}  // namespace etraj

@@ -1,0 +1,20 @@
+"""etraj_hip -- the HIP/gfx950 trajectory engine behind eTraj.jl's
+perform_traj_simulation, with the reference's Lasers / Targets /
+ElectronSamplers interface.
+
+Every compute entry point calls libetraj_hip.so (include/etraj.h); there is
+no CPU fallback.
+"""
+from . import _abi
+from ._abi import Context, EngineError, EngineUnavailable, load_library
+from .lasers import (Laser, MonochromaticLaser, Cos2Laser, Cos4Laser, GaussianLaser, BichromaticLaser, LaserFx,
+                     LaserFy, LaserAx, LaserAy, LaserF0, LaserA0, PeakInt, WaveLen, CycNum, SpreadCycNum, Ellipticity,
+                     Azimuth, AngFreq, Period, CEP, TimeShift, KeldyshParameter, SpreadDuration, FWHM_Duration, Laser1,
+                     Laser2, Delay21)
+from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, get_atom, get_available_atoms,
+                      hartree_asymp_coeff, IonPotential, AsympNuclCharge, AngularQuantumNumber, MagneticQuantumNumber,
+                      AsympCoeff, SoftCore, TargetName, QuantizationAxisOrientaion, TargetPotential, TargetForce)
+from .samplers import ElectronSampler, ADKSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
+from .simulation import perform_traj_simulation, run_desc
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,170 @@
+"""ElectronSamplers: init_sampler / batch_num / gen_electron_batch
+(src/ElectronSamplers/ElectronSamplers.jl:9-26, ADKSampler.jl).
+
+`gen_electron_batch` runs the device sampler kernel and returns the same
+rows x N column-major matrix the reference returns (rows x,y,kx,ky,t0,rate
+[,phase] in 2D; x,y,z,kx,ky,kz,t0,rate[,phase] in 3D), or None for an empty
+batch. Monte-Carlo sampling is seeded (Philox4x64-10 keyed by `seed`)
+instead of the reference's unseeded global rand().
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+from . import _abi
+from .lasers import BichromaticLaser, MonochromaticLaser, KeldyshParameter
+from .targets import SAEAtomBase, IonPotential
+
+_SPA = {"SPA", "MOSPA", "SFA", "MOSFA"}
+_SPANE = {"SPANE", "MOSPANE", "SFAAE", "MOSFAAE"}
+_ADK = {"ADK", "MOADK"}
+
+
+def _sym(x):
+    return x.lstrip(":") if isinstance(x, str) else x
+
+
+def resolve_rate_prefix(rate_prefix, laser) -> int:
+    """ADKSampler.jl:49-77 -> ETRAJ_PREFIX_* bits."""
+    names = {"Pre": _abi.PREFIX_PRE, "PreCC": _abi.PREFIX_PRECC, "Jac": _abi.PREFIX_JAC}
+    if isinstance(rate_prefix, str):
+        p = _sym(rate_prefix)
+        if p == "Exp":
+            s = set()
+        elif p == "Full":
+            s = {"PreCC", "Jac"}
+        elif p in names:
+            s = {p}
+        else:
+            raise ValueError(f"[ADKSampler] Undefined tunneling rate prefix [{rate_prefix}].")
+    else:
+        s = {_sym(p) for p in rate_prefix}
+        if not all(p in names for p in s):
+            raise ValueError(f"[ADKSampler] Undefined tunneling rate prefix [{rate_prefix}].")
+        if "Pre" in s and "PreCC" in s:
+            raise ValueError("[ADKSampler] Rate prefixes [Pre] & [PreCC] conflict.")
+    if "PreCC" in s and not isinstance(laser, MonochromaticLaser):
+        warnings.warn("[ADKSampler] Laser is not monochromatic, Coulomb correction in rate prefix is unavailable.")
+        s = (s - {"PreCC"}) | {"Pre"}
+    bits = 0
+    for p in s:
+        bits |= names[p]
+    return bits
+
+
+class ElectronSampler:
+    """ElectronSamplers.jl:28."""
+
+
+class ADKSampler(ElectronSampler):
+    """ADKSampler.jl:3-150 (atoms); device sampling."""
+
+    def __init__(self, *, laser, target, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit=1e-16,
+                 sample_monte_carlo=False, traj_phase_method="CTMC", rate_prefix="Full", ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
+                 ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, mol_orbit_ridx=0, seed=0,
+                 sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, **_):
+        if not isinstance(target, SAEAtomBase):
+            raise NotImplementedError("[ADKSampler] the engine samples atomic targets (SAEAtomBase) only")
+        pm = _sym(traj_phase_method)
+        if pm not in _abi.PHASE_METHODS:
+            raise ValueError(f"[ADKSampler] Undefined phase method [{traj_phase_method}].")
+        self.laser, self.target = laser, target
+        d = _abi.SamplerDesc()
+        d.init_method = _abi.INIT_ADK
+        d.dimension = int(dimension)
+        d.phase_method = _abi.PHASE_METHODS[pm]
+        d.rate_prefix = resolve_rate_prefix(rate_prefix, laser)
+        assert sample_t_num > 0, f"[ADKSampler] Invalid time sample number {sample_t_num}."
+        assert sample_cutoff_limit >= 0, f"[ADKSampler] Invalid cut-off limit {sample_cutoff_limit}."
+        d.sample_t_num = int(sample_t_num)
+        d.sample_t_lo, d.sample_t_hi = float(sample_t_intv[0]), float(sample_t_intv[1])
+        d.cutoff_limit = float(sample_cutoff_limit)
+        d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        if sample_stratified:
+            d.sampling = _abi.SAMPLING_STRATIFIED
+            d.mc_kt_num = int(mc_kt_num)
+            d.kd_sigma, d.kz_sigma = float(kd_sigma), float(kz_sigma)
+            assert mc_kt_num > 0 and kd_sigma > 0, "stratified sampling needs mc_kt_num > 0 and kd_sigma > 0"
+        elif sample_monte_carlo:
+            d.sampling = _abi.SAMPLING_MONTECARLO
+            assert sample_t_intv[0] < sample_t_intv[1], f"[ADKSampler] Invalid sampling time interval {sample_t_intv}."
+            assert mc_kt_num > 0, f"[ADKSampler] Invalid sampling kt_num {mc_kt_num}."
+            if dimension == 3:
+                assert mc_kd_max > 0 or mc_kz_max > 0, "[ADKSampler] Invalid kd,kz sample boundaries."
+            else:
+                assert mc_kd_max > 0, f"[ADKSampler] Invalid kd sample boundaries {mc_kd_max}."
+            d.mc_kt_num, d.mc_kd_max, d.mc_kz_max = int(mc_kt_num), float(mc_kd_max), float(mc_kz_max)
+        else:
+            d.sampling = _abi.SAMPLING_STEP
+            if dimension == 3:
+                assert ss_kd_num > 0 and ss_kz_num > 0, f"[ADKSampler] Invalid kd,kz sample number {ss_kd_num},{ss_kz_num}."
+                assert ss_kd_max > 0 or ss_kz_max > 0, "[ADKSampler] Invalid kd,kz sample boundaries."
+                d.ss_kz_num, d.ss_kz_max = int(ss_kz_num), float(ss_kz_max)
+            else:
+                assert ss_kd_num > 0, f"[ADKSampler] Invalid kd sample number {ss_kd_num}."
+                assert ss_kd_max > 0, f"[ADKSampler] Invalid kd sample boundaries {ss_kd_max}."
+                d.ss_kz_num, d.ss_kz_max = 1, 0.0
+            d.ss_kd_num, d.ss_kd_max = int(ss_kd_num), float(ss_kd_max)
+        # Keldysh warning (ADKSampler.jl:116-131)
+        Ip = IonPotential(target)
+        if isinstance(laser, MonochromaticLaser):
+            g = KeldyshParameter(laser, Ip)
+            if g >= 0.6:
+                warnings.warn(f"[ADKSampler] Keldysh parameter γ={g:.4f}, adiabatic (tunneling) condition [γ<<1] "
+                              "not sufficiently satisfied.")
+        elif isinstance(laser, BichromaticLaser):
+            g = max(KeldyshParameter(laser.l1, Ip), KeldyshParameter(laser.l2, Ip))
+            if g >= 0.6:
+                warnings.warn(f"[ADKSampler] Keldysh parameter γ={g:.4f}, adiabatic (tunneling) condition [γ<<1] "
+                              "not sufficiently satisfied.")
+        self.desc = d
+
+    @property
+    def rows(self) -> int:
+        d = self.desc
+        return (8 if d.phase_method == _abi.PHASE_CTMC else 9) if d.dimension == 3 else \
+            (6 if d.phase_method == _abi.PHASE_CTMC else 7)
+
+    @property
+    def t_samples(self) -> np.ndarray:
+        ctx = _abi.default_context()
+        t = np.empty(self.desc.sample_t_num)
+        ctx.check(ctx.lib.etraj_batch_times(ctx.handle, self.desc, _abi.dptr(t)))
+        return t
+
+
+def init_sampler(**kwargs) -> ElectronSampler:
+    """ElectronSamplers.jl:11-24."""
+    m = _sym(kwargs["init_cond_method"])
+    if m in _ADK:
+        return ADKSampler(**kwargs)
+    if m in _SPA or m in _SPANE or m == "WFAT":
+        raise NotImplementedError(f"[ElectronSamplers] initial condition method [{m}] is not implemented by the "
+                                  "engine (ADK only)")
+    raise ValueError(f"[ElectronSamplers] Undefined initial condition method [{m}].")
+
+
+def batch_num(sp: ElectronSampler) -> int:
+    """ADKSampler.jl:153-155."""
+    return int(sp.desc.sample_t_num)
+
+
+def batch_max_size(sp: ElectronSampler) -> int:
+    """ADKSampler.jl:158-164."""
+    return int(_abi.load_library().etraj_batch_max_size(sp.desc))
+
+
+def gen_electron_batch(sp: ElectronSampler, batch_id: int):
+    """ADKSampler.jl:167-396 on the device; batch_id is 1-based as in Julia."""
+    ctx = _abi.default_context()
+    cap = max(batch_max_size(sp), 1)
+    out = np.empty((cap, sp.rows))  # C-order (cap, rows) == Julia column-major rows x cap
+    cnt = _abi.C.c_int64()
+    ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, sp.laser._desc(), sp.target._desc(), sp.desc,
+                                               int(batch_id), _abi.dptr(out), cap, _abi.C.byref(cnt)))
+    n = cnt.value
+    if n == 0:
+        return None
+    return np.asfortranarray(out[:n].T)

@@ -1,0 +1,158 @@
+"""perform_traj_simulation with the HIP engine (TrajectorySimulation.jl:103-161).
+
+Keyword interface of the reference, plus engine keywords:
+  traj_integrator  "rk4" (fixed step, default; the engine's algorithm)
+  traj_dt          RK4 nominal step (a.u., default 0.1)
+  precision        "mixed" (FP32 EOM, FP64 phase/Kepler/re-step) or "fp64"
+  restep_radius    FP64 re-step radius of the mixed mode (a.u., default 2)
+  granularity      "job": the whole batch loop on the device (etraj_run_job);
+                   "batch": per batch gen_electron_batch + integrate_and_bin,
+                   the drop-in for launch_and_collect! (:300-401)
+  seed, sample_stratified, kd_sigma, kz_sigma   seeded sampling modes
+  reproduce_quirks reproduce the reference's 3D SCTS / SAE 2D SCTS formulas
+Returns the fields write_output! stores (:516-543) as a dict; when
+output_path is given they are also saved as .npz (JLD2/HDF5 writing is not
+part of the engine).
+"""
+from __future__ import annotations
+
+import os
+import time
+import warnings
+
+import numpy as np
+
+from . import _abi
+from .lasers import LaserFx, LaserFy, Serialize as _ser_laser
+from .samplers import init_sampler, batch_num, gen_electron_batch, _sym
+from .targets import Serialize as _ser_target
+
+
+def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num, traj_integrator="rk4",
+             traj_dt=0.1, traj_rtol=1e-6, precision="mixed", restep_radius=2.0, reproduce_quirks=True, **_):
+    d = _abi.RunDesc()
+    d.dimension = int(dimension)
+    d.phase_method = _abi.PHASE_METHODS[_sym(traj_phase_method)]
+    integ = _sym(traj_integrator).lower()
+    if integ not in ("rk4", "tsit5"):
+        raise ValueError(f"unknown traj_integrator {traj_integrator}")
+    d.integrator = _abi.INTEG_RK4 if integ == "rk4" else _abi.INTEG_TSIT5
+    prec = _sym(precision).lower()
+    if prec not in ("mixed", "fp64"):
+        raise ValueError(f"unknown precision {precision}")
+    d.precision = _abi.PREC_MIXED if prec == "mixed" else _abi.PREC_FP64
+    d.traj_t_final = float(traj_t_final)
+    d.dt = float(traj_dt)
+    d.rtol = float(traj_rtol)
+    d.atol = 1e-6
+    d.restep_radius = float(restep_radius)
+    for k in range(dimension):
+        d.final_p_max[k] = float(final_p_max[k])
+        d.final_p_num[k] = int(final_p_num[k])
+    for k in range(dimension, 3):
+        d.final_p_max[k] = 1.0
+        d.final_p_num[k] = 1
+    d.quirks = _abi.QUIRKS_REFERENCE if reproduce_quirks else 0
+    d.maxiters = 1000000
+    return d
+
+
+def _axis(pmax, num):
+    return np.linspace(-pmax, pmax, num) if num > 1 else np.array([0.0])
+
+
+def default_filename():
+    return time.strftime("TrajectorySimulation-%Y%m%d.%H%M%S")
+
+
+def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sample_t_intv, sample_t_num,
+                            traj_t_final, final_p_max, final_p_num, ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
+                            ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, traj_phase_method="CTMC",
+                            traj_rtol=1e-6, output_fmt="npz", output_compress=True, output_path=None,
+                            sample_monte_carlo=False, sample_cutoff_limit=1e-16, rate_prefix="Full",
+                            mol_orbit_ridx=0, show_progress=False, traj_integrator="rk4", traj_dt=0.1,
+                            precision="mixed", restep_radius=2.0, granularity="job", seed=0,
+                            sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, reproduce_quirks=True,
+                            backend="hip"):
+    if _sym(backend) != "hip":
+        raise ValueError("this package is the HIP backend; the CPU method is the reference itself")
+    # TrajectorySimulationJob checks (:175-186)
+    assert sample_t_intv[0] < sample_t_intv[1], "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval."
+    assert traj_t_final > sample_t_intv[1], \
+        "[TrajectorySimulationJob] `traj_t_final` must be greater than the last time point of `sample_t_intv`."
+    assert dimension in (2, 3), "[TrajectorySimulationJob] `dimension` must be either 2 or 3."
+    assert len(final_p_max) == len(final_p_num) == dimension, \
+        "[TrajectorySimulationJob] `length(final_p_max)` and `length(final_p_num)` should match `dimension`."
+    Fx, Fy = LaserFx(laser), LaserFy(laser)
+    for tt, what in ((sample_t_intv[0], "beginning of the sampling time interval"),
+                     (sample_t_intv[1], "end of the sampling time interval"), (traj_t_final, "end of the trajectory simulation")):
+        if Fx(tt) ** 2 + Fy(tt) ** 2 > 0.005 ** 2:
+            warnings.warn(f"[TrajectorySimulationJob] The laser field is not vanishingly small at the {what}.")
+    sp = init_sampler(init_cond_method=init_cond_method, laser=laser, target=target, dimension=dimension,
+                      sample_t_intv=sample_t_intv, sample_t_num=sample_t_num, sample_cutoff_limit=sample_cutoff_limit,
+                      sample_monte_carlo=sample_monte_carlo, traj_phase_method=traj_phase_method,
+                      rate_prefix=rate_prefix, ss_kd_max=ss_kd_max, ss_kd_num=ss_kd_num, ss_kz_max=ss_kz_max,
+                      ss_kz_num=ss_kz_num, mc_kt_num=mc_kt_num, mc_kd_max=mc_kd_max, mc_kz_max=mc_kz_max,
+                      mol_orbit_ridx=mol_orbit_ridx, seed=seed, sample_stratified=sample_stratified,
+                      kd_sigma=kd_sigma, kz_sigma=kz_sigma)
+    rd = run_desc(dimension=dimension, traj_phase_method=traj_phase_method, traj_t_final=traj_t_final,
+                  final_p_max=final_p_max, final_p_num=final_p_num, traj_integrator=traj_integrator,
+                  traj_dt=traj_dt, traj_rtol=traj_rtol, precision=precision, restep_radius=restep_radius,
+                  reproduce_quirks=reproduce_quirks)
+    ctx = _abi.default_context()
+    npix = int(np.prod(final_p_num))
+    coherent = rd.phase_method != _abi.PHASE_CTMC
+    stats = {}
+    if granularity == "job":
+        spec = np.empty(npix)
+        st = _abi.Stats()
+        ctx.check(ctx.lib.etraj_run_job(ctx.handle, laser._desc(), target._desc(), sp.desc, rd, _abi.dptr(spec),
+                                        _abi.C.byref(st)))
+        stats = st.as_dict()
+        ion_prob, ion_unc, n_eff = st.ion_prob, st.ion_prob_uncollected, st.num_effective_traj
+    elif granularity == "batch":
+        acc = np.zeros(2 * npix if coherent else npix)
+        ip, iu = _abi.C.c_double(0.0), _abi.C.c_double(0.0)
+        nf = _abi.C.c_int64(0)
+        n_eff = 0
+        for b in range(1, batch_num(sp) + 1):
+            init = gen_electron_batch(sp, b)
+            if init is None:
+                continue
+            cols = np.ascontiguousarray(init.T)
+            ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, laser._desc(), target._desc(), rd, _abi.dptr(cols),
+                                                      cols.shape[0], cols.shape[1], acc.ctypes.data_as(_abi.C.c_void_p),
+                                                      _abi.C.byref(ip), _abi.C.byref(iu), _abi.C.byref(nf)))
+            n_eff += cols.shape[0]
+        spec = (acc[0::2] ** 2 + acc[1::2] ** 2) if coherent else acc
+        ion_prob, ion_unc = ip.value, iu.value
+    else:
+        raise ValueError("granularity must be 'job' or 'batch'")
+    # write_output! layout: px fastest (Julia column-major)
+    shape = tuple(int(n) for n in final_p_num)
+    result = {
+        "px": _axis(final_p_max[0], final_p_num[0]),
+        "py": _axis(final_p_max[1], final_p_num[1]),
+        "momentum_spec": spec.reshape(shape[::-1]).T,
+        "ion_prob": ion_prob,
+        "ion_prob_uncollected": ion_unc,
+        "num_effective_traj": n_eff,
+        "params": {"init_cond_method": init_cond_method, "laser": _ser_laser(laser), "target": _ser_target(target),
+                   "sample_t_intv": sample_t_intv, "sample_t_num": sample_t_num, "traj_phase_method": traj_phase_method,
+                   "traj_t_final": traj_t_final, "final_p_max": final_p_max, "final_p_num": final_p_num,
+                   "traj_integrator": traj_integrator, "traj_dt": traj_dt, "precision": precision},
+        "stats": stats,
+    }
+    if dimension == 3:
+        result["pz"] = _axis(final_p_max[2], final_p_num[2])
+    if output_path is not None:
+        path = output_path if output_path.endswith(".npz") else output_path + ".npz"
+        if os.path.exists(path):
+            warnings.warn(f"[TrajectorySimulationJob] File `{path}` already exists, will save at default filename.")
+            path = default_filename() + ".npz"
+        tmp = path + "!"
+        with open(tmp, "wb") as fh:
+            np.savez_compressed(fh, **{k: v for k, v in result.items() if isinstance(v, (np.ndarray, float, int))})
+        os.replace(tmp, path)
+        result["output_path"] = path
+    return result

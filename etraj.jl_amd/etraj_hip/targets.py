@@ -1,0 +1,179 @@
+"""Targets: hydrogen-like and SAE atoms (src/Targets/).
+
+Constructors mirror HydrogenLikeAtom.jl:49-64 and SAEAtom.jl:56-72 (same
+keywords, defaults and assertions; Ip in a.u.). `TargetPotential(t)` and
+`TargetForce(t)` evaluate the device functions through `etraj_target_eval`.
+`get_atom` uses the Tong-Lin SAE parameter table of AtomDatabase.jl:54-275
+(J. Phys. B 38, 2593 (2005)).
+"""
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+
+from . import _abi
+
+
+class Target:
+    """Target.jl: abstract target."""
+
+
+class SAEAtomBase(Target):
+    def _desc(self) -> _abi.TargetDesc:
+        d = _abi.TargetDesc()
+        d.kind = self._kind
+        d.nucl_charge = self.nucl_charge
+        d.l, d.m = self.l, self.m
+        d.Ip = self.Ip
+        d.asymp_coeff = self.asymp_coeff
+        d.quan_ax_theta, d.quan_ax_phi = self.quan_ax_θ, self.quan_ax_ϕ
+        d.soft_core = self.soft_core
+        for k in ("a1", "b1", "a2", "b2", "a3", "b3"):
+            setattr(d, k, getattr(self, k, 0.0))
+        return d
+
+    def __eq__(self, o):
+        return type(o) is type(self) and self.__dict__ == o.__dict__
+
+
+def hartree_asymp_coeff(Z, Ip, l):
+    """SAEAtomBase_shared.jl:3-10."""
+    if Z / math.sqrt(2 * Ip) <= l:
+        warnings.warn("[hartree_asymp_coeff] Z/κ<=l, unable to determine the asymptotic coefficient using Hartree "
+                      "formula. Using the l=0 case instead.")
+        l = 0
+    n = Z / math.sqrt(2 * Ip)
+    return 2 ** (n - 1) / math.sqrt(n * math.gamma(n + l + 1) * math.gamma(n - l))
+
+
+def _coeff(asymp_coeff, Z, Ip, l, who):
+    assert asymp_coeff == "hartree" or (not isinstance(asymp_coeff, str) and asymp_coeff > 0), \
+        f"[{who}] asymp_coeff should be either `:hartree` or a positive number."
+    return hartree_asymp_coeff(Z, Ip, l) if asymp_coeff == "hartree" else float(asymp_coeff)
+
+
+class HydrogenLikeAtom(SAEAtomBase):
+    """HydrogenLikeAtom.jl:7-64: V = -Z (r^2 + soft_core)^(-1/2)."""
+    _kind = _abi.TARGET_HYDROGEN_LIKE
+
+    def __init__(self, *, Ip, Z, l=0, m=0, asymp_coeff="hartree", quan_ax_θ=0.0, quan_ax_ϕ=0.0, soft_core=1e-10,
+                 name="[NA]"):
+        assert Ip > 0, "[HydrogenLikeAtom] `Ip` should be positive."
+        assert l >= 0 and l >= abs(m), "[HydrogenLikeAtom] Invalid (l,m)."
+        assert soft_core >= 0, "[HydrogenLikeAtom] `soft_core` should be non-negative."
+        self.Ip, self.nucl_charge, self.l, self.m = float(Ip), int(Z), int(l), int(m)
+        self.asymp_coeff = _coeff(asymp_coeff, Z, Ip, l, "HydrogenLikeAtom")
+        self.quan_ax_θ, self.quan_ax_ϕ, self.soft_core, self.name = float(quan_ax_θ), float(quan_ax_ϕ), \
+            float(soft_core), name
+
+    def __repr__(self):
+        return f"[HydrogenLikeAtom] Atom {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}"
+
+
+class SAEAtom(SAEAtomBase):
+    """SAEAtom.jl:7-72: Tong-Lin model potential."""
+    _kind = _abi.TARGET_SAE
+
+    def __init__(self, *, Ip, Z, l=0, m=0, asymp_coeff="hartree", quan_ax_θ=0.0, quan_ax_ϕ=0.0, a1=0.0, b1=0.0,
+                 a2=0.0, b2=0.0, a3=0.0, b3=0.0, soft_core=1e-10, name="[NA]"):
+        assert Ip > 0, "[SAEAtom] Ip should be positive."
+        assert l >= 0 and l >= abs(m), "[SAEAtom] Invalid (l,m)."
+        assert b1 >= 0 and b2 >= 0 and b3 >= 0, "[SAEAtom] b1,b2,b3 should be non-negative."
+        assert soft_core > 0, "[SAEAtom] soft_core should be positive."
+        self.Ip, self.nucl_charge, self.l, self.m = float(Ip), int(Z), int(l), int(m)
+        self.asymp_coeff = _coeff(asymp_coeff, Z, Ip, l, "SAEAtom")
+        self.quan_ax_θ, self.quan_ax_ϕ = float(quan_ax_θ), float(quan_ax_ϕ)
+        self.a1, self.b1, self.a2, self.b2, self.a3, self.b3 = map(float, (a1, b1, a2, b2, a3, b3))
+        self.soft_core, self.name = float(soft_core), name
+
+    def __repr__(self):
+        return f"[SAEAtom] Atom {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}"
+
+
+# ---- accessors (Targets/exports.jl) ------------------------------------------
+def IonPotential(t): return t.Ip
+def AsympNuclCharge(t): return t.nucl_charge
+def AngularQuantumNumber(t): return t.l
+def MagneticQuantumNumber(t): return t.m
+def AsympCoeff(t): return t.asymp_coeff
+def SoftCore(t): return t.soft_core
+def TargetName(t): return t.name
+def QuantizationAxisOrientaion(t): return (t.quan_ax_θ, t.quan_ax_ϕ)
+
+
+def _eval(t, pts):
+    ctx = _abi.default_context()
+    xyz = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+    n = xyz.shape[0]
+    V = np.empty(n)
+    F = np.empty((n, 3))
+    ctx.check(ctx.lib.etraj_target_eval(ctx.handle, t._desc(), _abi.dptr(xyz), n, _abi.dptr(V), _abi.dptr(F)))
+    return V, F
+
+
+def TargetPotential(t):
+    """(x,y,z) -> V, device evaluation (HydrogenLikeAtom.jl:75, SAEAtom.jl:84)."""
+    def V(x, y, z):
+        v, _ = _eval(t, np.stack(np.broadcast_arrays(x, y, z), axis=-1))
+        return v if np.ndim(x) or np.ndim(y) or np.ndim(z) else float(v[0])
+    return V
+
+
+def TargetForce(t):
+    """(x,y,z) -> (Fx,Fy,Fz), device evaluation (HydrogenLikeAtom.jl:76, SAEAtom.jl:91)."""
+    def F(x, y, z):
+        _, f = _eval(t, np.stack(np.broadcast_arrays(x, y, z), axis=-1))
+        return tuple(f[:, k] for k in range(3)) if np.ndim(x) or np.ndim(y) or np.ndim(z) else tuple(f[0])
+    return F
+
+
+def Serialize(t) -> dict:
+    d = {"type": type(t).__name__}
+    d.update(t.__dict__)
+    return d
+
+
+# ---- AtomDatabase.jl:54-275 -------------------------------------------------
+_HYD = {"H": (0.5, 1, "H"), "He1p": (1.0, 2, "He⁺"), "Li2p": (1.5, 3, "Li²⁺")}
+# key: (Ip, Z, a1, b1, a2, b2, a3, b3, l, name)
+_SAE = {
+    "He": (0.9035698802, 1, 1.230723, 0.6620055, -1.325040, 1.236224, -0.2307230, 0.4804286, 0, "He"),
+    "Ne": (0.79248225, 1, 8.069321, 2.148149, -3.569610, 1.985509, 0.9306791, 0.6018819, 1, "Ne"),
+    "Ne1p": (1.505361, 2, 8.042896, 2.714635, 0.5064300, 0.9818268, -0.04289633, 0.4012269, 1, "Ne⁺"),
+    "Ne2p": (2.3307635, 3, 7.000000, 3.007574, 0.6521344, 1.048617, 0.0, 0.0, 1, "Ne²⁺"),
+    "Ar": (0.579155055, 1, 16.03862, 2.007423, -25.54278, 4.524663, 0.9613848, 0.4426447, 1, "Ar"),
+    "Ar1p": (1.0153715, 2, 14.98869, 2.217293, -23.60558, 4.584761, 1.011314, 0.5510695, 1, "Ar⁺"),
+    "Ar2p": (1.497, 3, 15.00000, 3.588835, 5.951071, 1.692914, 0.0, 0.0, 1, "Ar²⁺"),
+    "V": (0.2479178, 1, 21.27861, 1.810071, -30.02658, 3.201500, 0.7213853, 0.4534235, 2, "V"),
+    "Ni": (0.28076035, 1, 26.81231, 1.937326, -39.55201, 3.161884, 0.1876890, 0.2626299, 2, "Ni"),
+    "Kr": (1.02895202, 1, 23.6188, 10.8391, 109.956, 6.55432, 11.8312, 1.27723, 1, "Kr"),
+    "Kr1p": (1.790416, 2, 23.1460, 11.0416, 110.447, 6.67063, 10.8540, 1.41524, 1, "Kr⁺"),
+    "Rb": (0.153506625, 1, 24.02321, 11.10663, 115.1998, 6.629123, 11.97679, 1.244882, 0, "Rb"),
+    "Nb": (0.248383, 1, 4.000000, 1.379674, -6.356689, 2.245839, 0.0, 0.0, 2, "Nb"),
+    "Pd": (0.3063732, 1, 43.83606, 2.395037, -93.35653, 4.907321, 1.163943, 0.3767380, 2, "Pd"),
+    "Xe": (0.445763535, 1, 51.35554, 2.111554, -99.92747, 3.737221, 1.644457, 0.4306465, 1, "Xe"),
+    "Xe1p": (0.7708, 2, 37.49194, 9.452474, 103.1967, 5.305537, 14.50806, 1.315683, 1, "Xe⁺"),
+    "Ta": (0.27744165, 1, 67.70192, 2.565180, -135.5191, 4.088800, 4.298078, 0.7303277, 2, "Ta"),
+}
+
+
+def get_available_atoms():
+    return list(_HYD) + list(_SAE)
+
+
+def get_atom(name: str, charge: int = 0, **kwargs):
+    """AtomDatabase.jl:33-47."""
+    key = name + (f"{charge}p" if charge > 0 else f"{-charge}n" if charge < 0 else "")
+    if key in _HYD:
+        Ip, Z, nm = _HYD[key]
+        args = dict(Ip=Ip, Z=Z, name=nm)
+        args.update(kwargs)
+        return HydrogenLikeAtom(**args)
+    if key in _SAE:
+        Ip, Z, a1, b1, a2, b2, a3, b3, l, nm = _SAE[key]
+        args = dict(Ip=Ip, Z=Z, a1=a1, b1=b1, a2=a2, b2=b2, a3=a3, b3=b3, l=l, name=nm)
+        args.update(kwargs)
+        return SAEAtom(**args)
+    raise KeyError(f"[get_atom] Atom name key `{key}` not found in database.")

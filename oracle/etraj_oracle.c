@@ -1,0 +1,1031 @@
+/*
+ * etraj_oracle.c -- CPU FP64 restatement of eTraj.jl's trajectory-simulation
+ * path, used ONLY as test infrastructure: by tests/, by the smoke check and by
+ * bench.py's cpu_baseline leg. The product (libetraj_hip.so) never links,
+ * loads or calls it.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the eTraj.jl source tree). Compiled with -ffp-contract=off so every
+ * operation is a plain IEEE FP64 operation in the order written.
+ *
+ * What pins each part (tests under tests/):
+ *   philox4x64_10        numpy.random.Philox (same generator, independent code)
+ *   laser_*              dA/dt = -F by 8th-order central differences (Lasers_test.jl:40-54),
+ *                        property identities (Lasers_test.jl:29-39), zero outside the window
+ *   target_*             V(1,1,1) closed forms and grad V = -F (Targets_test.jl:31-36,59-64)
+ *   adk_gen_batch        the reference's survivor-count KATs (ElectronSamplers_test.jl:53-130)
+ *   rk4 / tsit5          tableau order conditions, convergence order, Z=0 closed form
+ *   kepler_post          p_inf invariance along a field-free Coulomb orbit, |p_inf|=sqrt(2E)
+ *   scts asymptote       Phi(t)+S_C(t)+E t constant after the pulse
+ *   bin_index            hand-built edge cases (ties-to-even, index 0, num == 1)
+ * Parity unpinned: the exact Tsit5 step sequence of OrdinaryDiffEq 6.89 (the
+ * reference cannot run here), so Tsit5 output is pinned only by the
+ * mathematical checks above, never by reference vectors.
+ */
+#include "etraj_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ========================================================================== */
+/* Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based RNG.      */
+/* ========================================================================== */
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+    unsigned __int128 p = (unsigned __int128)a * (unsigned __int128)b;
+    *hi = (uint64_t)(p >> 64);
+    *lo = (uint64_t)p;
+}
+
+void oracle_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+    uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint64_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B97F4A7C15ULL;
+            k1 += 0xBB67AE8584CAA73BULL;
+        }
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ULL, c0, &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ULL, c2, &hi1, &lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0;
+        uint64_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* stream tags (counter word 3) */
+#define TAG_T0 1ULL
+#define TAG_K  2ULL
+
+static inline double u01_open_right(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }       /* [0,1) */
+static inline double u01_open_left(uint64_t x) { return ((double)(x >> 11) + 1.0) * 0x1.0p-53; } /* (0,1] */
+
+/* ========================================================================== */
+/* Lasers                                                                      */
+/* ========================================================================== */
+typedef struct {
+    int kind;
+    double F0, A0, w, N, cep, dt_shift, ellip, azi, k, sigma;
+} mono_t;
+
+typedef struct {
+    int n;
+    double delay;
+    mono_t c[2];
+} laser_t;
+
+/* LaserF0/AngFreq/Period etc.: Cos2Laser.jl:81-86, Cos4Laser.jl (same lines),
+ * GaussianLaser.jl:119-136 (SpreadDuration = spread_cyc_num * Period). */
+static int mono_init(const etraj_mono_laser* d, mono_t* m) {
+    if (d->kind != ETRAJ_LASER_COS2 && d->kind != ETRAJ_LASER_COS4 && d->kind != ETRAJ_LASER_GAUSSIAN) return -1;
+    if (!(d->wave_len > 0) || !(d->cyc_num > 0)) return -1;
+    m->kind = d->kind;
+    m->w = 45.563352525 / d->wave_len;
+    double period = 2 * M_PI / m->w;
+    m->F0 = sqrt(d->peak_int / (1.0 + d->ellip * d->ellip) / 3.50944521e16);
+    m->A0 = m->F0 / m->w;
+    m->N = d->cyc_num;
+    m->cep = d->cep;
+    m->dt_shift = d->t_shift;
+    m->ellip = d->ellip;
+    m->azi = d->azi;
+    m->k = 200 / period;
+    m->sigma = d->cyc_num * period;
+    return 0;
+}
+
+static int laser_init(const etraj_laser_desc* d, laser_t* L) {
+    if (d->n_colors != 1 && d->n_colors != 2) return -1;
+    L->n = d->n_colors;
+    L->delay = d->delay21;
+    for (int i = 0; i < L->n; ++i)
+        if (mono_init(&d->color[i], &L->c[i])) return -1;
+    return 0;
+}
+
+/* smooth cut-off window of the cos^n envelopes (Cos2Laser.jl:93) */
+static inline double cosn_window(const mono_t* m, double t) {
+    return (0.5 + 0.5 * tanh(m->k * (t + m->N * M_PI / m->w))) * (0.5 + 0.5 * tanh(-m->k * (t - m->N * M_PI / m->w)));
+}
+
+/* UnitEnvelope: Cos2Laser.jl:89-95, Cos4Laser.jl:89-95, GaussianLaser.jl:138-144 */
+static double mono_envelope(const mono_t* m, double t) {
+    t -= m->dt_shift;
+    if (m->kind == ETRAJ_LASER_GAUSSIAN) return exp(-t * t / 2 / (m->sigma * m->sigma));
+    double c = cos(m->w * t / (2 * m->N));
+    double cn = (m->kind == ETRAJ_LASER_COS2) ? c * c : (c * c) * (c * c);
+    return cn * cosn_window(m, t);
+}
+
+/* LaserFx/LaserFy: Cos2Laser.jl:125-152, Cos4Laser.jl:125-152, GaussianLaser.jl:176-203 */
+static void mono_F(const mono_t* m, double t, double* Fx, double* Fy) {
+    t -= m->dt_shift;
+    const double w = m->w, N = m->N, phi = m->cep, e = m->ellip, az = m->azi;
+    if (m->kind == ETRAJ_LASER_GAUSSIAN) {
+        double s2 = m->sigma * m->sigma;
+        double g = m->A0 * exp(-t * t / 2 / s2);
+        double cw = cos(w * t + phi), sw = sin(w * t + phi);
+        double a = t / s2 * cw + w * sw;   /* x-like component */
+        double b = t / s2 * sw - w * cw;   /* y-like component */
+        if (az == 0) {
+            *Fx = g * a;
+            *Fy = g * b * e;
+        } else {
+            *Fx = g * (a * cos(az) + b * e * sin(az));
+            *Fy = g * (a * -sin(az) + b * e * cos(az));
+        }
+        return;
+    }
+    double ch = cos(w * t / (2 * N)), sh = sin(w * t / (2 * N));
+    double cw = cos(w * t + phi), sw = sin(w * t + phi);
+    double win = cosn_window(m, t);
+    double pre, a, b;
+    if (m->kind == ETRAJ_LASER_COS2) {
+        pre = m->F0 * ch * win;
+        a = ch * sw + 1 / N * sh * cw;        /* Cos2Laser.jl:130 */
+        b = -ch * cw + 1 / N * sh * sw;       /* Cos2Laser.jl:144 */
+    } else {
+        pre = m->F0 * (ch * ch * ch) * win;
+        a = ch * sw + 2 / N * sh * cw;        /* Cos4Laser.jl:130 */
+        b = -ch * cw + 2 / N * sh * sw;       /* Cos4Laser.jl:144 */
+    }
+    if (az == 0) {
+        *Fx = pre * a;
+        *Fy = pre * b * e;
+    } else {
+        *Fx = pre * (a * cos(az) + b * e * sin(az));
+        *Fy = pre * (a * -sin(az) + b * e * cos(az));
+    }
+}
+
+/* LaserAx/LaserAy: Cos2Laser.jl:97-124, Cos4Laser.jl:97-124, GaussianLaser.jl:146-175 */
+static void mono_A(const mono_t* m, double t, double* Ax, double* Ay) {
+    double env = mono_envelope(m, t);
+    t -= m->dt_shift;
+    double cw = cos(m->w * t + m->cep), sw = sin(m->w * t + m->cep);
+    double g = m->A0 * env;
+    if (m->azi == 0) {
+        *Ax = g * cw;
+        *Ay = g * sw * m->ellip;
+    } else {
+        *Ax = g * (cw * cos(m->azi) + sw * m->ellip * sin(m->azi));
+        *Ay = g * (cw * -sin(m->azi) + sw * m->ellip * cos(m->azi));
+    }
+}
+
+static inline void laser_F(const laser_t* L, double t, double* Fx, double* Fy) {
+    mono_F(&L->c[0], t, Fx, Fy);
+    if (L->n == 2) { /* BichromaticLaser.jl:73-85 */
+        double fx2, fy2;
+        mono_F(&L->c[1], t - L->delay, &fx2, &fy2);
+        *Fx += fx2;
+        *Fy += fy2;
+    }
+}
+
+static inline void laser_A(const laser_t* L, double t, double* Ax, double* Ay) {
+    mono_A(&L->c[0], t, Ax, Ay);
+    if (L->n == 2) { /* BichromaticLaser.jl:59-71 */
+        double ax2, ay2;
+        mono_A(&L->c[1], t - L->delay, &ax2, &ay2);
+        *Ax += ax2;
+        *Ay += ay2;
+    }
+}
+
+int oracle_laser_field(const etraj_laser_desc* d, const double* t, int64_t n,
+                       double* Fx, double* Fy, double* Ax, double* Ay) {
+    laser_t L;
+    if (laser_init(d, &L)) return ETRAJ_EINVAL;
+    for (int64_t i = 0; i < n; ++i) {
+        double fx, fy, ax, ay;
+        laser_F(&L, t[i], &fx, &fy);
+        laser_A(&L, t[i], &ax, &ay);
+        if (Fx) Fx[i] = fx;
+        if (Fy) Fy[i] = fy;
+        if (Ax) Ax[i] = ax;
+        if (Ay) Ay[i] = ay;
+    }
+    return ETRAJ_OK;
+}
+
+double oracle_unit_envelope(const etraj_laser_desc* d, double t) {
+    laser_t L;
+    if (laser_init(d, &L)) return NAN;
+    return mono_envelope(&L.c[0], t);
+}
+
+/* ========================================================================== */
+/* Targets                                                                     */
+/* ========================================================================== */
+typedef struct {
+    int kind, Z, l, m;
+    double Ip, C, sc, a1, b1, a2, b2, a3, b3;
+} target_t;
+
+static int target_init(const etraj_target_desc* d, target_t* T) {
+    if (d->kind != ETRAJ_TARGET_HYDROGEN_LIKE && d->kind != ETRAJ_TARGET_SAE) return -1;
+    if (!(d->Ip > 0) || d->soft_core < 0) return -1;
+    T->kind = d->kind; T->Z = d->nucl_charge; T->l = d->l; T->m = d->m;
+    T->Ip = d->Ip; T->C = d->asymp_coeff; T->sc = d->soft_core;
+    T->a1 = d->a1; T->b1 = d->b1; T->a2 = d->a2; T->b2 = d->b2; T->a3 = d->a3; T->b3 = d->b3;
+    return 0;
+}
+
+/* TargetPotential: HydrogenLikeAtom.jl:75, SAEAtom.jl:84-89 */
+static inline double target_V(const target_t* T, double x, double y, double z) {
+    if (T->kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        return -T->Z * pow(x * x + y * y + z * z + T->sc, -0.5);
+    double r = sqrt(x * x + y * y + z * z);
+    return -(T->Z + T->a1 * exp(-T->b1 * r) + T->a2 * r * exp(-T->b2 * r) + T->a3 * exp(-T->b3 * r)) / sqrt(r * r + T->sc);
+}
+
+/* TargetForce: HydrogenLikeAtom.jl:76, SAEAtom.jl:91-96 (returns the scalar s with F = s * (x,y,z)) */
+static inline double target_Fscale(const target_t* T, double x, double y, double z) {
+    if (T->kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        return -T->Z * pow(x * x + y * y + z * z + T->sc, -1.5);
+    double r = sqrt(x * x + y * y + z * z);
+    return -(pow(r * r + T->sc, -1.5) * (T->Z + T->a1 * (1 + T->b1 * r) * exp(-T->b1 * r) + T->a3 * (1 + T->b3 * r) * exp(-T->b3 * r))
+             + T->a2 * T->b2 / sqrt(r * r + T->sc) * exp(-T->b2 * r));
+}
+
+int oracle_target_eval(const etraj_target_desc* d, const double* xyz, int64_t n, double* V, double* F) {
+    target_t T;
+    if (target_init(d, &T)) return ETRAJ_EINVAL;
+    for (int64_t i = 0; i < n; ++i) {
+        double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (V) V[i] = target_V(&T, x, y, z);
+        if (F) {
+            double s = target_Fscale(&T, x, y, z);
+            F[3 * i] = s * x; F[3 * i + 1] = s * y; F[3 * i + 2] = s * z;
+        }
+    }
+    return ETRAJ_OK;
+}
+
+/* ========================================================================== */
+/* Equations of motion: TrajectoryFunction                                     */
+/*   HydrogenLikeAtom.jl:78-159, SAEAtom.jl:98-186                             */
+/* state (2D) x y px py [phi];  (3D) x y z px py pz [phi]                      */
+/* ========================================================================== */
+typedef struct {
+    const laser_t* L;
+    const target_t* T;
+    int dim, phase, quirks;
+} eom_t;
+
+static void eom_rhs(const eom_t* E, double t, const double* u, double* du) {
+    const target_t* T = E->T;
+    double Fx, Fy;
+    laser_F(E->L, t, &Fx, &Fy);
+    double x = u[0], y = u[1], z = 0, px, py, pz = 0;
+    if (E->dim == 2) { px = u[2]; py = u[3]; }
+    else { z = u[2]; px = u[3]; py = u[4]; pz = u[5]; }
+    double tFx, tFy, tFz = 0, V = 0;
+    if (T->kind == ETRAJ_TARGET_HYDROGEN_LIKE) {
+        double r2 = (E->dim == 2) ? x * x + y * y + T->sc : x * x + y * y + z * z + T->sc;
+        double s = -T->Z * pow(r2, -1.5);
+        tFx = s * x; tFy = s * y; tFz = s * z;
+        if (E->phase != ETRAJ_PHASE_CTMC) V = -T->Z * pow(r2, -0.5);
+    } else {
+        double r = (E->dim == 2) ? sqrt(x * x + y * y) : sqrt(x * x + y * y + z * z);
+        double s = -(pow(r * r + T->sc, -1.5) * (T->Z + T->a1 * (1 + T->b1 * r) * exp(-T->b1 * r) + T->a3 * (1 + T->b3 * r) * exp(-T->b3 * r))
+                     + T->a2 * T->b2 / sqrt(r * r + T->sc) * exp(-T->b2 * r));
+        tFx = s * x; tFy = s * y; tFz = s * z;
+        if (E->phase != ETRAJ_PHASE_CTMC) {
+            double num = T->Z + T->a1 * exp(-T->b1 * r) + T->a2 * r * exp(-T->b2 * r) + T->a3 * exp(-T->b3 * r);
+            /* SAEAtom.jl:136 divides by (r^2+sc) in 2D SCTS; :124/:166/:181 use sqrt */
+            if (E->dim == 2 && E->phase == ETRAJ_PHASE_SCTS && (E->quirks & ETRAJ_QUIRK_SAE2D_SCTS))
+                V = -num / (r * r + T->sc);
+            else
+                V = -num / sqrt(r * r + T->sc);
+        }
+    }
+    if (E->dim == 2) {
+        du[0] = px; du[1] = py; du[2] = tFx - Fx; du[3] = tFy - Fy;
+        double k2 = (px * px + py * py) / 2;
+        if (E->phase == ETRAJ_PHASE_QTMC) du[4] = -(T->Ip + k2 + V);                 /* :101 */
+        else if (E->phase == ETRAJ_PHASE_SCTS) du[4] = -(k2 + V + (x * tFx + y * tFy)); /* :112 */
+    } else {
+        du[0] = px; du[1] = py; du[2] = pz; du[3] = tFx - Fx; du[4] = tFy - Fy; du[5] = tFz;
+        double k2 = (px * px + py * py + pz * pz) / 2;
+        if (E->phase == ETRAJ_PHASE_QTMC) du[6] = -(T->Ip + k2 + V);                                /* :140 */
+        else if (E->phase == ETRAJ_PHASE_SCTS) du[6] = -(k2 + V + (x * tFx + y * tFy + z * tFz));     /* :154 */
+    }
+}
+
+/* ========================================================================== */
+/* Integrators                                                                 */
+/* ========================================================================== */
+
+/* Fixed-step classical RK4, the engine's extension of the reference's ODE
+ * solve (TrajectorySimulation.jl:342 uses Tsit5): n = ceil((tf-t0)/dt) equal
+ * steps of h = (tf-t0)/n; stage times t0 + j*(h/2) for integer j. */
+static int integrate_rk4(const eom_t* E, int nv, double t0, double tf, double dt, double* u, int64_t* nsteps) {
+    double span = tf - t0;
+    int64_t n = (int64_t)ceil(span / dt);
+    if (n < 1) n = 1;
+    double h = span / (double)n;
+    double hh = h * 0.5;
+    double h6 = h / 6.0;
+    double k1[7], k2[7], k3[7], k4[7], y[7];
+    for (int64_t s = 0; s < n; ++s) {
+        double t = t0 + (double)(2 * s) * hh;
+        double tm = t0 + (double)(2 * s + 1) * hh;
+        double tn = t0 + (double)(2 * s + 2) * hh;
+        eom_rhs(E, t, u, k1);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + hh * k1[i];
+        eom_rhs(E, tm, y, k2);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + hh * k2[i];
+        eom_rhs(E, tm, y, k3);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + h * k3[i];
+        eom_rhs(E, tn, y, k4);
+        for (int i = 0; i < nv; ++i) u[i] = u[i] + h6 * (k1[i] + 2.0 * (k2[i] + k3[i]) + k4[i]);
+    }
+    *nsteps = n;
+    for (int i = 0; i < nv; ++i)
+        if (isnan(u[i])) return 1;
+    return 0;
+}
+
+/* Tsit5 (Tsitouras 2011, Comput. Math. Appl. 62, 770) coefficients. */
+static const double TS_c2 = 0.161, TS_c3 = 0.327, TS_c4 = 0.9, TS_c5 = 0.9800255409045097;
+static const double TS_a21 = 0.161;
+static const double TS_a31 = -0.008480655492356989, TS_a32 = 0.335480655492357;
+static const double TS_a41 = 2.897153057105493, TS_a42 = -6.359448489975075, TS_a43 = 4.3622954328695815;
+static const double TS_a51 = 5.325864828439257, TS_a52 = -11.748883564062828, TS_a53 = 7.4955393428898365, TS_a54 = -0.09249506636175525;
+static const double TS_a61 = 5.86145544294642, TS_a62 = -12.92096931784711, TS_a63 = 8.159367898576159, TS_a64 = -0.071584973281401, TS_a65 = -0.028269050394068383;
+static const double TS_a71 = 0.09646076681806523, TS_a72 = 0.01, TS_a73 = 0.4798896504144996, TS_a74 = 1.379008574103742, TS_a75 = -3.290069515436081, TS_a76 = 2.324710524099774;
+static const double TS_bt1 = -0.00178001105222577714, TS_bt2 = -0.0008164344596567469, TS_bt3 = 0.007880878010261995,
+                    TS_bt4 = -0.1447110071732629, TS_bt5 = 0.5823571654525552, TS_bt6 = -0.45808210592918697, TS_bt7 = 0.015151515151515152;
+
+void oracle_tsit5_tableau(double* c, double* a, double* b, double* btilde) {
+    /* c[7], a[7*7] row-major strictly lower, b[7] (= a7 row, FSAL), btilde[7] */
+    const double cc[7] = {0, TS_c2, TS_c3, TS_c4, TS_c5, 1, 1};
+    double aa[49] = {0};
+    aa[1 * 7 + 0] = TS_a21;
+    aa[2 * 7 + 0] = TS_a31; aa[2 * 7 + 1] = TS_a32;
+    aa[3 * 7 + 0] = TS_a41; aa[3 * 7 + 1] = TS_a42; aa[3 * 7 + 2] = TS_a43;
+    aa[4 * 7 + 0] = TS_a51; aa[4 * 7 + 1] = TS_a52; aa[4 * 7 + 2] = TS_a53; aa[4 * 7 + 3] = TS_a54;
+    aa[5 * 7 + 0] = TS_a61; aa[5 * 7 + 1] = TS_a62; aa[5 * 7 + 2] = TS_a63; aa[5 * 7 + 3] = TS_a64; aa[5 * 7 + 4] = TS_a65;
+    aa[6 * 7 + 0] = TS_a71; aa[6 * 7 + 1] = TS_a72; aa[6 * 7 + 2] = TS_a73; aa[6 * 7 + 3] = TS_a74; aa[6 * 7 + 4] = TS_a75; aa[6 * 7 + 5] = TS_a76;
+    const double bt[7] = {TS_bt1, TS_bt2, TS_bt3, TS_bt4, TS_bt5, TS_bt6, TS_bt7};
+    for (int i = 0; i < 7; ++i) {
+        c[i] = cc[i];
+        b[i] = aa[6 * 7 + i];
+        btilde[i] = bt[i];
+    }
+    memcpy(a, aa, sizeof aa);
+}
+
+static inline double rms_norm(const double* v, int n) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += v[i] * v[i];
+    return sqrt(s / n);
+}
+
+/* Adaptive Tsit5 with OrdinaryDiffEq's defaults for an out-of-place
+ * explicit method (restated, parity unpinned): Hairer initial dt,
+ * PI controller beta1 = 7/50, beta2 = 2/25, qmin = 1/5, qmax = 10,
+ * gamma = 9/10, qoldinit = 1e-4, RMS error norm, maxiters, NaN check.
+ * Called as solve(..., Tsit5(), adaptive=true, reltol=traj_rtol)
+ * (TrajectorySimulation.jl:342). Returns 0 on success, 1 on failure. */
+static int integrate_tsit5(const eom_t* E, int nv, double t0, double tf, double rtol, double atol,
+                           int64_t maxiters, double* u, int64_t* nsteps) {
+    const double qmin = 0.2, qmax = 10.0, gamma = 0.9, beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qoldinit = 1e-4;
+    const double dtmax = tf - t0;
+    const double dtmin = fmax(fabs(t0), fabs(tf)) * 0x1.0p-52;   /* ~eps(max(|t0|,|tf|)) */
+    double k1[7], k2[7], k3[7], k4[7], k5[7], k6[7], k7[7], y[7], un[7], sk[7], tmp[7];
+    double t = t0;
+    /* ode_determine_initdt (Hairer & Wanner, Solving ODEs I, II.4) */
+    double dtmin_init = nextafter(fmax(0.0, fabs(t) * 0x1.0p-52), INFINITY);
+    double smalldt = fmax(dtmin_init, 1e-6);
+    for (int i = 0; i < nv; ++i) sk[i] = atol + fabs(u[i]) * rtol;
+    for (int i = 0; i < nv; ++i) tmp[i] = u[i] / sk[i];
+    double d0 = rms_norm(tmp, nv);
+    eom_rhs(E, t, u, k1);
+    for (int i = 0; i < nv; ++i) tmp[i] = k1[i] / sk[i];
+    double d1 = rms_norm(tmp, nv);
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
+    dt0 = fmin(dt0, dtmax);
+    for (int i = 0; i < nv; ++i) y[i] = u[i] + dt0 * k1[i];
+    eom_rhs(E, t + dt0, y, k2);
+    for (int i = 0; i < nv; ++i) tmp[i] = (k2[i] - k1[i]) / sk[i];
+    double d2 = rms_norm(tmp, nv) / dt0;
+    double mx = fmax(d1, d2);
+    double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(10.0, -(2 + log10(mx)) / 5.0);
+    double dt = fmax(dtmin_init, fmin(fmin(100 * dt0, dt1), dtmax));
+
+    double qold = qoldinit, q11 = 1.0;
+    int64_t iter = 0, accepted = 0;
+    while (t < tf) {
+        if (tf - t < dt) dt = tf - t;
+        if (++iter > maxiters) { *nsteps = accepted; return 1; }
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + dt * (TS_a21 * k1[i]);
+        eom_rhs(E, t + TS_c2 * dt, y, k2);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + dt * (TS_a31 * k1[i] + TS_a32 * k2[i]);
+        eom_rhs(E, t + TS_c3 * dt, y, k3);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + dt * (TS_a41 * k1[i] + TS_a42 * k2[i] + TS_a43 * k3[i]);
+        eom_rhs(E, t + TS_c4 * dt, y, k4);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + dt * (TS_a51 * k1[i] + TS_a52 * k2[i] + TS_a53 * k3[i] + TS_a54 * k4[i]);
+        eom_rhs(E, t + TS_c5 * dt, y, k5);
+        for (int i = 0; i < nv; ++i) y[i] = u[i] + dt * (TS_a61 * k1[i] + TS_a62 * k2[i] + TS_a63 * k3[i] + TS_a64 * k4[i] + TS_a65 * k5[i]);
+        eom_rhs(E, t + dt, y, k6);
+        for (int i = 0; i < nv; ++i)
+            un[i] = u[i] + dt * (TS_a71 * k1[i] + TS_a72 * k2[i] + TS_a73 * k3[i] + TS_a74 * k4[i] + TS_a75 * k5[i] + TS_a76 * k6[i]);
+        eom_rhs(E, t + dt, un, k7);
+        for (int i = 0; i < nv; ++i) {
+            double ut = dt * (TS_bt1 * k1[i] + TS_bt2 * k2[i] + TS_bt3 * k3[i] + TS_bt4 * k4[i] + TS_bt5 * k5[i] + TS_bt6 * k6[i] + TS_bt7 * k7[i]);
+            tmp[i] = ut / (atol + fmax(fabs(u[i]), fabs(un[i])) * rtol);
+        }
+        double EEst = rms_norm(tmp, nv);
+        if (isnan(EEst)) { *nsteps = accepted; return 1; }
+        double q;
+        if (EEst == 0) {
+            q = 1.0 / qmax;
+        } else {
+            q11 = pow(EEst, beta1);
+            q = q11 / pow(qold, beta2);
+            q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+        }
+        if (EEst <= 1.0) {
+            t = (tf - t == dt) ? tf : t + dt;
+            memcpy(u, un, sizeof(double) * nv);
+            memcpy(k1, k7, sizeof(double) * nv);
+            ++accepted;
+            if (q == 1.0) q = 1.0;  /* qsteady_min = qsteady_max = 1 */
+            qold = fmax(EEst, qoldinit);
+            dt = fmin(dt / q, dtmax);
+        } else {
+            dt = dt / fmin(1.0 / qmin, q11 / gamma);
+        }
+        if (fabs(dt) < dtmin && t < tf) { *nsteps = accepted; return 1; }
+    }
+    *nsteps = accepted;
+    return 0;
+}
+
+/* ========================================================================== */
+/* Post-processing: Kepler asymptote, SCTS post-pulse phase, binning           */
+/*   TrajectorySimulation.jl:356-396 (2D) / :451-496 (3D)                      */
+/* ========================================================================== */
+
+/* round(Int, (p + pmax) / (pmax / num * 2)), RoundNearest = ties to even;
+ * returns the 1-based Julia index or 0 when not representable. */
+int64_t oracle_bin_index(double p, double pmax, int32_t num) {
+    if (num <= 1) return 1;
+    double q = (p + pmax) / (pmax / num * 2);
+    if (!(fabs(q) < 9.0e15)) return 0;
+    return (int64_t)nearbyint(q);
+}
+
+typedef struct {
+    double rec[12];
+    int status; /* 0 E>=0, 1 failed, 2 bound */
+} post_t;
+
+static void post_process(const target_t* T, int dim, int phase_method, int quirks,
+                         const double* u, double t0, post_t* out) {
+    double x = u[0], y = u[1], z = 0, px, py, pz = 0, phase = 0;
+    if (dim == 2) { px = u[2]; py = u[3]; if (phase_method != ETRAJ_PHASE_CTMC) phase = u[4]; }
+    else { z = u[2]; px = u[3]; py = u[4]; pz = u[5]; if (phase_method != ETRAJ_PHASE_CTMC) phase = u[6]; }
+    double rec0[7] = {x, y, z, px, py, pz, phase};
+    memcpy(out->rec, rec0, sizeof rec0);
+    double E = (px * px + py * py + pz * pz) / 2 + target_V(T, x, y, z);
+    double Lx = y * pz - z * py, Ly = z * px - x * pz, Lz = x * py - y * px;
+    double L2 = Lx * Lx + Ly * Ly + Lz * Lz;
+    double Z = (double)T->Z;
+    if (E > 0 && phase_method == ETRAJ_PHASE_SCTS) {
+        phase += T->Ip * t0;
+        double sqrtb = pow(2 * E, -0.5);
+        double g = sqrt(1 + 2 * E * L2);
+        double rp;
+        if (dim == 2) rp = x * px + y * py;
+        else rp = (quirks & ETRAJ_QUIRK_SCTS3D_XPY) ? x * py + y * py + z * pz : x * px + y * py + z * pz;
+        phase -= Z * sqrtb * (log(g) + asinh(rp / (g * sqrtb)));
+    }
+    out->rec[7] = E;
+    out->rec[11] = phase;
+    if (E >= 0) {
+        double pinf = sqrt(2 * E);
+        double r = sqrt(x * x + y * y + z * z);
+        /* a = p x L - Z r/|r| */
+        double ax = (py * Lz - pz * Ly) - Z * x / r;
+        double ay = (pz * Lx - px * Lz) - Z * y / r;
+        double az = (px * Ly - py * Lx) - Z * z / r;
+        /* L x a */
+        double lax = Ly * az - Lz * ay, lay = Lz * ax - Lx * az, laz = Lx * ay - Ly * ax;
+        double f = pinf / (1 + pinf * pinf * L2);
+        out->rec[8] = f * (pinf * lax - ax);
+        out->rec[9] = f * (pinf * lay - ay);
+        out->rec[10] = f * (pinf * laz - az);
+        out->status = 0;
+    } else {
+        out->rec[8] = out->rec[9] = out->rec[10] = NAN;
+        out->status = 2;
+    }
+}
+
+/* ========================================================================== */
+/* ADK sampler: ADKSampler.jl:19-396 (atoms, l = 0)                            */
+/* ========================================================================== */
+typedef struct {
+    const etraj_sampler_desc* sd;
+    laser_t L;
+    target_t T;
+    int prefix;        /* resolved ETRAJ_PREFIX_* bits */
+    double* tsamp;     /* t_samples, length sample_t_num */
+    double tstep;      /* step(t_samples) = (max-min)/length  (ADKSampler.jl:267) */
+    double c_pre;      /* c (ADKSampler.jl:233) */
+} sampler_t;
+
+/* i-th point (0-based) of range(lo, hi; length=n): (lo*(n-1-i) + hi*i)/(n-1),
+ * the correctly rounded rational Julia's range produces for integer endpoints. */
+double oracle_grid_point(double lo, double hi, int n, int i) {
+    if (n <= 1) return lo;
+    return (lo * (double)(n - 1 - i) + hi * (double)i) / (double)(n - 1);
+}
+
+static int cmp_double(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+/* t_samples: range(lo,hi;length=n) (ADKSampler.jl:136), sorted seeded
+ * uniforms (:141, MONTECARLO), or one jittered point per stratum (STRATIFIED). */
+int oracle_batch_times(const etraj_sampler_desc* sd, double* t) {
+    int n = sd->sample_t_num;
+    if (n <= 0) return ETRAJ_EINVAL;
+    double lo = sd->sample_t_lo, hi = sd->sample_t_hi;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        for (int i = 0; i < n; ++i) t[i] = oracle_grid_point(lo, hi, n, i);
+    } else if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        for (int i = 0; i < n; ++i) {
+            uint64_t c[4] = {(uint64_t)i, 0, 0, TAG_T0}, k[2] = {sd->seed, 0}, o[4];
+            oracle_philox4x64_10(c, k, o);
+            t[i] = u01_open_right(o[0]) * (hi - lo) + lo;
+        }
+        qsort(t, n, sizeof(double), cmp_double);
+    } else if (sd->sampling == ETRAJ_SAMPLING_STRATIFIED) {
+        double w = (hi - lo) / (double)n;
+        for (int i = 0; i < n; ++i) {
+            uint64_t c[4] = {(uint64_t)i, 0, 0, TAG_T0}, k[2] = {sd->seed, 0}, o[4];
+            oracle_philox4x64_10(c, k, o);
+            t[i] = lo + ((double)i + u01_open_right(o[0])) * w;
+        }
+    } else {
+        return ETRAJ_EINVAL;
+    }
+    return ETRAJ_OK;
+}
+
+int64_t oracle_batch_max_size(const etraj_sampler_desc* sd) {
+    if (sd->sampling == ETRAJ_SAMPLING_STEP)
+        return (int64_t)sd->ss_kd_num * (sd->dimension == 3 ? (int64_t)sd->ss_kz_num : 1);
+    return sd->mc_kt_num;
+}
+
+static int resolve_prefix(int bits, const laser_t* L) {
+    if ((bits & ETRAJ_PREFIX_PRE) && (bits & ETRAJ_PREFIX_PRECC)) return -1; /* ADKSampler.jl:66-68 */
+    if ((bits & ETRAJ_PREFIX_PRECC) && L->n != 1) {                          /* :72-77, Q13 */
+        bits &= ~ETRAJ_PREFIX_PRECC;
+        bits |= ETRAJ_PREFIX_PRE;
+    }
+    return bits;
+}
+
+static int sampler_init(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd, sampler_t* S) {
+    memset(S, 0, sizeof *S);
+    S->sd = sd;
+    if (laser_init(ld, &S->L) || target_init(td, &S->T)) return ETRAJ_EINVAL;
+    if (sd->init_method != ETRAJ_INIT_ADK) return ETRAJ_EUNSUPPORTED;
+    if (S->T.l != 0) return ETRAJ_EUNSUPPORTED;
+    if (sd->dimension != 2 && sd->dimension != 3) return ETRAJ_EINVAL;
+    if (sd->phase_method < 0 || sd->phase_method > 2) return ETRAJ_EINVAL;
+    if (sd->sample_t_num <= 0 || sd->cutoff_limit < 0) return ETRAJ_EINVAL;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        if (sd->ss_kd_num <= 0) return ETRAJ_EINVAL;
+        if (sd->dimension == 2 && !(sd->ss_kd_max > 0)) return ETRAJ_EINVAL;
+        if (sd->dimension == 3 && (sd->ss_kz_num <= 0 || !(sd->ss_kd_max > 0 || sd->ss_kz_max > 0))) return ETRAJ_EINVAL;
+    } else {
+        if (sd->mc_kt_num <= 0 || !(sd->sample_t_lo < sd->sample_t_hi)) return ETRAJ_EINVAL;
+    }
+    int p = resolve_prefix(sd->rate_prefix, &S->L);
+    if (p < 0) return ETRAJ_EINVAL;
+    S->prefix = p;
+    S->tsamp = (double*)malloc(sizeof(double) * sd->sample_t_num);
+    if (!S->tsamp) return ETRAJ_ENOMEM;
+    oracle_batch_times(sd, S->tsamp);
+    double tmin = S->tsamp[0], tmax = S->tsamp[0];
+    for (int i = 1; i < sd->sample_t_num; ++i) {
+        tmin = fmin(tmin, S->tsamp[i]);
+        tmax = fmax(tmax, S->tsamp[i]);
+    }
+    S->tstep = (tmax - tmin) / sd->sample_t_num;
+    double kappa = sqrt(2 * S->T.Ip), n = S->T.Z / kappa;
+    S->c_pre = pow(2, n / 2 + 1) * pow(kappa, 2 * n + 0.5) * tgamma(n / 2 + 1);
+    return ETRAJ_OK;
+}
+
+static void sampler_free(sampler_t* S) { free(S->tsamp); S->tsamp = NULL; }
+
+static inline double gauss_pdf(double x, double s) { return exp(-0.5 * (x / s) * (x / s)) / (s * sqrt(2 * M_PI)); }
+
+/* gen_electron_batch(sp::ADKSampler, batch_id) for an atom with l = 0
+ * (ADKSampler.jl:167-396). For l = 0 the Wigner-D factor is 1 and Y_00 =
+ * 1/sqrt(4 pi), so the prefactor is real and positive: angle(amp) = 0. */
+static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out, int64_t cap) {
+    const etraj_sampler_desc* sd = S->sd;
+    const int dim = sd->dimension, ph = sd->phase_method;
+    const int rows = (dim == 3) ? (ph == ETRAJ_PHASE_CTMC ? 8 : 9) : (ph == ETRAJ_PHASE_CTMC ? 6 : 7);
+    const double tr = S->tsamp[b];
+    double Fxtr, Fytr;
+    laser_F(&S->L, tr, &Fxtr, &Fytr);
+    const double Ftr = hypot(Fxtr, Fytr);
+    const double phi = atan2(-Fytr, -Fxtr);                       /* :173 */
+    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    double gam = 0.0;                                              /* :183-187 */
+    if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
+    const double kappa = sqrt(2 * Ip), n = Z / kappa;
+    const double e0 = 2.71828182845904523;
+    const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n); /* :235 */
+    const double Y00 = sqrt(1.0 / (4 * M_PI));                     /* shared_methods.jl:21, l = m = 0 */
+    const double numer = S->T.C * 1.0 * Y00;                       /* pre_numerator :255, D^0_00 = 1 */
+    const double jac = Ftr;                                        /* :266 */
+    double dkdt;                                                   /* :268-276 */
+    int64_t nk;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        double kd_step = sd->ss_kd_num > 1 ? (2 * fabs(sd->ss_kd_max)) / sd->ss_kd_num : 1.0;
+        double kz_step = (dim == 3 && sd->ss_kz_num > 1) ? (2 * fabs(sd->ss_kz_max)) / sd->ss_kz_num : 1.0;
+        dkdt = S->tstep * kd_step * kz_step;
+        nk = (int64_t)sd->ss_kd_num * (dim == 3 ? sd->ss_kz_num : 1);
+    } else if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        double kd_span = sd->mc_kd_max > 0 ? 2 * sd->mc_kd_max : 1.0;
+        double kz_span = sd->mc_kz_max > 0 ? 2 * sd->mc_kz_max : 1.0;   /* no dimension check in :274 */
+        dkdt = S->tstep * kd_span * kz_span / sd->mc_kt_num;
+        nk = sd->mc_kt_num;
+    } else {
+        dkdt = (sd->sample_t_hi - sd->sample_t_lo) / sd->sample_t_num / sd->mc_kt_num; /* divided by pdf below */
+        nk = sd->mc_kt_num;
+    }
+    int64_t count = 0;
+    const int kzn = (dim == 3) ? sd->ss_kz_num : 1;
+    for (int64_t j = 0; j < nk; ++j) {
+        double kd0, kz0 = 0, w = dkdt;
+        if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+            int64_t ikd = j / kzn, ikz = j % kzn;     /* loop order ikd outer, ikz inner (:318-319) */
+            double kdm = fabs(sd->ss_kd_max), kzm = fabs(sd->ss_kz_max);
+            kd0 = oracle_grid_point(-kdm, kdm, sd->ss_kd_num, (int)ikd);
+            if (dim == 3) kz0 = oracle_grid_point(-kzm, kzm, kzn, (int)ikz);
+        } else {
+            uint64_t c[4] = {(uint64_t)j, (uint64_t)b, 0, TAG_K}, k[2] = {sd->seed, 0}, o[4];
+            oracle_philox4x64_10(c, k, o);
+            if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+                /* gen_rand_pt_1d / gen_rand_pt_2dsq (shared_methods.jl:4-12) */
+                kd0 = (u01_open_right(o[0]) - 0.5) * 2 * sd->mc_kd_max;
+                if (dim == 3) {
+                    kz0 = (u01_open_right(o[1]) - 0.5) * 2 * sd->mc_kz_max;
+                    if (sd->mc_kd_max == 0) kd0 = 0.0;
+                    if (sd->mc_kz_max == 0) kz0 = 0.0;
+                }
+            } else {
+                double r = sqrt(-2.0 * log(u01_open_left(o[0])));
+                double a = 2 * M_PI * u01_open_right(o[1]);
+                kd0 = sd->kd_sigma * (r * cos(a));
+                w = w / gauss_pdf(kd0, sd->kd_sigma);
+                if (dim == 3) {
+                    kz0 = sd->kz_sigma * (r * sin(a));
+                    w = w / gauss_pdf(kz0, sd->kz_sigma);
+                }
+            }
+        }
+        if (fabs(kd0) < 1e-4) continue;                              /* :312,:321 (Q5) */
+        double kx0 = kd0 * -sin(phi), ky0 = kd0 * cos(phi);           /* :324-325 */
+        double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / Ftr;         /* :326 */
+        double x0 = r0 * cos(phi), y0 = r0 * sin(phi);
+        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * Ftr));  /* ADKAmpExp :198 */
+        if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
+            /* pre/pre_cc take (kx,ky,kz) (:258-265, called at :283-291) */
+            double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
+            amp *= cpre * numer / pow((kx0 * kx0 + ky0 * ky0 + kz0 * kz0 + 2 * Ip) * (Ftr * Ftr), (n + 1) / 4);
+        }
+        if (S->prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(jac);
+        double rate = amp * amp;                                             /* abs2 :331 */
+        if (isnan(rate) || rate < sd->cutoff_limit) continue;
+        if (count >= cap) return -1;
+        double* col = out + count * rows;
+        if (dim == 3) {
+            double v[8] = {x0, y0, 0.0, kx0, ky0, kz0, tr, rate};
+            memcpy(col, v, sizeof v);
+            if (ph != ETRAJ_PHASE_CTMC) col[8] = 0.0;                    /* angle(amp) of a positive real */
+        } else {
+            double v[6] = {x0, y0, kx0, ky0, tr, rate};
+            memcpy(col, v, sizeof v);
+            if (ph != ETRAJ_PHASE_CTMC) col[6] = 0.0;
+        }
+        ++count;
+    }
+    return count;
+}
+
+int oracle_gen_electron_batch(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                              int32_t batch_id, double* out, int64_t capacity, int64_t* count) {
+    sampler_t S;
+    int rc = sampler_init(ld, td, sd, &S);
+    if (rc) { sampler_free(&S); return rc; }
+    if (batch_id < 1 || batch_id > sd->sample_t_num) { sampler_free(&S); return ETRAJ_EINVAL; }
+    int64_t c = adk_gen_batch(&S, batch_id - 1, out, capacity);
+    sampler_free(&S);
+    if (c < 0) return ETRAJ_ECAPACITY;
+    *count = c;
+    return ETRAJ_OK;
+}
+
+/* ========================================================================== */
+/* Integration of a sampled batch                                              */
+/* ========================================================================== */
+typedef struct {
+    laser_t L;
+    target_t T;
+    const etraj_run_desc* rd;
+    eom_t E;
+    int nv, rows;
+} runner_t;
+
+static int runner_init(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_run_desc* rd, int32_t rows, runner_t* R) {
+    if (laser_init(ld, &R->L) || target_init(td, &R->T)) return ETRAJ_EINVAL;
+    if (rd->dimension != 2 && rd->dimension != 3) return ETRAJ_EINVAL;
+    if (rd->phase_method < 0 || rd->phase_method > 2) return ETRAJ_EINVAL;
+    int exp_rows = (rd->dimension == 3) ? (rd->phase_method == ETRAJ_PHASE_CTMC ? 8 : 9) : (rd->phase_method == ETRAJ_PHASE_CTMC ? 6 : 7);
+    if (rows != exp_rows) return ETRAJ_EINVAL;
+    if (rd->integrator == ETRAJ_INTEG_RK4 && !(rd->dt > 0)) return ETRAJ_EINVAL;
+    if (rd->integrator == ETRAJ_INTEG_TSIT5 && !(rd->rtol > 0)) return ETRAJ_EINVAL;
+    R->rd = rd;
+    R->E.L = &R->L; R->E.T = &R->T; R->E.dim = rd->dimension; R->E.phase = rd->phase_method; R->E.quirks = rd->quirks;
+    R->nv = 2 * rd->dimension + (rd->phase_method == ETRAJ_PHASE_CTMC ? 0 : 1);
+    R->rows = rows;
+    return ETRAJ_OK;
+}
+
+/* one electron: TrajectorySimulation.jl:337-342 (remake u0, tspan) + :344-374 */
+static void run_one(const runner_t* R, const double* col, post_t* P, int64_t* nsteps) {
+    const etraj_run_desc* rd = R->rd;
+    double u[7];
+    int dim = rd->dimension;
+    double t0, rate;
+    for (int i = 0; i < 2 * dim; ++i) u[i] = col[i];
+    t0 = col[2 * dim];
+    rate = col[2 * dim + 1];
+    (void)rate;
+    if (rd->phase_method != ETRAJ_PHASE_CTMC) u[2 * dim] = col[2 * dim + 2];
+    int fail;
+    if (rd->integrator == ETRAJ_INTEG_RK4)
+        fail = integrate_rk4(&R->E, R->nv, t0, rd->traj_t_final, rd->dt, u, nsteps);
+    else
+        fail = integrate_tsit5(&R->E, R->nv, t0, rd->traj_t_final, rd->rtol, rd->atol > 0 ? rd->atol : 1e-6,
+                               rd->maxiters > 0 ? rd->maxiters : 1000000, u, nsteps);
+    post_process(&R->T, dim, rd->phase_method, rd->quirks, u, t0, P);
+    if (fail) P->status = 1;
+}
+
+int oracle_integrate_records(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_run_desc* rd,
+                             const double* init, int64_t n, int32_t rows, double* records, int32_t* status) {
+    runner_t R;
+    int rc = runner_init(ld, td, rd, rows, &R);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; ++i) {
+        post_t P;
+        int64_t ns;
+        run_one(&R, init + i * rows, &P, &ns);
+        memcpy(records + 12 * i, P.rec, sizeof P.rec);
+        status[i] = P.status;
+    }
+    return ETRAJ_OK;
+}
+
+/* binning of one electron (TrajectorySimulation.jl:370-396) */
+typedef struct {
+    double* pmd;      /* CTMC: n doubles; coherent: 2n (re, im) */
+    double ion_prob, ion_unc;
+    int64_t failed, bound, eff, steps;
+} acc_t;
+
+static void bin_one(const runner_t* R, const post_t* P, double prob, acc_t* A) {
+    const etraj_run_desc* rd = R->rd;
+    if (P->status == 1) { A->failed++; return; }
+    if (P->status == 2) { A->bound++; return; }
+    A->ion_prob += prob;
+    int dim = rd->dimension;
+    int64_t idx[3] = {1, 1, 1};
+    int inb = 1;
+    for (int d = 0; d < dim; ++d) {
+        idx[d] = oracle_bin_index(P->rec[8 + d], rd->final_p_max[d], rd->final_p_num[d]);
+        if (idx[d] < 1 || idx[d] > rd->final_p_num[d]) inb = 0;
+    }
+    if (!inb) { A->ion_unc += prob; return; }
+    int64_t lin = (idx[0] - 1) + (int64_t)rd->final_p_num[0] * ((idx[1] - 1) + (dim == 3 ? (int64_t)rd->final_p_num[1] * (idx[2] - 1) : 0));
+    if (rd->phase_method == ETRAJ_PHASE_CTMC) {
+        A->pmd[lin] += prob;
+    } else {
+        double s = sqrt(prob);
+        double complex a = s * cexp(I * P->rec[11]);
+        A->pmd[2 * lin] += creal(a);
+        A->pmd[2 * lin + 1] += cimag(a);
+    }
+}
+
+int oracle_integrate_and_bin(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_run_desc* rd,
+                             const double* init, int64_t n, int32_t rows, double* pmd, double* ion_prob,
+                             double* ion_prob_uncollected, int64_t* n_failed) {
+    runner_t R;
+    int rc = runner_init(ld, td, rd, rows, &R);
+    if (rc) return rc;
+    acc_t A;
+    memset(&A, 0, sizeof A);
+    A.pmd = pmd;
+    for (int64_t i = 0; i < n; ++i) {
+        post_t P;
+        int64_t ns;
+        const double* col = init + i * rows;
+        run_one(&R, col, &P, &ns);
+        bin_one(&R, &P, col[2 * rd->dimension + 1], &A);
+    }
+    *ion_prob += A.ion_prob;
+    *ion_prob_uncollected += A.ion_unc;
+    *n_failed = A.failed;
+    return ETRAJ_OK;
+}
+
+/* ========================================================================== */
+/* Whole job: Threads.@threads over batches with thread-private PMDs and one   */
+/* final sum (TrajectorySimulation.jl:154-157, :262-267, :521-540).            */
+/* ========================================================================== */
+typedef struct {
+    const sampler_t* S;
+    const runner_t* R;
+    int b_begin, b_end, b_stride;
+    int tid, nthreads;
+    int64_t pmd_len;
+    acc_t A;
+    int rc;
+} worker_t;
+
+static void* worker_main(void* arg) {
+    worker_t* W = (worker_t*)arg;
+    int64_t cap = oracle_batch_max_size(W->S->sd);
+    int rows = W->R->rows;
+    double* buf = (double*)malloc(sizeof(double) * rows * (cap > 0 ? cap : 1));
+    if (!buf) { W->rc = ETRAJ_ENOMEM; return NULL; }
+    /* static, contiguous partition of the selected batches over threads */
+    int64_t nsel = (W->b_end - W->b_begin + W->b_stride - 1) / W->b_stride;
+    int64_t lo = nsel * W->tid / W->nthreads, hi = nsel * (W->tid + 1) / W->nthreads;
+    for (int64_t s = lo; s < hi; ++s) {
+        int b = W->b_begin + (int)(s * W->b_stride);
+        int64_t c = adk_gen_batch(W->S, b, buf, cap);
+        if (c <= 0) continue;
+        for (int64_t i = 0; i < c; ++i) {
+            post_t P;
+            int64_t ns = 0;
+            const double* col = buf + i * rows;
+            run_one(W->R, col, &P, &ns);
+            W->A.steps += ns;
+            bin_one(W->R, &P, col[2 * W->R->rd->dimension + 1], &W->A);
+        }
+        W->A.eff += c;
+    }
+    free(buf);
+    return NULL;
+}
+
+int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                   const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end, int32_t batch_stride,
+                   double* pmd_out, etraj_stats* st) {
+    sampler_t S;
+    runner_t R;
+    int rc = sampler_init(ld, td, sd, &S);
+    if (rc) { sampler_free(&S); return rc; }
+    if (sd->dimension != rd->dimension || sd->phase_method != rd->phase_method) { sampler_free(&S); return ETRAJ_EINVAL; }
+    int rows = (sd->dimension == 3) ? (sd->phase_method == ETRAJ_PHASE_CTMC ? 8 : 9) : (sd->phase_method == ETRAJ_PHASE_CTMC ? 6 : 7);
+    rc = runner_init(ld, td, rd, rows, &R);
+    if (rc) { sampler_free(&S); return rc; }
+    if (batch_end <= 0 || batch_end > sd->sample_t_num) batch_end = sd->sample_t_num;
+    if (batch_begin < 0) batch_begin = 0;
+    if (batch_stride <= 0) batch_stride = 1;
+    if (nthreads <= 0) nthreads = 1;
+    int64_t npix = (int64_t)rd->final_p_num[0] * rd->final_p_num[1] * (rd->dimension == 3 ? rd->final_p_num[2] : 1);
+    int coh = rd->phase_method != ETRAJ_PHASE_CTMC;
+    int64_t len = coh ? 2 * npix : npix;
+    worker_t* W = (worker_t*)calloc(nthreads, sizeof(worker_t));
+    pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+    if (!W || !th) { free(W); free(th); sampler_free(&S); return ETRAJ_ENOMEM; }
+    for (int i = 0; i < nthreads; ++i) {
+        W[i].S = &S; W[i].R = &R;
+        W[i].b_begin = batch_begin; W[i].b_end = batch_end; W[i].b_stride = batch_stride;
+        W[i].tid = i; W[i].nthreads = nthreads; W[i].pmd_len = len;
+        W[i].A.pmd = (double*)calloc(len, sizeof(double));
+        if (!W[i].A.pmd) rc = ETRAJ_ENOMEM;
+    }
+    if (!rc) {
+        for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, worker_main, &W[i]);
+        for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    }
+    etraj_stats s;
+    memset(&s, 0, sizeof s);
+    if (!rc) {
+        memset(pmd_out, 0, sizeof(double) * npix);
+        double* sum = (double*)calloc(len, sizeof(double));
+        for (int i = 0; i < nthreads; ++i) {
+            if (W[i].rc) rc = W[i].rc;
+            for (int64_t k = 0; k < len; ++k) sum[k] += W[i].A.pmd[k];
+            s.ion_prob += W[i].A.ion_prob;
+            s.ion_prob_uncollected += W[i].A.ion_unc;
+            s.num_effective_traj += W[i].A.eff;
+            s.n_failed += W[i].A.failed;
+            s.n_bound += W[i].A.bound;
+            s.n_steps += W[i].A.steps;
+        }
+        for (int64_t k = 0; k < npix; ++k)
+            pmd_out[k] = coh ? sum[2 * k] * sum[2 * k] + sum[2 * k + 1] * sum[2 * k + 1] : sum[k];
+        free(sum);
+    }
+    for (int i = 0; i < nthreads; ++i) free(W[i].A.pmd);
+    free(W);
+    free(th);
+    sampler_free(&S);
+    if (st) *st = s;
+    return rc;
+}
+
+/* amplitude-level job output (before abs2), for coherent-sum checks */
+int oracle_run_job_amplitude(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                             const etraj_run_desc* rd, int nthreads, double* amp_out, etraj_stats* st) {
+    sampler_t S;
+    runner_t R;
+    int rc = sampler_init(ld, td, sd, &S);
+    if (rc) { sampler_free(&S); return rc; }
+    int rows = (sd->dimension == 3) ? (sd->phase_method == ETRAJ_PHASE_CTMC ? 8 : 9) : (sd->phase_method == ETRAJ_PHASE_CTMC ? 6 : 7);
+    rc = runner_init(ld, td, rd, rows, &R);
+    if (rc) { sampler_free(&S); return rc; }
+    int64_t npix = (int64_t)rd->final_p_num[0] * rd->final_p_num[1] * (rd->dimension == 3 ? rd->final_p_num[2] : 1);
+    int64_t len = rd->phase_method != ETRAJ_PHASE_CTMC ? 2 * npix : npix;
+    worker_t W;
+    memset(&W, 0, sizeof W);
+    W.S = &S; W.R = &R; W.b_begin = 0; W.b_end = sd->sample_t_num; W.b_stride = 1; W.tid = 0; W.nthreads = 1;
+    W.A.pmd = amp_out;
+    memset(amp_out, 0, sizeof(double) * len);
+    (void)nthreads;
+    worker_main(&W);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->ion_prob = W.A.ion_prob; st->ion_prob_uncollected = W.A.ion_unc;
+        st->num_effective_traj = W.A.eff; st->n_failed = W.A.failed; st->n_bound = W.A.bound; st->n_steps = W.A.steps;
+    }
+    sampler_free(&S);
+    return W.rc;
+}
+
+/* single-trajectory integrator access for the convergence / invariance tests:
+ * integrates u (nv values) from t0 to tf, returns steps in *nsteps */
+int oracle_integrate_state(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_run_desc* rd,
+                           double t0, double* u, int64_t* nsteps) {
+    runner_t R;
+    int rows = (rd->dimension == 3) ? (rd->phase_method == ETRAJ_PHASE_CTMC ? 8 : 9) : (rd->phase_method == ETRAJ_PHASE_CTMC ? 6 : 7);
+    int rc = runner_init(ld, td, rd, rows, &R);
+    if (rc) return rc;
+    int fail = (rd->integrator == ETRAJ_INTEG_RK4)
+                   ? integrate_rk4(&R.E, R.nv, t0, rd->traj_t_final, rd->dt, u, nsteps)
+                   : integrate_tsit5(&R.E, R.nv, t0, rd->traj_t_final, rd->rtol, rd->atol > 0 ? rd->atol : 1e-6,
+                                     rd->maxiters > 0 ? rd->maxiters : 1000000, u, nsteps);
+    return fail ? 1 : 0;
+}
+
+/* Kepler / SCTS post-processing of a given final state (for invariance tests) */
+int oracle_post_process(const etraj_target_desc* td, int32_t dim, int32_t phase_method, int32_t quirks,
+                        const double* u, double t0, double* rec, int32_t* status) {
+    target_t T;
+    if (target_init(td, &T)) return ETRAJ_EINVAL;
+    post_t P;
+    post_process(&T, dim, phase_method, quirks, u, t0, &P);
+    memcpy(rec, P.rec, sizeof P.rec);
+    *status = P.status;
+    return ETRAJ_OK;
+}
+
+/* the RHS itself (for direct checks) */
+int oracle_eom_rhs(const etraj_laser_desc* ld, const etraj_target_desc* td, int32_t dim, int32_t phase_method,
+                   int32_t quirks, double t, const double* u, double* du) {
+    laser_t L;
+    target_t T;
+    if (laser_init(ld, &L) || target_init(td, &T)) return ETRAJ_EINVAL;
+    eom_t E = {&L, &T, dim, phase_method, quirks};
+    eom_rhs(&E, t, u, du);
+    return ETRAJ_OK;
+}

@@ -1,0 +1,189 @@
+"""ctypes access to the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+and by bench.py's cpu_baseline leg as the checker / CPU baseline, never by
+the product. Descriptors are the product's C-ABI structs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PKG = os.path.join(ROOT, "etraj.jl_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from etraj_hip import _abi  # noqa: E402  (descriptor structs only)
+from etraj_hip._abi import LaserDesc, TargetDesc, SamplerDesc, RunDesc, Stats  # noqa: E402
+
+LIB = os.path.join(HERE, "liboracle.so")
+
+P = C.POINTER
+DP = P(C.c_double)
+
+
+def build(force: bool = False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.oracle_philox4x64_10.argtypes = [P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]
+        L.oracle_philox4x64_10.restype = None
+        L.oracle_grid_point.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int]
+        L.oracle_grid_point.restype = C.c_double
+        L.oracle_bin_index.argtypes = [C.c_double, C.c_double, C.c_int32]
+        L.oracle_bin_index.restype = C.c_int64
+        L.oracle_tsit5_tableau.argtypes = [DP, DP, DP, DP]
+        L.oracle_tsit5_tableau.restype = None
+        L.oracle_laser_field.argtypes = [P(LaserDesc), DP, C.c_int64, DP, DP, DP, DP]
+        L.oracle_unit_envelope.argtypes = [P(LaserDesc), C.c_double]
+        L.oracle_unit_envelope.restype = C.c_double
+        L.oracle_target_eval.argtypes = [P(TargetDesc), DP, C.c_int64, DP, DP]
+        L.oracle_eom_rhs.argtypes = [P(LaserDesc), P(TargetDesc), C.c_int32, C.c_int32, C.c_int32, C.c_double, DP, DP]
+        L.oracle_batch_times.argtypes = [P(SamplerDesc), DP]
+        L.oracle_batch_max_size.argtypes = [P(SamplerDesc)]
+        L.oracle_batch_max_size.restype = C.c_int64
+        L.oracle_gen_electron_batch.argtypes = [P(LaserDesc), P(TargetDesc), P(SamplerDesc), C.c_int32, DP,
+                                                C.c_int64, P(C.c_int64)]
+        L.oracle_integrate_state.argtypes = [P(LaserDesc), P(TargetDesc), P(RunDesc), C.c_double, DP, P(C.c_int64)]
+        L.oracle_post_process.argtypes = [P(TargetDesc), C.c_int32, C.c_int32, C.c_int32, DP, C.c_double, DP,
+                                          P(C.c_int32)]
+        L.oracle_integrate_records.argtypes = [P(LaserDesc), P(TargetDesc), P(RunDesc), DP, C.c_int64, C.c_int32,
+                                               DP, P(C.c_int32)]
+        L.oracle_integrate_and_bin.argtypes = [P(LaserDesc), P(TargetDesc), P(RunDesc), DP, C.c_int64, C.c_int32,
+                                               DP, DP, DP, P(C.c_int64)]
+        L.oracle_run_job.argtypes = [P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), C.c_int, C.c_int32,
+                                     C.c_int32, C.c_int32, DP, P(Stats)]
+        L.oracle_run_job_amplitude.argtypes = [P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), C.c_int, DP,
+                                               P(Stats)]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(DP)
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}")
+
+
+def philox(ctr, key):
+    c = (C.c_uint64 * 4)(*ctr)
+    k = (C.c_uint64 * 2)(*key)
+    o = (C.c_uint64 * 4)()
+    lib().oracle_philox4x64_10(c, k, o)
+    return [int(x) for x in o]
+
+
+def laser_field(ld, t):
+    t = np.ascontiguousarray(np.atleast_1d(np.asarray(t, float)))
+    out = [np.empty_like(t) for _ in range(4)]
+    _chk(lib().oracle_laser_field(ld, _d(t), t.size, *[_d(o) for o in out]))
+    return out
+
+
+def target_eval(td, xyz):
+    xyz = np.ascontiguousarray(np.asarray(xyz, float).reshape(-1, 3))
+    V = np.empty(len(xyz))
+    F = np.empty((len(xyz), 3))
+    _chk(lib().oracle_target_eval(td, _d(xyz), len(xyz), _d(V), _d(F)))
+    return V, F
+
+
+def eom_rhs(ld, td, dim, phase, quirks, t, u):
+    u = np.ascontiguousarray(u, float)
+    du = np.zeros(7)
+    _chk(lib().oracle_eom_rhs(ld, td, dim, phase, quirks, t, _d(u), _d(du)))
+    return du[:len(u)]
+
+
+def batch_times(sd):
+    t = np.empty(sd.sample_t_num)
+    _chk(lib().oracle_batch_times(sd, _d(t)))
+    return t
+
+
+def rows_of(dim, phase):
+    return (8 if phase == _abi.PHASE_CTMC else 9) if dim == 3 else (6 if phase == _abi.PHASE_CTMC else 7)
+
+
+def gen_electron_batch(ld, td, sd, batch_id):
+    """returns (N, rows) C-order == Julia rows x N, or None"""
+    cap = max(int(lib().oracle_batch_max_size(sd)), 1)
+    rows = rows_of(sd.dimension, sd.phase_method)
+    out = np.empty((cap, rows))
+    n = C.c_int64()
+    _chk(lib().oracle_gen_electron_batch(ld, td, sd, batch_id, _d(out), cap, C.byref(n)))
+    return out[:n.value].copy() if n.value else None
+
+
+def integrate_state(ld, td, rd, t0, u):
+    u = np.ascontiguousarray(u, float).copy()
+    ns = C.c_int64()
+    rc = lib().oracle_integrate_state(ld, td, rd, t0, _d(u), C.byref(ns))
+    return u, ns.value, rc
+
+
+def post_process(td, dim, phase, quirks, u, t0):
+    u = np.ascontiguousarray(u, float)
+    rec = np.empty(12)
+    st = C.c_int32()
+    _chk(lib().oracle_post_process(td, dim, phase, quirks, _d(u), t0, _d(rec), C.byref(st)))
+    return rec, st.value
+
+
+def integrate_records(ld, td, rd, cols):
+    cols = np.ascontiguousarray(cols, float)
+    n, rows = cols.shape
+    rec = np.empty((n, 12))
+    st = np.empty(n, np.int32)
+    _chk(lib().oracle_integrate_records(ld, td, rd, _d(cols), n, rows, _d(rec), st.ctypes.data_as(P(C.c_int32))))
+    return rec, st
+
+
+def integrate_and_bin(ld, td, rd, cols, pmd):
+    cols = np.ascontiguousarray(cols, float)
+    ip, iu, nf = C.c_double(0), C.c_double(0), C.c_int64(0)
+    _chk(lib().oracle_integrate_and_bin(ld, td, rd, _d(cols), cols.shape[0], cols.shape[1], _d(pmd), C.byref(ip),
+                                        C.byref(iu), C.byref(nf)))
+    return ip.value, iu.value, nf.value
+
+
+def run_job(ld, td, sd, rd, nthreads=1, batch_begin=0, batch_end=0, batch_stride=1):
+    npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
+    out = np.empty(npix)
+    st = Stats()
+    _chk(lib().oracle_run_job(ld, td, sd, rd, nthreads, batch_begin, batch_end, batch_stride, _d(out), C.byref(st)))
+    return out, st.as_dict()
+
+
+def run_job_amplitude(ld, td, sd, rd):
+    npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
+    n = npix * (2 if rd.phase_method != _abi.PHASE_CTMC else 1)
+    out = np.empty(n)
+    st = Stats()
+    _chk(lib().oracle_run_job_amplitude(ld, td, sd, rd, 1, _d(out), C.byref(st)))
+    return out, st.as_dict()
+
+
+def tsit5_tableau():
+    c, a, b, bt = np.empty(7), np.empty(49), np.empty(7), np.empty(7)
+    lib().oracle_tsit5_tableau(_d(c), _d(a), _d(b), _d(bt))
+    return c, a.reshape(7, 7), b, bt

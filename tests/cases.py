@@ -1,0 +1,36 @@
+"""Descriptor builders shared by the tests: one keyword set -> the C-ABI
+descriptors that drive both the engine and the oracle."""
+import warnings
+
+import numpy as np
+
+from etraj_hip import _abi, workloads
+from etraj_hip.samplers import ADKSampler
+from etraj_hip.simulation import run_desc
+
+
+def descriptors(kw):
+    kw = dict(kw)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sp = ADKSampler(**{k: v for k, v in kw.items() if k not in ("init_cond_method",)})
+    rd = run_desc(**kw)
+    return kw["laser"]._desc(), kw["target"]._desc(), sp.desc, rd
+
+
+def small_ctmc(n_t=40, n_kd=24, precision="fp64", **over):
+    kw = workloads.h_lp_ctmc_grid(n_t=n_t, n_kd=n_kd, precision=precision)
+    kw.update(over)
+    return kw
+
+
+def small_scts(n_t=24, n_k=96, precision="fp64", p_num=200, **over):
+    kw = workloads.h_lp_scts_stratified(n_t=n_t, n_k=n_k, precision=precision, p_num=p_num)
+    kw.update(over)
+    return kw
+
+
+def l1(p, q):
+    p = np.asarray(p, float).ravel()
+    q = np.asarray(q, float).ravel()
+    return float(np.abs(p - q).sum() / np.abs(q).sum())

@@ -1,0 +1,266 @@
+"""GPU parity: the engine (through the C ABI) against the CPU FP64 oracle.
+
+Tolerances:
+  fields / forces / sampler columns          1e-13 relative (FP64, different libm)
+  FP64 RK4 per-electron asymptotic momentum  median 1e-12, <= 1 % of electrons above 1e-8
+                                             (chaotic rescattering amplifies ulp differences)
+  FP64 PMD                                   relative L1 <= 1e-6
+  mixed FP32/FP64 PMD                        relative L1 <= 1e-3 (the engine's stated bound)
+"""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import descriptors, l1, small_ctmc, small_scts
+from etraj_hip import _abi, Cos2Laser, Cos4Laser, GaussianLaser, BichromaticLaser, get_atom
+from etraj_hip.samplers import ADKSampler
+
+pytestmark = pytest.mark.gpu
+
+
+def _lasers():
+    return [
+        Cos2Laser(peak_int=4e14, wave_len=800.0, cyc_num=2.0, ellip=1.0, azi=math.pi / 4, cep=math.pi, t_shift=10.0),
+        Cos2Laser(peak_int=1e14, wave_len=800.0, cyc_num=2.0, ellip=0.0),
+        Cos4Laser(peak_int=1e15, wave_len=800.0, cyc_num=2, ellip=1.0),
+        Cos4Laser(peak_int=4e14, wave_len=800.0, cyc_num=2.0, ellip=0.5, azi=0.3, cep=1.0, t_shift=-5.0),
+        GaussianLaser(peak_int=4e14, wave_len=800.0, spread_cyc_num=2.0, ellip=1.0, azi=math.pi / 4, cep=math.pi,
+                      t_shift=10.0),
+        BichromaticLaser(Cos2Laser(peak_int=2e14, wave_len=800.0, cyc_num=8, ellip=1.0),
+                         Cos2Laser(peak_int=2e14, wave_len=400.0, cyc_num=16, ellip=-1.0), 12.0),
+    ]
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_laser_field_matches_oracle(ctx, k):
+    l = _lasers()[k]
+    t = np.linspace(-450.0, 450.0, 4001)
+    ref = O.laser_field(l._desc(), t)
+    got = [np.empty_like(t) for _ in range(4)]
+    ctx.check(ctx.lib.etraj_laser_field(ctx.handle, l._desc(), _abi.dptr(t), t.size, *[_abi.dptr(g) for g in got]))
+    for g, r in zip(got, ref):
+        scale = np.abs(r).max()
+        assert np.abs(g - r).max() <= 1e-13 * scale + 1e-300
+
+
+@pytest.mark.parametrize("name", ["H", "He", "Ar", "Xe"])
+def test_target_eval_matches_oracle(ctx, name):
+    t = get_atom(name)
+    rng = np.random.default_rng(1)
+    xyz = rng.normal(size=(500, 3)) * 3
+    xyz[0] = (1.0, 1.0, 1.0)
+    V0, F0 = O.target_eval(t._desc(), xyz)
+    V = np.empty(500)
+    F = np.empty((500, 3))
+    ctx.check(ctx.lib.etraj_target_eval(ctx.handle, t._desc(), _abi.dptr(np.ascontiguousarray(xyz)), 500,
+                                        _abi.dptr(V), _abi.dptr(F)))
+    np.testing.assert_allclose(V, V0, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(F, F0, rtol=1e-12, atol=1e-15)
+
+
+KAT_BASE = dict(dimension=2, sample_t_intv=(-100, 100), sample_t_num=400, sample_cutoff_limit=1e-14,
+                sample_monte_carlo=False, traj_phase_method="CTMC", rate_prefix="Full", ss_kd_max=2.0,
+                ss_kd_num=800, ss_kz_max=2.0, ss_kz_num=0, mc_kt_num=800, mc_kd_max=2.0, mc_kz_max=2.0)
+KATS = [  # ElectronSamplers_test.jl:53-128: size(gen_electron_batch(sp, 200))
+    ({}, (6, 590)), ({"traj_phase_method": "QTMC"}, (7, 590)), ({"traj_phase_method": "SCTS"}, (7, 590)),
+    ({"dimension": 3, "ss_kd_num": 30, "ss_kz_num": 30}, (8, 376)),
+    ({"dimension": 3, "ss_kd_num": 30, "ss_kz_num": 30, "traj_phase_method": "QTMC"}, (9, 376)),
+    ({"dimension": 3, "ss_kd_num": 30, "ss_kz_num": 30, "traj_phase_method": "SCTS"}, (9, 376)),
+    ({"rate_prefix": "Exp"}, (6, 536)), ({"rate_prefix": "Pre"}, (6, 562)), ({"rate_prefix": "PreCC"}, (6, 610)),
+    ({"rate_prefix": "Jac"}, (6, 514)), ({"rate_prefix": {"Pre", "Jac"}}, (6, 540)),
+]
+
+
+def _kat_sampler(over):
+    l = Cos4Laser(peak_int=1e15, wave_len=800.0, cyc_num=2, ellip=1.0)
+    t = get_atom("H")
+    kw = dict(KAT_BASE)
+    kw.update(over)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sp = ADKSampler(laser=l, target=t, **kw)
+    return l, t, sp
+
+
+def _device_batch(ctx, l, t, sp, b):
+    cap = max(int(ctx.lib.etraj_batch_max_size(sp.desc)), 1)
+    out = np.empty((cap, sp.rows))
+    n = _abi.C.c_int64()
+    ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, l._desc(), t._desc(), sp.desc, b, _abi.dptr(out), cap,
+                                               _abi.C.byref(n)))
+    return out[:n.value]
+
+
+@pytest.mark.parametrize("over,expect", KATS)
+def test_sampler_kats_on_device(ctx, over, expect):
+    l, t, sp = _kat_sampler(over)
+    got = _device_batch(ctx, l, t, sp, 200)
+    assert (got.shape[1], got.shape[0]) == expect
+    ref = O.gen_electron_batch(l._desc(), t._desc(), sp.desc, 200)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("over", [{"sample_monte_carlo": True}, {"sample_monte_carlo": True, "dimension": 3},
+                                  {"sample_stratified": True, "kd_sigma": 0.5, "mc_kt_num": 700},
+                                  {"sample_stratified": True, "kd_sigma": 0.5, "kz_sigma": 0.3, "mc_kt_num": 700,
+                                   "dimension": 3}])
+def test_seeded_sampling_matches_oracle(ctx, over):
+    l, t, sp = _kat_sampler(dict(over, seed=1234))
+    tt = np.empty(sp.desc.sample_t_num)
+    ctx.check(ctx.lib.etraj_batch_times(ctx.handle, sp.desc, _abi.dptr(tt)))
+    np.testing.assert_array_equal(tt, O.batch_times(sp.desc))
+    for b in (1, 150, 200, 333):
+        got = _device_batch(ctx, l, t, sp, b)
+        ref = O.gen_electron_batch(l._desc(), t._desc(), sp.desc, b)
+        ref = np.zeros((0, sp.rows)) if ref is None else ref
+        assert got.shape == ref.shape
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-300)
+
+
+def _columns(kw, batches):
+    ld, td, sd, rd = descriptors(kw)
+    cols = [O.gen_electron_batch(ld, td, sd, b) for b in batches]
+    cols = np.concatenate([c for c in cols if c is not None])
+    return ld, td, sd, rd, cols
+
+
+def _device_records(ctx, ld, td, rd, cols):
+    n, rows = cols.shape
+    rec = np.empty((n, 12))
+    st = np.empty(n, np.int32)
+    ctx.check(ctx.lib.etraj_integrate_records(ctx.handle, ld, td, rd, _abi.dptr(np.ascontiguousarray(cols)), n, rows,
+                                              _abi.dptr(rec), st.ctypes.data_as(_abi.C.POINTER(_abi.C.c_int32))))
+    return rec, st
+
+
+def _momentum_errors(rec, st, rrec, rst):
+    both = (st == 0) & (rst == 0)
+    p, q = rec[both, 8:11], rrec[both, 8:11]
+    return np.linalg.norm(p - q, axis=1) / np.maximum(np.linalg.norm(q, axis=1), 1e-300), both
+
+
+@pytest.mark.parametrize("phase", ["CTMC", "QTMC", "SCTS"])
+def test_records_fp64_match_oracle(ctx, phase):
+    kw = small_ctmc(n_t=60, n_kd=40, traj_phase_method=phase)
+    ld, td, sd, rd, cols = _columns(kw, range(8, 60, 4))
+    rec, st = _device_records(ctx, ld, td, rd, cols)
+    rrec, rst = O.integrate_records(ld, td, rd, cols)
+    assert (st != rst).mean() <= 0.01
+    err, both = _momentum_errors(rec, st, rrec, rst)
+    assert np.median(err) < 1e-12
+    assert (err > 1e-8).mean() <= 0.01
+    if phase != "CTMC":
+        dphi = np.angle(np.exp(1j * (rec[both, 11] - rrec[both, 11])))
+        assert np.median(np.abs(dphi)) < 1e-9
+
+
+def test_records_mixed_close_to_oracle(ctx):
+    kw = small_ctmc(n_t=60, n_kd=40, traj_phase_method="SCTS", precision="mixed")
+    ld, td, sd, rd, cols = _columns(kw, range(8, 60, 4))
+    rec, st = _device_records(ctx, ld, td, rd, cols)
+    rd64 = descriptors(dict(kw, precision="fp64"))[3]
+    rrec, rst = O.integrate_records(ld, td, rd64, cols)
+    assert (st != rst).mean() <= 0.02
+    err, both = _momentum_errors(rec, st, rrec, rst)
+    print("mixed: median momentum rel err", np.median(err), "p90", np.quantile(err, 0.9))
+    assert np.median(err) < 1e-5
+    dphi = np.angle(np.exp(1j * (rec[both, 11] - rrec[both, 11])))
+    print("mixed: median |dphi|", np.median(np.abs(dphi)))
+    assert np.median(np.abs(dphi)) < 1e-3
+
+
+@pytest.mark.parametrize("phase", ["CTMC", "SCTS"])
+def test_integrate_and_bin_matches_oracle(ctx, phase):
+    kw = small_ctmc(n_t=60, n_kd=40, traj_phase_method=phase)
+    ld, td, sd, rd, cols = _columns(kw, range(10, 50, 3))
+    coh = phase != "CTMC"
+    npix = 500 * 500
+    pmd = np.zeros(2 * npix if coh else npix)
+    ip, iu, nf = _abi.C.c_double(0.0), _abi.C.c_double(0.0), _abi.C.c_int64(0)
+    ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ld, td, rd, _abi.dptr(np.ascontiguousarray(cols)),
+                                              cols.shape[0], cols.shape[1], pmd.ctypes.data_as(_abi.C.c_void_p),
+                                              _abi.C.byref(ip), _abi.C.byref(iu), _abi.C.byref(nf)))
+    ref = np.zeros_like(pmd)
+    rip, riu, rnf = O.integrate_and_bin(ld, td, rd, cols, ref)
+    assert nf.value == rnf
+    assert ip.value == pytest.approx(rip, rel=1e-10)
+    assert l1(pmd, ref) <= 1e-6
+
+
+def test_integrate_and_bin_empty_and_ragged(ctx):
+    kw = small_ctmc(n_t=60, n_kd=40)
+    ld, td, sd, rd, cols = _columns(kw, [20, 21, 30])
+    pmd = np.zeros(500 * 500)
+    ip, iu, nf = _abi.C.c_double(0.0), _abi.C.c_double(0.0), _abi.C.c_int64(0)
+    # n = 0 is a no-op
+    ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ld, td, rd, _abi.dptr(np.ascontiguousarray(cols)), 0,
+                                              cols.shape[1], pmd.ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
+                                              _abi.C.byref(iu), _abi.C.byref(nf)))
+    assert ip.value == 0.0 and not pmd.any()
+    # interleaved t0 (ragged segments, one electron per wavefront)
+    perm = np.random.default_rng(3).permutation(len(cols))
+    shuffled = np.ascontiguousarray(cols[perm])
+    rec, st = _device_records(ctx, ld, td, rd, shuffled)
+    rrec, rst = O.integrate_records(ld, td, rd, shuffled)
+    err, both = _momentum_errors(rec, st, rrec, rst)
+    assert np.median(err) < 1e-12
+
+
+def test_failed_electrons_are_counted(ctx):
+    kw = small_ctmc(n_t=60, n_kd=40)
+    ld, td, sd, rd, cols = _columns(kw, [25])
+    cols = cols.copy()
+    cols[::7, 2] = np.nan                      # NaN initial momentum -> integrator failure
+    pmd = np.zeros(500 * 500)
+    ip, iu, nf = _abi.C.c_double(0.0), _abi.C.c_double(0.0), _abi.C.c_int64(0)
+    ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ld, td, rd, _abi.dptr(cols), cols.shape[0],
+                                              cols.shape[1], pmd.ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
+                                              _abi.C.byref(iu), _abi.C.byref(nf)))
+    ref = np.zeros_like(pmd)
+    rip, riu, rnf = O.integrate_and_bin(ld, td, rd, cols, ref)
+    assert nf.value == rnf == len(range(0, len(cols), 7))
+    assert l1(pmd, ref) <= 1e-6
+
+
+def _device_job(ctx, ld, td, sd, rd):
+    npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
+    out = np.empty(npix)
+    st = _abi.Stats()
+    ctx.check(ctx.lib.etraj_run_job(ctx.handle, ld, td, sd, rd, _abi.dptr(out), _abi.C.byref(st)))
+    return out, st.as_dict()
+
+
+def test_run_job_ctmc_grid_matches_oracle(ctx):
+    kw = small_ctmc(n_t=100, n_kd=32)
+    ld, td, sd, rd = descriptors(kw)
+    got, st = _device_job(ctx, ld, td, sd, rd)
+    ref, rst = O.run_job(ld, td, sd, rd, nthreads=4)
+    assert st["num_effective_traj"] == rst["num_effective_traj"]
+    assert st["ion_prob"] == pytest.approx(rst["ion_prob"], rel=1e-9)
+    assert l1(got, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("mixed", 1e-3)])
+def test_run_job_scts_stratified_matches_oracle(ctx, precision, tol):
+    kw = small_scts(n_t=40, n_k=256, precision=precision)
+    ld, td, sd, rd = descriptors(kw)
+    got, st = _device_job(ctx, ld, td, sd, rd)
+    rd64 = descriptors(dict(kw, precision="fp64"))[3]
+    ref, rst = O.run_job(ld, td, sd, rd64, nthreads=4)
+    assert st["num_effective_traj"] == rst["num_effective_traj"]
+    err = l1(got, ref)
+    print(precision, "PMD L1 vs oracle", err, "restep", st["n_restep"])
+    assert err <= tol
+
+
+def test_run_job_3d_qtmc(ctx):
+    kw = small_ctmc(n_t=30, n_kd=12, dimension=3, ss_kz_num=9, ss_kz_max=0.6, final_p_max=(1.5, 1.5, 1.0),
+                    final_p_num=(60, 60, 20), traj_phase_method="QTMC")
+    ld, td, sd, rd = descriptors(kw)
+    got, st = _device_job(ctx, ld, td, sd, rd)
+    ref, rst = O.run_job(ld, td, sd, rd, nthreads=4)
+    assert st["num_effective_traj"] == rst["num_effective_traj"]
+    assert l1(got, ref) <= 1e-6

@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the trajectory engine on one GPU.
+
+Workload: H atom (Ip 0.5, Z 1), ADK (:Full prefix), Cos2 800 nm 1e14 W/cm^2 2-cycle
+linearly polarized pulse, SCTS phase; 1e8 electrons = 1e4 t0 strata (one seeded
+jitter each) x 1e4 Gaussian k_perp (sigma 0.4 a.u.) per stratum; fixed-step RK4
+dt = 0.1 a.u. to t_f = 120.3 a.u.; coherent 1000 x 1000 PMD over +-1.5 a.u.
+One step = one whole perform_traj_simulation pass (device sampler -> RK4 ->
+SCTS phase -> Kepler -> binning -> |amplitude|^2), output left in HBM.
+
+Also measured (outside the timed steps): the same job in all-FP64, the PMD
+relative L1 between the mixed and FP64 results, and, on a bounded sample of the
+same workload, the CPU FP64 oracle (same RK4 algorithm, and Tsit5 as the
+reference's algorithm) timed on the host cores with the GPU's L1 against it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "etraj.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+from etraj_hip import _abi, workloads  # noqa: E402
+from etraj_hip.samplers import ADKSampler  # noqa: E402
+from etraj_hip.simulation import run_desc  # noqa: E402
+
+# Algorithmic FLOPs of one RK4 step of the 2D SCTS hydrogen-like kernel (add/mul = 1,
+# FMA = 2, rsqrt = 1): 4 RHS x 20 + stage states 24 + combination 20 + phase 5 = 129,
+# plus the state update (mixed: 4 x Kahan 4 + FP64 Kahan phase 4 = 20; FP64: 5) and the
+# shared laser field (one evaluation of ~26 per lane per 32 steps ~ 1).
+FLOPS_PER_STEP = {"mixed": 150.0, "fp64": 135.0}
+PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}   # MI355X vector FP32 / FP64 (published)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def descs(kw):
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sp = ADKSampler(**{k: v for k, v in kw.items() if k != "init_cond_method"})
+    return kw["laser"]._desc(), kw["target"]._desc(), sp.desc, run_desc(**kw)
+
+
+def run_device(ctx, d):
+    ptr = _abi.C.POINTER(_abi.C.c_double)()
+    st = _abi.Stats()
+    ctx.check(ctx.lib.etraj_run_job_device(ctx.handle, *d, _abi.C.byref(ptr), _abi.C.byref(st)))
+    return st
+
+
+def run_host(ctx, d, npix):
+    out = np.empty(npix)
+    st = _abi.Stats()
+    ctx.check(ctx.lib.etraj_run_job(ctx.handle, *d, _abi.dptr(out), _abi.C.byref(st)))
+    return out, st
+
+
+def l1(p, q):
+    return float(np.abs(p - q).sum() / np.abs(q).sum())
+
+
+def cpu_baseline(kw, ctx, n_batches_sample, threads):
+    """Oracle (FP64 RK4, same algorithm; and Tsit5 rtol 1e-6, the reference's algorithm) on
+    every (n_t / n_batches_sample)-th stratum; the GPU runs the same strata through
+    gen_electron_batch + integrate_and_bin for the L1 comparison."""
+    import oracle as O
+    ld, td, sd, rd = descs(dict(kw, precision="fp64"))
+    stride = max(1, sd.sample_t_num // n_batches_sample)
+    t = time.perf_counter()
+    ref, st = O.run_job(ld, td, sd, rd, nthreads=threads, batch_stride=stride)
+    dt_rk4 = time.perf_counter() - t
+    n = st["num_effective_traj"]
+    # reference algorithm on a tenth of that sample
+    ldt, tdt, sdt, rdt = descs(dict(kw, precision="fp64", traj_integrator="tsit5"))
+    t = time.perf_counter()
+    _, stt = O.run_job(ldt, tdt, sdt, rdt, nthreads=threads, batch_stride=stride * 10)
+    dt_ts = time.perf_counter() - t
+    # the GPU (mixed) on the same strata
+    ldm, tdm, sdm, rdm = descs(kw)
+    npix = rdm.final_p_num[0] * rdm.final_p_num[1]
+    amp = np.zeros(2 * npix)
+    ip, iu, nf = _abi.C.c_double(0), _abi.C.c_double(0), _abi.C.c_int64(0)
+    cap = int(ctx.lib.etraj_batch_max_size(sdm))
+    buf = np.empty((cap, 7))
+    cnt = _abi.C.c_int64()
+    for b in range(0, sdm.sample_t_num, stride):
+        ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, ldm, tdm, sdm, b + 1, _abi.dptr(buf), cap,
+                                                   _abi.C.byref(cnt)))
+        if cnt.value:
+            ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ldm, tdm, rdm, _abi.dptr(buf), cnt.value, 7,
+                                                      amp.ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
+                                                      _abi.C.byref(iu), _abi.C.byref(nf)))
+    gpu = amp[0::2] ** 2 + amp[1::2] ** 2
+    return {
+        "value": n / dt_rk4, "unit": "trajectories/s", "cores": threads, "kind": "port",
+        "sample": f"{n} electrons = every {stride}-th of the {sd.sample_t_num} t0 strata, FP64 RK4 dt 0.1 "
+                  f"(same algorithm as the GPU), {dt_rk4:.1f} s",
+        "reference_algorithm_value": stt["num_effective_traj"] / dt_ts,
+        "reference_algorithm_sample": f"{stt['num_effective_traj']} electrons, Tsit5 rtol 1e-6 (the reference's "
+                                      f"integrator), {dt_ts:.1f} s",
+        "gpu_mixed_pmd_l1_vs_cpu_fp64": l1(gpu, ref),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-t", type=int, default=10000)
+    ap.add_argument("--n-k", type=int, default=10000)
+    ap.add_argument("--cpu-batches", type=int, default=8, help="strata in the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
+              "t0 strata with one PMD reduction, which this build does not run across GPUs.", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = _abi.default_context()
+    kw = workloads.h_lp_scts_stratified(n_t=args.n_t, n_k=args.n_k, precision="mixed")
+    d_mixed = descs(kw)
+    npix = kw["final_p_num"][0] * kw["final_p_num"][1]
+
+    for i in range(args.warmup):
+        st = run_device(ctx, d_mixed)
+        log(f"warmup {i}: {st.ms_total:.1f} ms, {st.num_effective_traj} electrons")
+    ms, ms_int, steps, restep, n_el = [], [], 0, 0, 0
+    hip = ctx.lib
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = run_device(ctx, d_mixed)
+        ms.append(st.ms_total)
+        ms_int.append(st.ms_integrate)
+        steps += st.n_steps
+        restep += st.n_restep
+        n_el = st.num_effective_traj
+        log(f"step {i}: {st.ms_total:.1f} ms (integrate kernel {st.ms_integrate:.1f} ms), restep {st.n_restep}")
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    value = n_el / (ms_step / 1e3)
+
+    n_launch = max(1, -(-kw["sample_t_num"] // max(1, (1 << 25) // kw["mc_kt_num"])))
+    flops = FLOPS_PER_STEP["mixed"] * steps / args.steps
+    achieved = flops / (np.mean(ms_int) / 1e3) / 1e12
+    result = {
+        "metric": "trajectories/s + PMD wall time, mixed-prec vs FP64 GPU at PMD L1<=1e-3",
+        "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic (seeded Philox sampling of the ADK distribution)",
+        "config": {"workload": "H-atom ADK+SCTS, 800 nm 1e14 W/cm^2 2-cycle LP cos2, 1e8 electrons "
+                               f"({kw['sample_t_num']} t0 strata x {kw['mc_kt_num']} Gaussian k_perp), RK4 dt 0.1, "
+                               "coherent PMD 1000x1000",
+                   "electrons": n_el, "precision": "mixed FP32 EOM / FP64 phase, Kepler, re-step r<2"},
+        "pmd_wall_ms": ms_step,
+        "integrate_kernel_ms": float(np.mean(ms_int)),
+        "restep_fraction_of_steps": restep / max(steps, 1),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS["mixed"], "traffic": None,
+                     "flops_per_step": FLOPS_PER_STEP["mixed"], "rk4_steps_per_job": steps // args.steps,
+                     "launches_per_job": n_launch},
+    }
+    if not args.no_fp64:
+        d64 = descs(dict(kw, precision="fp64"))
+        p64, s64 = run_host(ctx, d64, npix)
+        pmx, smx = run_host(ctx, d_mixed, npix)
+        result["fp64_gpu"] = {"value": s64.num_effective_traj / (s64.ms_total / 1e3), "pmd_wall_ms": s64.ms_total,
+                              "integrate_kernel_ms": s64.ms_integrate,
+                              "achieved_tflops": FLOPS_PER_STEP["fp64"] * s64.n_steps / (s64.ms_integrate / 1e3) / 1e12}
+        result["mixed_over_fp64"] = (smx.num_effective_traj / smx.ms_total) / (s64.num_effective_traj / s64.ms_total)
+        result["pmd_l1_mixed_vs_fp64_gpu"] = l1(pmx, p64)
+        result["ion_prob"] = {"mixed": smx.ion_prob, "fp64": s64.ion_prob}
+        log(f"fp64: {s64.ms_total:.1f} ms; L1(mixed, fp64) = {result['pmd_l1_mixed_vs_fp64_gpu']:.3e}")
+    if args.cpu_batches > 0:
+        threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+        result["cpu_baseline"] = cpu_baseline(kw, ctx, args.cpu_batches, threads)
+        log("cpu baseline:", result["cpu_baseline"]["sample"])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -296,7 +296,7 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
 
 // 64 electrons of one segment (equal t0) per wavefront. Segment s covers
 // columns [seg_start[s], seg_start[s] + seg_count[s]).
-template <int DIM, int PHASE, bool MIXED, int OUT>
+template <int DIM, int PHASE, bool MIXED, int OUT, int TGT>
 __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
                                              const int64_t* __restrict__ seg_start,
                                              const int32_t* __restrict__ seg_count, int64_t n_seg,
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
                 const double F1x = readlane_f64(fx, 2 * s + 1), F1y = readlane_f64(fy, 2 * s + 1);
                 State<DIM, double> d;
                 double r2m;
-                double dph = rk4_incr<DIM, PHASE, double>(P.T, saeq, u, h, Fpx, Fpy, Fmx, Fmy, F1x, F1y, d, r2m);
+                double dph = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, u, h, Fpx, Fpy, Fmx, Fmy, F1x, F1y, d, r2m);
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) {
                     u.r[i] = u.r[i] + d.r[i];
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
                 const float F1x = readlane_f32(fxf, 2 * s + 1), F1y = readlane_f32(fyf, 2 * s + 1);
                 State<DIM, float> d;
                 float r2m;
-                float dph = rk4_incr<DIM, PHASE, float>(P.T, saeq, us, hf, Fpxf, Fpyf, Fmx, Fmy, F1x, F1y, d, r2m);
+                float dph = rk4_incr<DIM, PHASE, TGT, float>(P.T, saeq, us, hf, Fpxf, Fpyf, Fmx, Fmy, F1x, F1y, d, r2m);
                 double dphd;
                 if (r2m < rc2) {
                     // close encounter: redo this step in FP64 from s - c
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
                         ud.p[i] = (double)us.p[i] - (double)uc.p[i];
                     }
                     double r2d;
-                    dphd = rk4_incr<DIM, PHASE, double>(P.T, saeq, ud, h, D0x, D0y, Dmx, Dmy, D1x, D1y, dd, r2d);
+                    dphd = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, ud, h, D0x, D0y, Dmx, Dmy, D1x, D1y, dd, r2d);
 #pragma unroll
                     for (int i = 0; i < DIM; ++i) {
                         double nr = ud.r[i] + dd.r[i], np = ud.p[i] + dd.p[i];
@@ -489,8 +489,12 @@ static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, c
                                const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps) {
     int64_t waves = n_seg * (int64_t)cps;
     unsigned blocks = (unsigned)((waves + 3) / 4);
-    hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT>), dim3(blocks), dim3(256), 0, st, P, O, init, seg_start,
-                       seg_count, n_seg, cps);
+    if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st,
+                           P, O, init, seg_start, seg_count, n_seg, cps);
+    else
+        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O,
+                           init, seg_start, seg_count, n_seg, cps);
     return hipGetLastError();
 }
 

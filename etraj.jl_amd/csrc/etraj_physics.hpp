@@ -223,21 +223,23 @@ __host__ __device__ inline double rsqrt_t<double>(double x) {
     return 1.0 / sqrt(x);
 #endif
 }
+// FP32: the hardware reciprocal square root (v_rsq_f32, 1 ulp); arguments are
+// r^2 + soft_core >= 1e-10, far from the denormal range.
 template <>
 __host__ __device__ inline float rsqrt_t<float>(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return rsqrtf(x);
+    return __builtin_amdgcn_rsqf(x);
 #else
     return 1.0f / sqrtf(x);
 #endif
 }
 
 // Potential V(r) and force scale s with F_target = s * r_vec, evaluated in
-// real type R. `sae_quirk_r2` reproduces SAEAtom.jl:136 (2D SCTS divides
-// by r^2+a instead of sqrt(r^2+a)) for the potential used in the phase.
-template <typename R>
-__host__ __device__ inline void target_vs(const TargetP& T, R r2n, bool want_v, bool sae_quirk_r2, R& V, R& s) {
-    if (T.kind == ETRAJ_TARGET_HYDROGEN_LIKE) {
+// real type R, for target kind TGT. `sae_quirk_r2` reproduces SAEAtom.jl:136
+// (2D SCTS divides by r^2+a instead of sqrt(r^2+a)) for the phase potential.
+template <int TGT, typename R>
+__host__ __device__ inline void target_vs_k(const TargetP& T, R r2n, bool want_v, bool sae_quirk_r2, R& V, R& s) {
+    if (TGT == ETRAJ_TARGET_HYDROGEN_LIKE) {
         R ir = rsqrt_t<R>(r2n + (R)T.sc);
         R zr = (R)T.Zd * ir;
         s = -(zr * ir * ir);
@@ -258,6 +260,14 @@ __host__ __device__ inline void target_vs(const TargetP& T, R r2n, bool want_v, 
     }
 }
 
+template <typename R>
+__host__ __device__ inline void target_vs(const TargetP& T, R r2n, bool want_v, bool sae_quirk_r2, R& V, R& s) {
+    if (T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        target_vs_k<ETRAJ_TARGET_HYDROGEN_LIKE, R>(T, r2n, want_v, sae_quirk_r2, V, s);
+    else
+        target_vs_k<ETRAJ_TARGET_SAE, R>(T, r2n, want_v, sae_quirk_r2, V, s);
+}
+
 // ---------------------------------------------------------------------------
 // Equations of motion (TrajectoryFunction, HydrogenLikeAtom.jl:78-159,
 // SAEAtom.jl:98-186): r' = p, p' = F_target(r) - F_laser(t) (z: target
@@ -269,14 +279,14 @@ struct State {
     R p[DIM];
 };
 
-template <int DIM, int PHASE, typename R>
+template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R Fx, R Fy,
                                     State<DIM, R>& du, R& dphi, R& r2out) {
     R r2n = u.r[0] * u.r[0] + u.r[1] * u.r[1];
     if (DIM == 3) r2n += u.r[DIM - 1] * u.r[DIM - 1];
     r2out = r2n;
     R V, s;
-    target_vs<R>(T, r2n, PHASE != ETRAJ_PHASE_CTMC, sae_quirk, V, s);
+    target_vs_k<TGT, R>(T, r2n, PHASE != ETRAJ_PHASE_CTMC, sae_quirk, V, s);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) du.r[i] = u.p[i];
     du.p[0] = s * u.r[0] - Fx;
@@ -298,7 +308,7 @@ __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const Stat
 // Fm at t+h/2, F1 at t+h). Returns the state increment d, so that
 // u_next = u + d with d = h/6 (k1 + 2 (k2 + k3) + k4), the phase increment
 // and the minimum |r|^2 over the four stages.
-template <int DIM, int PHASE, typename R>
+template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R h,
                                       R F0x, R F0y, R Fmx, R Fmy, R F1x, R F1y,
                                       State<DIM, R>& d, R& r2min) {
@@ -306,7 +316,7 @@ __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const St
     const R h6 = h / (R)6;
     State<DIM, R> k, y, k2;
     R ph, ph2, phacc, r2;
-    eom<DIM, PHASE, R>(T, sae_quirk, u, F0x, F0y, k, ph, r2);
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, u, F0x, F0y, k, ph, r2);
     r2min = r2;
     phacc = ph;
 #pragma unroll
@@ -316,14 +326,14 @@ __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const St
         y.r[i] = u.r[i] + hh * k.r[i];
         y.p[i] = u.p[i] + hh * k.p[i];
     }
-    eom<DIM, PHASE, R>(T, sae_quirk, y, Fmx, Fmy, k2, ph2, r2);
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k2, ph2, r2);
     r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         y.r[i] = u.r[i] + hh * k2.r[i];
         y.p[i] = u.p[i] + hh * k2.p[i];
     }
-    eom<DIM, PHASE, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
     r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
@@ -333,7 +343,7 @@ __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const St
         y.p[i] = u.p[i] + h * k.p[i];
     }
     phacc += (R)2 * (ph2 + ph);
-    eom<DIM, PHASE, R>(T, sae_quirk, y, F1x, F1y, k, ph, r2);
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, F1x, F1y, k, ph, r2);
     r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {

@@ -1,0 +1,163 @@
+#=
+eTrajHIP.jl -- Julia binding of libetraj_hip.so for eTraj.jl.
+
+Adds a `backend = :hip` path to `perform_traj_simulation` (eTraj.jl
+src/TrajectorySimulation.jl:103-161) without touching the Lasers / Targets /
+ElectronSamplers types: the Julia objects are flattened into the POD
+descriptors of include/etraj.h and the device does the work.
+
+  * fine path   -- `launch_and_collect_hip!(job, batch_id)` replaces the body of
+                   `launch_and_collect_2D!/3D!` (TrajectorySimulation.jl:308-500):
+                   the Julia sampler still produces the batch, the device integrates
+                   and bins it into the thread's `spec_collect` slot.
+  * coarse path -- `run_job_hip(; kwargs...)` replaces the whole `@threads` loop
+                   and the sum of `write_output!` (:154-157, :521).
+
+Usage (in eTraj's module, after `include("TrajectorySimulation.jl")`):
+    include("eTrajHIP.jl"); using .eTrajHIP
+=#
+module eTrajHIP
+
+using ..Lasers
+using ..Targets
+
+const libetraj = get(ENV, "ETRAJ_HIP_LIB", joinpath(@__DIR__, "..", "etraj_hip", "libetraj_hip.so"))
+
+# ---- include/etraj.h descriptors (field order and types must match) ----------
+struct MonoLaser
+    kind::Int32; reserved::Int32
+    peak_int::Float64; wave_len::Float64; cyc_num::Float64
+    ellip::Float64; azi::Float64; cep::Float64; t_shift::Float64
+end
+struct LaserDesc
+    n_colors::Int32; reserved::Int32; delay21::Float64
+    color1::MonoLaser; color2::MonoLaser
+end
+struct TargetDesc
+    kind::Int32; nucl_charge::Int32; l::Int32; m::Int32
+    Ip::Float64; asymp_coeff::Float64; quan_ax_theta::Float64; quan_ax_phi::Float64; soft_core::Float64
+    a1::Float64; b1::Float64; a2::Float64; b2::Float64; a3::Float64; b3::Float64
+end
+struct SamplerDesc
+    init_method::Int32; dimension::Int32; phase_method::Int32; rate_prefix::Int32
+    sampling::Int32; sample_t_num::Int32; ss_kd_num::Int32; ss_kz_num::Int32
+    mc_kt_num::Int64
+    sample_t_lo::Float64; sample_t_hi::Float64; ss_kd_max::Float64; ss_kz_max::Float64
+    mc_kd_max::Float64; mc_kz_max::Float64; kd_sigma::Float64; kz_sigma::Float64
+    cutoff_limit::Float64; seed::UInt64
+end
+struct RunDesc
+    dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
+    traj_t_final::Float64; dt::Float64; rtol::Float64; atol::Float64; restep_radius::Float64
+    final_p_max::NTuple{3,Float64}; final_p_num::NTuple{3,Int32}; quirks::Int32; maxiters::Int64
+end
+mutable struct Stats
+    ion_prob::Float64; ion_prob_uncollected::Float64
+    num_effective_traj::Int64; n_failed::Int64; n_bound::Int64; n_restep::Int64; n_steps::Int64
+    ms_sample::Float64; ms_integrate::Float64; ms_total::Float64
+    Stats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+end
+
+const PHASE = Dict(:CTMC => Int32(0), :QTMC => Int32(1), :SCTS => Int32(2))
+
+mono(l::Cos2Laser) = MonoLaser(1, 0, PeakInt(l), WaveLen(l), CycNum(l), Ellipticity(l), Azimuth(l), CEP(l), TimeShift(l))
+mono(l::Cos4Laser) = MonoLaser(2, 0, PeakInt(l), WaveLen(l), CycNum(l), Ellipticity(l), Azimuth(l), CEP(l), TimeShift(l))
+mono(l::GaussianLaser) = MonoLaser(3, 0, PeakInt(l), WaveLen(l), SpreadCycNum(l), Ellipticity(l), Azimuth(l), CEP(l), TimeShift(l))
+const NOLASER = MonoLaser(0, 0, 0, 0, 0, 0, 0, 0, 0)
+laser_desc(l::MonochromaticLaser) = LaserDesc(1, 0, 0.0, mono(l), NOLASER)
+laser_desc(l::BichromaticLaser) = LaserDesc(2, 0, Delay21(l), mono(Laser1(l)), mono(Laser2(l)))
+
+target_desc(t::HydrogenLikeAtom) = TargetDesc(1, AsympNuclCharge(t), AngularQuantumNumber(t), MagneticQuantumNumber(t),
+    IonPotential(t), AsympCoeff(t), QuantizationAxisOrientaion(t)..., SoftCore(t), 0, 0, 0, 0, 0, 0)
+target_desc(t::SAEAtom) = TargetDesc(2, AsympNuclCharge(t), AngularQuantumNumber(t), MagneticQuantumNumber(t),
+    IonPotential(t), AsympCoeff(t), QuantizationAxisOrientaion(t)..., SoftCore(t), t.a1, t.b1, t.a2, t.b2, t.a3, t.b3)
+
+function run_desc(dimension, phase, traj_t_final, final_p_max, final_p_num; dt=0.1, precision=:mixed,
+                  restep_radius=2.0, reproduce_quirks=true)
+    pmax = ntuple(i -> i <= dimension ? Float64(final_p_max[i]) : 1.0, 3)
+    pnum = ntuple(i -> i <= dimension ? Int32(final_p_num[i]) : Int32(1), 3)
+    RunDesc(dimension, PHASE[phase], 0, precision == :mixed ? 1 : 0, traj_t_final, dt, 1e-6, 1e-6, restep_radius,
+            pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000)
+end
+
+# ---- context ------------------------------------------------------------------
+mutable struct Ctx
+    h::Ptr{Cvoid}
+end
+function Ctx(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:etraj_ctx_create, libetraj), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, r)
+    rc == 0 || error("[eTrajHIP] etraj_ctx_create failed ($rc): no usable HIP device")
+    c = Ctx(r[])
+    finalizer(x -> ccall((:etraj_ctx_destroy, libetraj), Cvoid, (Ptr{Cvoid},), x.h), c)
+    return c
+end
+function check(c::Ctx, rc::Cint)
+    rc == 0 && return
+    msg = unsafe_string(ccall((:etraj_last_error, libetraj), Cstring, (Ptr{Cvoid},), c.h))
+    rc == -1 ? throw(AssertionError(msg)) : error("[eTrajHIP] $msg (code $rc)")
+end
+
+# One ctx per Julia thread: a ctx is single-threaded (include/etraj.h).
+const CTXS = Dict{Int,Ctx}()
+const CTX_LOCK = ReentrantLock()
+thread_ctx() = lock(CTX_LOCK) do
+    get!(() -> Ctx(0), CTXS, Threads.threadid())
+end
+
+# ---- fine path: drop-in for the per-batch body of launch_and_collect! ----------
+function launch_and_collect_hip!(job, batch_id::Integer; dt=0.1, precision=:mixed, restep_radius=2.0)
+    (job.batch_completion[batch_id] == 1) && return
+    init = gen_electron_batch(job.sp, batch_id)                         # the reference sampler, unchanged
+    isnothing(init) && (job.batch_completion[batch_id] = 1; return)
+    tid = Threads.threadid()
+    rd = Ref(run_desc(job.dimension, job.traj_phase_method, job.traj_t_final, job.final_p_max, job.final_p_num;
+                      dt, precision, restep_radius))
+    ld = Ref(laser_desc(job.laser)); td = Ref(target_desc(job.target))
+    ip = Ref(0.0); iu = Ref(0.0); nf = Ref(Int64(0))
+    spec = job.spec_collect[tid]                                        # Float64 or ComplexF64 array, px fastest
+    c = thread_ctx()
+    GC.@preserve init spec begin
+        rc = ccall((:etraj_integrate_and_bin, libetraj), Cint,
+                   (Ptr{Cvoid}, Ref{LaserDesc}, Ref{TargetDesc}, Ref{RunDesc}, Ptr{Float64}, Int64, Int32,
+                    Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ref{Int64}),
+                   c.h, ld, td, rd, init, size(init, 2), size(init, 1), spec, ip, iu, nf)
+        check(c, rc)
+    end
+    job.classical_prob[tid] += ip[]
+    job.classical_prob_uncollected[tid] += iu[]
+    nf[] > 0 && @warn "[launch_and_collect_hip!] $(nf[]) electrons failed in batch $batch_id."
+    job.batch_completion[batch_id] = 1
+    job.eff_trajs[batch_id] += size(init, 2)
+    return
+end
+
+# ---- coarse path: the whole loop on the device ----------------------------------
+prefix_bits(p::Symbol) = p == :Exp ? 0 : p == :Full ? 6 : p == :Pre ? 1 : p == :PreCC ? 2 : p == :Jac ? 4 :
+                         error("[ADKSampler] Undefined tunneling rate prefix [$p].")
+prefix_bits(p) = isempty(p) ? 0 : reduce(|, prefix_bits.(collect(p)))
+
+function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max, final_p_num,
+                     ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC,
+                     sample_cutoff_limit=1e-16, rate_prefix=:Full, dt=0.1, precision=:mixed, restep_radius=2.0,
+                     device=0)
+    sd = Ref(SamplerDesc(1, dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0, sample_t_num,
+                         ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
+                         ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0))
+    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
+    spec = zeros(Float64, final_p_num...)
+    st = Stats()
+    c = Ctx(device)
+    GC.@preserve spec st begin
+        rc = ccall((:etraj_run_job, libetraj), Cint,
+                   (Ptr{Cvoid}, Ref{LaserDesc}, Ref{TargetDesc}, Ref{SamplerDesc}, Ref{RunDesc}, Ptr{Float64}, Ref{Stats}),
+                   c.h, Ref(laser_desc(laser)), Ref(target_desc(target)), sd, rd, spec, st)
+        check(c, rc)
+    end
+    return (momentum_spec = spec, ion_prob = st.ion_prob, ion_prob_uncollected = st.ion_prob_uncollected,
+            num_effective_traj = st.num_effective_traj)
+end
+
+export launch_and_collect_hip!, run_job_hip
+
+end # module

@@ -55,7 +55,9 @@ struct etraj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s;
+    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0;
+    DevBuf<float4> ft32;
+    DevBuf<double2> ft64, ft0;
     DevBuf<int64_t> seg_start, cnt;
     DevBuf<int32_t> counts, status;
     DevBuf<Contrib> contrib;
@@ -253,6 +255,28 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     return ETRAJ_OK;
 }
 
+// RK4 steps of a segment starting at t0 (identical expression on the device)
+int64_t steps_of(double tf, double t0, double dt) {
+    int64_t n = (int64_t)std::ceil((tf - t0) / dt);
+    return n < 1 ? 1 : n;
+}
+
+constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
+constexpr int64_t kFieldTabBudget = (int64_t)1 << 30;  // bytes of field table per launch
+
+// Allocates and fills the field table of nseg segments whose t0 are in t0s_dev.
+int fill_field_table(etraj_ctx* c, const IntegP& P, const double* t0s_dev, int64_t nseg, int64_t stride, FieldTab& FT) {
+    HIP_TRY(c, c->ft32.ensure((size_t)nseg * stride));
+    HIP_TRY(c, c->ft64.ensure((size_t)2 * nseg * stride));
+    HIP_TRY(c, c->ft0.ensure((size_t)nseg));
+    FT.f32 = c->ft32.p;
+    FT.f64 = c->ft64.p;
+    FT.f0 = c->ft0.p;
+    FT.stride = stride;
+    HIP_TRY(c, launch_field_table(c->stream, P.L, t0s_dev, nseg, P.tf, P.dt, FT));
+    return ETRAJ_OK;
+}
+
 int64_t npix_of(const IntegP& P) { return (int64_t)P.pnum[0] * P.pnum[1] * (P.dim == 3 ? P.pnum[2] : 1); }
 
 // Segments of equal t0 in a Julia init matrix (columns of one
@@ -287,19 +311,38 @@ int check_init(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int
     return ETRAJ_OK;
 }
 
-// integrate n columns; out = OUT_RECORDS or OUT_CONTRIB
+// integrate n columns; out = OUT_RECORDS or OUT_CONTRIB. Segments are
+// processed in groups whose field tables fit kFieldTabBudget.
 int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int out, OutP& O) {
     std::vector<int64_t> ss;
     std::vector<int32_t> sc;
-    int cps = 0;
-    build_segments(init, n, P.rows, P.dim, ss, sc, cps);
+    int cps_all = 0;
+    build_segments(init, n, P.rows, P.dim, ss, sc, cps_all);
+    const int64_t nseg = (int64_t)ss.size();
+    std::vector<double> t0(nseg);
+    int64_t stride = 1;
+    for (int64_t k = 0; k < nseg; ++k) {
+        t0[k] = init[ss[k] * P.rows + 2 * P.dim];
+        stride = std::max(stride, steps_of(P.tf, t0[k], P.dt));
+    }
+    const int64_t group = std::max<int64_t>(1, std::min<int64_t>(nseg, kFieldTabBudget / (stride * kFieldTabBytesPerStep)));
     HIP_TRY(c, c->init.ensure((size_t)n * P.rows));
-    HIP_TRY(c, c->seg_start.ensure(ss.size()));
-    HIP_TRY(c, c->counts.ensure(sc.size()));
+    HIP_TRY(c, c->seg_start.ensure(nseg));
+    HIP_TRY(c, c->counts.ensure(nseg));
+    HIP_TRY(c, c->seg_t0.ensure(nseg));
     HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * ss.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, c->seg_start.p, c->counts.p, (int64_t)ss.size(), cps));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_t0.p, t0.data(), sizeof(double) * nseg, hipMemcpyHostToDevice, c->stream));
+    for (int64_t g0 = 0; g0 < nseg; g0 += group) {
+        const int64_t ng = std::min(group, nseg - g0);
+        int32_t maxc = 0;
+        for (int64_t k = g0; k < g0 + ng; ++k) maxc = std::max(maxc, sc[k]);
+        FieldTab FT{};
+        if (int rc = fill_field_table(c, P, c->seg_t0.p + g0, ng, stride, FT)) return rc;
+        HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, c->seg_start.p + g0, c->counts.p + g0, ng,
+                              (int)((maxc + 63) / 64), FT));
+    }
     return ETRAJ_OK;
 }
 
@@ -333,6 +376,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->init.release(); c->pmd.release(); c->pmd_out.release(); c->io1.release(); c->io2.release();
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
+    c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -529,7 +573,10 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
     const int64_t plen = coh ? 2 * npix : npix;
     // slabs of batches bound the init buffer to ~2^25 columns
     const int64_t max_bs = std::max<int64_t>(S.max_bs, 1);
+    int64_t stride = 1;
+    for (double tb : t) stride = std::max(stride, steps_of(P.tf, tb, P.dt));
     int64_t per_slab = std::max<int64_t>(1, ((int64_t)1 << 25) / max_bs);
+    per_slab = std::min<int64_t>(per_slab, std::max<int64_t>(1, kFieldTabBudget / (stride * kFieldTabBytesPerStep)));
     per_slab = std::min<int64_t>(per_slab, S.t_num);
     const int cps = (int)((max_bs + 63) / 64);
 
@@ -561,10 +608,17 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
     for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
         const int b0 = (int)(k * per_slab);
         const int nb = (int)std::min<int64_t>(per_slab, S.t_num - b0);
+        FieldTab FT{};
         hipError_t e = hipEventRecord(ev[3 * k], c->stream);
+        if (e == hipSuccess) {
+            if (int frc = fill_field_table(c, P, c->t0s.p + b0, nb, stride, FT)) {
+                rc = frc;
+                break;
+            }
+        }
         if (e == hipSuccess) e = launch_adk_sample(c->stream, S, L, T, c->t0s.p, b0, nb, c->init.p, c->counts.p);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], c->stream);
-        if (e == hipSuccess) e = launch_rk4(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->counts.p, nb, cps);
+        if (e == hipSuccess) e = launch_rk4(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->counts.p, nb, cps, FT);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(hcounts.data(), c->counts.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, c->stream);

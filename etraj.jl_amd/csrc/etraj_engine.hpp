@@ -47,6 +47,15 @@ struct Contrib {
 enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
 enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_N = 4 };
 
+// Per-segment laser field table (k_field_table): entry s of a segment holds
+// the field at half-steps 2s+1 and 2s+2 of that segment's step grid.
+struct FieldTab {
+    const float4* f32;   // stride entries per segment: (Fx, Fy) at 2s+1, (Fx, Fy) at 2s+2
+    const double2* f64;  // 2*stride entries per segment, same values in FP64
+    const double2* f0;   // F(t0) per segment
+    int64_t stride;      // >= max steps of any segment
+};
+
 struct OutP {
     double* pmd;       // CTMC: npix; coherent: 2*npix (re, im)
     double* scal;      // [ion_prob, ion_prob_uncollected]
@@ -62,7 +71,10 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
 hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L, const TargetP& T, const double* t0s,
                              int b_first, int nb, double* out, int32_t* counts);
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg);
+                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
+                      const FieldTab& FT);
+hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,
+                              const FieldTab& FT);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
 
 }  // namespace etraj

@@ -294,15 +294,45 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
     }
 }
 
+// Field table of one segment (equal t0): the segment's step is
+// h = (t_f - t0)/n, n = ceil((t_f - t0)/dt), and step s needs the field at
+// half-steps 2s+1 and 2s+2 (t0 + j h/2). Entry s holds both, as FP32
+// (Fx, Fy, Fx, Fy) and FP64 (two double2); f0 holds F(t0).
+__global__ __launch_bounds__(256) void k_field_table(LaserP L, const double* __restrict__ t0s, int64_t nseg,
+                                                     double tf, double dt, int64_t stride, float4* __restrict__ f32,
+                                                     double2* __restrict__ f64, double2* __restrict__ f0) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg * stride) return;
+    const int64_t seg = i / stride, s = i - seg * stride;
+    const double t0 = t0s[seg];
+    const double span = tf - t0;
+    int64_t n = (int64_t)ceil(span / dt);
+    if (n < 1) n = 1;
+    if (s >= n) return;
+    const double hh = (span / (double)n) * 0.5;
+    double mx, my, ex, ey;
+    laser_field(L, t0 + (double)(2 * s + 1) * hh, mx, my);
+    laser_field(L, t0 + (double)(2 * s + 2) * hh, ex, ey);
+    f32[i] = make_float4((float)mx, (float)my, (float)ex, (float)ey);
+    f64[2 * i] = make_double2(mx, my);
+    f64[2 * i + 1] = make_double2(ex, ey);
+    if (s == 0) {
+        double ax, ay;
+        laser_field(L, t0, ax, ay);
+        f0[seg] = make_double2(ax, ay);
+    }
+}
+
 // 64 electrons of one segment (equal t0) per wavefront. Segment s covers
-// columns [seg_start[s], seg_start[s] + seg_count[s]).
+// columns [seg_start[s], seg_start[s] + seg_count[s]); its field table row is
+// FT.f32/f64 + s * FT.stride, read with wave-uniform (scalar) loads.
 template <int DIM, int PHASE, bool MIXED, int OUT, int TGT>
 __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
                                              const int64_t* __restrict__ seg_start,
                                              const int32_t* __restrict__ seg_count, int64_t n_seg,
-                                             int chunks_per_seg) {
+                                             int chunks_per_seg, FieldTab FT) {
     const int lane = lane_id();
-    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int64_t seg = gw / chunks_per_seg;
     const int chunk = (int)(gw - seg * chunks_per_seg);
     if (seg >= n_seg) return;
@@ -320,11 +350,10 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
     int64_t n = (int64_t)ceil(span / P.dt);
     if (n < 1) n = 1;
     const double h = span / (double)n;
-    const double hh = h * 0.5;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
-
-    double Fpx, Fpy;  // field at the start of the current step
-    laser_field(P.L, t0, Fpx, Fpy);
+    const float4* __restrict__ T32 = FT.f32 + seg * FT.stride;
+    const double2* __restrict__ T64 = FT.f64 + 2 * seg * FT.stride;
+    const double2 F0 = FT.f0[seg];
     int64_t nre = 0;
     bool bad = false;
 
@@ -334,31 +363,27 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
 #pragma unroll
         for (int i = 0; i < DIM; ++i) { u.r[i] = e.r[i]; u.p[i] = e.p[i]; }
         phase = e.phase0;
-        for (int64_t base = 0; base < n; base += 32) {
-            double fx, fy;
-            laser_field(P.L, t0 + (double)(2 * base + 1 + lane) * hh, fx, fy);
-            const int nst = (int)((n - base) < 32 ? (n - base) : 32);
-            for (int s = 0; s < nst; ++s) {
-                const double Fmx = readlane_f64(fx, 2 * s), Fmy = readlane_f64(fy, 2 * s);
-                const double F1x = readlane_f64(fx, 2 * s + 1), F1y = readlane_f64(fy, 2 * s + 1);
-                State<DIM, double> d;
-                double r2m;
-                double dph = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, u, h, Fpx, Fpy, Fmx, Fmy, F1x, F1y, d, r2m);
+        double Fpx = F0.x, Fpy = F0.y;
+        for (int64_t s = 0; s < n; ++s) {
+            const double2 Fm = T64[2 * s], F1 = T64[2 * s + 1];
+            State<DIM, double> d;
+            double r2m;
+            double dph = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, u, h, Fpx, Fpy, Fm.x, Fm.y, F1.x, F1.y, d, r2m);
 #pragma unroll
-                for (int i = 0; i < DIM; ++i) {
-                    u.r[i] = u.r[i] + d.r[i];
-                    u.p[i] = u.p[i] + d.p[i];
-                }
-                phase = phase + dph;
-                Fpx = F1x;
-                Fpy = F1y;
+            for (int i = 0; i < DIM; ++i) {
+                u.r[i] = u.r[i] + d.r[i];
+                u.p[i] = u.p[i] + d.p[i];
             }
+            phase = phase + dph;
+            Fpx = F1.x;
+            Fpy = F1.y;
         }
 #pragma unroll
         for (int i = 0; i < DIM; ++i) { rf[i] = u.r[i]; pf[i] = u.p[i]; }
     } else {
         // FP32 state s with Kahan compensation c (value = s - c), FP64
-        // Kahan-compensated phase, FP64 re-step when any stage has r < r_c.
+        // Kahan-compensated phase, FP64 re-step of any step that starts
+        // within restep_radius of the core.
         State<DIM, float> us, uc;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
@@ -371,66 +396,55 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
         double phc = 0.0;
         const float hf = (float)h;
         const float rc2 = (float)(P.restep_radius * P.restep_radius);
-        float Fpxf = (float)Fpx, Fpyf = (float)Fpy;
-        for (int64_t base = 0; base < n; base += 32) {
-            double fx, fy;
-            laser_field(P.L, t0 + (double)(2 * base + 1 + lane) * hh, fx, fy);
-            const float fxf = (float)fx, fyf = (float)fy;
-            const int nst = (int)((n - base) < 32 ? (n - base) : 32);
-            for (int s = 0; s < nst; ++s) {
-                const float Fmx = readlane_f32(fxf, 2 * s), Fmy = readlane_f32(fyf, 2 * s);
-                const float F1x = readlane_f32(fxf, 2 * s + 1), F1y = readlane_f32(fyf, 2 * s + 1);
-                State<DIM, float> d;
-                float r2m;
-                float dph = rk4_incr<DIM, PHASE, TGT, float>(P.T, saeq, us, hf, Fpxf, Fpyf, Fmx, Fmy, F1x, F1y, d, r2m);
-                double dphd;
-                if (r2m < rc2) {
-                    // close encounter: redo this step in FP64 from s - c
-                    const double Dmx = readlane_f64(fx, 2 * s), Dmy = readlane_f64(fy, 2 * s);
-                    const double D1x = readlane_f64(fx, 2 * s + 1), D1y = readlane_f64(fy, 2 * s + 1);
-                    const double D0x = (s == 0) ? Fpx : readlane_f64(fx, 2 * s - 1);
-                    const double D0y = (s == 0) ? Fpy : readlane_f64(fy, 2 * s - 1);
-                    State<DIM, double> ud, dd;
+        float Fpx = (float)F0.x, Fpy = (float)F0.y;
+        for (int64_t s = 0; s < n; ++s) {
+            const float4 f = T32[s];
+            State<DIM, float> d;
+            float r2m;
+            const float dph = rk4_incr<DIM, PHASE, TGT, float>(P.T, saeq, us, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, d, r2m);
+            double dphd;
+            if (r2m < rc2) {
+                // close encounter: redo this step in FP64 from s - c
+                const double2 D0 = (s == 0) ? F0 : T64[2 * s - 1];
+                const double2 Dm = T64[2 * s], D1 = T64[2 * s + 1];
+                State<DIM, double> ud, dd;
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        ud.r[i] = (double)us.r[i] - (double)uc.r[i];
-                        ud.p[i] = (double)us.p[i] - (double)uc.p[i];
-                    }
-                    double r2d;
-                    dphd = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, ud, h, D0x, D0y, Dmx, Dmy, D1x, D1y, dd, r2d);
-#pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        double nr = ud.r[i] + dd.r[i], np = ud.p[i] + dd.p[i];
-                        us.r[i] = (float)nr;
-                        uc.r[i] = -(float)(nr - (double)us.r[i]);
-                        us.p[i] = (float)np;
-                        uc.p[i] = -(float)(np - (double)us.p[i]);
-                    }
-                    ++nre;
-                } else {
-#pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        float y = d.r[i] - uc.r[i];
-                        float t = us.r[i] + y;
-                        uc.r[i] = (t - us.r[i]) - y;
-                        us.r[i] = t;
-                        y = d.p[i] - uc.p[i];
-                        t = us.p[i] + y;
-                        uc.p[i] = (t - us.p[i]) - y;
-                        us.p[i] = t;
-                    }
-                    dphd = (double)dph;
+                for (int i = 0; i < DIM; ++i) {
+                    ud.r[i] = (double)us.r[i] - (double)uc.r[i];
+                    ud.p[i] = (double)us.p[i] - (double)uc.p[i];
                 }
-                // Kahan-compensated FP64 phase sum
-                double y = dphd - phc;
-                double t = phase + y;
-                phc = (t - phase) - y;
-                phase = t;
-                Fpxf = F1x;
-                Fpyf = F1y;
+                double r2d;
+                dphd = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, ud, h, D0.x, D0.y, Dm.x, Dm.y, D1.x, D1.y, dd, r2d);
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    double nr = ud.r[i] + dd.r[i], np = ud.p[i] + dd.p[i];
+                    us.r[i] = (float)nr;
+                    uc.r[i] = -(float)(nr - (double)us.r[i]);
+                    us.p[i] = (float)np;
+                    uc.p[i] = -(float)(np - (double)us.p[i]);
+                }
+                ++nre;
+            } else {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    float y = d.r[i] - uc.r[i];
+                    float t = us.r[i] + y;
+                    uc.r[i] = (t - us.r[i]) - y;
+                    us.r[i] = t;
+                    y = d.p[i] - uc.p[i];
+                    t = us.p[i] + y;
+                    uc.p[i] = (t - us.p[i]) - y;
+                    us.p[i] = t;
+                }
+                dphd = (double)dph;
             }
-            Fpx = readlane_f64(fx, 2 * nst - 1);
-            Fpy = readlane_f64(fy, 2 * nst - 1);
+            // Kahan-compensated FP64 phase sum
+            double y = dphd - phc;
+            double t = phase + y;
+            phc = (t - phase) - y;
+            phase = t;
+            Fpx = f.z;
+            Fpy = f.w;
         }
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
@@ -486,50 +500,61 @@ hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L,
 
 template <int DIM, int PHASE, bool MIXED, int OUT>
 static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                               const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps) {
+                               const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
+                               const FieldTab& FT) {
     int64_t waves = n_seg * (int64_t)cps;
     unsigned blocks = (unsigned)((waves + 3) / 4);
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
         hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st,
-                           P, O, init, seg_start, seg_count, n_seg, cps);
+                           P, O, init, seg_start, seg_count, n_seg, cps, FT);
     else
         hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O,
-                           init, seg_start, seg_count, n_seg, cps);
+                           init, seg_start, seg_count, n_seg, cps, FT);
     return hipGetLastError();
 }
 
 template <int DIM, int PHASE, bool MIXED>
 static hipError_t launch_rk4_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
     switch (out) {
-        case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps);
-        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps);
-        default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps);
+        case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps, FT);
+        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps, FT);
+        default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps, FT);
     }
 }
 
 template <int DIM, int PHASE>
 static hipError_t launch_rk4_m(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
-    return P.mixed ? launch_rk4_o<DIM, PHASE, true>(st, P, O, out, init, ss, sc, ns, cps)
-                   : launch_rk4_o<DIM, PHASE, false>(st, P, O, out, init, ss, sc, ns, cps);
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
+    return P.mixed ? launch_rk4_o<DIM, PHASE, true>(st, P, O, out, init, ss, sc, ns, cps, FT)
+                   : launch_rk4_o<DIM, PHASE, false>(st, P, O, out, init, ss, sc, ns, cps, FT);
 }
 
 template <int DIM>
 static hipError_t launch_rk4_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps) {
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
     switch (P.phase) {
-        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps);
-        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps);
-        default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps);
+        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps, FT);
+        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps, FT);
+        default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps, FT);
     }
 }
 
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps) {
+                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
+                      const FieldTab& FT) {
     if (n_seg <= 0 || cps <= 0) return hipSuccess;
-    return P.dim == 3 ? launch_rk4_p<3>(st, P, O, out, init, seg_start, seg_count, n_seg, cps)
-                      : launch_rk4_p<2>(st, P, O, out, init, seg_start, seg_count, n_seg, cps);
+    return P.dim == 3 ? launch_rk4_p<3>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT)
+                      : launch_rk4_p<2>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT);
+}
+
+hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,
+                              const FieldTab& FT) {
+    if (nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_table, dim3(grid_for(nseg * FT.stride, 256)), dim3(256), 0, st, L, t0s, nseg, tf, dt,
+                       FT.stride, const_cast<float4*>(FT.f32), const_cast<double2*>(FT.f64),
+                       const_cast<double2*>(FT.f0));
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out) {

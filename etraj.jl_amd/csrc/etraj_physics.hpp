@@ -282,6 +282,31 @@ struct State {
 template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R Fx, R Fy,
                                     State<DIM, R>& du, R& dphi, R& r2out) {
+    if (sizeof(R) == 4 && TGT == ETRAJ_TARGET_HYDROGEN_LIKE) {
+        // FP32 form: the soft core is folded into r^2 = |r|^2 + a once; then
+        // r.F_target = -Z r^2/(r^2)^(3/2) = -Z/r, so the SCTS integrand
+        // -(p^2/2 + V + r.F) is 2Z/r - p^2/2 (the a/r^2 ~ 1e-10 difference is
+        // below FP32 resolution).
+        R r2 = fma(u.r[1], u.r[1], (R)T.sc);
+        if (DIM == 3) r2 = fma(u.r[DIM - 1], u.r[DIM - 1], r2);
+        r2 = fma(u.r[0], u.r[0], r2);
+        r2out = r2;
+        const R ir = rsqrt_t<R>(r2);
+        const R zr = (R)T.Zd * ir;
+        const R s = -(zr * (ir * ir));
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) du.r[i] = u.p[i];
+        du.p[0] = fma(s, u.r[0], -Fx);
+        du.p[1] = fma(s, u.r[1], -Fy);
+        if (DIM == 3) du.p[DIM - 1] = s * u.r[DIM - 1];
+        R k2 = u.p[1] * u.p[1];
+        if (DIM == 3) k2 = fma(u.p[DIM - 1], u.p[DIM - 1], k2);
+        k2 = fma(u.p[0], u.p[0], k2);
+        if (PHASE == ETRAJ_PHASE_QTMC) dphi = fma(k2, (R)-0.5, zr - (R)T.Ip);
+        else if (PHASE == ETRAJ_PHASE_SCTS) dphi = fma(k2, (R)-0.5, zr + zr);
+        else dphi = 0;
+        return;
+    }
     R r2n = u.r[0] * u.r[0] + u.r[1] * u.r[1];
     if (DIM == 3) r2n += u.r[DIM - 1] * u.r[DIM - 1];
     r2out = r2n;
@@ -307,7 +332,7 @@ __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const Stat
 // One classical RK4 step of size h from state u (stage fields F0 at t,
 // Fm at t+h/2, F1 at t+h). Returns the state increment d, so that
 // u_next = u + d with d = h/6 (k1 + 2 (k2 + k3) + k4), the phase increment
-// and the minimum |r|^2 over the four stages.
+// and |r|^2 at the start of the step (r2min).
 template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R h,
                                       R F0x, R F0y, R Fmx, R Fmy, R F1x, R F1y,
@@ -327,14 +352,12 @@ __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const St
         y.p[i] = u.p[i] + hh * k.p[i];
     }
     eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k2, ph2, r2);
-    r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         y.r[i] = u.r[i] + hh * k2.r[i];
         y.p[i] = u.p[i] + hh * k2.p[i];
     }
     eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
-    r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         d.r[i] += (R)2 * (k2.r[i] + k.r[i]);
@@ -344,7 +367,6 @@ __host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const St
     }
     phacc += (R)2 * (ph2 + ph);
     eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, F1x, F1y, k, ph, r2);
-    r2min = r2 < r2min ? r2 : r2min;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         d.r[i] = h6 * (d.r[i] + k.r[i]);

@@ -58,8 +58,10 @@ struct etraj_ctx {
     DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0;
     DevBuf<float4> ft32;
     DevBuf<double2> ft64, ft0;
-    DevBuf<int64_t> seg_start, cnt;
-    DevBuf<int32_t> counts, status;
+    DevBuf<int64_t> seg_start, cnt, pre;
+    DevBuf<int32_t> counts, counts2, status;
+    DevBuf<uint8_t> flags;
+    DevBuf<int64_t> orig;
     DevBuf<Contrib> contrib;
 };
 
@@ -225,10 +227,14 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     if (r.dimension != 2 && r.dimension != 3)
         return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `dimension` must be either 2 or 3.");
     if (r.phase_method < 0 || r.phase_method > 2) return fail(c, ETRAJ_EINVAL, "undefined phase method");
-    if (r.integrator == ETRAJ_INTEG_TSIT5)
-        return fail(c, ETRAJ_EUNSUPPORTED, "the adaptive Tsit5 integrator is not available in this engine build");
-    if (r.integrator != ETRAJ_INTEG_RK4) return fail(c, ETRAJ_EINVAL, "unknown integrator");
-    if (!(r.dt > 0)) return fail(c, ETRAJ_EINVAL, "RK4 needs dt > 0");
+    if (r.integrator != ETRAJ_INTEG_RK4 && r.integrator != ETRAJ_INTEG_TSIT5)
+        return fail(c, ETRAJ_EINVAL, "unknown integrator");
+    if (r.integrator == ETRAJ_INTEG_RK4 && !(r.dt > 0)) return fail(c, ETRAJ_EINVAL, "RK4 needs dt > 0");
+    if (r.integrator == ETRAJ_INTEG_TSIT5) {
+        if (!(r.rtol > 0)) return fail(c, ETRAJ_EINVAL, "Tsit5 needs traj_rtol > 0");
+        if (r.precision != ETRAJ_PREC_FP64)
+            return fail(c, ETRAJ_EUNSUPPORTED, "the Tsit5 integrator runs in FP64 only (precision = fp64)");
+    }
     if (r.precision != ETRAJ_PREC_FP64 && r.precision != ETRAJ_PREC_MIXED) return fail(c, ETRAJ_EINVAL, "unknown precision");
     if (r.precision == ETRAJ_PREC_MIXED && !(r.restep_radius >= 0)) return fail(c, ETRAJ_EINVAL, "restep_radius must be >= 0");
     for (int d = 0; d < r.dimension; ++d)
@@ -252,6 +258,10 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     P.tf = r.traj_t_final;
     P.dt = r.dt;
     P.restep_radius = r.restep_radius;
+    P.integrator = r.integrator;
+    P.rtol = r.rtol;
+    P.atol = r.atol > 0 ? r.atol : 1e-6;          // OrdinaryDiffEq's default abstol
+    P.maxiters = r.maxiters > 0 ? r.maxiters : 1000000;
     return ETRAJ_OK;
 }
 
@@ -260,6 +270,8 @@ int64_t steps_of(double tf, double t0, double dt) {
     int64_t n = (int64_t)std::ceil((tf - t0) / dt);
     return n < 1 ? 1 : n;
 }
+
+constexpr int kTsit5PerWave = 64 * 8;  // electrons per wavefront for lane refill
 
 constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
 constexpr int64_t kFieldTabBudget = (int64_t)1 << 30;  // bytes of field table per launch
@@ -274,6 +286,30 @@ int fill_field_table(etraj_ctx* c, const IntegP& P, const double* t0s_dev, int64
     FT.f0 = c->ft0.p;
     FT.stride = stride;
     HIP_TRY(c, launch_field_table(c->stream, P.L, t0s_dev, nseg, P.tf, P.dt, FT));
+    return ETRAJ_OK;
+}
+
+// Fixed-step RK4 over nseg segments. FP64: one pass. MIXED: the FP32 pass
+// flags the electrons that come within restep_radius of the core, which are
+// compacted in place per segment and integrated again in FP64. `ncols` is the
+// number of columns the init buffer spans (for the flag array).
+int rk4_passes(etraj_ctx* c, const IntegP& P, const OutP& O, int out, const int64_t* seg_start, int32_t* counts,
+               int64_t nseg, int cps, const FieldTab& FT, int64_t ncols) {
+    if (!P.mixed) {
+        HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, nullptr, nullptr));
+        return ETRAJ_OK;
+    }
+    HIP_TRY(c, c->flags.ensure((size_t)ncols));
+    HIP_TRY(c, c->counts2.ensure((size_t)nseg));
+    const bool need_orig = out != OUT_PMD;
+    if (need_orig) HIP_TRY(c, c->orig.ensure((size_t)ncols));
+    HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, c->flags.p, nullptr));
+    HIP_TRY(c, launch_compact_flagged(c->stream, c->init.p, P.rows, seg_start, counts, nseg, c->flags.p, nullptr,
+                                      c->counts2.p, need_orig ? c->orig.p : nullptr));
+    IntegP P64 = P;
+    P64.mixed = 0;
+    HIP_TRY(c, launch_rk4(c->stream, P64, O, out, c->init.p, seg_start, c->counts2.p, nseg, cps, FT, nullptr,
+                          need_orig ? c->orig.p : nullptr));
     return ETRAJ_OK;
 }
 
@@ -319,6 +355,18 @@ int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t
     int cps_all = 0;
     build_segments(init, n, P.rows, P.dim, ss, sc, cps_all);
     const int64_t nseg = (int64_t)ss.size();
+    if (P.integrator == ETRAJ_INTEG_TSIT5) {
+        std::vector<int64_t> pre(nseg + 1, 0);
+        for (int64_t k = 0; k < nseg; ++k) pre[k + 1] = pre[k] + sc[k];
+        HIP_TRY(c, c->init.ensure((size_t)n * P.rows));
+        HIP_TRY(c, c->seg_start.ensure(nseg));
+        HIP_TRY(c, c->pre.ensure(nseg + 1));
+        HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pre.p, pre.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave));
+        return ETRAJ_OK;
+    }
     std::vector<double> t0(nseg);
     int64_t stride = 1;
     for (int64_t k = 0; k < nseg; ++k) {
@@ -340,8 +388,8 @@ int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t
         for (int64_t k = g0; k < g0 + ng; ++k) maxc = std::max(maxc, sc[k]);
         FieldTab FT{};
         if (int rc = fill_field_table(c, P, c->seg_t0.p + g0, ng, stride, FT)) return rc;
-        HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, c->seg_start.p + g0, c->counts.p + g0, ng,
-                              (int)((maxc + 63) / 64), FT));
+        if (int rc = rk4_passes(c, P, O, out, c->seg_start.p + g0, c->counts.p + g0, ng, (int)((maxc + 63) / 64), FT, n))
+            return rc;
     }
     return ETRAJ_OK;
 }
@@ -376,7 +424,8 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->init.release(); c->pmd.release(); c->pmd_out.release(); c->io1.release(); c->io2.release();
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
-    c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release();
+    c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
+    c->counts2.release(); c->flags.release(); c->orig.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -587,6 +636,7 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
     HIP_TRY(c, c->init.ensure((size_t)per_slab * max_bs * S.rows));
     HIP_TRY(c, c->counts.ensure(per_slab));
     HIP_TRY(c, c->seg_start.ensure(per_slab));
+    HIP_TRY(c, c->pre.ensure(per_slab + 1));
     std::vector<int64_t> ss(per_slab);
     for (int64_t i = 0; i < per_slab; ++i) ss[i] = i * max_bs;
     HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * per_slab, hipMemcpyHostToDevice, c->stream));
@@ -609,8 +659,9 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         const int b0 = (int)(k * per_slab);
         const int nb = (int)std::min<int64_t>(per_slab, S.t_num - b0);
         FieldTab FT{};
+        const bool rk4 = P.integrator == ETRAJ_INTEG_RK4;
         hipError_t e = hipEventRecord(ev[3 * k], c->stream);
-        if (e == hipSuccess) {
+        if (e == hipSuccess && rk4) {
             if (int frc = fill_field_table(c, P, c->t0s.p + b0, nb, stride, FT)) {
                 rc = frc;
                 break;
@@ -618,7 +669,18 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         }
         if (e == hipSuccess) e = launch_adk_sample(c->stream, S, L, T, c->t0s.p, b0, nb, c->init.p, c->counts.p);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], c->stream);
-        if (e == hipSuccess) e = launch_rk4(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->counts.p, nb, cps, FT);
+        if (e == hipSuccess && rk4) {
+            if (int prc = rk4_passes(c, P, O, OUT_PMD, c->seg_start.p, c->counts.p, nb, cps, FT, (int64_t)nb * max_bs)) {
+                rc = prc;
+                break;
+            }
+        }
+        if (e == hipSuccess && !rk4) {
+            e = launch_scan_counts(c->stream, c->counts.p, nb, c->pre.p);
+            if (e == hipSuccess)
+                e = launch_tsit5(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->pre.p, nb, (int64_t)nb * max_bs,
+                                 kTsit5PerWave);
+        }
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(hcounts.data(), c->counts.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, c->stream);
@@ -664,6 +726,7 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         stats->n_bound = cnt[CNT_BOUND];
         stats->n_restep = cnt[CNT_RESTEP];
         stats->n_steps = cnt[CNT_STEPS];
+        stats->n_steps_fp64 = cnt[CNT_STEPS64];
         stats->ms_sample = ms_s;
         stats->ms_integrate = ms_i;
         stats->ms_total =

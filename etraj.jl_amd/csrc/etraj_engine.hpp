@@ -33,6 +33,9 @@ struct IntegP {
     int pnum[3];
     double pmax[3];
     double tf, dt, restep_radius;
+    double rtol, atol;
+    int64_t maxiters;
+    int integrator;
 };
 
 // per-electron binning record for host-side accumulation in column order
@@ -45,7 +48,8 @@ struct Contrib {
 };
 
 enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
-enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_N = 4 };
+// CNT_RESTEP counts the electrons of the mixed pass re-integrated in FP64
+enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_N = 5 };
 
 // Per-segment laser field table (k_field_table): entry s of a segment holds
 // the field at half-steps 2s+1 and 2s+2 of that segment's step grid.
@@ -72,9 +76,15 @@ hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L,
                              int b_first, int nb, double* out, int32_t* counts);
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
-                      const FieldTab& FT);
+                      const FieldTab& FT, uint8_t* flags, const int64_t* out_index);
+hipError_t launch_compact_flagged(hipStream_t st, double* init, int rows, const int64_t* seg_start,
+                                  const int32_t* seg_count, int64_t n_seg, const uint8_t* flags,
+                                  const int64_t* out_index, int32_t* count_out, int64_t* orig);
 hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,
                               const FieldTab& FT);
+hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave);
+hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
 
 }  // namespace etraj

@@ -326,11 +326,20 @@ __global__ __launch_bounds__(256) void k_field_table(LaserP L, const double* __r
 // 64 electrons of one segment (equal t0) per wavefront. Segment s covers
 // columns [seg_start[s], seg_start[s] + seg_count[s]); its field table row is
 // FT.f32/f64 + s * FT.stride, read with wave-uniform (scalar) loads.
+//
+// MIXED: the electron state is FP32 with a Kahan compensation term per
+// component and the phase is summed in FP64 (Kahan). An electron that starts
+// any step within restep_radius of the core is not binned: flags[col] = 1
+// marks it for the FP64 pass (k_compact_flagged + this kernel with
+// MIXED = false), which integrates it again from t0. Errors a close encounter
+// amplifies are accumulated before it, so the whole trajectory is redone.
+// out_index (may be NULL) maps a column to the index of its record/contrib.
 template <int DIM, int PHASE, bool MIXED, int OUT, int TGT>
 __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
                                              const int64_t* __restrict__ seg_start,
                                              const int32_t* __restrict__ seg_count, int64_t n_seg,
-                                             int chunks_per_seg, FieldTab FT) {
+                                             int chunks_per_seg, FieldTab FT, uint8_t* __restrict__ flags,
+                                             const int64_t* __restrict__ out_index) {
     const int lane = lane_id();
     const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int64_t seg = gw / chunks_per_seg;
@@ -354,8 +363,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
     const float4* __restrict__ T32 = FT.f32 + seg * FT.stride;
     const double2* __restrict__ T64 = FT.f64 + 2 * seg * FT.stride;
     const double2 F0 = FT.f0[seg];
-    int64_t nre = 0;
-    bool bad = false;
+    bool bad = false, close = false;
 
     double rf[DIM], pf[DIM], phase;
     if (!MIXED) {
@@ -381,10 +389,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
 #pragma unroll
         for (int i = 0; i < DIM; ++i) { rf[i] = u.r[i]; pf[i] = u.p[i]; }
     } else {
-        // FP32 state s with Kahan compensation c (value = s - c), FP64
-        // Kahan-compensated phase, FP64 re-step of any step that starts
-        // within restep_radius of the core.
-        State<DIM, float> us, uc;
+        State<DIM, float> us, uc;  // value = us - uc
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             us.r[i] = (float)e.r[i];
@@ -402,45 +407,20 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
             State<DIM, float> d;
             float r2m;
             const float dph = rk4_incr<DIM, PHASE, TGT, float>(P.T, saeq, us, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, d, r2m);
-            double dphd;
-            if (r2m < rc2) {
-                // close encounter: redo this step in FP64 from s - c
-                const double2 D0 = (s == 0) ? F0 : T64[2 * s - 1];
-                const double2 Dm = T64[2 * s], D1 = T64[2 * s + 1];
-                State<DIM, double> ud, dd;
+            close = close || (r2m < rc2);
 #pragma unroll
-                for (int i = 0; i < DIM; ++i) {
-                    ud.r[i] = (double)us.r[i] - (double)uc.r[i];
-                    ud.p[i] = (double)us.p[i] - (double)uc.p[i];
-                }
-                double r2d;
-                dphd = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, ud, h, D0.x, D0.y, Dm.x, Dm.y, D1.x, D1.y, dd, r2d);
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) {
-                    double nr = ud.r[i] + dd.r[i], np = ud.p[i] + dd.p[i];
-                    us.r[i] = (float)nr;
-                    uc.r[i] = -(float)(nr - (double)us.r[i]);
-                    us.p[i] = (float)np;
-                    uc.p[i] = -(float)(np - (double)us.p[i]);
-                }
-                ++nre;
-            } else {
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) {
-                    float y = d.r[i] - uc.r[i];
-                    float t = us.r[i] + y;
-                    uc.r[i] = (t - us.r[i]) - y;
-                    us.r[i] = t;
-                    y = d.p[i] - uc.p[i];
-                    t = us.p[i] + y;
-                    uc.p[i] = (t - us.p[i]) - y;
-                    us.p[i] = t;
-                }
-                dphd = (double)dph;
+            for (int i = 0; i < DIM; ++i) {
+                float y = d.r[i] - uc.r[i];
+                float t = us.r[i] + y;
+                uc.r[i] = (t - us.r[i]) - y;
+                us.r[i] = t;
+                y = d.p[i] - uc.p[i];
+                t = us.p[i] + y;
+                uc.p[i] = (t - us.p[i]) - y;
+                us.p[i] = t;
             }
-            // Kahan-compensated FP64 phase sum
-            double y = dphd - phc;
-            double t = phase + y;
+            const double y = (double)dph - phc;   // Kahan-compensated FP64 phase sum
+            const double t = phase + y;
             phc = (t - phase) - y;
             phase = t;
             Fpx = f.z;
@@ -451,19 +431,256 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
             rf[i] = (double)us.r[i] - (double)uc.r[i];
             pf[i] = (double)us.p[i] - (double)uc.p[i];
         }
+        if (active) flags[col_idx] = close ? 1 : 0;
     }
 #pragma unroll
     for (int i = 0; i < DIM; ++i) bad = bad || isnan(rf[i]) || isnan(pf[i]);
     if (PHASE != ETRAJ_PHASE_CTMC) bad = bad || isnan(phase);
-    emit<DIM, PHASE, OUT>(P, O, col_idx, active, bad, rf, pf, phase, t0, e.prob);
+    const bool done = active && !close;
+    const int64_t out_idx = (out_index && done) ? out_index[col_idx] : col_idx;
+    emit<DIM, PHASE, OUT>(P, O, out_idx, done, bad, rf, pf, phase, t0, e.prob);
     if (OUT == OUT_PMD) {
-        int64_t nact = wave_sum_i64(active ? 1 : 0);
-        int64_t nr = wave_sum_i64(active ? nre : 0);
+        // steps: every active lane of this pass; FP64 steps also counted apart
+        const int64_t nact = wave_sum_i64(active ? 1 : 0);
+        const int64_t nclose = wave_sum_i64((active && close) ? 1 : 0);
         if (lane == 0) {
             atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)(n * nact));
-            if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nr);
+            if (!MIXED) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS64), (unsigned long long)(n * nact));
+            if (nclose) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nclose);
         }
     }
+}
+
+// In-place, order-preserving compaction of the flagged columns of each
+// segment (one workgroup per segment): count_out[s] receives their number and
+// orig[seg_start[s] + k] the original column (or its out_index entry) of the
+// k-th. Every column of a chunk is read before any is written, and writes
+// never pass the read position, so the compaction can overwrite in place.
+__global__ __launch_bounds__(256) void k_compact_flagged(double* __restrict__ init, int rows,
+                                                         const int64_t* __restrict__ seg_start,
+                                                         const int32_t* __restrict__ seg_count,
+                                                         const uint8_t* __restrict__ flags,
+                                                         const int64_t* __restrict__ out_index,
+                                                         int32_t* __restrict__ count_out,
+                                                         int64_t* __restrict__ orig) {
+    __shared__ int wave_tot[4];
+    __shared__ int base_sh;
+    const int64_t seg = blockIdx.x;
+    const int64_t base = seg_start[seg];
+    const int32_t cnt = seg_count[seg];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) base_sh = 0;
+    __syncthreads();
+    double col[9];
+    for (int j0 = 0; j0 < cnt; j0 += 256) {
+        const int j = j0 + tid;
+        const bool keep = j < cnt && flags[base + j] != 0;
+        if (keep)
+            for (int r = 0; r < rows; ++r) col[r] = init[(base + j) * rows + r];
+        const uint64_t m = __ballot(keep);
+        const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (lane == 0) wave_tot[wv] = __popcll(m);
+        __syncthreads();
+        int off = base_sh;
+        for (int w = 0; w < wv; ++w) off += wave_tot[w];
+        const int tot = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        if (keep) {
+            const int64_t dst = base + off + pre;
+            for (int r = 0; r < rows; ++r) init[dst * rows + r] = col[r];
+            if (orig) orig[dst] = out_index ? out_index[base + j] : base + j;
+        }
+        __syncthreads();
+        if (tid == 0) base_sh += tot;
+        __syncthreads();
+    }
+    if (tid == 0) count_out[seg] = base_sh;
+}
+
+// Per-lane output of one finished electron for kernels whose lanes finish at
+// different times (no wave-wide operations inside). Accumulates the scalar
+// counters into `acc` for one reduction per wave at the end.
+struct LaneAcc {
+    double ip = 0.0, iu = 0.0;
+    int64_t failed = 0, bound = 0, steps = 0;
+};
+
+template <int DIM, int PHASE, int OUT>
+__device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx, bool failed, const double* r,
+                                 const double* p, double phase, double t0, double prob, LaneAcc& acc) {
+    Final f = kepler_final<DIM, PHASE>(P.T, P.quirks, r, p, phase, t0);
+    if (failed) f.ionized = false;
+    int64_t lin = -1;
+    bool inb = false;
+    double cre = 0.0, cim = 0.0;
+    if (f.ionized) {
+        int64_t idx[3] = {1, 1, 1};
+        inb = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            idx[d] = bin_index(f.pinf[d], P.pmax[d], P.pnum[d]);
+            if (idx[d] < 1 || idx[d] > P.pnum[d]) inb = false;
+        }
+        if (inb) {
+            lin = (idx[0] - 1) + (int64_t)P.pnum[0] * ((idx[1] - 1) + (DIM == 3 ? (int64_t)P.pnum[1] * (idx[2] - 1) : 0));
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                cre = prob;
+            } else {
+                double sq = sqrt(prob), sp, cp;
+                sincos(f.phase, &sp, &cp);
+                cre = sq * cp;
+                cim = sq * sp;
+            }
+        }
+    }
+    if (failed) acc.failed++;
+    else if (!f.ionized) acc.bound++;
+    else {
+        acc.ip += prob;
+        if (!inb) acc.iu += prob;
+    }
+    if (OUT == OUT_PMD) {
+        if (inb) {
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                unsafeAtomicAdd(O.pmd + lin, cre);
+            } else {
+                unsafeAtomicAdd(O.pmd + 2 * lin, cre);
+                unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
+            }
+        }
+    } else if (OUT == OUT_RECORDS) {
+        double* rec = O.records + 12 * col_idx;
+        rec[0] = r[0]; rec[1] = r[1]; rec[2] = DIM == 3 ? r[2] : 0.0;
+        rec[3] = p[0]; rec[4] = p[1]; rec[5] = DIM == 3 ? p[2] : 0.0;
+        rec[6] = phase; rec[7] = f.E;
+        rec[8] = f.pinf[0]; rec[9] = f.pinf[1]; rec[10] = f.pinf[2];
+        rec[11] = f.phase;
+        O.status[col_idx] = failed ? 1 : (f.ionized ? 0 : 2);
+    } else {
+        Contrib c;
+        c.lin = lin;
+        c.re = cre;
+        c.im = cim;
+        c.prob = prob;
+        c.status = failed ? 1 : (f.ionized ? (inb ? 0 : 3) : 2);
+        c.pad = 0;
+        O.contrib[col_idx] = c;
+    }
+}
+
+__device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
+    const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
+    const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
+    if (lane_id() == 0) {
+        if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
+        if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+        if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
+        if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
+        if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
+    }
+}
+
+// Exclusive prefix sum of the segment counts (one workgroup):
+// pre[k] = sum_{j<k} count[j], pre[n] = total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* __restrict__ count, int64_t n,
+                                                       int64_t* __restrict__ pre) {
+    __shared__ int64_t part[1024];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t b = 0; b < n + 1; b += 1024) {
+        const int64_t i = b + threadIdx.x;
+        part[threadIdx.x] = (i < n) ? (int64_t)count[i] : 0;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+            int64_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int64_t excl = carry + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+        if (i <= n) pre[i] = excl;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += part[1023];
+        __syncthreads();
+    }
+}
+
+// item (0-based over all electrons of all segments) -> column index
+__device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t n_seg, const int64_t* seg_start) {
+    int64_t lo = 0, hi = n_seg;  // pre[lo] <= item < pre[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= item) lo = mid; else hi = mid;
+    }
+    return seg_start[lo] + (item - pre[lo]);
+}
+
+// Adaptive Tsit5, one electron per lane. A wavefront owns `per_wave`
+// consecutive items; a lane whose electron has finished takes the next item
+// of its wavefront (ballot + mbcnt), so short and long trajectories share
+// the wave without idling until the range is exhausted.
+template <int DIM, int PHASE, int OUT, int TGT>
+__global__ __launch_bounds__(256) void k_tsit5(IntegP P, OutP O, const double* __restrict__ init,
+                                               const int64_t* __restrict__ seg_start, const int64_t* __restrict__ pre,
+                                               int64_t n_seg, int per_wave) {
+    const int lane = lane_id();
+    const int64_t total = pre[n_seg];
+    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int64_t wbeg = gw * per_wave;
+    if (wbeg >= total) return;
+    const int64_t wend = (wbeg + per_wave < total) ? wbeg + per_wave : total;
+    const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
+    const double atol = P.atol;
+    LaneAcc acc;
+    Ts5Lane<DIM, PHASE> st;
+    int64_t item = wbeg + lane, next = wbeg + 64, col = 0;
+    bool valid = item < wend;
+    double t0 = 0.0, prob = 0.0;
+    auto start = [&]() {
+        col = item_column(item, pre, n_seg, seg_start);
+        const double* c = init + col * P.rows;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            st.u.v[i] = c[i];
+            st.u.v[DIM + i] = c[DIM + i];
+        }
+        t0 = c[2 * DIM];
+        prob = c[2 * DIM + 1];
+        if (PHASE != ETRAJ_PHASE_CTMC) st.u.v[2 * DIM] = c[2 * DIM + 2];
+        ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+    };
+    if (valid) start();
+    while (__ballot(valid) != 0) {
+        bool done = false;
+        if (valid) {
+            const int rc = ts5_attempt<DIM, PHASE, TGT>(P.L, P.T, saeq, P.rtol, atol, P.maxiters, st);
+            if (rc != 0) {
+                done = true;
+                bool failed = rc < 0;
+                double r[DIM], p[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    r[i] = st.u.v[i];
+                    p[i] = st.u.v[DIM + i];
+                    failed = failed || isnan(r[i]) || isnan(p[i]);
+                }
+                const double ph = (PHASE != ETRAJ_PHASE_CTMC) ? st.u.v[2 * DIM] : 0.0;
+                acc.steps += st.accepted;
+                emit_lane<DIM, PHASE, OUT>(P, O, col, failed, r, p, ph, t0, prob, acc);
+            }
+        }
+        const uint64_t m = __ballot(done);
+        if (m) {
+            const int pre_l = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (done) {
+                item = next + pre_l;
+                valid = item < wend;
+                if (valid) start();
+            }
+            next += __popcll(m);
+        }
+    }
+    if (OUT == OUT_PMD) flush_lane_acc(O, acc);
 }
 
 __global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, double* __restrict__ out) {
@@ -501,51 +718,111 @@ hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L,
 template <int DIM, int PHASE, bool MIXED, int OUT>
 static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
                                const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
-                               const FieldTab& FT) {
+                               const FieldTab& FT, uint8_t* flags, const int64_t* out_index) {
     int64_t waves = n_seg * (int64_t)cps;
     unsigned blocks = (unsigned)((waves + 3) / 4);
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
         hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st,
-                           P, O, init, seg_start, seg_count, n_seg, cps, FT);
+                           P, O, init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
     else
         hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O,
-                           init, seg_start, seg_count, n_seg, cps, FT);
+                           init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
     return hipGetLastError();
 }
 
 template <int DIM, int PHASE, bool MIXED>
 static hipError_t launch_rk4_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT,
+                               uint8_t* fl, const int64_t* oi) {
     switch (out) {
-        case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps, FT);
-        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps, FT);
-        default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps, FT);
+        case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
+        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
+        default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
     }
 }
 
 template <int DIM, int PHASE>
 static hipError_t launch_rk4_m(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
-    return P.mixed ? launch_rk4_o<DIM, PHASE, true>(st, P, O, out, init, ss, sc, ns, cps, FT)
-                   : launch_rk4_o<DIM, PHASE, false>(st, P, O, out, init, ss, sc, ns, cps, FT);
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT,
+                               uint8_t* fl, const int64_t* oi) {
+    return P.mixed ? launch_rk4_o<DIM, PHASE, true>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi)
+                   : launch_rk4_o<DIM, PHASE, false>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
 }
 
 template <int DIM>
 static hipError_t launch_rk4_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT) {
+                               const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT,
+                               uint8_t* fl, const int64_t* oi) {
     switch (P.phase) {
-        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps, FT);
-        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps, FT);
-        default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps, FT);
+        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
+        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
+        default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
     }
 }
 
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
-                      const FieldTab& FT) {
+                      const FieldTab& FT, uint8_t* flags, const int64_t* out_index) {
     if (n_seg <= 0 || cps <= 0) return hipSuccess;
-    return P.dim == 3 ? launch_rk4_p<3>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT)
-                      : launch_rk4_p<2>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT);
+    return P.dim == 3 ? launch_rk4_p<3>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT, flags, out_index)
+                      : launch_rk4_p<2>(st, P, O, out, init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
+}
+
+hipError_t launch_compact_flagged(hipStream_t st, double* init, int rows, const int64_t* seg_start,
+                                  const int32_t* seg_count, int64_t n_seg, const uint8_t* flags,
+                                  const int64_t* out_index, int32_t* count_out, int64_t* orig) {
+    if (n_seg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_flagged, dim3((unsigned)n_seg), dim3(256), 0, st, init, rows, seg_start, seg_count,
+                       flags, out_index, count_out, orig);
+    return hipGetLastError();
+}
+
+template <int DIM, int PHASE, int OUT>
+static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                                 const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                 int per_wave) {
+    const int64_t waves = (max_items + per_wave - 1) / per_wave;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st, P, O,
+                           init, seg_start, pre, n_seg, per_wave);
+    else
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O, init,
+                           seg_start, pre, n_seg, per_wave);
+    return hipGetLastError();
+}
+
+template <int DIM, int PHASE>
+static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+    switch (out) {
+        case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD>(st, P, O, init, ss, pre, ns, mi, pw);
+        case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS>(st, P, O, init, ss, pre, ns, mi, pw);
+        default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB>(st, P, O, init, ss, pre, ns, mi, pw);
+    }
+}
+
+template <int DIM>
+static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+    switch (P.phase) {
+        case ETRAJ_PHASE_CTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
+        case ETRAJ_PHASE_QTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
+        default: return launch_tsit5_o<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, pre, ns, mi, pw);
+    }
+}
+
+hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave) {
+    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
+    return P.dim == 3 ? launch_tsit5_p<3>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave)
+                      : launch_tsit5_p<2>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave);
+}
+
+hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, count, n, pre);
+    return hipGetLastError();
 }
 
 hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,

@@ -439,3 +439,197 @@ __host__ __device__ inline int64_t bin_index(double p, double pmax, int num) {
 }
 
 }  // namespace etraj
+
+namespace etraj {
+
+// ---------------------------------------------------------------------------
+// Adaptive Tsit5 (Tsitouras 2011) with the defaults OrdinaryDiffEq applies to
+// solve(prob, Tsit5(), adaptive=true, reltol=traj_rtol) (TrajectorySimulation.jl:342):
+// Hairer's initial step, PI controller beta1 = 7/50, beta2 = 2/25,
+// qmin = 1/5, qmax = 10, gamma = 9/10, qoldinit = 1e-4, RMS error norm of
+// utilde / (atol + max(|u|, |u_new|) rtol), NaN and maxiters checks.
+// The state vector is (r, p[, phase]); the phase takes part in the error norm
+// as in the reference's ODE.
+// ---------------------------------------------------------------------------
+namespace ts5 {
+constexpr double c2 = 0.161, c3 = 0.327, c4 = 0.9, c5 = 0.9800255409045097;
+constexpr double a21 = 0.161;
+constexpr double a31 = -0.008480655492356989, a32 = 0.335480655492357;
+constexpr double a41 = 2.897153057105493, a42 = -6.359448489975075, a43 = 4.3622954328695815;
+constexpr double a51 = 5.325864828439257, a52 = -11.748883564062828, a53 = 7.4955393428898365,
+                 a54 = -0.09249506636175525;
+constexpr double a61 = 5.86145544294642, a62 = -12.92096931784711, a63 = 8.159367898576159,
+                 a64 = -0.071584973281401, a65 = -0.028269050394068383;
+constexpr double a71 = 0.09646076681806523, a72 = 0.01, a73 = 0.4798896504144996, a74 = 1.379008574103742,
+                 a75 = -3.290069515436081, a76 = 2.324710524099774;
+constexpr double b1 = -0.00178001105222577714, b2 = -0.0008164344596567469, b3 = 0.007880878010261995,
+                 b4 = -0.1447110071732629, b5 = 0.5823571654525552, b6 = -0.45808210592918697,
+                 b7 = 0.015151515151515152;
+}  // namespace ts5
+
+template <int DIM, int PHASE>
+struct Vec {
+    static constexpr int N = 2 * DIM + (PHASE != ETRAJ_PHASE_CTMC ? 1 : 0);
+    double v[N];
+};
+
+template <int DIM, int PHASE, int TGT>
+__host__ __device__ inline void rhs_f(const TargetP& T, bool saeq, double Fx, double Fy, const Vec<DIM, PHASE>& u,
+                                      Vec<DIM, PHASE>& du) {
+    State<DIM, double> s, ds;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        s.r[i] = u.v[i];
+        s.p[i] = u.v[DIM + i];
+    }
+    double dphi, r2;
+    eom<DIM, PHASE, TGT, double>(T, saeq, s, Fx, Fy, ds, dphi, r2);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        du.v[i] = ds.r[i];
+        du.v[DIM + i] = ds.p[i];
+    }
+    if (PHASE != ETRAJ_PHASE_CTMC) du.v[2 * DIM] = dphi;
+}
+
+template <int DIM, int PHASE, int TGT>
+__host__ __device__ inline void rhs_full(const LaserP& L, const TargetP& T, bool saeq, double t,
+                                         const Vec<DIM, PHASE>& u, Vec<DIM, PHASE>& du) {
+    double Fx, Fy;
+    laser_field(L, t, Fx, Fy);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx, Fy, u, du);
+}
+
+template <int N>
+__host__ __device__ inline double rms(const double* x) {
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += x[i] * x[i];
+    return sqrt(s / N);
+}
+
+// Per-electron Tsit5 integration state, advanced one attempted step at a
+// time so that a wavefront can refill lanes whose electron has finished.
+template <int DIM, int PHASE>
+struct Ts5Lane {
+    Vec<DIM, PHASE> u, k1;
+    double t, tf, dt, dtmax, dtmin, qold, q11;
+    int64_t accepted, attempted;
+};
+
+template <int DIM, int PHASE, int TGT>
+__host__ __device__ inline void ts5_init(const LaserP& L, const TargetP& T, bool saeq, double t0, double tf, double rtol,
+                                         double atol, Ts5Lane<DIM, PHASE>& s) {
+    using V = Vec<DIM, PHASE>;
+    constexpr int N = V::N;
+    s.t = t0;
+    s.tf = tf;
+    s.dtmax = tf - t0;
+    s.dtmin = fmax(fabs(t0), fabs(tf)) * 0x1.0p-52;
+    s.qold = 1e-4;
+    s.q11 = 1.0;
+    s.accepted = 0;
+    s.attempted = 0;
+    // initial step (Hairer & Wanner, Solving ODEs I, sec. II.4)
+    double sk[N], tmp[N];
+    const double dtmin0 = fmax(0.0, fabs(t0) * 0x1.0p-52);
+    const double smalldt = fmax(dtmin0, 1e-6);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        sk[i] = atol + fabs(s.u.v[i]) * rtol;
+        tmp[i] = s.u.v[i] / sk[i];
+    }
+    const double d0 = rms<N>(tmp);
+    rhs_full<DIM, PHASE, TGT>(L, T, saeq, t0, s.u, s.k1);
+#pragma unroll
+    for (int i = 0; i < N; ++i) tmp[i] = s.k1.v[i] / sk[i];
+    const double d1 = rms<N>(tmp);
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
+    dt0 = fmin(dt0, s.dtmax);
+    V y, k2;
+#pragma unroll
+    for (int i = 0; i < N; ++i) y.v[i] = s.u.v[i] + dt0 * s.k1.v[i];
+    rhs_full<DIM, PHASE, TGT>(L, T, saeq, t0 + dt0, y, k2);
+#pragma unroll
+    for (int i = 0; i < N; ++i) tmp[i] = (k2.v[i] - s.k1.v[i]) / sk[i];
+    const double d2 = rms<N>(tmp) / dt0;
+    const double mx = fmax(d1, d2);
+    const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(10.0, -(2 + log10(mx)) / 5.0);
+    s.dt = fmax(dtmin0, fmin(fmin(100 * dt0, dt1), s.dtmax));
+}
+
+// One attempted step. Returns 0 while running, 1 when t reached tf, -1 on
+// failure (maxiters, NaN, dt below dtmin).
+template <int DIM, int PHASE, int TGT>
+__host__ __device__ inline int ts5_attempt(const LaserP& L, const TargetP& T, bool saeq, double rtol, double atol,
+                                           int64_t maxiters, Ts5Lane<DIM, PHASE>& s) {
+    using V = Vec<DIM, PHASE>;
+    constexpr int N = V::N;
+    double dt = s.dt;
+    const double t = s.t;
+    if (s.tf - t < dt) dt = s.tf - t;
+    if (++s.attempted > maxiters) return -1;
+    // the five distinct stage times t + c_i dt (c6 = c7 = 1): one field loop
+    double Fx[5], Fy[5];
+    const double cs[5] = {ts5::c2, ts5::c3, ts5::c4, ts5::c5, 1.0};
+#pragma unroll 1
+    for (int j = 0; j < 5; ++j) laser_field(L, t + cs[j] * dt, Fx[j], Fy[j]);
+    V y, k2, k3, k4, k5, k6, k7, un;
+    const V& u = s.u;
+    const V& k1 = s.k1;
+#pragma unroll
+    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a21 * k1.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[0], Fy[0], y, k2);
+#pragma unroll
+    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a31 * k1.v[i] + ts5::a32 * k2.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[1], Fy[1], y, k3);
+#pragma unroll
+    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a41 * k1.v[i] + ts5::a42 * k2.v[i] + ts5::a43 * k3.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[2], Fy[2], y, k4);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        y.v[i] = u.v[i] + dt * (ts5::a51 * k1.v[i] + ts5::a52 * k2.v[i] + ts5::a53 * k3.v[i] + ts5::a54 * k4.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[3], Fy[3], y, k5);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        y.v[i] = u.v[i] + dt * (ts5::a61 * k1.v[i] + ts5::a62 * k2.v[i] + ts5::a63 * k3.v[i] + ts5::a64 * k4.v[i] +
+                                ts5::a65 * k5.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[4], Fy[4], y, k6);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        un.v[i] = u.v[i] + dt * (ts5::a71 * k1.v[i] + ts5::a72 * k2.v[i] + ts5::a73 * k3.v[i] + ts5::a74 * k4.v[i] +
+                                 ts5::a75 * k5.v[i] + ts5::a76 * k6.v[i]);
+    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[4], Fy[4], un, k7);
+    double tmp[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double ut = dt * (ts5::b1 * k1.v[i] + ts5::b2 * k2.v[i] + ts5::b3 * k3.v[i] + ts5::b4 * k4.v[i] +
+                                ts5::b5 * k5.v[i] + ts5::b6 * k6.v[i] + ts5::b7 * k7.v[i]);
+        tmp[i] = ut / (atol + fmax(fabs(u.v[i]), fabs(un.v[i])) * rtol);
+    }
+    const double EEst = rms<N>(tmp);
+    if (isnan(EEst)) return -1;
+    double q;
+    if (EEst == 0) {
+        q = 1.0 / 10.0;
+    } else {
+        s.q11 = pow(EEst, 7.0 / 50.0);
+        q = s.q11 / pow(s.qold, 2.0 / 25.0);
+        q = fmax(1.0 / 10.0, fmin(1.0 / 0.2, q / 0.9));
+    }
+    if (EEst <= 1.0) {
+        s.t = (s.tf - t == dt) ? s.tf : t + dt;
+        s.u = un;
+        s.k1 = k7;
+        ++s.accepted;
+        s.qold = fmax(EEst, 1e-4);
+        s.dt = fmin(dt / q, s.dtmax);
+    } else {
+        s.dt = dt / fmin(1.0 / 0.2, s.q11 / 0.9);
+    }
+    if (!(s.t < s.tf)) return 1;
+    if (fabs(s.dt) < s.dtmin) return -1;
+    return 0;
+}
+
+}  // namespace etraj

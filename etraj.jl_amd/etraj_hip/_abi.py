@@ -74,7 +74,7 @@ class RunDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("ion_prob", C.c_double), ("ion_prob_uncollected", C.c_double),
                 ("num_effective_traj", C.c_int64), ("n_failed", C.c_int64), ("n_bound", C.c_int64),
-                ("n_restep", C.c_int64), ("n_steps", C.c_int64), ("ms_sample", C.c_double),
+                ("n_restep", C.c_int64), ("n_steps", C.c_int64), ("n_steps_fp64", C.c_int64), ("ms_sample", C.c_double),
                 ("ms_integrate", C.c_double), ("ms_total", C.c_double)]
 
     def as_dict(self) -> dict:

@@ -29,7 +29,7 @@ from .targets import Serialize as _ser_target
 
 
 def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num, traj_integrator="rk4",
-             traj_dt=0.1, traj_rtol=1e-6, precision="mixed", restep_radius=2.0, reproduce_quirks=True, **_):
+             traj_dt=0.1, traj_rtol=1e-6, precision=None, restep_radius=2.0, reproduce_quirks=True, **_):
     d = _abi.RunDesc()
     d.dimension = int(dimension)
     d.phase_method = _abi.PHASE_METHODS[_sym(traj_phase_method)]
@@ -37,6 +37,8 @@ def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p
     if integ not in ("rk4", "tsit5"):
         raise ValueError(f"unknown traj_integrator {traj_integrator}")
     d.integrator = _abi.INTEG_RK4 if integ == "rk4" else _abi.INTEG_TSIT5
+    if precision is None:   # mixed for the fixed-step RK4, FP64 for Tsit5 (its only form)
+        precision = "mixed" if integ == "rk4" else "fp64"
     prec = _sym(precision).lower()
     if prec not in ("mixed", "fp64"):
         raise ValueError(f"unknown precision {precision}")
@@ -71,7 +73,7 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
                             traj_rtol=1e-6, output_fmt="npz", output_compress=True, output_path=None,
                             sample_monte_carlo=False, sample_cutoff_limit=1e-16, rate_prefix="Full",
                             mol_orbit_ridx=0, show_progress=False, traj_integrator="rk4", traj_dt=0.1,
-                            precision="mixed", restep_radius=2.0, granularity="job", seed=0,
+                            precision=None, restep_radius=2.0, granularity="job", seed=0,
                             sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, reproduce_quirks=True,
                             backend="hip"):
     if _sym(backend) != "hip":

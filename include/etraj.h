@@ -128,8 +128,10 @@ typedef struct etraj_sampler_desc {
 enum { ETRAJ_INTEG_RK4 = 0, ETRAJ_INTEG_TSIT5 = 1 };
 enum {
     ETRAJ_PREC_FP64  = 0,   /* every operation in FP64                                   */
-    ETRAJ_PREC_MIXED = 1    /* FP32 EOM, Kahan FP64 phase, FP64 re-step when r < restep_radius,
-                               FP64 Kepler asymptote and FP64 PMD sums                   */
+    ETRAJ_PREC_MIXED = 1    /* RK4 only: FP32 EOM with Kahan-compensated state, Kahan FP64
+                               phase; electrons that start any step within restep_radius of
+                               the core are integrated again from t0 in FP64; FP64 Kepler
+                               asymptote and FP64 PMD sums                                 */
 };
 /* quirk bits: set = reproduce the reference's behaviour (default) */
 enum {
@@ -147,11 +149,11 @@ typedef struct etraj_run_desc {
     double  dt;                /* RK4: nominal step; the step is (tf-t0)/ceil((tf-t0)/dt)  */
     double  rtol;              /* Tsit5: traj_rtol                                         */
     double  atol;              /* Tsit5: absolute tolerance (OrdinaryDiffEq default 1e-6)  */
-    double  restep_radius;     /* MIXED: FP64 re-step below this radius (a.u.)             */
+    double  restep_radius;     /* MIXED: FP64 re-integration radius (a.u.)                 */
     double  final_p_max[3];
     int32_t final_p_num[3];
     int32_t quirks;            /* ETRAJ_QUIRK_* bits                                       */
-    int64_t maxiters;          /* Tsit5 step limit (OrdinaryDiffEq default 100000)          */
+    int64_t maxiters;          /* Tsit5 step-attempt limit (<= 0: 1000000)                  */
 } etraj_run_desc;
 
 typedef struct etraj_stats {
@@ -160,8 +162,9 @@ typedef struct etraj_stats {
     int64_t num_effective_traj;     /* launched trajectories, failed included (:399) */
     int64_t n_failed;               /* integrator failures (skipped)        */
     int64_t n_bound;                /* E < 0 at the end (dropped)           */
-    int64_t n_restep;               /* FP64 re-steps taken (MIXED)           */
-    int64_t n_steps;                /* integrator steps taken (all lanes)    */
+    int64_t n_restep;               /* MIXED: electrons re-integrated in FP64 */
+    int64_t n_steps;                /* integrator steps, all passes           */
+    int64_t n_steps_fp64;           /* of which steps in FP64 (MIXED: the re-integration pass) */
     double  ms_sample;              /* device time of the sampler stage      */
     double  ms_integrate;           /* device time of integrate+phase+bin    */
     double  ms_total;               /* wall time of the call                 */

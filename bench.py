@@ -114,6 +114,36 @@ def cpu_baseline(kw, ctx, n_batches_sample, threads):
     }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_rk4_h_lp_scts_strat_1e8_pmc.md")
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the FP32 pass, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (the json block of the profile file), or None if that file is absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["traffic_bytes_per_launch"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(mib):
+    """One hipMemcpy device-to-device of `mib` MiB: a copy kernel of known traffic (mib MiB read +
+    mib MiB written) whose FETCH_SIZE / WRITE_SIZE counters scale the integrator's."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0          # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +153,8 @@ def main():
     ap.add_argument("--n-k", type=int, default=32000)
     ap.add_argument("--cpu-batches", type=int, default=24, help="strata in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
@@ -130,6 +162,8 @@ def main():
         sys.exit(2)
 
     ctx = _abi.default_context()
+    if args.calibrate_copy > 0:
+        copy_calibration(args.calibrate_copy)
     kw = workloads.h_lp_scts_stratified(n_t=args.n_t, n_k=args.n_k, precision="mixed")
     d_mixed = descs(kw)
     npix = kw["final_p_num"][0] * kw["final_p_num"][1]
@@ -137,12 +171,14 @@ def main():
     for i in range(args.warmup):
         st = run_device(ctx, d_mixed)
         log(f"warmup {i}: {st.ms_total:.1f} ms, {st.num_effective_traj} electrons")
-    ms, ms_int, steps, steps64, restep, n_el = [], [], 0, 0, 0, 0
+    ms, ms_int, ms_main, steps, steps64, restep, n_el, n_slabs = [], [], [], 0, 0, 0, 0, 1
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = run_device(ctx, d_mixed)
         ms.append(st.ms_total)
         ms_int.append(st.ms_integrate)
+        ms_main.append(st.ms_integrate_main)
+        n_slabs = st.n_slabs
         steps += st.n_steps
         steps64 += st.n_steps_fp64
         restep += st.n_restep
@@ -153,9 +189,12 @@ def main():
     ms_step = wall * 1e3 / args.steps
     value = n_el / (ms_step / 1e3)
 
-    # FP32 pass: every electron; FP64 pass: the electrons that came within restep_radius
-    flops = (FLOPS_PER_STEP["mixed"] * (steps - steps64) + FLOPS_PER_STEP["fp64"] * steps64) / args.steps
-    achieved = flops / (np.mean(ms_int) / 1e3) / 1e12
+    # Dominant kernel: the FP32 pass (every electron, one launch per slab), timed with HIP events on the
+    # ctx stream it runs on. The FP64 pass re-integrates the electrons that came within restep_radius.
+    flops32 = FLOPS_PER_STEP["mixed"] * (steps - steps64) / args.steps
+    flops_all = flops32 + FLOPS_PER_STEP["fp64"] * steps64 / args.steps
+    launch_ms = float(np.mean(ms_main)) / n_slabs
+    achieved = flops32 / n_slabs / (launch_ms / 1e3) / 1e12
     result = {
         "metric": "trajectories/s + PMD wall time, mixed-prec vs FP64 GPU at PMD L1<=1e-3",
         "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -171,9 +210,12 @@ def main():
         "integrate_kernel_ms": float(np.mean(ms_int)),
         "fp64_reintegrated_fraction": restep / max(n_el * args.steps, 1),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["mixed"], "traffic": None,
+                     "frac": achieved / PEAK_TFLOPS["mixed"], "traffic": traffic_per_launch(),
+                     "kernel": "k_rk4 FP32 pass", "launches_per_job": n_slabs, "avg_launch_ms": launch_ms,
+                     "flops_per_launch": flops32 / n_slabs,
                      "flops_per_step": {"fp32_pass": FLOPS_PER_STEP["mixed"], "fp64": FLOPS_PER_STEP["fp64"]},
-                     "rk4_steps_per_job": steps // args.steps, "fp64_steps_per_job": steps64 // args.steps},
+                     "rk4_steps_per_job": steps // args.steps, "fp64_steps_per_job": steps64 // args.steps,
+                     "both_passes_tflops": flops_all / (np.mean(ms_int) / 1e3) / 1e12},
     }
     if not args.no_fp64:
         d64 = descs(dict(kw, precision="fp64"))

@@ -55,7 +55,7 @@ struct etraj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0;
+    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat;
     DevBuf<float4> ft32;
     DevBuf<double2> ft64, ft0;
     DevBuf<int64_t> seg_start, cnt, pre;
@@ -145,8 +145,10 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
                  const std::vector<double>& t, SamplerP& S) {
     if (!sd) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
     const etraj_sampler_desc& s = *sd;
-    if (s.init_method != ETRAJ_INIT_ADK)
-        return fail(c, ETRAJ_EUNSUPPORTED, "only init_cond_method = :ADK is implemented by the engine");
+    if (s.init_method != ETRAJ_INIT_ADK && s.init_method != ETRAJ_INIT_WFAT)
+        return fail(c, ETRAJ_EUNSUPPORTED, "init_cond_method: the engine implements :ADK and :WFAT");
+    if (s.init_method == ETRAJ_INIT_WFAT && (s.wfat_na < 1 || s.wfat_nb < 1 || !s.wfat_coeff))
+        return fail(c, ETRAJ_EINVAL, "[WFATSampler] the structure-factor table (wfat_na, wfat_nb, wfat_coeff) is missing");
     if (s.phase_method < 0 || s.phase_method > 2) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Undefined phase method");
     if (s.dimension != 2 && s.dimension != 3) return fail(c, ETRAJ_EINVAL, "`dimension` must be either 2 or 3");
     if (s.rate_prefix & ~(ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC | ETRAJ_PREFIX_JAC))
@@ -218,6 +220,16 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     S.nstar = nn;
     S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
     S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
+    S.method = s.init_method;
+    if (s.init_method == ETRAJ_INIT_WFAT) {
+        S.wfat_na = s.wfat_na;
+        S.wfat_nb = s.wfat_nb;
+        for (int k = 0; k < 3; ++k) S.mol_rot[k] = s.mol_rot[k];
+        const size_t nc = (size_t)2 * s.wfat_na * s.wfat_nb;
+        HIP_TRY(c, c->wfat.ensure(nc));
+        HIP_TRY(c, hipMemcpyAsync(c->wfat.p, s.wfat_coeff, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+        S.wfat_coeff = c->wfat.p;
+    }
     return ETRAJ_OK;
 }
 
@@ -294,9 +306,10 @@ int fill_field_table(etraj_ctx* c, const IntegP& P, const double* t0s_dev, int64
 // compacted in place per segment and integrated again in FP64. `ncols` is the
 // number of columns the init buffer spans (for the flag array).
 int rk4_passes(etraj_ctx* c, const IntegP& P, const OutP& O, int out, const int64_t* seg_start, int32_t* counts,
-               int64_t nseg, int cps, const FieldTab& FT, int64_t ncols) {
+               int64_t nseg, int cps, const FieldTab& FT, int64_t ncols, hipEvent_t after_main = nullptr) {
     if (!P.mixed) {
         HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, nullptr, nullptr));
+        if (after_main) HIP_TRY(c, hipEventRecord(after_main, c->stream));
         return ETRAJ_OK;
     }
     HIP_TRY(c, c->flags.ensure((size_t)ncols));
@@ -304,6 +317,7 @@ int rk4_passes(etraj_ctx* c, const IntegP& P, const OutP& O, int out, const int6
     const bool need_orig = out != OUT_PMD;
     if (need_orig) HIP_TRY(c, c->orig.ensure((size_t)ncols));
     HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, c->flags.p, nullptr));
+    if (after_main) HIP_TRY(c, hipEventRecord(after_main, c->stream));
     HIP_TRY(c, launch_compact_flagged(c->stream, c->init.p, P.rows, seg_start, counts, nseg, c->flags.p, nullptr,
                                       c->counts2.p, need_orig ? c->orig.p : nullptr));
     IntegP P64 = P;
@@ -425,7 +439,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
-    c->counts2.release(); c->flags.release(); c->orig.release();
+    c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -494,10 +508,11 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     if (int rc = check_laser(c, laser, L)) return rc;
     if (int rc = check_target(c, target, T)) return rc;
     if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
-    if (target->l != 0)
+    if (target->l != 0 && sampler->init_method == ETRAJ_INIT_ADK)
         return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
     std::vector<double> t;
     if (int rc = batch_times_of(c, *sampler, t)) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
     if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
     if (batch_id < 1 || batch_id > S.t_num) return fail(c, ETRAJ_EINVAL, "batch_id out of range");
     if (!count || (capacity > 0 && !out)) return fail(c, ETRAJ_EINVAL, "NULL output");
@@ -603,10 +618,11 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
     if (int rc = check_laser(c, laser, L)) return rc;
     if (int rc = check_target(c, target, T)) return rc;
     if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
-    if (target->l != 0)
+    if (target->l != 0 && sampler->init_method == ETRAJ_INIT_ADK)
         return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
     std::vector<double> t;
     if (int rc = batch_times_of(c, *sampler, t)) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
     if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
     if (int rc = make_integ(c, run, L, T, P)) return rc;
     if (sampler->dimension != run->dimension || sampler->phase_method != run->phase_method)
@@ -650,7 +666,8 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
     O.scal = c->scal.p;
     O.cnt = c->cnt.p;
     const int nslab = (int)((S.t_num + per_slab - 1) / per_slab);
-    std::vector<hipEvent_t> ev(3 * nslab, nullptr);
+    // per slab: start, sampler done, main integrator kernel done, slab done
+    std::vector<hipEvent_t> ev(4 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
     int64_t launched = 0;
     std::vector<int32_t> hcounts(per_slab);
@@ -660,7 +677,7 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         const int nb = (int)std::min<int64_t>(per_slab, S.t_num - b0);
         FieldTab FT{};
         const bool rk4 = P.integrator == ETRAJ_INTEG_RK4;
-        hipError_t e = hipEventRecord(ev[3 * k], c->stream);
+        hipError_t e = hipEventRecord(ev[4 * k], c->stream);
         if (e == hipSuccess && rk4) {
             if (int frc = fill_field_table(c, P, c->t0s.p + b0, nb, stride, FT)) {
                 rc = frc;
@@ -668,9 +685,10 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
             }
         }
         if (e == hipSuccess) e = launch_adk_sample(c->stream, S, L, T, c->t0s.p, b0, nb, c->init.p, c->counts.p);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 1], c->stream);
         if (e == hipSuccess && rk4) {
-            if (int prc = rk4_passes(c, P, O, OUT_PMD, c->seg_start.p, c->counts.p, nb, cps, FT, (int64_t)nb * max_bs)) {
+            if (int prc = rk4_passes(c, P, O, OUT_PMD, c->seg_start.p, c->counts.p, nb, cps, FT, (int64_t)nb * max_bs,
+                                     ev[4 * k + 2])) {
                 rc = prc;
                 break;
             }
@@ -680,8 +698,9 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
             if (e == hipSuccess)
                 e = launch_tsit5(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->pre.p, nb, (int64_t)nb * max_bs,
                                  kTsit5PerWave);
+            if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 2], c->stream);
         }
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 3], c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(hcounts.data(), c->counts.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -691,14 +710,16 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         }
         for (int i = 0; i < nb; ++i) launched += hcounts[i];
     }
-    double ms_s = 0, ms_i = 0;
+    double ms_s = 0, ms_i = 0, ms_m = 0;
     if (rc == ETRAJ_OK) {
         for (int k = 0; k < nslab; ++k) {
-            float a = 0, b = 0;
-            (void)hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
-            (void)hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+            float a = 0, b = 0, m = 0;
+            (void)hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]);
+            (void)hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 3]);
+            (void)hipEventElapsedTime(&m, ev[4 * k + 1], ev[4 * k + 2]);
             ms_s += a;
             ms_i += b;
+            ms_m += m;
         }
     }
     for (auto& e : ev)
@@ -729,6 +750,8 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
         stats->n_steps_fp64 = cnt[CNT_STEPS64];
         stats->ms_sample = ms_s;
         stats->ms_integrate = ms_i;
+        stats->ms_integrate_main = ms_m;
+        stats->n_slabs = nslab;
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     }

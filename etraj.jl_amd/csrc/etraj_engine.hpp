@@ -24,6 +24,10 @@ struct SamplerP {
     double numer;      // C * D^0_00 * Y_00 (:255)
     double nstar;      // n* = Z/kappa (:232)
     uint64_t seed;
+    int method;        // ETRAJ_INIT_ADK / ETRAJ_INIT_WFAT
+    int wfat_na, wfat_nb;
+    double mol_rot[3];
+    const double* wfat_coeff;  // device copy of the G00 table (re, im), a-major
 };
 
 struct IntegP {

@@ -90,7 +90,56 @@ __global__ void k_target_eval(TargetP T, const double* __restrict__ xyz, int64_t
 // ---------------------------------------------------------------------------
 struct AdkBatch {
     double tr, F, cphi, sphi, c_cc, dkdt;
+    double g2;   // WFAT: |G00(beta, gamma)|^2 of this batch's field direction
 };
+
+// params(RotZYZ(MF^-1 FF)) (shared_methods.jl:31-38, Rotations.jl ZYZ
+// decomposition with beta in [0, pi]); returns (beta, gamma).
+__device__ inline void ff_mf_euler(const double* mr, double Fx, double Fy, double& beta, double& gamma) {
+    const double F = hypot(Fx, Fy);
+    const double FF[3][3] = {{0, Fy / F, Fx / F}, {0, -Fx / F, Fy / F}, {1, 0, 0}};
+    double sa, ca, sb, cb, sc, cc;
+    sincos(mr[0], &sa, &ca);
+    sincos(mr[1], &sb, &cb);
+    sincos(mr[2], &sc, &cc);
+    // MF = Rz(a) Ry(b) Rz(c)
+    const double MF[3][3] = {{ca * cb * cc - sa * sc, -ca * cb * sc - sa * cc, ca * sb},
+                             {sa * cb * cc + ca * sc, -sa * cb * sc + ca * cc, sa * sb},
+                             {-sb * cc, sb * sc, cb}};
+    double R[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) R[i][j] = MF[0][i] * FF[0][j] + MF[1][i] * FF[1][j] + MF[2][i] * FF[2][j];
+    const double t1 = atan2(R[1][2], R[0][2]);
+    double st, ct;
+    sincos(t1, &st, &ct);
+    beta = atan2(ct * R[0][2] + st * R[1][2], R[2][2]);
+    gamma = atan2(-st * R[0][0] + ct * R[1][0], -st * R[0][1] + ct * R[1][1]);
+}
+
+// |G00|^2 with G00 = sum_{a,b} c_ab P_a(cos beta) e^{-i b gamma}
+// (Legendre recurrence in a, complex power recurrence in b)
+__device__ inline double wfat_g2(const SamplerP& S, double beta, double gamma) {
+    const double x = cos(beta);
+    double sg, cg;
+    sincos(gamma, &sg, &cg);
+    double gre = 0.0, gim = 0.0, pm1 = 0.0, p = 1.0;
+    for (int a = 0; a < S.wfat_na; ++a) {
+        if (a == 1) { pm1 = 1.0; p = x; }
+        else if (a > 1) { const double pn = ((2 * a - 1) * x * p - (a - 1) * pm1) / a; pm1 = p; p = pn; }
+        double er = 1.0, ei = 0.0;   // e^{-i b gamma}
+        for (int b = 0; b < S.wfat_nb; ++b) {
+            const double* c = S.wfat_coeff + 2 * (a * S.wfat_nb + b);
+            gre += p * (c[0] * er - c[1] * ei);
+            gim += p * (c[0] * ei + c[1] * er);
+            const double nr = er * cg + ei * sg, ni = ei * cg - er * sg;
+            er = nr;
+            ei = ni;
+        }
+    }
+    return gre * gre + gim * gim;
+}
 
 __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, const TargetP& T, double tr) {
     AdkBatch B;
@@ -106,10 +155,13 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     double kappa = sqrt(2 * Ip), nn = T.Zd / kappa;
     const double e0 = 2.71828182845904523;
     B.c_cc = pow(2.0, 3 * nn / 2 + 1) * pow(kappa, 5 * nn + 0.5) * pow(B.F, -nn) * pow(1 + 2 * gam / e0, -nn);  // :235
-    if (S.sampling == ETRAJ_SAMPLING_STRATIFIED)
-        B.dkdt = (S.t_hi - S.t_lo) / S.t_num / (double)S.nk;
-    else
-        B.dkdt = S.dkdt_base;
+    B.dkdt = S.dkdt_base;
+    B.g2 = 0.0;
+    if (S.method == ETRAJ_INIT_WFAT) {
+        double beta, gamma;
+        ff_mf_euler(S.mol_rot, Fx, Fy, beta, gamma);   // WFATSampler.jl:143-145
+        B.g2 = wfat_g2(S, beta, gamma);                // :151-155
+    }
     return B;
 }
 
@@ -142,17 +194,28 @@ __device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const Adk
             }
         }
     }
-    if (fabs(kd0) < 1e-4) return false;                           // :312,:321
     const double Ip = T.Ip;
+    double rate;
+    if (S.method == ETRAJ_INIT_WFAT) {
+        if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) return false;    // WFATSampler.jl:188
+        // W(0, 0, kd, kz) (WFATSampler.jl:157)
+        const double kappa = sqrt(2 * Ip), k2 = kappa * kappa;
+        const double W = (k2 / 2 / B.F / 1.0) * pow(4 * k2 / B.F, 2 * S.nstar - 1) *
+                         exp(-2 * pow(k2 + kd0 * kd0 + kz0 * kz0, 1.5) / (3 * B.F));
+        rate = B.g2 * W * w;
+    } else {
+        if (fabs(kd0) < 1e-4) return false;                           // ADKSampler.jl:312,:321
+        const double kx = kd0 * -B.sphi, ky = kd0 * B.cphi;
+        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * B.F));  // :198
+        if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
+            double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
+            amp *= cpre * S.numer / pow((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
+        }
+        if (S.prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(B.F);             // :266,:283
+        rate = amp * amp;
+    }
     double kx0 = kd0 * -B.sphi, ky0 = kd0 * B.cphi;                // :324-325
     double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / B.F;          // :326
-    double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * B.F));  // :198
-    if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
-        double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
-        amp *= cpre * S.numer / pow((kx0 * kx0 + ky0 * ky0 + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
-    }
-    if (S.prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(B.F);             // :266,:283
-    double rate = amp * amp;
     if (isnan(rate) || rate < S.cutoff) return false;              // :332
     if (S.dim == 3) {
         col[0] = r0 * B.cphi; col[1] = r0 * B.sphi; col[2] = 0.0;
@@ -328,7 +391,8 @@ __global__ __launch_bounds__(256) void k_field_table(LaserP L, const double* __r
 // FT.f32/f64 + s * FT.stride, read with wave-uniform (scalar) loads.
 //
 // MIXED: the electron state is FP32 with a Kahan compensation term per
-// component and the phase is summed in FP64 (Kahan). An electron that starts
+// component; phase increments are Kahan-summed in FP32 over 32 steps and the
+// partial sums Kahan-summed in FP64 (the SCTS phase reaches 1e3 rad). An electron that starts
 // any step within restep_radius of the core is not binned: flags[col] = 1
 // marks it for the FP64 pass (k_compact_flagged + this kernel with
 // MIXED = false), which integrates it again from t0. Errors a close encounter
@@ -399,6 +463,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
         }
         phase = e.phase0;
         double phc = 0.0;
+        float qs = 0.0f, qc = 0.0f;
         const float hf = (float)h;
         const float rc2 = (float)(P.restep_radius * P.restep_radius);
         float Fpx = (float)F0.x, Fpy = (float)F0.y;
@@ -419,10 +484,20 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
                 uc.p[i] = (t - us.p[i]) - y;
                 us.p[i] = t;
             }
-            const double y = (double)dph - phc;   // Kahan-compensated FP64 phase sum
-            const double t = phase + y;
-            phc = (t - phase) - y;
-            phase = t;
+            {   // FP32 Kahan partial sum of the phase over <= 32 steps ...
+                const float y = dph - qc;
+                const float t = qs + y;
+                qc = (t - qs) - y;
+                qs = t;
+            }
+            if ((s & 31) == 31 || s == n - 1) {  // ... flushed into the FP64 Kahan sum
+                const double y = ((double)qs - (double)qc) - phc;
+                const double t = phase + y;
+                phc = (t - phase) - y;
+                phase = t;
+                qs = 0.0f;
+                qc = 0.0f;
+            }
             Fpx = f.z;
             Fpy = f.w;
         }

@@ -11,10 +11,11 @@ from .lasers import (Laser, MonochromaticLaser, Cos2Laser, Cos4Laser, GaussianLa
                      LaserFy, LaserAx, LaserAy, LaserF0, LaserA0, PeakInt, WaveLen, CycNum, SpreadCycNum, Ellipticity,
                      Azimuth, AngFreq, Period, CEP, TimeShift, KeldyshParameter, SpreadDuration, FWHM_Duration, Laser1,
                      Laser2, Delay21)
-from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, get_atom, get_available_atoms,
+from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, MoleculeBase, TabulatedMolecule, MolRotation,
+                      synthetic_g00_table, get_atom, get_available_atoms,
                       hartree_asymp_coeff, IonPotential, AsympNuclCharge, AngularQuantumNumber, MagneticQuantumNumber,
                       AsympCoeff, SoftCore, TargetName, QuantizationAxisOrientaion, TargetPotential, TargetForce)
-from .samplers import ElectronSampler, ADKSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
+from .samplers import ElectronSampler, ADKSampler, WFATSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
 from .simulation import perform_traj_simulation, run_desc
 
 __all__ = [n for n in dir() if not n.startswith("_")]

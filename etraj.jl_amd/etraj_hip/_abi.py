@@ -21,7 +21,7 @@ OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY = 0, -1, -2, -3, -4, -5
 
 LASER_COS2, LASER_COS4, LASER_GAUSSIAN = 1, 2, 3
 TARGET_HYDROGEN_LIKE, TARGET_SAE = 1, 2
-INIT_ADK = 1
+INIT_ADK, INIT_WFAT = 1, 4
 PHASE_CTMC, PHASE_QTMC, PHASE_SCTS = 0, 1, 2
 PREFIX_PRE, PREFIX_PRECC, PREFIX_JAC = 1, 2, 4
 SAMPLING_STEP, SAMPLING_MONTECARLO, SAMPLING_STRATIFIED = 0, 1, 2
@@ -60,7 +60,8 @@ class SamplerDesc(C.Structure):
                 ("ss_kd_max", C.c_double), ("ss_kz_max", C.c_double),
                 ("mc_kd_max", C.c_double), ("mc_kz_max", C.c_double),
                 ("kd_sigma", C.c_double), ("kz_sigma", C.c_double),
-                ("cutoff_limit", C.c_double), ("seed", C.c_uint64)]
+                ("cutoff_limit", C.c_double), ("seed", C.c_uint64), ("mol_rot", C.c_double * 3),
+                ("wfat_na", C.c_int32), ("wfat_nb", C.c_int32), ("wfat_coeff", C.POINTER(C.c_double))]
 
 
 class RunDesc(C.Structure):
@@ -74,8 +75,10 @@ class RunDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("ion_prob", C.c_double), ("ion_prob_uncollected", C.c_double),
                 ("num_effective_traj", C.c_int64), ("n_failed", C.c_int64), ("n_bound", C.c_int64),
-                ("n_restep", C.c_int64), ("n_steps", C.c_int64), ("n_steps_fp64", C.c_int64), ("ms_sample", C.c_double),
-                ("ms_integrate", C.c_double), ("ms_total", C.c_double)]
+                ("n_restep", C.c_int64), ("n_steps", C.c_int64), ("n_steps_fp64", C.c_int64),
+                ("ms_sample", C.c_double),
+                ("ms_integrate", C.c_double), ("ms_integrate_main", C.c_double), ("ms_total", C.c_double),
+                ("n_slabs", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -1,5 +1,5 @@
 """ElectronSamplers: init_sampler / batch_num / gen_electron_batch
-(src/ElectronSamplers/ElectronSamplers.jl:9-26, ADKSampler.jl).
+(src/ElectronSamplers/ElectronSamplers.jl:9-26, ADKSampler.jl, WFATSampler.jl).
 
 `gen_electron_batch` runs the device sampler kernel and returns the same
 rows x N column-major matrix the reference returns (rows x,y,kx,ky,t0,rate
@@ -54,6 +54,57 @@ def resolve_rate_prefix(rate_prefix, laser) -> int:
     return bits
 
 
+def _fill_sampling(d, who, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit, sample_monte_carlo,
+                   sample_stratified, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num, mc_kd_max, mc_kz_max,
+                   kd_sigma, kz_sigma, seed):
+    """Sampling-mode fields and their assertions, shared by the samplers (ADKSampler.jl:78-110,
+    WFATSampler.jl:45-72)."""
+    assert sample_t_num > 0, f"[{who}] Invalid time sample number {sample_t_num}."
+    assert sample_cutoff_limit >= 0, f"[{who}] Invalid cut-off limit {sample_cutoff_limit}."
+    d.sample_t_num = int(sample_t_num)
+    d.sample_t_lo, d.sample_t_hi = float(sample_t_intv[0]), float(sample_t_intv[1])
+    d.cutoff_limit = float(sample_cutoff_limit)
+    d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    if sample_stratified:
+        d.sampling = _abi.SAMPLING_STRATIFIED
+        d.mc_kt_num = int(mc_kt_num)
+        d.kd_sigma, d.kz_sigma = float(kd_sigma), float(kz_sigma)
+        assert mc_kt_num > 0 and kd_sigma > 0, "stratified sampling needs mc_kt_num > 0 and kd_sigma > 0"
+    elif sample_monte_carlo:
+        d.sampling = _abi.SAMPLING_MONTECARLO
+        assert sample_t_intv[0] < sample_t_intv[1], f"[{who}] Invalid sampling time interval {sample_t_intv}."
+        assert mc_kt_num > 0, f"[{who}] Invalid sampling kt_num {mc_kt_num}."
+        if dimension == 3:
+            assert mc_kd_max > 0 or mc_kz_max > 0, f"[{who}] Invalid kd,kz sample boundaries."
+        else:
+            assert mc_kd_max > 0, f"[{who}] Invalid kd sample boundaries {mc_kd_max}."
+        d.mc_kt_num, d.mc_kd_max, d.mc_kz_max = int(mc_kt_num), float(mc_kd_max), float(mc_kz_max)
+    else:
+        d.sampling = _abi.SAMPLING_STEP
+        if dimension == 3:
+            assert ss_kd_num > 0 and ss_kz_num > 0, f"[{who}] Invalid kd,kz sample number {ss_kd_num},{ss_kz_num}."
+            assert ss_kd_max > 0 or ss_kz_max > 0, f"[{who}] Invalid kd,kz sample boundaries."
+            d.ss_kz_num, d.ss_kz_max = int(ss_kz_num), float(ss_kz_max)
+        else:
+            assert ss_kd_num > 0, f"[{who}] Invalid kd sample number {ss_kd_num}."
+            assert ss_kd_max > 0, f"[{who}] Invalid kd sample boundaries {ss_kd_max}."
+            d.ss_kz_num, d.ss_kz_max = 1, 0.0
+        d.ss_kd_num, d.ss_kd_max = int(ss_kd_num), float(ss_kd_max)
+
+
+def _keldysh_warning(who, laser, Ip):
+    """ADKSampler.jl:116-131, WFATSampler.jl:80-95 (the second branch of each is unreachable there)."""
+    if isinstance(laser, MonochromaticLaser):
+        g = KeldyshParameter(laser, Ip)
+    elif isinstance(laser, BichromaticLaser):
+        g = max(KeldyshParameter(laser.l1, Ip), KeldyshParameter(laser.l2, Ip))
+    else:
+        return
+    if g >= 0.6:
+        warnings.warn(f"[{who}] Keldysh parameter γ={g:.4f}, adiabatic (tunneling) condition [γ<<1] "
+                      "not sufficiently satisfied.")
+
+
 class ElectronSampler:
     """ElectronSamplers.jl:28."""
 
@@ -76,49 +127,10 @@ class ADKSampler(ElectronSampler):
         d.dimension = int(dimension)
         d.phase_method = _abi.PHASE_METHODS[pm]
         d.rate_prefix = resolve_rate_prefix(rate_prefix, laser)
-        assert sample_t_num > 0, f"[ADKSampler] Invalid time sample number {sample_t_num}."
-        assert sample_cutoff_limit >= 0, f"[ADKSampler] Invalid cut-off limit {sample_cutoff_limit}."
-        d.sample_t_num = int(sample_t_num)
-        d.sample_t_lo, d.sample_t_hi = float(sample_t_intv[0]), float(sample_t_intv[1])
-        d.cutoff_limit = float(sample_cutoff_limit)
-        d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
-        if sample_stratified:
-            d.sampling = _abi.SAMPLING_STRATIFIED
-            d.mc_kt_num = int(mc_kt_num)
-            d.kd_sigma, d.kz_sigma = float(kd_sigma), float(kz_sigma)
-            assert mc_kt_num > 0 and kd_sigma > 0, "stratified sampling needs mc_kt_num > 0 and kd_sigma > 0"
-        elif sample_monte_carlo:
-            d.sampling = _abi.SAMPLING_MONTECARLO
-            assert sample_t_intv[0] < sample_t_intv[1], f"[ADKSampler] Invalid sampling time interval {sample_t_intv}."
-            assert mc_kt_num > 0, f"[ADKSampler] Invalid sampling kt_num {mc_kt_num}."
-            if dimension == 3:
-                assert mc_kd_max > 0 or mc_kz_max > 0, "[ADKSampler] Invalid kd,kz sample boundaries."
-            else:
-                assert mc_kd_max > 0, f"[ADKSampler] Invalid kd sample boundaries {mc_kd_max}."
-            d.mc_kt_num, d.mc_kd_max, d.mc_kz_max = int(mc_kt_num), float(mc_kd_max), float(mc_kz_max)
-        else:
-            d.sampling = _abi.SAMPLING_STEP
-            if dimension == 3:
-                assert ss_kd_num > 0 and ss_kz_num > 0, f"[ADKSampler] Invalid kd,kz sample number {ss_kd_num},{ss_kz_num}."
-                assert ss_kd_max > 0 or ss_kz_max > 0, "[ADKSampler] Invalid kd,kz sample boundaries."
-                d.ss_kz_num, d.ss_kz_max = int(ss_kz_num), float(ss_kz_max)
-            else:
-                assert ss_kd_num > 0, f"[ADKSampler] Invalid kd sample number {ss_kd_num}."
-                assert ss_kd_max > 0, f"[ADKSampler] Invalid kd sample boundaries {ss_kd_max}."
-                d.ss_kz_num, d.ss_kz_max = 1, 0.0
-            d.ss_kd_num, d.ss_kd_max = int(ss_kd_num), float(ss_kd_max)
-        # Keldysh warning (ADKSampler.jl:116-131)
-        Ip = IonPotential(target)
-        if isinstance(laser, MonochromaticLaser):
-            g = KeldyshParameter(laser, Ip)
-            if g >= 0.6:
-                warnings.warn(f"[ADKSampler] Keldysh parameter γ={g:.4f}, adiabatic (tunneling) condition [γ<<1] "
-                              "not sufficiently satisfied.")
-        elif isinstance(laser, BichromaticLaser):
-            g = max(KeldyshParameter(laser.l1, Ip), KeldyshParameter(laser.l2, Ip))
-            if g >= 0.6:
-                warnings.warn(f"[ADKSampler] Keldysh parameter γ={g:.4f}, adiabatic (tunneling) condition [γ<<1] "
-                              "not sufficiently satisfied.")
+        _fill_sampling(d, "ADKSampler", dimension, sample_t_intv, sample_t_num, sample_cutoff_limit,
+                       sample_monte_carlo, sample_stratified, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num,
+                       mc_kd_max, mc_kz_max, kd_sigma, kz_sigma, seed)
+        _keldysh_warning("ADKSampler", laser, IonPotential(target))
         self.desc = d
 
     @property
@@ -135,14 +147,54 @@ class ADKSampler(ElectronSampler):
         return t
 
 
+class WFATSampler(ADKSampler):
+    """WFATSampler.jl:3-117 (molecules, channel (0,0)); device sampling.
+
+    The structure factor comes from the target's G00 table
+    (`TabulatedMolecule`). The reference accepts the :CTMC phase method only
+    (WFATSampler.jl:41-44); :QTMC and :SCTS are accepted here as an extension
+    (the initial phase row is 0, as in ADK)."""
+
+    def __init__(self, *, laser, target, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit=1e-16,
+                 sample_monte_carlo=False, traj_phase_method="CTMC", mol_orbit_ridx=0, ss_kd_max=0.0, ss_kd_num=0,
+                 ss_kz_max=0.0, ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, seed=0,
+                 sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, **_):
+        from .targets import MoleculeBase, MolRotation
+        if not isinstance(target, MoleculeBase):
+            raise TypeError("[WFATSampler] WFAT only supports molecular targets (MoleculeBase).")
+        pm = _sym(traj_phase_method)
+        if pm not in _abi.PHASE_METHODS:
+            raise ValueError(f"[WFATSampler] Unsupported phase method [{traj_phase_method}].")
+        if IonPotential(target) <= 0:
+            raise ValueError("[WFATSampler] The energy of the ionizing orbit is non-negative.")
+        self.laser, self.target = laser, target
+        d = _abi.SamplerDesc()
+        d.init_method = _abi.INIT_WFAT
+        d.dimension = int(dimension)
+        d.phase_method = _abi.PHASE_METHODS[pm]
+        _fill_sampling(d, "WFATSampler", dimension, sample_t_intv, sample_t_num, sample_cutoff_limit,
+                       sample_monte_carlo, sample_stratified, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num,
+                       mc_kd_max, mc_kz_max, kd_sigma, kz_sigma, seed)
+        for i, a in enumerate(MolRotation(target)):
+            d.mol_rot[i] = a
+        coeff = target.coeff_table()
+        d.wfat_na, d.wfat_nb = target.g00_coeff.shape
+        d.wfat_coeff = _abi.dptr(coeff)
+        d._keep_coeff = coeff                       # the descriptor points into it
+        _keldysh_warning("WFATSampler", laser, IonPotential(target))
+        self.desc = d
+
+
 def init_sampler(**kwargs) -> ElectronSampler:
     """ElectronSamplers.jl:11-24."""
     m = _sym(kwargs["init_cond_method"])
     if m in _ADK:
         return ADKSampler(**kwargs)
-    if m in _SPA or m in _SPANE or m == "WFAT":
+    if m == "WFAT":
+        return WFATSampler(**kwargs)
+    if m in _SPA or m in _SPANE:
         raise NotImplementedError(f"[ElectronSamplers] initial condition method [{m}] is not implemented by the "
-                                  "engine (ADK only)")
+                                  "engine (ADK and WFAT only)")
     raise ValueError(f"[ElectronSamplers] Undefined initial condition method [{m}].")
 
 

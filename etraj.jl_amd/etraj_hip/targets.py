@@ -92,6 +92,74 @@ class SAEAtom(SAEAtomBase):
         return f"[SAEAtom] Atom {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}"
 
 
+class MoleculeBase(Target):
+    """Targets.jl: abstract molecular target."""
+
+
+class TabulatedMolecule(MoleculeBase):
+    """A molecule reduced to what the WFAT trajectory path reads from
+    GenericMolecule (GenericMolecule.jl): the ionizing orbital's Ip, the
+    asymptotic charge Z = charge + 1, the orientation (rot_α, rot_β, rot_γ)
+    (z-y'-z'' Euler angles in radians) and the channel-(0,0) structure factor
+    G00(β, γ) = Σ_{a,b} c_ab P_a(cos β) e^{-i b γ}, given as the complex
+    table `g00_coeff[a, b]` in place of the WFAT integrals of a
+    quantum-chemistry run (GenericMolecule.jl:236-260).
+
+    The EOM is GenericMolecule's single-centre soft-core Coulomb potential
+    V = -Z (r^2 + 1)^(-1/2) (GenericMolecule.jl:453-454); `soft_core` may be
+    changed.
+    """
+
+    def __init__(self, *, Ip, charge=0, g00_coeff, rot_α=0.0, rot_β=0.0, rot_γ=0.0, soft_core=1.0, name="[NA]"):
+        assert Ip > 0, "[TabulatedMolecule] the energy of the ionizing orbit is non-negative."
+        assert soft_core >= 0, "[TabulatedMolecule] `soft_core` should be non-negative."
+        c = np.asarray(g00_coeff, dtype=np.complex128)
+        assert c.ndim == 2 and c.size > 0, "[TabulatedMolecule] g00_coeff should be a non-empty 2D table."
+        self.Ip, self.charge, self.nucl_charge = float(Ip), int(charge), int(charge) + 1
+        self.g00_coeff = c
+        self.rot = (float(rot_α), float(rot_β), float(rot_γ))
+        self.soft_core, self.name = float(soft_core), name
+        self.l = self.m = 0
+        self.asymp_coeff = 1.0
+        self.quan_ax_θ = self.quan_ax_ϕ = 0.0
+
+    def _desc(self) -> _abi.TargetDesc:
+        d = _abi.TargetDesc()
+        d.kind = _abi.TARGET_HYDROGEN_LIKE
+        d.nucl_charge = self.nucl_charge
+        d.Ip = self.Ip
+        d.asymp_coeff = 1.0
+        d.soft_core = self.soft_core
+        return d
+
+    def coeff_table(self) -> np.ndarray:
+        """(re, im) pairs, index a * nb + b (the sampler descriptor's layout)."""
+        return np.ascontiguousarray(np.stack([self.g00_coeff.real, self.g00_coeff.imag], axis=-1).ravel())
+
+    def with_rotation(self, rot_α, rot_β, rot_γ) -> "TabulatedMolecule":
+        return TabulatedMolecule(Ip=self.Ip, charge=self.charge, g00_coeff=self.g00_coeff, rot_α=rot_α, rot_β=rot_β,
+                                 rot_γ=rot_γ, soft_core=self.soft_core, name=self.name)
+
+    def __eq__(self, o):
+        return (type(o) is type(self) and self.Ip == o.Ip and self.charge == o.charge and self.rot == o.rot
+                and self.soft_core == o.soft_core and np.array_equal(self.g00_coeff, o.g00_coeff))
+
+    def __repr__(self):
+        return f"[TabulatedMolecule] {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}, rot={self.rot}"
+
+
+def MolRotation(mol: MoleculeBase):
+    """GenericMolecule.jl: (rot_α, rot_β, rot_γ) in radians."""
+    return mol.rot
+
+
+def synthetic_g00_table(seed: int, na: int = 8, nb: int = 16, mean: float = 1.0, sigma: float = 0.3) -> np.ndarray:
+    """A seeded G00 coefficient table c_ab ~ mean + sigma N(0,1) (real and imaginary parts independent, the
+    imaginary part centred on 0), for workloads that stand in for a quantum-chemistry structure factor."""
+    rng = np.random.default_rng(seed)
+    return (mean + sigma * rng.standard_normal((na, nb))) + 1j * sigma * rng.standard_normal((na, nb))
+
+
 # ---- accessors (Targets/exports.jl) ------------------------------------------
 def IonPotential(t): return t.Ip
 def AsympNuclCharge(t): return t.nucl_charge

@@ -45,6 +45,7 @@ struct SamplerDesc
     sample_t_lo::Float64; sample_t_hi::Float64; ss_kd_max::Float64; ss_kz_max::Float64
     mc_kd_max::Float64; mc_kz_max::Float64; kd_sigma::Float64; kz_sigma::Float64
     cutoff_limit::Float64; seed::UInt64
+    mol_rot::NTuple{3,Float64}; wfat_na::Int32; wfat_nb::Int32; wfat_coeff::Ptr{Float64}
 end
 struct RunDesc
     dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
@@ -53,9 +54,9 @@ struct RunDesc
 end
 mutable struct Stats
     ion_prob::Float64; ion_prob_uncollected::Float64
-    num_effective_traj::Int64; n_failed::Int64; n_bound::Int64; n_restep::Int64; n_steps::Int64
-    ms_sample::Float64; ms_integrate::Float64; ms_total::Float64
-    Stats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    num_effective_traj::Int64; n_failed::Int64; n_bound::Int64; n_restep::Int64; n_steps::Int64; n_steps_fp64::Int64
+    ms_sample::Float64; ms_integrate::Float64; ms_integrate_main::Float64; ms_total::Float64; n_slabs::Int64
+    Stats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
 end
 
 const PHASE = Dict(:CTMC => Int32(0), :QTMC => Int32(1), :SCTS => Int32(2))
@@ -143,7 +144,7 @@ function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, tr
                      device=0)
     sd = Ref(SamplerDesc(1, dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0, sample_t_num,
                          ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
-                         ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0))
+                         ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, (0.0, 0.0, 0.0), 0, 0, C_NULL))
     rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
     spec = zeros(Float64, final_p_num...)
     st = Stats()

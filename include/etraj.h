@@ -93,7 +93,13 @@ typedef struct etraj_target_desc {
 } etraj_target_desc;
 
 /* ---- sampler: src/ElectronSamplers ------------------------------------------ */
-enum { ETRAJ_INIT_ADK = 1 };                        /* ADKSampler.jl */
+enum {
+    ETRAJ_INIT_ADK  = 1,   /* ADKSampler.jl                                                  */
+    ETRAJ_INIT_WFAT = 4    /* WFATSampler.jl, channel (nxi, m) = (0, 0), structure factor
+                              G00(beta, gamma) = sum_{a<na, b<nb} c_ab P_a(cos beta) e^{-i b gamma}
+                              from the table `wfat_coeff` (complex, a-major), times
+                              exp(-kappa mu_z) with mu_z = 0                                 */
+};
 enum { ETRAJ_PHASE_CTMC = 0, ETRAJ_PHASE_QTMC = 1, ETRAJ_PHASE_SCTS = 2 };
 /* rate_prefix bit set (ADKSampler.jl:49-77): :Exp = 0, :Full = PRECC|JAC */
 enum { ETRAJ_PREFIX_PRE = 1, ETRAJ_PREFIX_PRECC = 2, ETRAJ_PREFIX_JAC = 4 };
@@ -122,6 +128,11 @@ typedef struct etraj_sampler_desc {
     double   kd_sigma, kz_sigma;
     double   cutoff_limit;
     uint64_t seed;
+    /* WFAT only: orientation of the molecular frame (ZYZ Euler angles, as
+     * MolRotation, GenericMolecule.jl) and the G00 coefficient table */
+    double   mol_rot[3];
+    int32_t  wfat_na, wfat_nb;
+    const double* wfat_coeff;  /* 2 * na * nb doubles: (re, im) of c_ab, index a * nb + b */
 } etraj_sampler_desc;
 
 /* ---- trajectory run ---------------------------------------------------------- */
@@ -167,7 +178,9 @@ typedef struct etraj_stats {
     int64_t n_steps_fp64;           /* of which steps in FP64 (MIXED: the re-integration pass) */
     double  ms_sample;              /* device time of the sampler stage      */
     double  ms_integrate;           /* device time of integrate+phase+bin    */
+    double  ms_integrate_main;      /* of which the main integrator kernel (MIXED: the FP32 pass) */
     double  ms_total;               /* wall time of the call                 */
+    int64_t n_slabs;                /* run_job: slabs = launches of the main integrator kernel */
 } etraj_stats;
 
 typedef struct etraj_ctx etraj_ctx;

@@ -403,7 +403,7 @@ static int integrate_tsit5(const eom_t* E, int nv, double t0, double tf, double 
     const double qmin = 0.2, qmax = 10.0, gamma = 0.9, beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qoldinit = 1e-4;
     const double dtmax = tf - t0;
     const double dtmin = fmax(fabs(t0), fabs(tf)) * 0x1.0p-52;   /* ~eps(max(|t0|,|tf|)) */
-    double k1[7], k2[7], k3[7], k4[7], k5[7], k6[7], k7[7], y[7], un[7], sk[7], tmp[7];
+    double k1[7], k2[7], k3[7], k4[7], k5[7], k6[7], k7[7], y[7] = {0}, un[7], sk[7], tmp[7];
     double t = t0;
     /* ode_determine_initdt (Hairer & Wanner, Solving ODEs I, II.4) */
     double dtmin_init = nextafter(fmax(0.0, fabs(t) * 0x1.0p-52), INFINITY);
@@ -606,8 +606,9 @@ static int sampler_init(const etraj_laser_desc* ld, const etraj_target_desc* td,
     memset(S, 0, sizeof *S);
     S->sd = sd;
     if (laser_init(ld, &S->L) || target_init(td, &S->T)) return ETRAJ_EINVAL;
-    if (sd->init_method != ETRAJ_INIT_ADK) return ETRAJ_EUNSUPPORTED;
-    if (S->T.l != 0) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method != ETRAJ_INIT_ADK && sd->init_method != ETRAJ_INIT_WFAT) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method == ETRAJ_INIT_ADK && S->T.l != 0) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method == ETRAJ_INIT_WFAT && (sd->wfat_na < 1 || sd->wfat_nb < 1 || !sd->wfat_coeff)) return ETRAJ_EINVAL;
     if (sd->dimension != 2 && sd->dimension != 3) return ETRAJ_EINVAL;
     if (sd->phase_method < 0 || sd->phase_method > 2) return ETRAJ_EINVAL;
     if (sd->sample_t_num <= 0 || sd->cutoff_limit < 0) return ETRAJ_EINVAL;
@@ -736,13 +737,171 @@ static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out,
     return count;
 }
 
+/* ---- WFAT (WFATSampler.jl:132-249), channel (0,0), tabulated G00 ---------- */
+
+/* params(RotZYZ(R)) of Rotations.jl for a proper rotation R (row-major):
+ * R = Rz(a) Ry(b) Rz(c) with b in [0, pi]. */
+static void zyz_params(const double R[3][3], double* a, double* b, double* c) {
+    double t1 = atan2(R[1][2], R[0][2]);
+    double ct = cos(t1), st = sin(t1);
+    *a = t1;
+    *b = atan2(ct * R[0][2] + st * R[1][2], R[2][2]);
+    *c = atan2(-st * R[0][0] + ct * R[1][0], -st * R[0][1] + ct * R[1][1]);
+}
+
+static void rot_zyz(double a, double b, double c, double M[3][3]) {
+    double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cc = cos(c), sc = sin(c);
+    double Rz1[3][3] = {{ca, -sa, 0}, {sa, ca, 0}, {0, 0, 1}};
+    double Ry[3][3] = {{cb, 0, sb}, {0, 1, 0}, {-sb, 0, cb}};
+    double Rz2[3][3] = {{cc, -sc, 0}, {sc, cc, 0}, {0, 0, 1}};
+    double T[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            T[i][j] = 0;
+            for (int k = 0; k < 3; ++k) T[i][j] += Rz1[i][k] * Ry[k][j];
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            M[i][j] = 0;
+            for (int k = 0; k < 3; ++k) M[i][j] += T[i][k] * Rz2[k][j];
+        }
+}
+
+/* obtain_FF_MF_Euler (shared_methods.jl:31-38): R = MF^-1 FF with
+ * FF = [0 Fy/F Fx/F; 0 -Fx/F Fy/F; 1 0 0] and MF = RotZYZ(mol_rot). */
+void oracle_ff_mf_euler(const double mol_rot[3], double Fx, double Fy, double out[3]) {
+    double F = hypot(Fx, Fy);
+    double FF[3][3] = {{0, Fy / F, Fx / F}, {0, -Fx / F, Fy / F}, {1, 0, 0}};
+    double MF[3][3], R[3][3];
+    rot_zyz(mol_rot[0], mol_rot[1], mol_rot[2], MF);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            R[i][j] = 0;
+            for (int k = 0; k < 3; ++k) R[i][j] += MF[k][i] * FF[k][j];   /* MF^T FF */
+        }
+    zyz_params(R, &out[0], &out[1], &out[2]);
+}
+
+/* G00(beta, gamma) = sum_{a,b} c_ab P_a(cos beta) e^{-i b gamma}, the
+ * tabulated stand-in for MolWFATStructureFactor_G(mol, ridx, 0, 0, beta, gamma)
+ * (GenericMolecule.jl:236-260), whose sum runs over Wigner-d functions and
+ * e^{-i m' chi} of the same angles. */
+static double complex wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma) {
+    double x = cos(beta);
+    double complex g = 0;
+    double pm1 = 0, p = 1;   /* P_{a-1}, P_a */
+    for (int a = 0; a < sd->wfat_na; ++a) {
+        if (a == 1) { pm1 = 1; p = x; }
+        else if (a > 1) { double pn = ((2 * a - 1) * x * p - (a - 1) * pm1) / a; pm1 = p; p = pn; }
+        for (int b = 0; b < sd->wfat_nb; ++b) {
+            const double* c = sd->wfat_coeff + 2 * (a * sd->wfat_nb + b);
+            g += (c[0] + I * c[1]) * p * cexp(-I * (double)b * gamma);
+        }
+    }
+    return g;
+}
+
+void oracle_wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma, double out[2]) {
+    double complex g = wfat_g00(sd, beta, gamma);
+    out[0] = creal(g);
+    out[1] = cimag(g);
+}
+
+static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t cap) {
+    const etraj_sampler_desc* sd = S->sd;
+    const int dim = sd->dimension, ph = sd->phase_method;
+    const int rows = (dim == 3) ? (ph == ETRAJ_PHASE_CTMC ? 8 : 9) : (ph == ETRAJ_PHASE_CTMC ? 6 : 7);
+    const double tr = S->tsamp[b];
+    double Fxtr, Fytr;
+    laser_F(&S->L, tr, &Fxtr, &Fytr);
+    const double Ftr = hypot(Fxtr, Fytr);
+    const double phi = atan2(-Fytr, -Fxtr);                          /* :137 */
+    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    double eul[3];
+    oracle_ff_mf_euler(sd->mol_rot, Fxtr, Fytr, eul);                 /* :143-145 */
+    const double complex G = wfat_g00(sd, eul[1], eul[2]);     /* :151-155 */
+    const double G2 = creal(G) * creal(G) + cimag(G) * cimag(G);
+    const double kappa = sqrt(2 * Ip), n = Z / kappa;
+    double dkdt;                                                      /* :159-168 */
+    int64_t nk;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        double kd_step = sd->ss_kd_num > 1 ? (2 * fabs(sd->ss_kd_max)) / sd->ss_kd_num : 1.0;
+        double kz_step = (dim == 3 && sd->ss_kz_num > 1) ? (2 * fabs(sd->ss_kz_max)) / sd->ss_kz_num : 1.0;
+        dkdt = S->tstep * kd_step * kz_step;
+        nk = (int64_t)sd->ss_kd_num * (dim == 3 ? sd->ss_kz_num : 1);
+    } else if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        double kd_span = sd->mc_kd_max > 0 ? 2 * sd->mc_kd_max : 1.0;
+        double kz_span = sd->mc_kz_max > 0 ? 2 * sd->mc_kz_max : 1.0;
+        dkdt = S->tstep * kd_span * kz_span / sd->mc_kt_num;
+        nk = sd->mc_kt_num;
+    } else {
+        dkdt = (sd->sample_t_hi - sd->sample_t_lo) / sd->sample_t_num / sd->mc_kt_num;
+        nk = sd->mc_kt_num;
+    }
+    const int kzn = (dim == 3) ? sd->ss_kz_num : 1;
+    int64_t count = 0;
+    for (int64_t j = 0; j < nk; ++j) {
+        double kd0, kz0 = 0, w = dkdt;
+        if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+            int64_t ikd = j / kzn, ikz = j % kzn;
+            kd0 = oracle_grid_point(-fabs(sd->ss_kd_max), fabs(sd->ss_kd_max), sd->ss_kd_num, (int)ikd);
+            if (dim == 3) kz0 = oracle_grid_point(-fabs(sd->ss_kz_max), fabs(sd->ss_kz_max), kzn, (int)ikz);
+        } else {
+            uint64_t c[4] = {(uint64_t)j, (uint64_t)b, 0, TAG_K}, k[2] = {sd->seed, 0}, o[4];
+            oracle_philox4x64_10(c, k, o);
+            if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+                kd0 = (u01_open_right(o[0]) - 0.5) * 2 * sd->mc_kd_max;
+                if (dim == 3) {
+                    kz0 = (u01_open_right(o[1]) - 0.5) * 2 * sd->mc_kz_max;
+                    if (sd->mc_kd_max == 0) kd0 = 0.0;
+                    if (sd->mc_kz_max == 0) kz0 = 0.0;
+                }
+            } else {
+                double r = sqrt(-2.0 * log(u01_open_left(o[0])));
+                double a = 2 * M_PI * u01_open_right(o[1]);
+                kd0 = sd->kd_sigma * (r * cos(a));
+                w = w / gauss_pdf(kd0, sd->kd_sigma);
+                if (dim == 3) {
+                    kz0 = sd->kz_sigma * (r * sin(a));
+                    w = w / gauss_pdf(kz0, sd->kz_sigma);
+                }
+            }
+        }
+        if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) continue;           /* :188 */
+        double kx0 = kd0 * -sin(phi), ky0 = kd0 * cos(phi);
+        double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / Ftr;
+        /* W(0, 0, kd, kz) (:157) */
+        double W = (kappa * kappa / 2 / Ftr / 1.0) * pow(4 * kappa * kappa / Ftr, 2 * n - 1) *
+                   exp(-2 * pow(kappa * kappa + kd0 * kd0 + kz0 * kz0, 1.5) / (3 * Ftr));
+        double rate = G2 * W * w;
+        if (isnan(rate) || rate < sd->cutoff_limit) continue;
+        if (count >= cap) return -1;
+        double* col = out + count * rows;
+        if (dim == 3) {
+            double v[8] = {r0 * cos(phi), r0 * sin(phi), 0.0, kx0, ky0, kz0, tr, rate};
+            memcpy(col, v, sizeof v);
+            if (ph != ETRAJ_PHASE_CTMC) col[8] = 0.0;
+        } else {
+            double v[6] = {r0 * cos(phi), r0 * sin(phi), kx0, ky0, tr, rate};
+            memcpy(col, v, sizeof v);
+            if (ph != ETRAJ_PHASE_CTMC) col[6] = 0.0;
+        }
+        ++count;
+    }
+    return count;
+}
+
+static int64_t gen_batch(const sampler_t* S, int b, double* out, int64_t cap) {
+    return S->sd->init_method == ETRAJ_INIT_WFAT ? wfat_gen_batch(S, b, out, cap) : adk_gen_batch(S, b, out, cap);
+}
+
 int oracle_gen_electron_batch(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
                               int32_t batch_id, double* out, int64_t capacity, int64_t* count) {
     sampler_t S;
     int rc = sampler_init(ld, td, sd, &S);
     if (rc) { sampler_free(&S); return rc; }
     if (batch_id < 1 || batch_id > sd->sample_t_num) { sampler_free(&S); return ETRAJ_EINVAL; }
-    int64_t c = adk_gen_batch(&S, batch_id - 1, out, capacity);
+    int64_t c = gen_batch(&S, batch_id - 1, out, capacity);
     sampler_free(&S);
     if (c < 0) return ETRAJ_ECAPACITY;
     *count = c;
@@ -889,7 +1048,7 @@ static void* worker_main(void* arg) {
     int64_t lo = nsel * W->tid / W->nthreads, hi = nsel * (W->tid + 1) / W->nthreads;
     for (int64_t s = lo; s < hi; ++s) {
         int b = W->b_begin + (int)(s * W->b_stride);
-        int64_t c = adk_gen_batch(W->S, b, buf, cap);
+        int64_t c = gen_batch(W->S, b, buf, cap);
         if (c <= 0) continue;
         for (int64_t i = 0; i < c; ++i) {
             post_t P;

@@ -26,6 +26,8 @@ int oracle_target_eval(const etraj_target_desc* d, const double* xyz, int64_t n,
 int oracle_eom_rhs(const etraj_laser_desc* ld, const etraj_target_desc* td, int32_t dim, int32_t phase_method,
                    int32_t quirks, double t, const double* u, double* du);
 
+void oracle_ff_mf_euler(const double mol_rot[3], double Fx, double Fy, double out[3]);
+void oracle_wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma, double out[2]);
 int oracle_batch_times(const etraj_sampler_desc* sd, double* t);
 int64_t oracle_batch_max_size(const etraj_sampler_desc* sd);
 int oracle_gen_electron_batch(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,

@@ -55,6 +55,10 @@ def lib():
         L.oracle_unit_envelope.restype = C.c_double
         L.oracle_target_eval.argtypes = [P(TargetDesc), DP, C.c_int64, DP, DP]
         L.oracle_eom_rhs.argtypes = [P(LaserDesc), P(TargetDesc), C.c_int32, C.c_int32, C.c_int32, C.c_double, DP, DP]
+        L.oracle_ff_mf_euler.argtypes = [DP, C.c_double, C.c_double, DP]
+        L.oracle_ff_mf_euler.restype = None
+        L.oracle_wfat_g00.argtypes = [P(SamplerDesc), C.c_double, C.c_double, DP]
+        L.oracle_wfat_g00.restype = None
         L.oracle_batch_times.argtypes = [P(SamplerDesc), DP]
         L.oracle_batch_max_size.argtypes = [P(SamplerDesc)]
         L.oracle_batch_max_size.restype = C.c_int64
@@ -112,6 +116,19 @@ def eom_rhs(ld, td, dim, phase, quirks, t, u):
     du = np.zeros(7)
     _chk(lib().oracle_eom_rhs(ld, td, dim, phase, quirks, t, _d(u), _d(du)))
     return du[:len(u)]
+
+
+def ff_mf_euler(mol_rot, Fx, Fy):
+    r = np.ascontiguousarray(mol_rot, dtype=np.float64)
+    out = np.empty(3)
+    lib().oracle_ff_mf_euler(_d(r), Fx, Fy, _d(out))
+    return out
+
+
+def wfat_g00(sd, beta, gamma):
+    out = np.empty(2)
+    lib().oracle_wfat_g00(sd, beta, gamma, _d(out))
+    return complex(out[0], out[1])
 
 
 def batch_times(sd):

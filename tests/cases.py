@@ -5,7 +5,7 @@ import warnings
 import numpy as np
 
 from etraj_hip import _abi, workloads
-from etraj_hip.samplers import ADKSampler
+from etraj_hip.samplers import init_sampler
 from etraj_hip.simulation import run_desc
 
 
@@ -13,7 +13,7 @@ def descriptors(kw):
     kw = dict(kw)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        sp = ADKSampler(**{k: v for k, v in kw.items() if k not in ("init_cond_method",)})
+        sp = init_sampler(**kw)
     rd = run_desc(**kw)
     return kw["laser"]._desc(), kw["target"]._desc(), sp.desc, rd
 
@@ -34,3 +34,9 @@ def l1(p, q):
     p = np.asarray(p, float).ravel()
     q = np.asarray(q, float).ravel()
     return float(np.abs(p - q).sum() / np.abs(q).sum())
+
+
+def small_wfat(n_t=24, n_k=96, precision="fp64", p_num=160, **over):
+    kw = workloads.mol_lp_scts_wfat(n_t=n_t, n_k=n_k, precision=precision, p_num=p_num, rot=(0.3, 1.1, -0.7))
+    kw.update(over)
+    return kw

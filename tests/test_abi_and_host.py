@@ -162,9 +162,11 @@ def test_init_sampler_dispatch():
         warnings.simplefilter("ignore")
         sp = init_sampler(init_cond_method=":ADK", **kw)
     assert sp.desc.sampling == _abi.SAMPLING_STEP and sp.rows == 6
-    for m in ("SPA", "SPANE", "WFAT"):
+    for m in ("SPA", "SPANE"):
         with pytest.raises(NotImplementedError):
             init_sampler(init_cond_method=m, **kw)
+    with pytest.raises(TypeError):              # WFAT only supports molecular targets
+        init_sampler(init_cond_method="WFAT", **kw)
     with pytest.raises(ValueError):
         init_sampler(init_cond_method="XYZ", **kw)
     with pytest.raises(AssertionError):

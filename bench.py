@@ -227,6 +227,8 @@ def main():
                               "frac_of_fp64_peak": FLOPS_PER_STEP["fp64"] * s64.n_steps / (s64.ms_integrate / 1e3)
                               / 1e12 / PEAK_TFLOPS["fp64"]}
         result["mixed_over_fp64"] = (smx.num_effective_traj / smx.ms_total) / (s64.num_effective_traj / s64.ms_total)
+        # etraj_run_job with the PMD returned in a host buffer (the drop-in's host hand-over)
+        result["host_output_value"] = smx.num_effective_traj / (smx.ms_total / 1e3)
         result["pmd_l1_mixed_vs_fp64_gpu"] = l1(pmx, p64)
         result["ion_prob"] = {"mixed": smx.ion_prob, "fp64": s64.ion_prob}
         log(f"fp64: {s64.ms_total:.1f} ms; L1(mixed, fp64) = {result['pmd_l1_mixed_vs_fp64_gpu']:.3e}")

@@ -465,7 +465,9 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
         double phc = 0.0;
         float qs = 0.0f, qc = 0.0f;
         const float hf = (float)h;
-        const float rc2 = (float)(P.restep_radius * P.restep_radius);
+        // the FP32 hydrogen-like EOM reports |r|^2 + soft core
+        const float rc2 = (float)(P.restep_radius * P.restep_radius +
+                                  (TGT == ETRAJ_TARGET_HYDROGEN_LIKE ? P.T.sc : 0.0));
         float Fpx = (float)F0.x, Fpy = (float)F0.y;
         for (int64_t s = 0; s < n; ++s) {
             const float4 f = T32[s];

@@ -283,10 +283,12 @@ template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R Fx, R Fy,
                                     State<DIM, R>& du, R& dphi, R& r2out) {
     if (sizeof(R) == 4 && TGT == ETRAJ_TARGET_HYDROGEN_LIKE) {
-        // FP32 form: the soft core is folded into r^2 = |r|^2 + a once; then
-        // r.F_target = -Z r^2/(r^2)^(3/2) = -Z/r, so the SCTS integrand
-        // -(p^2/2 + V + r.F) is 2Z/r - p^2/2 (the a/r^2 ~ 1e-10 difference is
-        // below FP32 resolution).
+        // FP32 form: the soft core a is folded into r2 = |r|^2 + a once, so
+        // V = -Z/sqrt(r2) and r.F_target = -Z (r2 - a)/r2^(3/2); the SCTS
+        // integrand -(p^2/2 + V + r.F) is Z/sqrt(r2) (2 - a/r2) - p^2/2. For
+        // the atomic default a = 1e-10 the a/r2 term is below FP32 resolution
+        // wherever the FP32 pass integrates (r2 >= the re-step radius^2) and is
+        // skipped; a molecular soft core (a = 1) keeps it.
         R r2 = fma(u.r[1], u.r[1], (R)T.sc);
         if (DIM == 3) r2 = fma(u.r[DIM - 1], u.r[DIM - 1], r2);
         r2 = fma(u.r[0], u.r[0], r2);
@@ -303,7 +305,11 @@ __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const Stat
         if (DIM == 3) k2 = fma(u.p[DIM - 1], u.p[DIM - 1], k2);
         k2 = fma(u.p[0], u.p[0], k2);
         if (PHASE == ETRAJ_PHASE_QTMC) dphi = fma(k2, (R)-0.5, zr - (R)T.Ip);
-        else if (PHASE == ETRAJ_PHASE_SCTS) dphi = fma(k2, (R)-0.5, zr + zr);
+        else if (PHASE == ETRAJ_PHASE_SCTS) {
+            R two = (R)2;
+            if (T.sc > 1e-6) two = fma(-(R)T.sc, ir * ir, two);
+            dphi = fma(k2, (R)-0.5, zr * two);
+        }
         else dphi = 0;
         return;
     }

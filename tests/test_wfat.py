@@ -162,14 +162,16 @@ def _device_batch(ctx, ld, td, sd, rows, b):
     return out[:n.value]
 
 
+D3 = {"dimension": 3, "final_p_max": (2.0, 2.0, 1.0), "final_p_num": (40, 40, 10)}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("over", [{}, {"dimension": 3, "ss_kz_num": 9, "ss_kz_max": 0.6},
+@pytest.mark.parametrize("over", [{}, dict(D3, ss_kz_num=9, ss_kz_max=0.6),
                                   {"sample_monte_carlo": True, "mc_kt_num": 500, "mc_kd_max": 1.0},
-                                  {"sample_monte_carlo": True, "mc_kt_num": 500, "mc_kd_max": 1.0, "mc_kz_max": 0.5,
-                                   "dimension": 3},
+                                  dict(D3, sample_monte_carlo=True, mc_kt_num=500, mc_kd_max=1.0, mc_kz_max=0.5),
                                   {"sample_stratified": True, "mc_kt_num": 500, "kd_sigma": 0.4},
-                                  {"sample_stratified": True, "mc_kt_num": 500, "kd_sigma": 0.4, "kz_sigma": 0.3,
-                                   "dimension": 3, "traj_phase_method": "SCTS"}])
+                                  dict(D3, sample_stratified=True, mc_kt_num=500, kd_sigma=0.4, kz_sigma=0.3,
+                                       traj_phase_method="SCTS")])
 def test_wfat_device_batches_match_oracle(ctx, over):
     kw = small_wfat(n_t=50, sample_stratified=False, ss_kd_max=1.2, ss_kd_num=64, seed=99, sample_cutoff_limit=1e-20)
     kw.update(over)

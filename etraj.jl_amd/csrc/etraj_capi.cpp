@@ -59,7 +59,9 @@ struct etraj_ctx {
     DevBuf<float4> ft32;
     DevBuf<double2> ft64, ft0;
     DevBuf<int64_t> seg_start, cnt, pre;
-    DevBuf<int32_t> counts, counts2, status;
+    DevBuf<int32_t> counts, counts2, status, seg_job, seg_b;
+    DevBuf<SamplerP> sps;
+    DevBuf<LaserP> lps;
     DevBuf<uint8_t> flags;
     DevBuf<int64_t> orig;
     DevBuf<Contrib> contrib;
@@ -221,14 +223,10 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
     S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
     S.method = s.init_method;
-    if (s.init_method == ETRAJ_INIT_WFAT) {
+    if (s.init_method == ETRAJ_INIT_WFAT) {   // the table is uploaded by the caller (S.wfat_coeff)
         S.wfat_na = s.wfat_na;
         S.wfat_nb = s.wfat_nb;
         for (int k = 0; k < 3; ++k) S.mol_rot[k] = s.mol_rot[k];
-        const size_t nc = (size_t)2 * s.wfat_na * s.wfat_nb;
-        HIP_TRY(c, c->wfat.ensure(nc));
-        HIP_TRY(c, hipMemcpyAsync(c->wfat.p, s.wfat_coeff, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
-        S.wfat_coeff = c->wfat.p;
     }
     return ETRAJ_OK;
 }
@@ -288,8 +286,10 @@ constexpr int kTsit5PerWave = 64 * 8;  // electrons per wavefront for lane refil
 constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
 constexpr int64_t kFieldTabBudget = (int64_t)1 << 30;  // bytes of field table per launch
 
-// Allocates and fills the field table of nseg segments whose t0 are in t0s_dev.
-int fill_field_table(etraj_ctx* c, const IntegP& P, const double* t0s_dev, int64_t nseg, int64_t stride, FieldTab& FT) {
+// Allocates and fills the field table of nseg segments whose t0 are in
+// t0s_dev and whose lasers are lps_dev[seg_job_dev[s]] (seg_job_dev NULL: lps_dev[0]).
+int fill_field_table(etraj_ctx* c, const IntegP& P, const LaserP* lps_dev, const int32_t* seg_job_dev,
+                     const double* t0s_dev, int64_t nseg, int64_t stride, FieldTab& FT) {
     HIP_TRY(c, c->ft32.ensure((size_t)nseg * stride));
     HIP_TRY(c, c->ft64.ensure((size_t)2 * nseg * stride));
     HIP_TRY(c, c->ft0.ensure((size_t)nseg));
@@ -297,7 +297,7 @@ int fill_field_table(etraj_ctx* c, const IntegP& P, const double* t0s_dev, int64
     FT.f64 = c->ft64.p;
     FT.f0 = c->ft0.p;
     FT.stride = stride;
-    HIP_TRY(c, launch_field_table(c->stream, P.L, t0s_dev, nseg, P.tf, P.dt, FT));
+    HIP_TRY(c, launch_field_table(c->stream, lps_dev, seg_job_dev, t0s_dev, nseg, P.tf, P.dt, FT));
     return ETRAJ_OK;
 }
 
@@ -396,12 +396,14 @@ int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t
     HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->seg_t0.p, t0.data(), sizeof(double) * nseg, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, c->lps.ensure(1));
+    HIP_TRY(c, hipMemcpyAsync(c->lps.p, &P.L, sizeof(LaserP), hipMemcpyHostToDevice, c->stream));
     for (int64_t g0 = 0; g0 < nseg; g0 += group) {
         const int64_t ng = std::min(group, nseg - g0);
         int32_t maxc = 0;
         for (int64_t k = g0; k < g0 + ng; ++k) maxc = std::max(maxc, sc[k]);
         FieldTab FT{};
-        if (int rc = fill_field_table(c, P, c->seg_t0.p + g0, ng, stride, FT)) return rc;
+        if (int rc = fill_field_table(c, P, c->lps.p, nullptr, c->seg_t0.p + g0, ng, stride, FT)) return rc;
         if (int rc = rk4_passes(c, P, O, out, c->seg_start.p + g0, c->counts.p + g0, ng, (int)((maxc + 63) / 64), FT, n))
             return rc;
     }
@@ -440,6 +442,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release();
+    c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -517,11 +520,31 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     if (batch_id < 1 || batch_id > S.t_num) return fail(c, ETRAJ_EINVAL, "batch_id out of range");
     if (!count || (capacity > 0 && !out)) return fail(c, ETRAJ_EINVAL, "NULL output");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, c->t0s.ensure(t.size()));
+    if (S.method == ETRAJ_INIT_WFAT) {
+        const size_t nc = (size_t)2 * S.wfat_na * S.wfat_nb;
+        HIP_TRY(c, c->wfat.ensure(nc));
+        HIP_TRY(c, hipMemcpyAsync(c->wfat.p, sampler->wfat_coeff, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+        S.wfat_coeff = c->wfat.p;
+    }
     HIP_TRY(c, c->init.ensure((size_t)S.max_bs * S.rows));
     HIP_TRY(c, c->counts.ensure(1));
-    HIP_TRY(c, hipMemcpyAsync(c->t0s.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_adk_sample(c->stream, S, L, T, c->t0s.p, batch_id - 1, 1, c->init.p, c->counts.p));
+    HIP_TRY(c, c->sps.ensure(1));
+    HIP_TRY(c, c->lps.ensure(1));
+    HIP_TRY(c, c->seg_t0.ensure(1));
+    HIP_TRY(c, c->seg_job.ensure(1));
+    HIP_TRY(c, c->seg_b.ensure(1));
+    HIP_TRY(c, c->seg_start.ensure(1));
+    const double t0 = t[batch_id - 1];
+    const int32_t job0 = 0, b = batch_id - 1;
+    const int64_t start0 = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->sps.p, &S, sizeof(SamplerP), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->lps.p, &L, sizeof(LaserP), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_t0.p, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_job.p, &job0, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_b.p, &b, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, &start0, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
+    HIP_TRY(c, launch_adk_sample(c->stream, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p));
     int32_t cnt = 0;
     HIP_TRY(c, hipMemcpyAsync(&cnt, c->counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -605,98 +628,207 @@ int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const e
     return ETRAJ_OK;
 }
 
-static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
-                        const etraj_sampler_desc* sampler, const etraj_run_desc* run, double* host_out,
-                        const double** dev_out, etraj_stats* stats) {
+// One job of a run or a scan: its laser, its sampler and its batch times.
+struct JobPlan {
+    LaserP L;
+    SamplerP S;
+    std::vector<double> t;
+};
+
+// Validates n_jobs (laser, sampler) pairs that share the target and the run
+// descriptor, as TrajectorySimulationJob does for one job
+// (TrajectorySimulation.jl:178-186), and uploads the WFAT tables of all jobs
+// into one device buffer.
+static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* lasers, const etraj_target_desc* target,
+                     const TargetP& T, const etraj_sampler_desc* samplers, const etraj_run_desc* run,
+                     std::vector<JobPlan>& jobs) {
+    if (n_jobs < 1) return fail(c, ETRAJ_EINVAL, "n_jobs must be positive");
+    if (!lasers || !samplers) return fail(c, ETRAJ_EINVAL, "laser / sampler descriptor array is NULL");
+    if (!run) return fail(c, ETRAJ_EINVAL, "run descriptor is NULL");
+    jobs.resize(n_jobs);
+    size_t ncoef = 0;
+    for (int32_t j = 0; j < n_jobs; ++j) {
+        JobPlan& J = jobs[j];
+        const etraj_sampler_desc& sd = samplers[j];
+        int rc = check_laser(c, &lasers[j], J.L);
+        if (!rc && target->l != 0 && sd.init_method == ETRAJ_INIT_ADK)
+            rc = fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
+        if (!rc) rc = batch_times_of(c, sd, J.t);
+        if (!rc) rc = make_sampler(c, &sd, J.L, T, J.t, J.S);
+        if (!rc && (sd.dimension != run->dimension || sd.phase_method != run->phase_method))
+            rc = fail(c, ETRAJ_EINVAL, "sampler and run disagree on dimension / phase method");
+        if (!rc && !(run->traj_t_final > sd.sample_t_hi))
+            rc = fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `traj_t_final` must be greater than the last time point of `sample_t_intv`.");
+        if (!rc && !(sd.sample_t_lo < sd.sample_t_hi) && sd.sample_t_num > 1)
+            rc = fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval.");
+        if (rc) {
+            if (n_jobs > 1) c->err = "job " + std::to_string(j) + ": " + c->err;
+            return rc;
+        }
+        if (J.S.method == ETRAJ_INIT_WFAT) ncoef += (size_t)2 * J.S.wfat_na * J.S.wfat_nb;
+    }
+    if (ncoef) {
+        HIP_TRY(c, c->wfat.ensure(ncoef));
+        size_t off = 0;
+        for (int32_t j = 0; j < n_jobs; ++j) {
+            SamplerP& S = jobs[j].S;
+            if (S.method != ETRAJ_INIT_WFAT) continue;
+            const size_t nc = (size_t)2 * S.wfat_na * S.wfat_nb;
+            HIP_TRY(c, hipMemcpyAsync(c->wfat.p + off, samplers[j].wfat_coeff, sizeof(double) * nc,
+                                      hipMemcpyHostToDevice, c->stream));
+            S.wfat_coeff = c->wfat.p + off;
+            off += nc;
+        }
+    }
+    return ETRAJ_OK;
+}
+
+// The batch loop of n_jobs independent jobs (TrajectorySimulation.jl:150-159
+// per job). The batches of all jobs, job-major, are cut into slabs of at most
+// 2^25 init columns and kFieldTabBudget bytes of field table; each slab is
+// one field-table launch, one sampler launch and one integrator launch (per
+// pass) over segments of any job. Tsit5 slabs hold one job each (its kernel
+// evaluates the job's laser directly). PMDs, scalars and counters are per job.
+static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* lasers,
+                         const etraj_target_desc* target, const etraj_sampler_desc* samplers,
+                         const etraj_run_desc* run, double* host_out, const double** dev_out, etraj_stats* stats) {
     if (!c) return ETRAJ_EINVAL;
     c->err.clear();
     auto wall0 = std::chrono::steady_clock::now();
-    LaserP L;
     TargetP T;
     IntegP P;
-    SamplerP S;
-    if (int rc = check_laser(c, laser, L)) return rc;
     if (int rc = check_target(c, target, T)) return rc;
-    if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
-    if (target->l != 0 && sampler->init_method == ETRAJ_INIT_ADK)
-        return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
-    std::vector<double> t;
-    if (int rc = batch_times_of(c, *sampler, t)) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
-    if (int rc = make_integ(c, run, L, T, P)) return rc;
-    if (sampler->dimension != run->dimension || sampler->phase_method != run->phase_method)
-        return fail(c, ETRAJ_EINVAL, "sampler and run disagree on dimension / phase method");
-    if (!(run->traj_t_final > sampler->sample_t_hi))
-        return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `traj_t_final` must be greater than the last time point of `sample_t_intv`.");
-    if (!(sampler->sample_t_lo < sampler->sample_t_hi) && sampler->sample_t_num > 1)
-        return fail(c, ETRAJ_EINVAL, "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval.");
-    HIP_TRY(c, hipSetDevice(c->device));
-
+    std::vector<JobPlan> jobs;
+    if (int rc = plan_jobs(c, n_jobs, lasers, target, T, samplers, run, jobs)) return rc;
+    if (int rc = make_integ(c, run, jobs[0].L, T, P)) return rc;
+    const bool rk4 = P.integrator == ETRAJ_INTEG_RK4;
     const bool coh = P.phase != ETRAJ_PHASE_CTMC;
     const int64_t npix = npix_of(P);
     const int64_t plen = coh ? 2 * npix : npix;
-    // slabs of batches bound the init buffer to ~2^25 columns
-    const int64_t max_bs = std::max<int64_t>(S.max_bs, 1);
-    int64_t stride = 1;
-    for (double tb : t) stride = std::max(stride, steps_of(P.tf, tb, P.dt));
-    int64_t per_slab = std::max<int64_t>(1, ((int64_t)1 << 25) / max_bs);
-    per_slab = std::min<int64_t>(per_slab, std::max<int64_t>(1, kFieldTabBudget / (stride * kFieldTabBytesPerStep)));
-    per_slab = std::min<int64_t>(per_slab, S.t_num);
-    const int cps = (int)((max_bs + 63) / 64);
+    const int rows = jobs[0].S.rows;
 
-    HIP_TRY(c, c->pmd.ensure(plen));
-    HIP_TRY(c, c->scal.ensure(2));
-    HIP_TRY(c, c->cnt.ensure(CNT_N));
-    HIP_TRY(c, c->t0s.ensure(t.size()));
-    HIP_TRY(c, c->init.ensure((size_t)per_slab * max_bs * S.rows));
-    HIP_TRY(c, c->counts.ensure(per_slab));
-    HIP_TRY(c, c->seg_start.ensure(per_slab));
-    HIP_TRY(c, c->pre.ensure(per_slab + 1));
-    std::vector<int64_t> ss(per_slab);
-    for (int64_t i = 0; i < per_slab; ++i) ss[i] = i * max_bs;
-    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * per_slab, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->t0s.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * plen, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N, c->stream));
+    // batches of all jobs, job-major
+    struct Item { int32_t job, b; double t0; int64_t cap; };
+    std::vector<Item> items;
+    int64_t stride = 1;
+    for (int32_t j = 0; j < n_jobs; ++j) {
+        const int64_t cap = std::max<int64_t>(jobs[j].S.max_bs, 1);
+        for (int32_t b = 0; b < (int32_t)jobs[j].t.size(); ++b) {
+            items.push_back({j, b, jobs[j].t[b], cap});
+            stride = std::max(stride, steps_of(P.tf, jobs[j].t[b], P.dt));
+        }
+    }
+    const int64_t kMaxCols = (int64_t)1 << 25;
+    const int64_t seg_max = std::max<int64_t>(1, kFieldTabBudget / (stride * kFieldTabBytesPerStep));
+    struct Slab { size_t i0, i1; int64_t cols; int64_t maxcap; };
+    std::vector<Slab> slabs;
+    for (size_t i = 0; i < items.size();) {
+        Slab sl{i, i, 0, 0};
+        while (sl.i1 < items.size()) {
+            const Item& it = items[sl.i1];
+            const bool empty = sl.i1 == sl.i0;
+            if (!empty && (sl.cols + it.cap > kMaxCols || (int64_t)(sl.i1 - sl.i0) >= seg_max)) break;
+            if (!empty && !rk4 && it.job != items[sl.i0].job) break;
+            sl.cols += it.cap;
+            sl.maxcap = std::max(sl.maxcap, it.cap);
+            ++sl.i1;
+        }
+        slabs.push_back(sl);
+        i = sl.i1;
+    }
+    int64_t max_cols = 0, max_nb = 0;
+    for (const Slab& sl : slabs) {
+        max_cols = std::max(max_cols, sl.cols);
+        max_nb = std::max<int64_t>(max_nb, (int64_t)(sl.i1 - sl.i0));
+    }
+
+    std::vector<SamplerP> hsp(n_jobs);
+    std::vector<LaserP> hlp(n_jobs);
+    for (int32_t j = 0; j < n_jobs; ++j) {
+        hsp[j] = jobs[j].S;
+        hlp[j] = jobs[j].L;
+    }
+    HIP_TRY(c, c->pmd.ensure((size_t)n_jobs * plen));
+    HIP_TRY(c, c->scal.ensure((size_t)2 * n_jobs));
+    HIP_TRY(c, c->cnt.ensure((size_t)CNT_N * n_jobs));
+    HIP_TRY(c, c->sps.ensure(n_jobs));
+    HIP_TRY(c, c->lps.ensure(n_jobs));
+    HIP_TRY(c, c->init.ensure((size_t)max_cols * rows));
+    HIP_TRY(c, c->counts.ensure(max_nb));
+    HIP_TRY(c, c->seg_start.ensure(max_nb));
+    HIP_TRY(c, c->seg_t0.ensure(max_nb));
+    HIP_TRY(c, c->seg_job.ensure(max_nb));
+    HIP_TRY(c, c->seg_b.ensure(max_nb));
+    HIP_TRY(c, c->pre.ensure(max_nb + 1));
+    HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->lps.p, hlp.data(), sizeof(LaserP) * n_jobs, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * n_jobs * plen, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2 * n_jobs, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N * n_jobs, c->stream));
 
     OutP O{};
     O.pmd = c->pmd.p;
     O.scal = c->scal.p;
     O.cnt = c->cnt.p;
-    const int nslab = (int)((S.t_num + per_slab - 1) / per_slab);
+    O.job_stride = plen;
+    const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
+    const int nslab = (int)slabs.size();
     // per slab: start, sampler done, main integrator kernel done, slab done
     std::vector<hipEvent_t> ev(4 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
-    int64_t launched = 0;
-    std::vector<int32_t> hcounts(per_slab);
+    std::vector<int64_t> launched(n_jobs, 0);
+    std::vector<double> h_t0;
+    std::vector<int32_t> h_job, h_b, hcounts;
+    std::vector<int64_t> h_start;
     int rc = ETRAJ_OK;
     for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
-        const int b0 = (int)(k * per_slab);
-        const int nb = (int)std::min<int64_t>(per_slab, S.t_num - b0);
+        const Slab& sl = slabs[k];
+        const int nb = (int)(sl.i1 - sl.i0);
+        h_t0.resize(nb); h_job.resize(nb); h_b.resize(nb); h_start.resize(nb); hcounts.resize(nb);
+        int64_t col = 0;
+        for (int i = 0; i < nb; ++i) {
+            const Item& it = items[sl.i0 + i];
+            h_t0[i] = it.t0;
+            h_job[i] = it.job;
+            h_b[i] = it.b;
+            h_start[i] = col;
+            col += it.cap;
+        }
+        hipError_t e = hipMemcpyAsync(c->seg_t0.p, h_t0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_job.p, h_job.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_b.p, h_b.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_start.p, h_start.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[4 * k], c->stream);
         FieldTab FT{};
-        const bool rk4 = P.integrator == ETRAJ_INTEG_RK4;
-        hipError_t e = hipEventRecord(ev[4 * k], c->stream);
         if (e == hipSuccess && rk4) {
-            if (int frc = fill_field_table(c, P, c->t0s.p + b0, nb, stride, FT)) {
+            if (int frc = fill_field_table(c, P, c->lps.p, c->seg_job.p, c->seg_t0.p, nb, stride, FT)) {
                 rc = frc;
                 break;
             }
         }
-        if (e == hipSuccess) e = launch_adk_sample(c->stream, S, L, T, c->t0s.p, b0, nb, c->init.p, c->counts.p);
+        if (e == hipSuccess) e = launch_adk_sample(c->stream, c->sps.p, c->lps.p, T, G, nb, c->init.p, c->counts.p);
         if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 1], c->stream);
         if (e == hipSuccess && rk4) {
-            if (int prc = rk4_passes(c, P, O, OUT_PMD, c->seg_start.p, c->counts.p, nb, cps, FT, (int64_t)nb * max_bs,
-                                     ev[4 * k + 2])) {
+            OutP Os = O;
+            Os.seg_job = c->seg_job.p;
+            if (int prc = rk4_passes(c, P, Os, OUT_PMD, c->seg_start.p, c->counts.p, nb, (int)((sl.maxcap + 63) / 64),
+                                     FT, sl.cols, ev[4 * k + 2])) {
                 rc = prc;
                 break;
             }
         }
         if (e == hipSuccess && !rk4) {
+            const int32_t j = items[sl.i0].job;
+            IntegP Pj = P;
+            Pj.L = jobs[j].L;
+            OutP Oj = O;
+            Oj.pmd = O.pmd + j * plen;
+            Oj.scal = O.scal + 2 * j;
+            Oj.cnt = O.cnt + (int64_t)j * CNT_N;
             e = launch_scan_counts(c->stream, c->counts.p, nb, c->pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(c->stream, P, O, OUT_PMD, c->init.p, c->seg_start.p, c->pre.p, nb, (int64_t)nb * max_bs,
+                e = launch_tsit5(c->stream, Pj, Oj, OUT_PMD, c->init.p, c->seg_start.p, c->pre.p, nb, sl.cols,
                                  kTsit5PerWave);
             if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 2], c->stream);
         }
@@ -708,7 +840,7 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
             rc = hip_fail(c, e, "run_job slab");
             break;
         }
-        for (int i = 0; i < nb; ++i) launched += hcounts[i];
+        for (int i = 0; i < nb; ++i) launched[items[sl.i0 + i].job] += hcounts[i];
     }
     double ms_s = 0, ms_i = 0, ms_m = 0;
     if (rc == ETRAJ_OK) {
@@ -728,32 +860,38 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
 
     const double* result = c->pmd.p;
     if (coh) {
-        HIP_TRY(c, c->pmd_out.ensure(npix));
-        HIP_TRY(c, launch_finalize_abs2(c->stream, c->pmd.p, npix, c->pmd_out.p));
+        HIP_TRY(c, c->pmd_out.ensure((size_t)n_jobs * npix));
+        HIP_TRY(c, launch_finalize_abs2(c->stream, c->pmd.p, (int64_t)n_jobs * npix, c->pmd_out.p));
         result = c->pmd_out.p;
     }
-    double scal[2];
-    int64_t cnt[CNT_N];
-    HIP_TRY(c, hipMemcpyAsync(scal, c->scal.p, sizeof scal, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(cnt, c->cnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    if (host_out) HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> scal(2 * n_jobs);
+    std::vector<int64_t> cnt((size_t)CNT_N * n_jobs);
+    HIP_TRY(c, hipMemcpyAsync(scal.data(), c->scal.p, sizeof(double) * 2 * n_jobs, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(cnt.data(), c->cnt.p, sizeof(int64_t) * CNT_N * n_jobs, hipMemcpyDeviceToHost, c->stream));
+    if (host_out)
+        HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix * n_jobs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (dev_out) *dev_out = result;
     if (stats) {
-        stats->ion_prob = scal[0];
-        stats->ion_prob_uncollected = scal[1];
-        stats->num_effective_traj = launched;
-        stats->n_failed = cnt[CNT_FAILED];
-        stats->n_bound = cnt[CNT_BOUND];
-        stats->n_restep = cnt[CNT_RESTEP];
-        stats->n_steps = cnt[CNT_STEPS];
-        stats->n_steps_fp64 = cnt[CNT_STEPS64];
-        stats->ms_sample = ms_s;
-        stats->ms_integrate = ms_i;
-        stats->ms_integrate_main = ms_m;
-        stats->n_slabs = nslab;
-        stats->ms_total =
+        const double ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        for (int32_t j = 0; j < n_jobs; ++j) {
+            etraj_stats& s = stats[j];
+            const int64_t* cj = cnt.data() + (size_t)CNT_N * j;
+            s.ion_prob = scal[2 * j];
+            s.ion_prob_uncollected = scal[2 * j + 1];
+            s.num_effective_traj = launched[j];
+            s.n_failed = cj[CNT_FAILED];
+            s.n_bound = cj[CNT_BOUND];
+            s.n_restep = cj[CNT_RESTEP];
+            s.n_steps = cj[CNT_STEPS];
+            s.n_steps_fp64 = cj[CNT_STEPS64];
+            s.ms_sample = ms_s;
+            s.ms_integrate = ms_i;
+            s.ms_integrate_main = ms_m;
+            s.ms_total = ms_total;
+            s.n_slabs = nslab;
+        }
     }
     return ETRAJ_OK;
 }
@@ -761,14 +899,30 @@ static int run_job_impl(etraj_ctx* c, const etraj_laser_desc* laser, const etraj
 int etraj_run_job(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
                   const etraj_sampler_desc* sampler, const etraj_run_desc* run, double* pmd_out, etraj_stats* stats) {
     if (c && !pmd_out) return fail(c, ETRAJ_EINVAL, "pmd_out is NULL");
-    return run_job_impl(c, laser, target, sampler, run, pmd_out, nullptr, stats);
+    if (c && (!laser || !sampler)) return fail(c, ETRAJ_EINVAL, "laser / sampler descriptor is NULL");
+    return run_scan_impl(c, 1, laser, target, sampler, run, pmd_out, nullptr, stats);
 }
 
 int etraj_run_job_device(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
                          const etraj_sampler_desc* sampler, const etraj_run_desc* run, const double** pmd_dev,
                          etraj_stats* stats) {
     if (c && !pmd_dev) return fail(c, ETRAJ_EINVAL, "pmd_dev is NULL");
-    return run_job_impl(c, laser, target, sampler, run, nullptr, pmd_dev, stats);
+    if (c && (!laser || !sampler)) return fail(c, ETRAJ_EINVAL, "laser / sampler descriptor is NULL");
+    return run_scan_impl(c, 1, laser, target, sampler, run, nullptr, pmd_dev, stats);
+}
+
+int etraj_run_scan(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* lasers, const etraj_target_desc* target,
+                   const etraj_sampler_desc* samplers, const etraj_run_desc* run, double* pmd_out,
+                   etraj_stats* stats) {
+    if (c && !pmd_out) return fail(c, ETRAJ_EINVAL, "pmd_out is NULL");
+    return run_scan_impl(c, n_jobs, lasers, target, samplers, run, pmd_out, nullptr, stats);
+}
+
+int etraj_run_scan_device(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* lasers,
+                          const etraj_target_desc* target, const etraj_sampler_desc* samplers,
+                          const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats) {
+    if (c && !pmd_dev) return fail(c, ETRAJ_EINVAL, "pmd_dev is NULL");
+    return run_scan_impl(c, n_jobs, lasers, target, samplers, run, nullptr, pmd_dev, stats);
 }
 
 }  // extern "C"

@@ -64,28 +64,46 @@ struct FieldTab {
     int64_t stride;      // >= max steps of any segment
 };
 
+// Outputs of the integrator kernels. For a parameter scan (several jobs in
+// one launch) seg_job maps a segment to its job, whose PMD starts at
+// pmd + job * job_stride and whose scalars and counters are scal + 2 * job and
+// cnt + job * CNT_N; seg_job == nullptr means one job.
 struct OutP {
-    double* pmd;       // CTMC: npix; coherent: 2*npix (re, im)
-    double* scal;      // [ion_prob, ion_prob_uncollected]
-    int64_t* cnt;      // CNT_*
+    double* pmd;       // CTMC: npix; coherent: 2*npix (re, im) per job
+    double* scal;      // [ion_prob, ion_prob_uncollected] per job
+    int64_t* cnt;      // CNT_* per job
     double* records;   // 12 per electron
     int32_t* status;
     Contrib* contrib;
+    const int32_t* seg_job;
+    int64_t job_stride;
+};
+
+// Per-segment (one t0 batch) description for the sampler and the field table:
+// the batch's t0, its job, its index within the job (the Philox counter) and
+// its first column in the init buffer.
+struct SegTab {
+    const double* t0;
+    const int32_t* job;
+    const int32_t* batch;
+    const int64_t* start;
 };
 
 hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, int64_t n, double* Fx, double* Fy,
                               double* Ax, double* Ay);
 hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F);
-hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L, const TargetP& T, const double* t0s,
-                             int b_first, int nb, double* out, int32_t* counts);
+// sps / lps: device arrays of per-job sampler and laser parameters
+hipError_t launch_adk_sample(hipStream_t st, const SamplerP* sps, const LaserP* lps, const TargetP& T, const SegTab& G,
+                             int nb, double* out, int32_t* counts);
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
                       const FieldTab& FT, uint8_t* flags, const int64_t* out_index);
 hipError_t launch_compact_flagged(hipStream_t st, double* init, int rows, const int64_t* seg_start,
                                   const int32_t* seg_count, int64_t n_seg, const uint8_t* flags,
                                   const int64_t* out_index, int32_t* count_out, int64_t* orig);
-hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,
-                              const FieldTab& FT);
+// lps: device array of per-job lasers; seg_job may be NULL (job 0)
+hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* seg_job, const double* t0s,
+                              int64_t nseg, double tf, double dt, const FieldTab& FT);
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);

@@ -231,14 +231,20 @@ __device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const Adk
 
 // One workgroup per batch; survivors are compacted in sample order with a
 // wave ballot + LDS scan, so column order equals the reference's loop order.
-__global__ __launch_bounds__(256) void k_adk_sample(SamplerP S, LaserP L, TargetP T, const double* __restrict__ t0s,
-                                                    int b_first, double* __restrict__ out, int32_t* __restrict__ counts) {
+// Workgroup g samples batch G.batch[g] of job G.job[g] (per-job parameters
+// in sps / lps) into the columns from G.start[g].
+__global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__ sps, const LaserP* __restrict__ lps,
+                                                    TargetP T, SegTab G, double* __restrict__ out,
+                                                    int32_t* __restrict__ counts) {
     __shared__ int wave_tot[4];
     __shared__ int base_sh;
-    const int b = b_first + blockIdx.x;
-    const AdkBatch B = adk_batch_setup(S, L, T, t0s[b]);
+    const int g = blockIdx.x;
+    const int job = G.job[g];
+    const int b = G.batch[g];
+    const SamplerP S = sps[job];
+    const AdkBatch B = adk_batch_setup(S, lps[job], T, G.t0[g]);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double* bout = out + (int64_t)blockIdx.x * S.max_bs * S.rows;
+    double* bout = out + G.start[g] * S.rows;
     if (tid == 0) base_sh = 0;
     __syncthreads();
     double col[9];
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(256) void k_adk_sample(SamplerP S, LaserP L, Target
         if (tid == 0) base_sh += tot;
         __syncthreads();
     }
-    if (tid == 0) counts[blockIdx.x] = base_sh;
+    if (tid == 0) counts[g] = base_sh;
 }
 
 // ---------------------------------------------------------------------------
@@ -361,12 +367,14 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
 // h = (t_f - t0)/n, n = ceil((t_f - t0)/dt), and step s needs the field at
 // half-steps 2s+1 and 2s+2 (t0 + j h/2). Entry s holds both, as FP32
 // (Fx, Fy, Fx, Fy) and FP64 (two double2); f0 holds F(t0).
-__global__ __launch_bounds__(256) void k_field_table(LaserP L, const double* __restrict__ t0s, int64_t nseg,
-                                                     double tf, double dt, int64_t stride, float4* __restrict__ f32,
+__global__ __launch_bounds__(256) void k_field_table(const LaserP* __restrict__ lps, const int32_t* __restrict__ seg_job,
+                                                     const double* __restrict__ t0s, int64_t nseg, double tf, double dt,
+                                                     int64_t stride, float4* __restrict__ f32,
                                                      double2* __restrict__ f64, double2* __restrict__ f0) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseg * stride) return;
     const int64_t seg = i / stride, s = i - seg * stride;
+    const LaserP& L = lps[seg_job ? seg_job[seg] : 0];
     const double t0 = t0s[seg];
     const double span = tf - t0;
     int64_t n = (int64_t)ceil(span / dt);
@@ -415,6 +423,13 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
     const int64_t cbase = seg_start[seg];
     const bool active = first + lane < cnt;
     const int64_t col_idx = cbase + first + (active ? lane : 0);
+    OutP Oj = O;
+    if (O.seg_job) {
+        const int job = O.seg_job[seg];
+        Oj.pmd = O.pmd + job * O.job_stride;
+        Oj.scal = O.scal + 2 * job;
+        Oj.cnt = O.cnt + (int64_t)job * CNT_N;
+    }
 
     Lane<DIM, PHASE> e;
     load_column<DIM, PHASE>(init + col_idx * P.rows, e);
@@ -515,15 +530,15 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
     if (PHASE != ETRAJ_PHASE_CTMC) bad = bad || isnan(phase);
     const bool done = active && !close;
     const int64_t out_idx = (out_index && done) ? out_index[col_idx] : col_idx;
-    emit<DIM, PHASE, OUT>(P, O, out_idx, done, bad, rf, pf, phase, t0, e.prob);
+    emit<DIM, PHASE, OUT>(P, Oj, out_idx, done, bad, rf, pf, phase, t0, e.prob);
     if (OUT == OUT_PMD) {
         // steps: every active lane of this pass; FP64 steps also counted apart
         const int64_t nact = wave_sum_i64(active ? 1 : 0);
         const int64_t nclose = wave_sum_i64((active && close) ? 1 : 0);
         if (lane == 0) {
-            atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)(n * nact));
-            if (!MIXED) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS64), (unsigned long long)(n * nact));
-            if (nclose) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nclose);
+            atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS), (unsigned long long)(n * nact));
+            if (!MIXED) atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS64), (unsigned long long)(n * nact));
+            if (nclose) atomicAdd((unsigned long long*)(Oj.cnt + CNT_RESTEP), (unsigned long long)nclose);
         }
     }
 }
@@ -785,10 +800,10 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
     return hipGetLastError();
 }
 
-hipError_t launch_adk_sample(hipStream_t st, const SamplerP& S, const LaserP& L, const TargetP& T, const double* t0s,
-                             int b_first, int nb, double* out, int32_t* counts) {
+hipError_t launch_adk_sample(hipStream_t st, const SamplerP* sps, const LaserP* lps, const TargetP& T, const SegTab& G,
+                             int nb, double* out, int32_t* counts) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adk_sample, dim3(nb), dim3(256), 0, st, S, L, T, t0s, b_first, out, counts);
+    hipLaunchKernelGGL(k_adk_sample, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
     return hipGetLastError();
 }
 
@@ -902,11 +917,11 @@ hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, i
     return hipGetLastError();
 }
 
-hipError_t launch_field_table(hipStream_t st, const LaserP& L, const double* t0s, int64_t nseg, double tf, double dt,
-                              const FieldTab& FT) {
+hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* seg_job, const double* t0s,
+                              int64_t nseg, double tf, double dt, const FieldTab& FT) {
     if (nseg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field_table, dim3(grid_for(nseg * FT.stride, 256)), dim3(256), 0, st, L, t0s, nseg, tf, dt,
-                       FT.stride, const_cast<float4*>(FT.f32), const_cast<double2*>(FT.f64),
+    hipLaunchKernelGGL(k_field_table, dim3(grid_for(nseg * FT.stride, 256)), dim3(256), 0, st, lps, seg_job, t0s, nseg,
+                       tf, dt, FT.stride, const_cast<float4*>(FT.f32), const_cast<double2*>(FT.f64),
                        const_cast<double2*>(FT.f0));
     return hipGetLastError();
 }

@@ -107,6 +107,10 @@ SIGNATURES = {
                                 P(Stats)]),
     "etraj_run_job_device": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc),
                                        P(DP), P(Stats)]),
+    "etraj_run_scan": (C.c_int, [C.c_void_p, C.c_int32, P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), DP,
+                                 P(Stats)]),
+    "etraj_run_scan_device": (C.c_int, [C.c_void_p, C.c_int32, P(LaserDesc), P(TargetDesc), P(SamplerDesc),
+                                        P(RunDesc), P(DP), P(Stats)]),
 }
 
 

@@ -130,7 +130,17 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
         ion_prob, ion_unc = ip.value, iu.value
     else:
         raise ValueError("granularity must be 'job' or 'batch'")
-    # write_output! layout: px fastest (Julia column-major)
+    result = _result(spec, ion_prob, ion_unc, n_eff, stats, init_cond_method=init_cond_method, laser=laser,
+                     target=target, dimension=dimension, sample_t_intv=sample_t_intv, sample_t_num=sample_t_num,
+                     traj_phase_method=traj_phase_method, traj_t_final=traj_t_final, final_p_max=final_p_max,
+                     final_p_num=final_p_num, traj_integrator=traj_integrator, traj_dt=traj_dt, precision=precision)
+    _save(result, output_path)
+    return result
+
+
+def _result(spec, ion_prob, ion_unc, n_eff, stats, **kw):
+    """The fields write_output! stores (:516-543); PMD layout px fastest (Julia column-major)."""
+    final_p_max, final_p_num = kw["final_p_max"], kw["final_p_num"]
     shape = tuple(int(n) for n in final_p_num)
     result = {
         "px": _axis(final_p_max[0], final_p_num[0]),
@@ -139,14 +149,19 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
         "ion_prob": ion_prob,
         "ion_prob_uncollected": ion_unc,
         "num_effective_traj": n_eff,
-        "params": {"init_cond_method": init_cond_method, "laser": _ser_laser(laser), "target": _ser_target(target),
-                   "sample_t_intv": sample_t_intv, "sample_t_num": sample_t_num, "traj_phase_method": traj_phase_method,
-                   "traj_t_final": traj_t_final, "final_p_max": final_p_max, "final_p_num": final_p_num,
-                   "traj_integrator": traj_integrator, "traj_dt": traj_dt, "precision": precision},
+        "params": {k: kw[k] for k in ("init_cond_method", "sample_t_intv", "sample_t_num", "traj_phase_method",
+                                      "traj_t_final", "final_p_max", "final_p_num", "traj_integrator", "traj_dt",
+                                      "precision")},
         "stats": stats,
     }
-    if dimension == 3:
+    result["params"]["laser"] = _ser_laser(kw["laser"])
+    result["params"]["target"] = _ser_target(kw["target"])
+    if kw["dimension"] == 3:
         result["pz"] = _axis(final_p_max[2], final_p_num[2])
+    return result
+
+
+def _save(result, output_path):
     if output_path is not None:
         path = output_path if output_path.endswith(".npz") else output_path + ".npz"
         if os.path.exists(path):
@@ -157,4 +172,59 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
             np.savez_compressed(fh, **{k: v for k, v in result.items() if isinstance(v, (np.ndarray, float, int))})
         os.replace(tmp, path)
         result["output_path"] = path
-    return result
+
+
+_SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0,
+                      mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=1e-6, sample_monte_carlo=False,
+                      sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="rk4",
+                      traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
+                      kz_sigma=0.0, reproduce_quirks=True, output_path=None)
+
+
+def perform_traj_scan(jobs):
+    """A parameter scan on the device: `jobs` is a list of perform_traj_simulation keyword sets, one per scan
+    point, as the reference's loops of sequential calls build them (examples/test_Bichromatic_CCP.jl:4-27 over
+    intensities, examples/test_Molecules.jl:20-36 over molecules). The points may differ in the laser and in the
+    sampler settings (counts, seeds, t0 grid, a molecule's orientation); they must share the target otherwise and
+    every setting of the trajectory run (dimension, phase method, integrator, t_final, final momentum grid,
+    precision). All points share the device launches (etraj_run_scan); each returns the result dict of
+    perform_traj_simulation, equal to what that call alone would return."""
+    if not jobs:
+        raise ValueError("perform_traj_scan needs at least one job")
+    prep = []
+    for kw in jobs:
+        kw = dict(_SCAN_DEFAULTS, **kw)
+        assert kw["sample_t_intv"][0] < kw["sample_t_intv"][1], \
+            "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval."
+        assert kw["traj_t_final"] > kw["sample_t_intv"][1], \
+            "[TrajectorySimulationJob] `traj_t_final` must be greater than the last time point of `sample_t_intv`."
+        assert kw["dimension"] in (2, 3), "[TrajectorySimulationJob] `dimension` must be either 2 or 3."
+        assert len(kw["final_p_max"]) == len(kw["final_p_num"]) == kw["dimension"], \
+            "[TrajectorySimulationJob] `length(final_p_max)` and `length(final_p_num)` should match `dimension`."
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            sp = init_sampler(**kw)
+        prep.append((kw, sp, run_desc(**kw)))
+    kw0, _, rd0 = prep[0]
+    td0 = kw0["target"]._desc()
+    for i, (kw, sp, rd) in enumerate(prep):
+        if bytes(rd) != bytes(rd0):
+            raise ValueError(f"scan job {i}: the trajectory-run settings differ from job 0")
+        if bytes(kw["target"]._desc()) != bytes(td0):
+            raise ValueError(f"scan job {i}: the target differs from job 0 (only a molecule's orientation may vary)")
+    n = len(prep)
+    lasers = (_abi.LaserDesc * n)(*[kw["laser"]._desc() for kw, _, _ in prep])
+    samplers = (_abi.SamplerDesc * n)(*[sp.desc for _, sp, _ in prep])   # the samplers keep their tables alive
+    stats = (_abi.Stats * n)()
+    npix = int(np.prod(kw0["final_p_num"]))
+    out = np.empty(n * npix)
+    ctx = _abi.default_context()
+    ctx.check(ctx.lib.etraj_run_scan(ctx.handle, n, lasers, td0, samplers, rd0, _abi.dptr(out), stats))
+    results = []
+    for j, (kw, sp, rd) in enumerate(prep):
+        st = stats[j]
+        r = _result(out[j * npix:(j + 1) * npix].copy(), st.ion_prob, st.ion_prob_uncollected,
+                    st.num_effective_traj, st.as_dict(), **kw)
+        _save(r, kw["output_path"])
+        results.append(r)
+    return results

@@ -262,6 +262,26 @@ int etraj_run_job_device(etraj_ctx* ctx, const etraj_laser_desc* laser,
                          const etraj_target_desc* target, const etraj_sampler_desc* sampler,
                          const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats);
 
+/* A parameter scan: n_jobs independent jobs, job j with lasers[j] and
+ * samplers[j] (intensity, polarization, orientation mol_rot, seeds, t0 grid
+ * or sample counts may differ), sharing the target and the run descriptor.
+ * Replaces the reference's sequential perform_traj_simulation calls over a
+ * scanned parameter (examples/test_Bichromatic_CCP.jl:4-27 over intensities,
+ * examples/test_Molecules.jl:20-36 over targets / orbitals): the batches of
+ * all jobs share the sampler and integrator launches, each job keeping its
+ * own PMD. Every job's result equals etraj_run_job of that job alone.
+ * pmd_out: n_jobs * prod(final_p_num) doubles, job-major. stats: NULL or
+ * n_jobs entries (the ms_* fields and n_slabs are those of the whole call). */
+int etraj_run_scan(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc* lasers,
+                   const etraj_target_desc* target, const etraj_sampler_desc* samplers,
+                   const etraj_run_desc* run, double* pmd_out, etraj_stats* stats);
+
+/* Same as etraj_run_scan with the n_jobs PMDs left in device memory owned by
+ * the ctx (*pmd_dev, valid until the next call on ctx). */
+int etraj_run_scan_device(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc* lasers,
+                          const etraj_target_desc* target, const etraj_sampler_desc* samplers,
+                          const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,8 +33,8 @@ from etraj_hip.simulation import run_desc  # noqa: E402
 
 # Algorithmic FLOPs of one RK4 step of the 2D SCTS hydrogen-like integrator kernel (add/mul = 1,
 # FMA = 2, rsqrt = 1): 4 RHS x 20 + stage states 24 + combination 20 + phase 5 = 129, plus the
-# state update (mixed: 4 x FP32 Kahan 4 + FP64 Kahan phase 4 = 20; FP64: 5). The laser field is
-# read from the per-batch table built by a separate kernel and is not counted here.
+# state update (FP32 pass: 4 x FP32 Kahan 4 + FP32 Kahan phase 4 = 20; FP64: 5). The laser field
+# is read from the per-batch table built by a separate kernel and is not counted here.
 FLOPS_PER_STEP = {"mixed": 149.0, "fp64": 134.0}
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}   # MI355X vector FP32 / FP64 (published)
 
@@ -204,7 +204,7 @@ def main():
                                f"{kw['sample_t_num']} t0 strata x {kw['mc_kt_num']} Gaussian k_perp samples "
                                f"({n_el} electrons above the 1e-16 cutoff), RK4 dt 0.1, coherent PMD 1000x1000",
                    "electrons": n_el,
-                   "precision": "mixed: FP32 EOM + Kahan, FP64 Kahan phase, FP64 re-integration of electrons "
+                   "precision": "mixed: FP32 EOM + Kahan, FP32/FP64 Kahan phase, FP64 re-integration of electrons "
                                 "within 2 a.u. of the core, FP64 Kepler and PMD"},
         "pmd_wall_ms": ms_step,
         "integrate_kernel_ms": float(np.mean(ms_int)),

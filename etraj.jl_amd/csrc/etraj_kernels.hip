@@ -36,6 +36,13 @@ __device__ inline double uniform_f64(double v) {
     int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
     return __hiloint2double(hi, lo);
 }
+// A wave-uniform value moved into a vector register (v_mov_b32 from an SGPR).
+__device__ inline float sgpr_to_vgpr(float x) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
 __device__ inline int64_t uniform_i64(int64_t v) {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
@@ -394,11 +401,90 @@ __global__ __launch_bounds__(256) void k_field_table(const LaserP* __restrict__ 
     }
 }
 
+// The FP32 pass of one electron over n steps from t0 (see k_rk4). TGT2 is
+// the target kind, kTargetHSoft for a hydrogen-like target whose soft core is
+// not negligible in FP32 (decided once per wave, outside the step loop).
+template <int DIM, int PHASE, int TGT2>
+__device__ inline void rk4_fp32_pass(const IntegP& P, bool saeq, const Lane<DIM, PHASE>& e, int64_t n, double h,
+                                     const float4* __restrict__ T32, double2 F0, bool& close, double* rf, double* pf,
+                                     double& phase) {
+    // FP32 state with a Kahan compensation term per component (value =
+    // us - uc). Without it the PMD error against FP64 grows ~5x (chaotic
+    // electrons passing just outside the re-step radius amplify the position
+    // rounding).
+    State<DIM, float> us, uc;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        us.r[i] = (float)e.r[i];
+        us.p[i] = (float)e.p[i];
+        uc.r[i] = -(float)(e.r[i] - (double)us.r[i]);
+        uc.p[i] = -(float)(e.p[i] - (double)us.p[i]);
+    }
+    phase = e.phase0;
+    double phc = 0.0;
+    const float hf = (float)h;
+    // the FP32 hydrogen-like EOM reports |r|^2 + soft core
+    const float rc2 = (float)(P.restep_radius * P.restep_radius +
+                              (TGT2 != ETRAJ_TARGET_SAE ? P.T.sc : 0.0));
+    float Fpx = (float)F0.x, Fpy = (float)F0.y;
+    for (int64_t s0 = 0; s0 < n; s0 += 32) {
+        // the phase: FP32 Kahan partial sum over <= 32 steps, then one
+        // FP64 Kahan addition per chunk (a wave-uniform loop nest, so the
+        // flush is not executed per step)
+        const int64_t s1 = s0 + 32 < n ? s0 + 32 : n;
+        float qs = 0.0f, qc = 0.0f;
+        for (int64_t s = s0; s < s1; ++s) {
+            // The table entry arrives in scalar registers (a wave-uniform
+            // load); it is copied to vector registers once, because VALU
+            // instructions that read an SGPR operand issue at a lower rate
+            // than VGPR-only ones (6 % of the whole pass on gfx950).
+            const float4 f = T32[s];
+            const float fmx = sgpr_to_vgpr(f.x), fmy = sgpr_to_vgpr(f.y);
+            const float f1x = sgpr_to_vgpr(f.z), f1y = sgpr_to_vgpr(f.w);
+            State<DIM, float> d;
+            float r2m;
+            const float dph =
+                rk4_incr<DIM, PHASE, TGT2, float>(P.T, saeq, us, hf, Fpx, Fpy, fmx, fmy, f1x, f1y, d, r2m);
+            close = close || (r2m < rc2);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                float y = d.r[i] - uc.r[i];
+                float t = us.r[i] + y;
+                uc.r[i] = (t - us.r[i]) - y;
+                us.r[i] = t;
+                y = d.p[i] - uc.p[i];
+                t = us.p[i] + y;
+                uc.p[i] = (t - us.p[i]) - y;
+                us.p[i] = t;
+            }
+            if (PHASE != ETRAJ_PHASE_CTMC) {
+                const float y = dph - qc;
+                const float t = qs + y;
+                qc = (t - qs) - y;
+                qs = t;
+            }
+            Fpx = f1x;
+            Fpy = f1y;
+        }
+        if (PHASE != ETRAJ_PHASE_CTMC) {
+            const double y = ((double)qs - (double)qc) - phc;
+            const double t = phase + y;
+            phc = (t - phase) - y;
+            phase = t;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        rf[i] = (double)us.r[i] - (double)uc.r[i];
+        pf[i] = (double)us.p[i] - (double)uc.p[i];
+    }
+}
+
 // 64 electrons of one segment (equal t0) per wavefront. Segment s covers
 // columns [seg_start[s], seg_start[s] + seg_count[s]); its field table row is
 // FT.f32/f64 + s * FT.stride, read with wave-uniform (scalar) loads.
 //
-// MIXED: the electron state is FP32 with a Kahan compensation term per
+// MIXED (rk4_fp32_pass): FP32 state with a Kahan compensation term per
 // component; phase increments are Kahan-summed in FP32 over 32 steps and the
 // partial sums Kahan-summed in FP64 (the SCTS phase reaches 1e3 rad). An electron that starts
 // any step within restep_radius of the core is not binned: flags[col] = 1
@@ -437,6 +523,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
     const double span = P.tf - t0;
     int64_t n = (int64_t)ceil(span / P.dt);
     if (n < 1) n = 1;
+    n = uniform_i64(n);   // wave-uniform: loop counters stay scalar
     const double h = span / (double)n;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
     const float4* __restrict__ T32 = FT.f32 + seg * FT.stride;
@@ -468,61 +555,10 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
 #pragma unroll
         for (int i = 0; i < DIM; ++i) { rf[i] = u.r[i]; pf[i] = u.p[i]; }
     } else {
-        State<DIM, float> us, uc;  // value = us - uc
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            us.r[i] = (float)e.r[i];
-            us.p[i] = (float)e.p[i];
-            uc.r[i] = -(float)(e.r[i] - (double)us.r[i]);
-            uc.p[i] = -(float)(e.p[i] - (double)us.p[i]);
-        }
-        phase = e.phase0;
-        double phc = 0.0;
-        float qs = 0.0f, qc = 0.0f;
-        const float hf = (float)h;
-        // the FP32 hydrogen-like EOM reports |r|^2 + soft core
-        const float rc2 = (float)(P.restep_radius * P.restep_radius +
-                                  (TGT == ETRAJ_TARGET_HYDROGEN_LIKE ? P.T.sc : 0.0));
-        float Fpx = (float)F0.x, Fpy = (float)F0.y;
-        for (int64_t s = 0; s < n; ++s) {
-            const float4 f = T32[s];
-            State<DIM, float> d;
-            float r2m;
-            const float dph = rk4_incr<DIM, PHASE, TGT, float>(P.T, saeq, us, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, d, r2m);
-            close = close || (r2m < rc2);
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                float y = d.r[i] - uc.r[i];
-                float t = us.r[i] + y;
-                uc.r[i] = (t - us.r[i]) - y;
-                us.r[i] = t;
-                y = d.p[i] - uc.p[i];
-                t = us.p[i] + y;
-                uc.p[i] = (t - us.p[i]) - y;
-                us.p[i] = t;
-            }
-            {   // FP32 Kahan partial sum of the phase over <= 32 steps ...
-                const float y = dph - qc;
-                const float t = qs + y;
-                qc = (t - qs) - y;
-                qs = t;
-            }
-            if ((s & 31) == 31 || s == n - 1) {  // ... flushed into the FP64 Kahan sum
-                const double y = ((double)qs - (double)qc) - phc;
-                const double t = phase + y;
-                phc = (t - phase) - y;
-                phase = t;
-                qs = 0.0f;
-                qc = 0.0f;
-            }
-            Fpx = f.z;
-            Fpy = f.w;
-        }
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            rf[i] = (double)us.r[i] - (double)uc.r[i];
-            pf[i] = (double)us.p[i] - (double)uc.p[i];
-        }
+        if (TGT == ETRAJ_TARGET_HYDROGEN_LIKE && P.T.sc > 1e-6)
+            rk4_fp32_pass<DIM, PHASE, kTargetHSoft>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+        else
+            rk4_fp32_pass<DIM, PHASE, TGT>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
         if (active) flags[col_idx] = close ? 1 : 0;
     }
 #pragma unroll

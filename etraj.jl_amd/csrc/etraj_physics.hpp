@@ -213,6 +213,12 @@ inline int make_target(const etraj_target_desc& d, TargetP& T) {
     return 0;
 }
 
+// Kernel-internal target kind: a hydrogen-like target whose soft core is not
+// negligible next to |r|^2 in FP32 (the molecular a = 1); same physics as
+// ETRAJ_TARGET_HYDROGEN_LIKE, selected once per wavefront.
+constexpr int kTargetHSoft = 3;
+constexpr bool is_hydrogen_like(int tgt) { return tgt == ETRAJ_TARGET_HYDROGEN_LIKE || tgt == kTargetHSoft; }
+
 template <typename R>
 __host__ __device__ inline R rsqrt_t(R x);
 template <>
@@ -239,7 +245,7 @@ __host__ __device__ inline float rsqrt_t<float>(float x) {
 // (2D SCTS divides by r^2+a instead of sqrt(r^2+a)) for the phase potential.
 template <int TGT, typename R>
 __host__ __device__ inline void target_vs_k(const TargetP& T, R r2n, bool want_v, bool sae_quirk_r2, R& V, R& s) {
-    if (TGT == ETRAJ_TARGET_HYDROGEN_LIKE) {
+    if (is_hydrogen_like(TGT)) {
         R ir = rsqrt_t<R>(r2n + (R)T.sc);
         R zr = (R)T.Zd * ir;
         s = -(zr * ir * ir);
@@ -282,33 +288,32 @@ struct State {
 template <int DIM, int PHASE, int TGT, typename R>
 __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R Fx, R Fy,
                                     State<DIM, R>& du, R& dphi, R& r2out) {
-    if (sizeof(R) == 4 && TGT == ETRAJ_TARGET_HYDROGEN_LIKE) {
+    if (sizeof(R) == 4 && is_hydrogen_like(TGT)) {
         // FP32 form: the soft core a is folded into r2 = |r|^2 + a once, so
         // V = -Z/sqrt(r2) and r.F_target = -Z (r2 - a)/r2^(3/2); the SCTS
         // integrand -(p^2/2 + V + r.F) is Z/sqrt(r2) (2 - a/r2) - p^2/2. For
         // the atomic default a = 1e-10 the a/r2 term is below FP32 resolution
         // wherever the FP32 pass integrates (r2 >= the re-step radius^2) and is
-        // skipped; a molecular soft core (a = 1) keeps it.
+        // skipped; kTargetHSoft (a molecular a = 1) keeps it.
         R r2 = fma(u.r[1], u.r[1], (R)T.sc);
         if (DIM == 3) r2 = fma(u.r[DIM - 1], u.r[DIM - 1], r2);
         r2 = fma(u.r[0], u.r[0], r2);
         r2out = r2;
         const R ir = rsqrt_t<R>(r2);
         const R zr = (R)T.Zd * ir;
-        const R s = -(zr * (ir * ir));
+        const R sm = zr * (ir * ir);   // -s; the sign goes into the FMAs' operand modifiers
 #pragma unroll
         for (int i = 0; i < DIM; ++i) du.r[i] = u.p[i];
-        du.p[0] = fma(s, u.r[0], -Fx);
-        du.p[1] = fma(s, u.r[1], -Fy);
-        if (DIM == 3) du.p[DIM - 1] = s * u.r[DIM - 1];
+        du.p[0] = fma(-sm, u.r[0], -Fx);
+        du.p[1] = fma(-sm, u.r[1], -Fy);
+        if (DIM == 3) du.p[DIM - 1] = -(sm * u.r[DIM - 1]);
         R k2 = u.p[1] * u.p[1];
         if (DIM == 3) k2 = fma(u.p[DIM - 1], u.p[DIM - 1], k2);
         k2 = fma(u.p[0], u.p[0], k2);
         if (PHASE == ETRAJ_PHASE_QTMC) dphi = fma(k2, (R)-0.5, zr - (R)T.Ip);
         else if (PHASE == ETRAJ_PHASE_SCTS) {
-            R two = (R)2;
-            if (T.sc > 1e-6) two = fma(-(R)T.sc, ir * ir, two);
-            dphi = fma(k2, (R)-0.5, zr * two);
+            if (TGT == kTargetHSoft) dphi = fma(k2, (R)-0.5, zr * fma(-(R)T.sc, ir * ir, (R)2));
+            else dphi = fma(k2, (R)-0.5, zr + zr);
         }
         else dphi = 0;
         return;

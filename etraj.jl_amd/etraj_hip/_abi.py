@@ -1,6 +1,6 @@
 """ctypes mirror of include/etraj.h and the loader of libetraj_hip.so.
 
-The library is loaded from this package directory only. There is no fallback:
+The library is loaded from this package directory (or from $ETRAJ_HIP_LIB). There is no fallback:
 if the shared object is missing or cannot be loaded, every entry point that
 needs the device raises :class:`EngineUnavailable`.
 """
@@ -13,7 +13,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libetraj_hip.so")
+# ETRAJ_HIP_LIB names another build of the same library (the Julia binding reads the same variable)
+LIB_PATH = os.environ.get("ETRAJ_HIP_LIB", os.path.join(HERE, "libetraj_hip.so"))
 
 ABI_VERSION = 1
 

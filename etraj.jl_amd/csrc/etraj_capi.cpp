@@ -61,6 +61,7 @@ struct etraj_ctx {
     DevBuf<int64_t> seg_start, cnt, pre;
     DevBuf<int32_t> counts, counts2, status, seg_job, seg_b;
     DevBuf<SamplerP> sps;
+    DevBuf<double> glq;   // 32-point Gauss-Legendre (node, weight) pairs for the SPA sampler
     DevBuf<LaserP> lps;
     DevBuf<uint8_t> flags;
     DevBuf<int64_t> orig;
@@ -143,12 +144,38 @@ int batch_times_of(etraj_ctx* c, const etraj_sampler_desc& s, std::vector<double
 
 // ADKSampler constructor checks (ADKSampler.jl:43-105) and the per-job
 // constants of gen_electron_batch.
+// 32-point Gauss-Legendre rule on [-1, 1]: Newton iteration on P_32 from
+// the Chebyshev-like initial guesses; (node, weight) pairs.
+void gauss_legendre_32(double* q) {
+    const int n = 32;
+    for (int i = 0; i < n / 2; ++i) {
+        double z = std::cos(kPi * (i + 0.75) / (n + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        const double w = 2 / ((1 - z * z) * dp * dp);
+        q[2 * i] = -z;
+        q[2 * i + 1] = w;
+        q[2 * (n - 1 - i)] = z;
+        q[2 * (n - 1 - i) + 1] = w;
+    }
+}
+
 int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, const TargetP& T,
                  const std::vector<double>& t, SamplerP& S) {
     if (!sd) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
     const etraj_sampler_desc& s = *sd;
-    if (s.init_method != ETRAJ_INIT_ADK && s.init_method != ETRAJ_INIT_WFAT)
-        return fail(c, ETRAJ_EUNSUPPORTED, "init_cond_method: the engine implements :ADK and :WFAT");
+    if (s.init_method < ETRAJ_INIT_ADK || s.init_method > ETRAJ_INIT_WFAT)
+        return fail(c, ETRAJ_EUNSUPPORTED, "init_cond_method: the engine implements :ADK, :SPA, :SPANE and :WFAT");
     if (s.init_method == ETRAJ_INIT_WFAT && (s.wfat_na < 1 || s.wfat_nb < 1 || !s.wfat_coeff))
         return fail(c, ETRAJ_EINVAL, "[WFATSampler] the structure-factor table (wfat_na, wfat_nb, wfat_coeff) is missing");
     if (s.phase_method < 0 || s.phase_method > 2) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Undefined phase method");
@@ -223,6 +250,8 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
     S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
     S.method = s.init_method;
+    S.omega = L.n == 1 ? L.c[0].w : (L.c[0].w + L.c[1].w) / 2;   // SPASampler.jl:160-166
+    S.glq = c->glq.p;
     if (s.init_method == ETRAJ_INIT_WFAT) {   // the table is uploaded by the caller (S.wfat_coeff)
         S.wfat_na = s.wfat_na;
         S.wfat_nb = s.wfat_nb;
@@ -429,6 +458,14 @@ int etraj_ctx_create(int device, etraj_ctx** out) {
         delete c;
         return ETRAJ_EHIP;
     }
+    double glq[64];
+    gauss_legendre_32(glq);
+    if (c->glq.ensure(64) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        c->glq.release();
+        delete c;
+        return ETRAJ_EHIP;
+    }
     *out = c;
     return ETRAJ_OK;
 }
@@ -442,7 +479,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release();
-    c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release();
+    c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -511,7 +548,7 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     if (int rc = check_laser(c, laser, L)) return rc;
     if (int rc = check_target(c, target, T)) return rc;
     if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
-    if (target->l != 0 && sampler->init_method == ETRAJ_INIT_ADK)
+    if (target->l != 0 && sampler->init_method != ETRAJ_INIT_WFAT)
         return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
     std::vector<double> t;
     if (int rc = batch_times_of(c, *sampler, t)) return rc;
@@ -544,7 +581,7 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     HIP_TRY(c, hipMemcpyAsync(c->seg_b.p, &b, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, &start0, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
-    HIP_TRY(c, launch_adk_sample(c->stream, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p));
+    HIP_TRY(c, launch_adk_sample(c->stream, S.method, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p));
     int32_t cnt = 0;
     HIP_TRY(c, hipMemcpyAsync(&cnt, c->counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -651,8 +688,10 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
         JobPlan& J = jobs[j];
         const etraj_sampler_desc& sd = samplers[j];
         int rc = check_laser(c, &lasers[j], J.L);
-        if (!rc && target->l != 0 && sd.init_method == ETRAJ_INIT_ADK)
+        if (!rc && target->l != 0 && sd.init_method != ETRAJ_INIT_WFAT)
             rc = fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
+        if (!rc && sd.init_method != samplers[0].init_method)
+            rc = fail(c, ETRAJ_EINVAL, "the jobs of a scan must share init_cond_method");
         if (!rc) rc = batch_times_of(c, sd, J.t);
         if (!rc) rc = make_sampler(c, &sd, J.L, T, J.t, J.S);
         if (!rc && (sd.dimension != run->dimension || sd.phase_method != run->phase_method))
@@ -807,7 +846,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                 break;
             }
         }
-        if (e == hipSuccess) e = launch_adk_sample(c->stream, c->sps.p, c->lps.p, T, G, nb, c->init.p, c->counts.p);
+        if (e == hipSuccess)
+            e = launch_adk_sample(c->stream, jobs[0].S.method, c->sps.p, c->lps.p, T, G, nb, c->init.p, c->counts.p);
         if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 1], c->stream);
         if (e == hipSuccess && rk4) {
             OutP Os = O;

@@ -24,7 +24,9 @@ struct SamplerP {
     double numer;      // C * D^0_00 * Y_00 (:255)
     double nstar;      // n* = Z/kappa (:232)
     uint64_t seed;
-    int method;        // ETRAJ_INIT_ADK / ETRAJ_INIT_WFAT
+    int method;        // ETRAJ_INIT_*
+    double omega;      // SPA: initial-guess frequency (SPASampler.jl:160-166)
+    const double* glq; // SPA: 32 Gauss-Legendre (node, weight) pairs on [-1, 1]
     int wfat_na, wfat_nb;
     double mol_rot[3];
     const double* wfat_coeff;  // device copy of the G00 table (re, im), a-major
@@ -93,8 +95,8 @@ hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, 
                               double* Ax, double* Ay);
 hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F);
 // sps / lps: device arrays of per-job sampler and laser parameters
-hipError_t launch_adk_sample(hipStream_t st, const SamplerP* sps, const LaserP* lps, const TargetP& T, const SegTab& G,
-                             int nb, double* out, int32_t* counts);
+hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
+                             const SegTab& G, int nb, double* out, int32_t* counts);
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
                       const FieldTab& FT, uint8_t* flags, const int64_t* out_index);

@@ -97,7 +97,10 @@ __global__ void k_target_eval(TargetP T, const double* __restrict__ xyz, int64_t
 // ---------------------------------------------------------------------------
 struct AdkBatch {
     double tr, F, cphi, sphi, c_cc, dkdt;
-    double g2;   // WFAT: |G00(beta, gamma)|^2 of this batch's field direction
+    double g2;          // WFAT: |G00(beta, gamma)|^2 of this batch's field direction
+    double dFx, dFy;    // SPANE: dF/dt at tr
+    double cross;       // SPANE: Fx dFy - dFx Fy (SPANESampler.jl:238)
+    bool skip;          // SPA: F(tr) <= 1e-3 empties the batch (SPASampler.jl:167-169)
 };
 
 // params(RotZYZ(MF^-1 FF)) (shared_methods.jl:31-38, Rotations.jl ZYZ
@@ -148,11 +151,18 @@ __device__ inline double wfat_g2(const SamplerP& S, double beta, double gamma) {
     return gre * gre + gim * gim;
 }
 
+template <int M>
 __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, const TargetP& T, double tr) {
     AdkBatch B;
     B.tr = tr;
     double Fx, Fy;
-    laser_field(L, tr, Fx, Fy);
+    B.dFx = B.dFy = B.cross = 0.0;
+    if (M == ETRAJ_INIT_SPANE) {
+        laser_field_deriv(L, tr, Fx, Fy, B.dFx, B.dFy);              // SPANESampler.jl:164-167
+        B.cross = Fx * B.dFy - B.dFx * Fy;
+    } else {
+        laser_field(L, tr, Fx, Fy);
+    }
     B.F = hypot(Fx, Fy);                                            // :172
     double phi = atan2(-Fy, -Fx);                                   // :173
     sincos(phi, &B.sphi, &B.cphi);
@@ -164,7 +174,8 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     B.c_cc = pow(2.0, 3 * nn / 2 + 1) * pow(kappa, 5 * nn + 0.5) * pow(B.F, -nn) * pow(1 + 2 * gam / e0, -nn);  // :235
     B.dkdt = S.dkdt_base;
     B.g2 = 0.0;
-    if (S.method == ETRAJ_INIT_WFAT) {
+    B.skip = (M == ETRAJ_INIT_SPA) && !(B.F > 1e-3);
+    if (M == ETRAJ_INIT_WFAT) {
         double beta, gamma;
         ff_mf_euler(S.mol_rot, Fx, Fy, beta, gamma);   // WFATSampler.jl:143-145
         B.g2 = wfat_g2(S, beta, gamma);                // :151-155
@@ -172,8 +183,112 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     return B;
 }
 
+// SFA-SPA (SPASampler.jl:230-292). With px, py of :231-232 the saddle-point
+// bracket (px + Ax)^2 + (py + Ay)^2 is real and equals kd^2 - |Im A(ts)|^2,
+// so the equation of :233-238 is f(ti) = (kd^2 + kz^2 - |Im A|^2)/2 + Ip = 0
+// with f'(ti) = Im Ax Re Fx + Im Ay Re Fy (dA/dti = -i F). Newton from the
+// reference's initial guess (:240); accepted if |f| <= 1e-8 (NLsolve's ftol)
+// and ti > 0 (:241-246), else 0.
+__device__ inline double spa_solve(const LaserP& L, double tr, double Ftr, double omega, double Ip, double kd,
+                                   double kz) {
+    const double k2 = kd * kd + kz * kz;
+    double ti = asinh(omega / Ftr * sqrt(k2 + 2 * Ip)) / omega;
+    Cx Fx, Fy, Ax, Ay;
+#pragma unroll 1
+    for (int it = 0; it < 100; ++it) {
+        laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+        const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
+        const double step = f / (Ax.im * Fx.re + Ay.im * Fy.re);
+        if (!isfinite(step)) return 0.0;
+        ti -= step;
+        if (fabs(step) <= 1e-15 * fabs(ti)) break;
+    }
+    laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+    const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
+    return (fabs(f) <= 1e-8 && ti > 0) ? ti : 0.0;
+}
+
+// px_/py_ (SPASampler.jl:231-232)
+__device__ inline void spa_p(double kd, Cx Ax, Cx Ay, double& px, double& py) {
+    const double rho = sqrt(Ax.im * Ax.im + Ay.im * Ay.im);
+    px = kd * Ay.im / rho - Ax.re;
+    py = -kd * Ax.im / rho - Ay.re;
+}
+
+// One SPA sample: saddle point, exit position x0 = int_0^ti Im Ax(tr + i tau)
+// (:350-351), the tunnelling integral G = int_0^ti g(tr + i tau) dtau with
+// g = ((px+Ax)^2 + (py+Ay)^2 + kz^2)/2 + Ip, so exp(i S_tun) = exp(-G) (:279),
+// by 32-point Gauss-Legendre on [0, ti] (S.glq: nodes, weights); the
+// prefactor k(ts).F(ts) (:271-278) and the finite-difference Jacobian
+// (:280-292). Returns false if the solve fails.
+__device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const TargetP& T, const AdkBatch& B, double kd,
+                                double kz, double w, double& rate, double& phase, double col[6]) {
+    const double Ip = T.Ip, tr = B.tr;
+    const double ti = spa_solve(L, tr, B.F, S.omega, Ip, kd, kz);
+    if (ti == 0.0) return false;
+    Cx Fx, Fy, Ax, Ay;
+    laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+    double px, py;
+    spa_p(kd, Ax, Ay, px, py);
+    const Cx kdotF = Cx{px + Ax.re, Ax.im} * Fx + Cx{py + Ay.re, Ay.im} * Fy;
+    double x0 = 0.0, y0 = 0.0;
+    Cx G{0.0, 0.0};
+    const double h = ti / 2;
+#pragma unroll 1
+    for (int i = 0; i < 32; ++i) {
+        Cx fx, fy, ax, ay;
+        laser_field_pot_c(L, Cx{tr, h + h * S.glq[2 * i]}, fx, fy, ax, ay);
+        const double wi = S.glq[2 * i + 1];
+        x0 += wi * ax.im;
+        y0 += wi * ay.im;
+        const Cx kx{px + ax.re, ax.im}, ky{py + ay.re, ay.im};
+        const Cx g = kx * kx + ky * ky;
+        G = G + wi * Cx{g.re / 2 + kz * kz / 2 + Ip, g.im / 2};
+    }
+    x0 *= h;
+    y0 *= h;
+    G = h * G;
+    double Axr, Ayr;
+    laser_potential(L, tr, Axr, Ayr);
+    // amp = sqrt(w) exp(-G) [c numer / (k.F)^((n+1)/2)] [sqrt(jac)]  (:296-309)
+    double lmag = 0.5 * log(w) - G.re, arg = -G.im;
+    if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
+        const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
+        const double p = (S.nstar + 1) / 2;
+        lmag += log(cpre * S.numer) - p * log(hypot(kdotF.re, kdotF.im));
+        arg -= p * atan2(kdotF.im, kdotF.re);
+    }
+    if (S.prefix & ETRAJ_PREFIX_JAC) {
+        // saddle points at tr +- dtr and kd +- dkd; a failed solve gives
+        // ti = 0 and a NaN momentum, as in the reference
+        const double dtr = 0.01, dkd = 0.01;
+        double pq[4][2];
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const double trq = tr + (q == 0 ? dtr : q == 1 ? -dtr : 0.0);
+            const double kdq = kd + (q == 2 ? dkd : q == 3 ? -dkd : 0.0);
+            const double tq = spa_solve(L, trq, B.F, S.omega, Ip, kdq, kz);
+            Cx f1, f2, a1, b1;
+            laser_field_pot_c(L, Cx{trq, tq}, f1, f2, a1, b1);
+            spa_p(kdq, a1, b1, pq[q][0], pq[q][1]);
+        }
+        const double dpxdtr = pq[0][0] - pq[1][0], dpydtr = pq[0][1] - pq[1][1];
+        const double dpxdkd = pq[2][0] - pq[3][0], dpydkd = pq[2][1] - pq[3][1];
+        lmag += 0.5 * log(fabs(dpxdtr * dpydkd - dpxdkd * dpydtr) / (4 * dtr * dkd));
+    }
+    rate = exp(2 * lmag);
+    phase = atan2(sin(arg), cos(arg));     // angle(amp) in (-pi, pi]
+    col[0] = x0;
+    col[1] = y0;
+    col[2] = px + Axr;                     // :352-353
+    col[3] = py + Ayr;
+    return true;
+}
+
 // Evaluates sample j of batch b; returns false if it is discarded.
-__device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const AdkBatch& B, int b, int64_t j, double* col) {
+template <int M>
+__device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const TargetP& T, const AdkBatch& B, int b,
+                                  int64_t j, double* col) {
     double kd0, kz0 = 0.0, w = B.dkdt;
     if (S.sampling == ETRAJ_SAMPLING_STEP) {
         int64_t ikd = j / S.kz_num, ikz = j - ikd * S.kz_num;      // ikd outer, ikz inner (:318-319)
@@ -202,8 +317,33 @@ __device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const Adk
         }
     }
     const double Ip = T.Ip;
-    double rate;
-    if (S.method == ETRAJ_INIT_WFAT) {
+    double rate, phase = 0.0;
+    double x0, y0, kx0, ky0;
+    if (M == ETRAJ_INIT_SPA) {
+        if (fabs(kd0) < 1e-4) return false;                           // SPASampler.jl:345
+        double v[6];
+        if (!spa_eval(S, L, T, B, kd0, kz0, w, rate, phase, v)) return false;
+        x0 = v[0]; y0 = v[1]; kx0 = v[2]; ky0 = v[3];
+    } else if (M == ETRAJ_INIT_SPANE) {
+        if (fabs(kd0) < 1e-4) return false;                           // SPANESampler.jl:323
+        kx0 = kd0 * -B.sphi;
+        ky0 = kd0 * B.cphi;
+        const double k2 = kx0 * kx0 + ky0 * ky0 + kz0 * kz0;
+        const double f2 = B.F * B.F - (kx0 * B.dFx + ky0 * B.dFy);   // F2eff :239
+        const double r0 = B.F / 2 * (k2 + 2 * Ip) / f2;               // :240
+        x0 = r0 * B.cphi;
+        y0 = r0 * B.sphi;
+        if (f2 <= B.F * B.F / (S.sampling == ETRAJ_SAMPLING_STEP ? 9.0 : 25.0)) return false;   // :332, :378
+        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));
+        if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
+            const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
+            amp *= cpre * S.numer / pow((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
+        }
+        if (S.prefix & ETRAJ_PREFIX_JAC)
+            amp *= sqrt(fabs(B.F + sqrt(kd0 * kd0 + kz0 * kz0) / (B.F * B.F) * B.cross));   // :270
+        rate = amp * amp;
+        if (isnan(rate) || rate / w < S.cutoff) return false;        // :337: rate/dkdt (Q7)
+    } else if (M == ETRAJ_INIT_WFAT) {
         if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) return false;    // WFATSampler.jl:188
         // W(0, 0, kd, kz) (WFATSampler.jl:157)
         const double kappa = sqrt(2 * Ip), k2 = kappa * kappa;
@@ -221,17 +361,23 @@ __device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const Adk
         if (S.prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(B.F);             // :266,:283
         rate = amp * amp;
     }
-    double kx0 = kd0 * -B.sphi, ky0 = kd0 * B.cphi;                // :324-325
-    double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / B.F;          // :326
-    if (isnan(rate) || rate < S.cutoff) return false;              // :332
+    if (M == ETRAJ_INIT_ADK || M == ETRAJ_INIT_WFAT) {
+        kx0 = kd0 * -B.sphi;                                          // :324-325
+        ky0 = kd0 * B.cphi;
+        const double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / B.F;  // :326
+        x0 = r0 * B.cphi;
+        y0 = r0 * B.sphi;
+    }
+    // :332 (SPANE applied its own rate/dkdt cut above, Q7)
+    if (M != ETRAJ_INIT_SPANE && (isnan(rate) || rate < S.cutoff)) return false;
     if (S.dim == 3) {
-        col[0] = r0 * B.cphi; col[1] = r0 * B.sphi; col[2] = 0.0;
+        col[0] = x0; col[1] = y0; col[2] = 0.0;
         col[3] = kx0; col[4] = ky0; col[5] = kz0; col[6] = B.tr; col[7] = rate;
-        if (S.phase != ETRAJ_PHASE_CTMC) col[8] = 0.0;
+        if (S.phase != ETRAJ_PHASE_CTMC) col[8] = phase;
     } else {
-        col[0] = r0 * B.cphi; col[1] = r0 * B.sphi;
+        col[0] = x0; col[1] = y0;
         col[2] = kx0; col[3] = ky0; col[4] = B.tr; col[5] = rate;
-        if (S.phase != ETRAJ_PHASE_CTMC) col[6] = 0.0;
+        if (S.phase != ETRAJ_PHASE_CTMC) col[6] = phase;
     }
     return true;
 }
@@ -240,6 +386,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const TargetP& T, const Adk
 // wave ballot + LDS scan, so column order equals the reference's loop order.
 // Workgroup g samples batch G.batch[g] of job G.job[g] (per-job parameters
 // in sps / lps) into the columns from G.start[g].
+template <int M>
 __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__ sps, const LaserP* __restrict__ lps,
                                                     TargetP T, SegTab G, double* __restrict__ out,
                                                     int32_t* __restrict__ counts) {
@@ -249,7 +396,8 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
     const int job = G.job[g];
     const int b = G.batch[g];
     const SamplerP S = sps[job];
-    const AdkBatch B = adk_batch_setup(S, lps[job], T, G.t0[g]);
+    const LaserP& L = lps[job];
+    const AdkBatch B = adk_batch_setup<M>(S, L, T, G.t0[g]);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double* bout = out + G.start[g] * S.rows;
     if (tid == 0) base_sh = 0;
@@ -257,7 +405,7 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
     double col[9];
     for (int64_t j0 = 0; j0 < S.nk; j0 += 256) {
         int64_t j = j0 + tid;
-        bool keep = (j < S.nk) && adk_sample(S, T, B, b, j, col);
+        bool keep = (j < S.nk) && !B.skip && adk_sample<M>(S, L, T, B, b, j, col);
         uint64_t m = __ballot(keep);
         int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
         if (lane == 0) wave_tot[wv] = __popcll(m);
@@ -836,10 +984,22 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
     return hipGetLastError();
 }
 
-hipError_t launch_adk_sample(hipStream_t st, const SamplerP* sps, const LaserP* lps, const TargetP& T, const SegTab& G,
-                             int nb, double* out, int32_t* counts) {
+hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
+                             const SegTab& G, int nb, double* out, int32_t* counts) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adk_sample, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+    switch (method) {
+        case ETRAJ_INIT_SPA:
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPA>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            break;
+        case ETRAJ_INIT_SPANE:
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPANE>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            break;
+        case ETRAJ_INIT_WFAT:
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_WFAT>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            break;
+        default:
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_ADK>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+    }
     return hipGetLastError();
 }
 

@@ -191,6 +191,146 @@ __host__ __device__ inline void laser_potential(const LaserP& L, double t, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Lasers at complex time (the SFA-SPA sampler) and the field's time derivative
+// (the SPANE sampler).
+// ---------------------------------------------------------------------------
+struct Cx {
+    double re, im;
+};
+__host__ __device__ inline Cx cx(double r, double i = 0.0) { return Cx{r, i}; }
+__host__ __device__ inline Cx operator+(Cx a, Cx b) { return Cx{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline Cx operator-(Cx a, Cx b) { return Cx{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline Cx operator*(Cx a, Cx b) { return Cx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__host__ __device__ inline Cx operator*(double s, Cx a) { return Cx{s * a.re, s * a.im}; }
+__host__ __device__ inline Cx operator/(Cx a, double s) { return Cx{a.re / s, a.im / s}; }
+// cos(a + ib) = cos a cosh b - i sin a sinh b;  sin(a + ib) = sin a cosh b + i cos a sinh b
+__host__ __device__ inline void csincos(Cx z, Cx& s, Cx& c) {
+    double sa, ca;
+    sincos(z.re, &sa, &ca);
+    const double chb = cosh(z.im), shb = sinh(z.im);
+    s = Cx{sa * chb, ca * shb};
+    c = Cx{ca * chb, -sa * shb};
+}
+__host__ __device__ inline Cx cexp_(Cx z) {
+    double sb, cb;
+    sincos(z.im, &sb, &cb);
+    const double e = exp(z.re);
+    return Cx{e * cb, e * sb};
+}
+
+// LaserFx/Fy/Ax/Ay (Cos2Laser.jl:97-152 and siblings) at complex t; the
+// cut-off window takes real(t) (Cos2Laser.jl:93), as in the reference.
+__host__ __device__ inline void mono_field_pot_c(const MonoP& m, Cx t, Cx& Fx, Cx& Fy, Cx& Ax, Cx& Ay) {
+    t.re -= m.tshift;
+    Cx sw, cw;
+    csincos(Cx{m.w * t.re + m.cep, m.w * t.im}, sw, cw);
+    Cx g, gA, a, b;
+    if (m.kind == ETRAJ_LASER_GAUSSIAN) {
+        const Cx t2 = t * t;
+        gA = m.A0 * cexp_(Cx{-t2.re / 2 / m.sigma2, -t2.im / 2 / m.sigma2});
+        g = gA;
+        const Cx ts = t / m.sigma2;
+        a = ts * cw + m.w * sw;
+        b = ts * sw - m.w * cw;
+    } else {
+        const double win = cos_window(m, t.re);
+        Cx sh, ch;
+        csincos(Cx{m.w * t.re / (2 * m.N), m.w * t.im / (2 * m.N)}, sh, ch);
+        const Cx c2 = ch * ch;
+        const bool cos2 = m.kind == ETRAJ_LASER_COS2;
+        gA = (m.A0 * win) * (cos2 ? c2 : c2 * c2);
+        g = (m.F0 * win) * (cos2 ? ch : c2 * ch);
+        a = ch * sw + m.coefN * (sh * cw);
+        b = m.coefN * (sh * sw) - ch * cw;
+    }
+    if (!m.azi_nz) {
+        Fx = g * a;
+        Fy = m.ellip * (g * b);
+        Ax = gA * cw;
+        Ay = m.ellip * (gA * sw);
+    } else {
+        Fx = g * (m.cazi * a + (m.ellip * m.sazi) * b);
+        Fy = g * ((-m.sazi) * a + (m.ellip * m.cazi) * b);
+        Ax = gA * (m.cazi * cw + (m.ellip * m.sazi) * sw);
+        Ay = gA * ((-m.sazi) * cw + (m.ellip * m.cazi) * sw);
+    }
+}
+
+// Not inlined: the SPA sampler calls it from several loops, and one shared
+// body keeps that kernel's register count within an occupancy of 2+.
+__host__ __device__ inline __attribute__((noinline)) void laser_field_pot_c(const LaserP& L, Cx t, Cx& Fx, Cx& Fy, Cx& Ax,
+                                                                     Cx& Ay) {
+    mono_field_pot_c(L.c[0], t, Fx, Fy, Ax, Ay);
+    if (L.n == 2) {
+        Cx fx, fy, ax, ay;
+        mono_field_pot_c(L.c[1], Cx{t.re - L.delay, t.im}, fx, fy, ax, ay);
+        Fx = Fx + fx; Fy = Fy + fy; Ax = Ax + ax; Ay = Ay + ay;
+    }
+}
+
+// (value, d/dt) pairs: the field and its full time derivative, window
+// included (SPANESampler.jl:164-165 differentiates the whole closed form).
+struct Dv {
+    double v, d;
+};
+__host__ __device__ inline Dv operator+(Dv a, Dv b) { return Dv{a.v + b.v, a.d + b.d}; }
+__host__ __device__ inline Dv operator*(Dv a, Dv b) { return Dv{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__host__ __device__ inline Dv operator*(double s, Dv a) { return Dv{s * a.v, s * a.d}; }
+__host__ __device__ inline void dsincos(Dv x, Dv& s, Dv& c) {
+    double sv, cv;
+    sincos(x.v, &sv, &cv);
+    s = Dv{sv, cv * x.d};
+    c = Dv{cv, -sv * x.d};
+}
+__host__ __device__ inline Dv dtanh_(Dv x) {
+    const double th = tanh(x.v);
+    return Dv{th, (1 - th * th) * x.d};
+}
+
+__host__ __device__ inline void mono_field_deriv(const MonoP& m, double t0, double& Fx, double& Fy, double& dFx,
+                                                 double& dFy) {
+    const Dv t{t0 - m.tshift, 1.0};
+    Dv sw, cw;
+    dsincos(m.w * t + Dv{m.cep, 0.0}, sw, cw);
+    Dv g, a, b;
+    if (m.kind == ETRAJ_LASER_GAUSSIAN) {
+        const double e = exp(-t.v * t.v / 2 / m.sigma2);
+        g = Dv{m.A0 * e, m.A0 * e * (-t.v / m.sigma2)};
+        const Dv ts = (1.0 / m.sigma2) * t;
+        a = ts * cw + m.w * sw;
+        b = ts * sw + (-m.w) * cw;
+    } else {
+        const Dv w1 = Dv{0.5, 0.0} + 0.5 * dtanh_(m.k * (t + Dv{m.halfw, 0.0}));
+        const Dv w2 = Dv{0.5, 0.0} + 0.5 * dtanh_((-m.k) * (t + Dv{-m.halfw, 0.0}));
+        Dv sh, ch;
+        dsincos((m.w / (2 * m.N)) * t, sh, ch);
+        const Dv env = (m.kind == ETRAJ_LASER_COS2) ? ch : ch * ch * ch;
+        g = (m.F0 * env) * w1 * w2;
+        a = ch * sw + m.coefN * (sh * cw);
+        b = m.coefN * (sh * sw) + (-1.0) * (ch * cw);
+    }
+    Dv fx, fy;
+    if (!m.azi_nz) {
+        fx = g * a;
+        fy = m.ellip * (g * b);
+    } else {
+        fx = g * (m.cazi * a + (m.ellip * m.sazi) * b);
+        fy = g * ((-m.sazi) * a + (m.ellip * m.cazi) * b);
+    }
+    Fx = fx.v; Fy = fy.v; dFx = fx.d; dFy = fy.d;
+}
+
+__host__ __device__ inline void laser_field_deriv(const LaserP& L, double t, double& Fx, double& Fy, double& dFx,
+                                                  double& dFy) {
+    mono_field_deriv(L.c[0], t, Fx, Fy, dFx, dFy);
+    if (L.n == 2) {
+        double a, b, c, d;
+        mono_field_deriv(L.c[1], t - L.delay, a, b, c, d);
+        Fx += a; Fy += b; dFx += c; dFy += d;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Targets (HydrogenLikeAtom.jl:75-76, SAEAtom.jl:84-96)
 // ---------------------------------------------------------------------------
 struct TargetP {

@@ -15,7 +15,7 @@ from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, MoleculeBa
                       synthetic_g00_table, get_atom, get_available_atoms,
                       hartree_asymp_coeff, IonPotential, AsympNuclCharge, AngularQuantumNumber, MagneticQuantumNumber,
                       AsympCoeff, SoftCore, TargetName, QuantizationAxisOrientaion, TargetPotential, TargetForce)
-from .samplers import ElectronSampler, ADKSampler, WFATSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
+from .samplers import ElectronSampler, ADKSampler, SPASampler, SPANESampler, WFATSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
 from .simulation import perform_traj_simulation, perform_traj_scan, run_desc
 
 __all__ = [n for n in dir() if not n.startswith("_")]

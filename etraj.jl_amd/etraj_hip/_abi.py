@@ -22,7 +22,7 @@ OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY = 0, -1, -2, -3, -4, -5
 
 LASER_COS2, LASER_COS4, LASER_GAUSSIAN = 1, 2, 3
 TARGET_HYDROGEN_LIKE, TARGET_SAE = 1, 2
-INIT_ADK, INIT_WFAT = 1, 4
+INIT_ADK, INIT_SPA, INIT_SPANE, INIT_WFAT = 1, 2, 3, 4
 PHASE_CTMC, PHASE_QTMC, PHASE_SCTS = 0, 1, 2
 PREFIX_PRE, PREFIX_PRECC, PREFIX_JAC = 1, 2, 4
 SAMPLING_STEP, SAMPLING_MONTECARLO, SAMPLING_STRATIFIED = 0, 1, 2

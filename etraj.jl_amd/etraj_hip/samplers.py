@@ -1,5 +1,6 @@
 """ElectronSamplers: init_sampler / batch_num / gen_electron_batch
-(src/ElectronSamplers/ElectronSamplers.jl:9-26, ADKSampler.jl, WFATSampler.jl).
+(src/ElectronSamplers/ElectronSamplers.jl:9-26, ADKSampler.jl, SPASampler.jl,
+SPANESampler.jl, WFATSampler.jl).
 
 `gen_electron_batch` runs the device sampler kernel and returns the same
 rows x N column-major matrix the reference returns (rows x,y,kx,ky,t0,rate
@@ -26,8 +27,8 @@ def _sym(x):
     return x.lstrip(":") if isinstance(x, str) else x
 
 
-def resolve_rate_prefix(rate_prefix, laser) -> int:
-    """ADKSampler.jl:49-77 -> ETRAJ_PREFIX_* bits."""
+def resolve_rate_prefix(rate_prefix, laser, who="ADKSampler") -> int:
+    """ADKSampler.jl:49-77 (and SPASampler.jl:44-71, SPANESampler.jl) -> ETRAJ_PREFIX_* bits."""
     names = {"Pre": _abi.PREFIX_PRE, "PreCC": _abi.PREFIX_PRECC, "Jac": _abi.PREFIX_JAC}
     if isinstance(rate_prefix, str):
         p = _sym(rate_prefix)
@@ -38,15 +39,15 @@ def resolve_rate_prefix(rate_prefix, laser) -> int:
         elif p in names:
             s = {p}
         else:
-            raise ValueError(f"[ADKSampler] Undefined tunneling rate prefix [{rate_prefix}].")
+            raise ValueError(f"[{who}] Undefined tunneling rate prefix [{rate_prefix}].")
     else:
         s = {_sym(p) for p in rate_prefix}
         if not all(p in names for p in s):
-            raise ValueError(f"[ADKSampler] Undefined tunneling rate prefix [{rate_prefix}].")
+            raise ValueError(f"[{who}] Undefined tunneling rate prefix [{rate_prefix}].")
         if "Pre" in s and "PreCC" in s:
-            raise ValueError("[ADKSampler] Rate prefixes [Pre] & [PreCC] conflict.")
+            raise ValueError(f"[{who}] Rate prefixes [Pre] & [PreCC] conflict.")
     if "PreCC" in s and not isinstance(laser, MonochromaticLaser):
-        warnings.warn("[ADKSampler] Laser is not monochromatic, Coulomb correction in rate prefix is unavailable.")
+        warnings.warn(f"[{who}] Laser is not monochromatic, Coulomb correction in rate prefix is unavailable.")
         s = (s - {"PreCC"}) | {"Pre"}
     bits = 0
     for p in s:
@@ -111,26 +112,30 @@ class ElectronSampler:
 
 class ADKSampler(ElectronSampler):
     """ADKSampler.jl:3-150 (atoms); device sampling."""
+    _method = _abi.INIT_ADK
+    _who = "ADKSampler"
 
     def __init__(self, *, laser, target, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit=1e-16,
                  sample_monte_carlo=False, traj_phase_method="CTMC", rate_prefix="Full", ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
                  ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, mol_orbit_ridx=0, seed=0,
                  sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, **_):
+        who = self._who
         if not isinstance(target, SAEAtomBase):
-            raise NotImplementedError("[ADKSampler] the engine samples atomic targets (SAEAtomBase) only")
+            raise NotImplementedError(f"[{who}] the engine samples atomic targets (SAEAtomBase) only")
         pm = _sym(traj_phase_method)
         if pm not in _abi.PHASE_METHODS:
-            raise ValueError(f"[ADKSampler] Undefined phase method [{traj_phase_method}].")
+            raise ValueError(f"[{who}] Undefined phase method [{traj_phase_method}].")
         self.laser, self.target = laser, target
         d = _abi.SamplerDesc()
-        d.init_method = _abi.INIT_ADK
+        d.init_method = self._method
         d.dimension = int(dimension)
         d.phase_method = _abi.PHASE_METHODS[pm]
-        d.rate_prefix = resolve_rate_prefix(rate_prefix, laser)
-        _fill_sampling(d, "ADKSampler", dimension, sample_t_intv, sample_t_num, sample_cutoff_limit,
+        d.rate_prefix = resolve_rate_prefix(rate_prefix, laser, who)
+        _fill_sampling(d, who, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit,
                        sample_monte_carlo, sample_stratified, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num,
                        mc_kd_max, mc_kz_max, kd_sigma, kz_sigma, seed)
-        _keldysh_warning("ADKSampler", laser, IonPotential(target))
+        if self._method == _abi.INIT_ADK:
+            _keldysh_warning(who, laser, IonPotential(target))
         self.desc = d
 
     @property
@@ -145,6 +150,20 @@ class ADKSampler(ElectronSampler):
         t = np.empty(self.desc.sample_t_num)
         ctx.check(ctx.lib.etraj_batch_times(ctx.handle, self.desc, _abi.dptr(t)))
         return t
+
+
+class SPASampler(ADKSampler):
+    """SPASampler.jl:1-148 (SFA-SPA; atoms with l = 0); device sampling. Each
+    sample solves the complex saddle-point equation for its tunnelling time."""
+    _method = _abi.INIT_SPA
+    _who = "SPASampler"
+
+
+class SPANESampler(ADKSampler):
+    """SPANESampler.jl:1-148 (SFA-SPA with non-adiabatic expansion; atoms with
+    l = 0); device sampling."""
+    _method = _abi.INIT_SPANE
+    _who = "SPANESampler"
 
 
 class WFATSampler(ADKSampler):
@@ -192,9 +211,10 @@ def init_sampler(**kwargs) -> ElectronSampler:
         return ADKSampler(**kwargs)
     if m == "WFAT":
         return WFATSampler(**kwargs)
-    if m in _SPA or m in _SPANE:
-        raise NotImplementedError(f"[ElectronSamplers] initial condition method [{m}] is not implemented by the "
-                                  "engine (ADK and WFAT only)")
+    if m in _SPA:
+        return SPASampler(**kwargs)
+    if m in _SPANE:
+        return SPANESampler(**kwargs)
     raise ValueError(f"[ElectronSamplers] Undefined initial condition method [{m}].")
 
 

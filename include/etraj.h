@@ -95,6 +95,8 @@ typedef struct etraj_target_desc {
 /* ---- sampler: src/ElectronSamplers ------------------------------------------ */
 enum {
     ETRAJ_INIT_ADK  = 1,   /* ADKSampler.jl                                                  */
+    ETRAJ_INIT_SPA  = 2,   /* SPASampler.jl (SFA-SPA): complex saddle point per sample       */
+    ETRAJ_INIT_SPANE = 3,  /* SPANESampler.jl (SFA-SPA with non-adiabatic expansion)         */
     ETRAJ_INIT_WFAT = 4    /* WFATSampler.jl, channel (nxi, m) = (0, 0), structure factor
                               G00(beta, gamma) = sum_{a<na, b<nb} c_ab P_a(cos beta) e^{-i b gamma}
                               from the table `wfat_coeff` (complex, a-major), times

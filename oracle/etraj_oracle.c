@@ -606,8 +606,8 @@ static int sampler_init(const etraj_laser_desc* ld, const etraj_target_desc* td,
     memset(S, 0, sizeof *S);
     S->sd = sd;
     if (laser_init(ld, &S->L) || target_init(td, &S->T)) return ETRAJ_EINVAL;
-    if (sd->init_method != ETRAJ_INIT_ADK && sd->init_method != ETRAJ_INIT_WFAT) return ETRAJ_EUNSUPPORTED;
-    if (sd->init_method == ETRAJ_INIT_ADK && S->T.l != 0) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method < ETRAJ_INIT_ADK || sd->init_method > ETRAJ_INIT_WFAT) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method != ETRAJ_INIT_WFAT && S->T.l != 0) return ETRAJ_EUNSUPPORTED;
     if (sd->init_method == ETRAJ_INIT_WFAT && (sd->wfat_na < 1 || sd->wfat_nb < 1 || !sd->wfat_coeff)) return ETRAJ_EINVAL;
     if (sd->dimension != 2 && sd->dimension != 3) return ETRAJ_EINVAL;
     if (sd->phase_method < 0 || sd->phase_method > 2) return ETRAJ_EINVAL;
@@ -891,8 +891,466 @@ static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t ca
     return count;
 }
 
+/* ---- lasers at complex time and dF/dt (SPASampler.jl, SPANESampler.jl) ---- */
+
+/* Complex-time LaserAx/Ay/Fx/Fy: the closed forms of Cos2Laser.jl:97-152
+ * (and Cos4, Gaussian) with complex t; the cut-off window takes real(t)
+ * (Cos2Laser.jl:93), so it is constant along t = tr + i tau. */
+static void mono_FA_c(const mono_t* m, double complex t, double complex* Fx, double complex* Fy,
+                      double complex* Ax, double complex* Ay) {
+    t -= m->dt_shift;
+    const double w = m->w, N = m->N, e = m->ellip, az = m->azi;
+    const double complex cw = ccos(w * t + m->cep), sw = csin(w * t + m->cep);
+    double complex g, gA, a, b;
+    if (m->kind == ETRAJ_LASER_GAUSSIAN) {
+        const double s2 = m->sigma * m->sigma;
+        gA = m->A0 * cexp(-t * t / 2 / s2);
+        g = gA;
+        a = t / s2 * cw + w * sw;
+        b = t / s2 * sw - w * cw;
+    } else {
+        const double win = cosn_window(m, creal(t));
+        const double complex ch = ccos(w * t / (2 * N)), sh = csin(w * t / (2 * N));
+        const double cN = (m->kind == ETRAJ_LASER_COS2) ? 1 / N : 2 / N;
+        const double complex env = (m->kind == ETRAJ_LASER_COS2) ? ch * ch : (ch * ch) * (ch * ch);
+        gA = m->A0 * env * win;
+        g = m->F0 * ((m->kind == ETRAJ_LASER_COS2) ? ch : ch * ch * ch) * win;
+        a = ch * sw + cN * sh * cw;
+        b = -ch * cw + cN * sh * sw;
+    }
+    if (az == 0) {
+        *Fx = g * a;
+        *Fy = g * b * e;
+        *Ax = gA * cw;
+        *Ay = gA * sw * e;
+    } else {
+        *Fx = g * (a * cos(az) + b * e * sin(az));
+        *Fy = g * (a * -sin(az) + b * e * cos(az));
+        *Ax = gA * (cw * cos(az) + sw * e * sin(az));
+        *Ay = gA * (cw * -sin(az) + sw * e * cos(az));
+    }
+}
+
+static void laser_FA_c(const laser_t* L, double complex t, double complex* Fx, double complex* Fy,
+                       double complex* Ax, double complex* Ay) {
+    mono_FA_c(&L->c[0], t, Fx, Fy, Ax, Ay);
+    if (L->n == 2) {
+        double complex f2x, f2y, a2x, a2y;
+        mono_FA_c(&L->c[1], t - L->delay, &f2x, &f2y, &a2x, &a2y);
+        *Fx += f2x; *Fy += f2y; *Ax += a2x; *Ay += a2y;
+    }
+}
+
+int oracle_laser_complex(const etraj_laser_desc* d, double t_re, double t_im, double out[8]) {
+    laser_t L;
+    if (laser_init(d, &L)) return ETRAJ_EINVAL;
+    double complex Fx, Fy, Ax, Ay;
+    laser_FA_c(&L, t_re + I * t_im, &Fx, &Fy, &Ax, &Ay);
+    double v[8] = {creal(Fx), cimag(Fx), creal(Fy), cimag(Fy), creal(Ax), cimag(Ax), creal(Ay), cimag(Ay)};
+    memcpy(out, v, sizeof v);
+    return ETRAJ_OK;
+}
+
+/* Forward-mode derivative (value, d/dt) of the field: what ForwardDiff's
+ * derivative(Fx, tr) gives in SPANESampler.jl:164-165 (the tanh window is
+ * differentiated too). */
+typedef struct { double v, d; } dual_t;
+static inline dual_t dl(double v, double d) { dual_t r = {v, d}; return r; }
+static inline dual_t dmul(dual_t a, dual_t b) { return dl(a.v * b.v, a.d * b.v + a.v * b.d); }
+static inline dual_t dadd(dual_t a, dual_t b) { return dl(a.v + b.v, a.d + b.d); }
+static inline dual_t dscale(double s, dual_t a) { return dl(s * a.v, s * a.d); }
+static inline dual_t dcos(dual_t a) { return dl(cos(a.v), -sin(a.v) * a.d); }
+static inline dual_t dsin(dual_t a) { return dl(sin(a.v), cos(a.v) * a.d); }
+static inline dual_t dexp(dual_t a) { double e = exp(a.v); return dl(e, e * a.d); }
+static inline dual_t dtanh(dual_t a) { double th = tanh(a.v); return dl(th, (1 - th * th) * a.d); }
+
+static void mono_F_dual(const mono_t* m, double t0, double* Fx, double* Fy, double* dFx, double* dFy) {
+    const dual_t t = dl(t0 - m->dt_shift, 1.0);
+    const double w = m->w, N = m->N, e = m->ellip, az = m->azi;
+    const dual_t wt = dadd(dscale(w, t), dl(m->cep, 0));
+    const dual_t cw = dcos(wt), sw = dsin(wt);
+    dual_t g, a, b;
+    if (m->kind == ETRAJ_LASER_GAUSSIAN) {
+        const double s2 = m->sigma * m->sigma;
+        g = dscale(m->A0, dexp(dscale(-1.0 / (2 * s2), dmul(t, t))));
+        a = dadd(dmul(dscale(1 / s2, t), cw), dscale(w, sw));
+        b = dadd(dmul(dscale(1 / s2, t), sw), dscale(-w, cw));
+    } else {
+        const double hw = N * M_PI / w;
+        const dual_t w1 = dadd(dl(0.5, 0), dscale(0.5, dtanh(dscale(m->k, dadd(t, dl(hw, 0))))));
+        const dual_t w2 = dadd(dl(0.5, 0), dscale(0.5, dtanh(dscale(-m->k, dadd(t, dl(-hw, 0))))));
+        const dual_t half = dscale(w / (2 * N), t);
+        const dual_t ch = dcos(half), sh = dsin(half);
+        const double cN = (m->kind == ETRAJ_LASER_COS2) ? 1 / N : 2 / N;
+        const dual_t env = (m->kind == ETRAJ_LASER_COS2) ? ch : dmul(dmul(ch, ch), ch);
+        g = dmul(dmul(dscale(m->F0, env), w1), w2);
+        a = dadd(dmul(ch, sw), dscale(cN, dmul(sh, cw)));
+        b = dadd(dscale(-1, dmul(ch, cw)), dscale(cN, dmul(sh, sw)));
+    }
+    dual_t fx, fy;
+    if (az == 0) {
+        fx = dmul(g, a);
+        fy = dscale(e, dmul(g, b));
+    } else {
+        fx = dmul(g, dadd(dscale(cos(az), a), dscale(e * sin(az), b)));
+        fy = dmul(g, dadd(dscale(-sin(az), a), dscale(e * cos(az), b)));
+    }
+    *Fx = fx.v; *Fy = fy.v; *dFx = fx.d; *dFy = fy.d;
+}
+
+static void laser_F_dual(const laser_t* L, double t, double* Fx, double* Fy, double* dFx, double* dFy) {
+    mono_F_dual(&L->c[0], t, Fx, Fy, dFx, dFy);
+    if (L->n == 2) {
+        double a, b, c, d;
+        mono_F_dual(&L->c[1], t - L->delay, &a, &b, &c, &d);
+        *Fx += a; *Fy += b; *dFx += c; *dFy += d;
+    }
+}
+
+int oracle_laser_field_deriv(const etraj_laser_desc* d, double t, double out[4]) {
+    laser_t L;
+    if (laser_init(d, &L)) return ETRAJ_EINVAL;
+    laser_F_dual(&L, t, &out[0], &out[1], &out[2], &out[3]);
+    return ETRAJ_OK;
+}
+
+/* dkdt of one sample (ADKSampler.jl:267-276 and its copies in SPA/SPANE/WFAT) */
+static double sample_dkdt(const sampler_t* S, int64_t* nk) {
+    const etraj_sampler_desc* sd = S->sd;
+    const int dim = sd->dimension;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        double kd_step = sd->ss_kd_num > 1 ? (2 * fabs(sd->ss_kd_max)) / sd->ss_kd_num : 1.0;
+        double kz_step = (dim == 3 && sd->ss_kz_num > 1) ? (2 * fabs(sd->ss_kz_max)) / sd->ss_kz_num : 1.0;
+        *nk = (int64_t)sd->ss_kd_num * (dim == 3 ? sd->ss_kz_num : 1);
+        return S->tstep * kd_step * kz_step;
+    }
+    *nk = sd->mc_kt_num;
+    if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        double kd_span = sd->mc_kd_max > 0 ? 2 * sd->mc_kd_max : 1.0;
+        double kz_span = sd->mc_kz_max > 0 ? 2 * sd->mc_kz_max : 1.0;
+        return S->tstep * kd_span * kz_span / sd->mc_kt_num;
+    }
+    return (sd->sample_t_hi - sd->sample_t_lo) / sd->sample_t_num / sd->mc_kt_num;
+}
+
+/* (kd, kz) of sample j of batch b and its weight factor w (dkdt, over the
+ * pdf for stratified sampling): the loops of ADKSampler.jl:315-380. */
+static void sample_k(const sampler_t* S, int b, int64_t j, double dkdt, double* kd, double* kz, double* w) {
+    const etraj_sampler_desc* sd = S->sd;
+    const int dim = sd->dimension;
+    *kz = 0.0;
+    *w = dkdt;
+    if (sd->sampling == ETRAJ_SAMPLING_STEP) {
+        const int kzn = (dim == 3) ? sd->ss_kz_num : 1;
+        int64_t ikd = j / kzn, ikz = j % kzn;
+        *kd = oracle_grid_point(-fabs(sd->ss_kd_max), fabs(sd->ss_kd_max), sd->ss_kd_num, (int)ikd);
+        if (dim == 3) *kz = oracle_grid_point(-fabs(sd->ss_kz_max), fabs(sd->ss_kz_max), kzn, (int)ikz);
+        return;
+    }
+    uint64_t c[4] = {(uint64_t)j, (uint64_t)b, 0, TAG_K}, k[2] = {sd->seed, 0}, o[4];
+    oracle_philox4x64_10(c, k, o);
+    if (sd->sampling == ETRAJ_SAMPLING_MONTECARLO) {
+        *kd = (u01_open_right(o[0]) - 0.5) * 2 * sd->mc_kd_max;
+        if (dim == 3) {
+            *kz = (u01_open_right(o[1]) - 0.5) * 2 * sd->mc_kz_max;
+            if (sd->mc_kd_max == 0) *kd = 0.0;
+            if (sd->mc_kz_max == 0) *kz = 0.0;
+        }
+        return;
+    }
+    double r = sqrt(-2.0 * log(u01_open_left(o[0])));
+    double a = 2 * M_PI * u01_open_right(o[1]);
+    *kd = sd->kd_sigma * (r * cos(a));
+    *w = *w / gauss_pdf(*kd, sd->kd_sigma);
+    if (dim == 3) {
+        *kz = sd->kz_sigma * (r * sin(a));
+        *w = *w / gauss_pdf(*kz, sd->kz_sigma);
+    }
+}
+
+static int put_column(const etraj_sampler_desc* sd, double* out, int64_t* count, int64_t cap, double x0, double y0,
+                      double kx0, double ky0, double kz0, double tr, double rate, double phase) {
+    const int dim = sd->dimension, ph = sd->phase_method;
+    const int rows = (dim == 3) ? (ph == ETRAJ_PHASE_CTMC ? 8 : 9) : (ph == ETRAJ_PHASE_CTMC ? 6 : 7);
+    if (*count >= cap) return -1;
+    double* col = out + *count * rows;
+    if (dim == 3) {
+        double v[9] = {x0, y0, 0.0, kx0, ky0, kz0, tr, rate, phase};
+        memcpy(col, v, sizeof(double) * rows);
+    } else {
+        double v[7] = {x0, y0, kx0, ky0, tr, rate, phase};
+        memcpy(col, v, sizeof(double) * rows);
+    }
+    ++*count;
+    return 0;
+}
+
+/* gen_electron_batch(sp::SPANESampler, batch_id) for an atom with l = 0
+ * (SPANESampler.jl:162-420): the non-adiabatic expansion of the SFA saddle
+ * point to first order in dF/dt. For l = 0 the amplitude is real and
+ * positive, so angle(amp) = 0. */
+static int64_t spane_gen_batch(const sampler_t* S, int b, double* out, int64_t cap) {
+    const etraj_sampler_desc* sd = S->sd;
+    const double tr = S->tsamp[b];
+    double Fx, Fy, dFx, dFy;
+    laser_F_dual(&S->L, tr, &Fx, &Fy, &dFx, &dFy);                    /* :164-167 */
+    const double Ftr = hypot(Fx, Fy), phi = atan2(-Fy, -Fx);           /* :168-169 */
+    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    double gam = 0.0;                                                  /* :178-182 */
+    if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
+    const double kappa = sqrt(2 * Ip), n = Z / kappa;
+    const double e0 = 2.71828182845904523;
+    const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n);
+    const double numer = S->T.C * sqrt(1.0 / (4 * M_PI));           /* C D^0_00 Y_00 */
+    const double cross = Fx * dFy - dFx * Fy;                          /* :238 */
+    int64_t nk;
+    const double dkdt = sample_dkdt(S, &nk);
+    const double f2min = Ftr * Ftr / (sd->sampling == ETRAJ_SAMPLING_STEP ? 9.0 : 25.0);   /* :332, :378 (Q7) */
+    int64_t count = 0;
+    for (int64_t j = 0; j < nk; ++j) {
+        double kd0, kz0, w;
+        sample_k(S, b, j, dkdt, &kd0, &kz0, &w);
+        if (fabs(kd0) < 1e-4) continue;                                /* :323 */
+        const double kx0 = kd0 * -sin(phi), ky0 = kd0 * cos(phi);
+        const double k2 = kx0 * kx0 + ky0 * ky0 + kz0 * kz0;
+        const double f2 = Ftr * Ftr - (kx0 * dFx + ky0 * dFy);         /* F2eff :239 */
+        const double r0 = Ftr / 2 * (k2 + 2 * Ip) / f2;                 /* :240 */
+        if (f2 <= f2min) continue;
+        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));  /* :275 */
+        if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
+            const double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
+            amp *= cpre * numer / pow((k2 + 2 * Ip) * f2, (n + 1) / 4);   /* :262-269 */
+        }
+        if (S->prefix & ETRAJ_PREFIX_JAC)
+            amp *= sqrt(fabs(Ftr + sqrt(kd0 * kd0 + kz0 * kz0) / (Ftr * Ftr) * cross));   /* :270 */
+        const double rate = amp * amp;
+        if (isnan(rate) || rate / w < sd->cutoff_limit) continue;      /* :337 (rate/dkdt, Q7) */
+        if (put_column(sd, out, &count, cap, r0 * cos(phi), r0 * sin(phi), kx0, ky0, kz0, tr, rate, 0.0)) return -1;
+    }
+    return count;
+}
+
+/* Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n). */
+static void gauss_legendre(int n, double* x, double* wt) {
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1, p2 = 0;
+            for (int k = 1; k <= n; ++k) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2 * k - 1) * z * p2 - (k - 1) * p3) / k;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1);
+            double dz = p1 / pp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        x[i] = -z;
+        x[n - 1 - i] = z;
+        wt[i] = wt[n - 1 - i] = 2 / ((1 - z * z) * pp * pp);
+    }
+}
+
+/* The SFA-SPA ingredients at complex time tr + i tau (SPASampler.jl:230-292). */
+typedef struct {
+    const laser_t* L;
+    double tr, Ip, kz;
+    double px, py;       /* canonical momentum (set after the saddle point) */
+} spa_ctx;
+
+/* px_/py_ of SPASampler.jl:231-232 */
+static void spa_p(double kd, double complex Ax, double complex Ay, double* px, double* py) {
+    const double rho = sqrt(cimag(Ax) * cimag(Ax) + cimag(Ay) * cimag(Ay));
+    *px = kd * cimag(Ay) / rho - creal(Ax);
+    *py = -kd * cimag(Ax) / rho - creal(Ay);
+}
+
+/* Saddle-point equation (SPASampler.jl:233-238) solved for ti > 0. With
+ * px, py of :231-232 the bracket (px + A)^2 + (py + A)^2 is real and equals
+ * kd^2 - |Im A(ts)|^2, so f(ti) = (kd^2 + kz^2 - |Im A|^2)/2 + Ip, and
+ * f'(ti) = Im Ax Re Fx + Im Ay Re Fy (dA/dti = -i F). Newton from the
+ * reference's initial guess asinh(w/F sqrt(k^2 + 2Ip))/w (:240); converged if
+ * |f| <= 1e-8 (NLsolve's ftol) and ti > 0 (:241-246); 0 otherwise. */
+static double spa_solve(const laser_t* L, double tr, double Ftr, double omega, double Ip, double kd, double kz) {
+    double ti = asinh(omega / Ftr * sqrt(kd * kd + kz * kz + 2 * Ip)) / omega;
+    double f = 0;
+    for (int it = 0; it < 100; ++it) {
+        double complex Fx, Fy, Ax, Ay;
+        laser_FA_c(L, tr + I * ti, &Fx, &Fy, &Ax, &Ay);
+        const double ia = cimag(Ax), ib = cimag(Ay);
+        f = (kd * kd + kz * kz - (ia * ia + ib * ib)) / 2 + Ip;
+        const double fp = ia * creal(Fx) + ib * creal(Fy);
+        const double step = f / fp;
+        if (!isfinite(step)) return 0.0;
+        ti -= step;
+        if (fabs(step) <= 1e-15 * fabs(ti)) break;
+    }
+    {
+        double complex Fx, Fy, Ax, Ay;
+        laser_FA_c(L, tr + I * ti, &Fx, &Fy, &Ax, &Ay);
+        f = (kd * kd + kz * kz - (cimag(Ax) * cimag(Ax) + cimag(Ay) * cimag(Ay))) / 2 + Ip;
+    }
+    if (!(fabs(f) <= 1e-8) || !(ti > 0)) return 0.0;
+    return ti;
+}
+
+/* Adaptive Gauss-Legendre (16 against 32 points, bisection) of three complex
+ * integrands over tau in [0, ti]: x0 = int Im Ax, y0 = int Im Ay (:350-351)
+ * and G = int g(tr + i tau) with g = ((px+Ax)^2 + (py+Ay)^2 + kz^2)/2 + Ip,
+ * so that S_tun = i G (:279: S_tun = -int_{ts}^{tr} g dt along the segment). */
+typedef struct { double x0, y0; double complex G; } spa_ints;
+static double gl16x[16], gl16w[16], gl32x[32], gl32w[32];
+static pthread_once_t gl_once = PTHREAD_ONCE_INIT;
+static void gl_init(void) { gauss_legendre(16, gl16x, gl16w); gauss_legendre(32, gl32x, gl32w); }
+
+static spa_ints spa_rule(const spa_ctx* C, double a, double b, int n, const double* x, const double* wt) {
+    spa_ints r = {0, 0, 0};
+    const double h = (b - a) / 2, m = (a + b) / 2;
+    for (int i = 0; i < n; ++i) {
+        double complex Fx, Fy, Ax, Ay;
+        laser_FA_c(C->L, C->tr + I * (m + h * x[i]), &Fx, &Fy, &Ax, &Ay);
+        const double complex kx = C->px + Ax, ky = C->py + Ay;
+        r.x0 += wt[i] * cimag(Ax);
+        r.y0 += wt[i] * cimag(Ay);
+        r.G += wt[i] * ((kx * kx + ky * ky + C->kz * C->kz) / 2 + C->Ip);
+    }
+    r.x0 *= h; r.y0 *= h; r.G *= h;
+    return r;
+}
+
+static spa_ints spa_integrate(const spa_ctx* C, double a, double b, int depth) {
+    spa_ints lo = spa_rule(C, a, b, 16, gl16x, gl16w), hi = spa_rule(C, a, b, 32, gl32x, gl32w);
+    const double err = fabs(hi.x0 - lo.x0) + fabs(hi.y0 - lo.y0) + cabs(hi.G - lo.G);
+    const double mag = fabs(hi.x0) + fabs(hi.y0) + cabs(hi.G);
+    if (err <= 1e-14 * mag || depth >= 30) return hi;
+    spa_ints l = spa_integrate(C, a, (a + b) / 2, depth + 1), r = spa_integrate(C, (a + b) / 2, b, depth + 1);
+    spa_ints s = {l.x0 + r.x0, l.y0 + r.y0, l.G + r.G};
+    return s;
+}
+
+typedef struct {
+    double ti, px, py, x0, y0, kx0, ky0;
+    double complex G;      /* S_tun = i G */
+    double complex kdotF;  /* k(ts) . F(ts) */
+    double jac;
+} spa_point_t;
+
+/* One sample's saddle point and tunnelling integrals; returns 0 if the
+ * saddle-point solve fails (the reference's `ti == 0 && continue`). */
+static int spa_point(const sampler_t* S, double tr, double Ftr, double omega, double kd, double kz, int want_jac,
+                     spa_point_t* P) {
+    const laser_t* L = &S->L;
+    const double Ip = S->T.Ip;
+    pthread_once(&gl_once, gl_init);
+    const double ti = spa_solve(L, tr, Ftr, omega, Ip, kd, kz);
+    if (ti == 0) return 0;
+    double complex Fx, Fy, Ax, Ay;
+    laser_FA_c(L, tr + I * ti, &Fx, &Fy, &Ax, &Ay);
+    P->ti = ti;
+    spa_p(kd, Ax, Ay, &P->px, &P->py);
+    spa_ctx C = {L, tr, Ip, kz, P->px, P->py};
+    spa_ints in = spa_integrate(&C, 0.0, ti, 0);
+    P->x0 = in.x0;
+    P->y0 = in.y0;
+    P->G = in.G;
+    double Axr, Ayr;
+    laser_A(L, tr, &Axr, &Ayr);
+    P->kx0 = P->px + Axr;                                             /* :352-353 */
+    P->ky0 = P->py + Ayr;
+    P->kdotF = (P->px + Ax) * Fx + (P->py + Ay) * Fy;                 /* :272-277 */
+    P->jac = 0;
+    if (want_jac) {                                                   /* :280-292 */
+        const double dtr = 0.01, dkd = 0.01;
+        const double t_tp = spa_solve(L, tr + dtr, Ftr, omega, Ip, kd, kz);
+        const double t_tm = spa_solve(L, tr - dtr, Ftr, omega, Ip, kd, kz);
+        const double t_kp = spa_solve(L, tr, Ftr, omega, Ip, kd + dkd, kz);
+        const double t_km = spa_solve(L, tr, Ftr, omega, Ip, kd - dkd, kz);
+        double complex f1, f2, a1, b1, a2, b2;
+        double pxa, pya, pxb, pyb;
+        laser_FA_c(L, tr + dtr + I * t_tp, &f1, &f2, &a1, &b1);
+        laser_FA_c(L, tr - dtr + I * t_tm, &f1, &f2, &a2, &b2);
+        spa_p(kd, a1, b1, &pxa, &pya);
+        spa_p(kd, a2, b2, &pxb, &pyb);
+        const double dpxdtr = pxa - pxb, dpydtr = pya - pyb;
+        laser_FA_c(L, tr + I * t_kp, &f1, &f2, &a1, &b1);
+        laser_FA_c(L, tr + I * t_km, &f1, &f2, &a2, &b2);
+        spa_p(kd + dkd, a1, b1, &pxa, &pya);
+        spa_p(kd - dkd, a2, b2, &pxb, &pyb);
+        const double dpxdkd = pxa - pxb, dpydkd = pya - pyb;
+        P->jac = fabs(dpxdtr * dpydkd - dpxdkd * dpydtr) / (4 * dtr * dkd);
+    }
+    return 1;
+}
+
+/* omega of SPASampler.jl:160-166 (the initial guess of the saddle point) */
+static double spa_omega(const laser_t* L) {
+    if (L->n == 1) return L->c[0].w;
+    return (L->c[0].w + L->c[1].w) / 2;
+}
+
+int oracle_spa_point(const etraj_laser_desc* ld, const etraj_target_desc* td, double tr, double kd, double kz,
+                     double out[12]) {
+    sampler_t S;
+    memset(&S, 0, sizeof S);
+    if (laser_init(ld, &S.L) || target_init(td, &S.T)) return ETRAJ_EINVAL;
+    double Fx, Fy;
+    laser_F(&S.L, tr, &Fx, &Fy);
+    spa_point_t P;
+    if (!spa_point(&S, tr, hypot(Fx, Fy), spa_omega(&S.L), kd, kz, 1, &P)) return 1;
+    double v[12] = {P.ti, P.px, P.py, P.x0, P.y0, P.kx0, P.ky0, creal(P.G), cimag(P.G), creal(P.kdotF),
+                    cimag(P.kdotF), P.jac};
+    memcpy(out, v, sizeof v);
+    return ETRAJ_OK;
+}
+
+/* gen_electron_batch(sp::SPASampler, batch_id) for an atom with l = 0
+ * (SPASampler.jl:150-431). */
+static int64_t spa_gen_batch(const sampler_t* S, int b, double* out, int64_t cap) {
+    const etraj_sampler_desc* sd = S->sd;
+    const double tr = S->tsamp[b];
+    double Fx, Fy;
+    laser_F(&S->L, tr, &Fx, &Fy);
+    const double Ftr = hypot(Fx, Fy);
+    const double omega = spa_omega(&S->L);
+    if (Ftr <= 1e-3) return 0;                                        /* :167-169 (Q8) */
+    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    double gam = 0.0;                                                 /* :177-181 */
+    if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
+    const double kappa = sqrt(2 * Ip), n = Z / kappa;
+    const double e0 = 2.71828182845904523;
+    const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n);
+    const double numer = S->T.C * sqrt(1.0 / (4 * M_PI));
+    int64_t nk;
+    const double dkdt = sample_dkdt(S, &nk);
+    const int jac = (S->prefix & ETRAJ_PREFIX_JAC) != 0;
+    int64_t count = 0;
+    for (int64_t j = 0; j < nk; ++j) {
+        double kd0, kz0, w;
+        sample_k(S, b, j, dkdt, &kd0, &kz0, &w);
+        if (fabs(kd0) < 1e-4) continue;                               /* :345 */
+        spa_point_t P;
+        if (!spa_point(S, tr, Ftr, omega, kd0, kz0, jac, &P)) continue;
+        double complex amp = sqrt(w) * cexp(-P.G);                    /* sqrt(dkdt) exp(i S_tun) */
+        if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
+            const double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
+            amp *= cpre * numer / cpow(P.kdotF, (n + 1) / 2);         /* :271-278 */
+        }
+        if (jac) amp *= sqrt(P.jac);
+        const double rate = creal(amp) * creal(amp) + cimag(amp) * cimag(amp);
+        if (isnan(rate) || rate < sd->cutoff_limit) continue;         /* :357 */
+        if (put_column(sd, out, &count, cap, P.x0, P.y0, P.kx0, P.ky0, kz0, tr, rate, carg(amp))) return -1;
+    }
+    return count;
+}
+
 static int64_t gen_batch(const sampler_t* S, int b, double* out, int64_t cap) {
-    return S->sd->init_method == ETRAJ_INIT_WFAT ? wfat_gen_batch(S, b, out, cap) : adk_gen_batch(S, b, out, cap);
+    switch (S->sd->init_method) {
+        case ETRAJ_INIT_WFAT: return wfat_gen_batch(S, b, out, cap);
+        case ETRAJ_INIT_SPA: return spa_gen_batch(S, b, out, cap);
+        case ETRAJ_INIT_SPANE: return spane_gen_batch(S, b, out, cap);
+        default: return adk_gen_batch(S, b, out, cap);
+    }
 }
 
 int oracle_gen_electron_batch(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,

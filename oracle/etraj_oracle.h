@@ -28,6 +28,10 @@ int oracle_eom_rhs(const etraj_laser_desc* ld, const etraj_target_desc* td, int3
 
 void oracle_ff_mf_euler(const double mol_rot[3], double Fx, double Fy, double out[3]);
 void oracle_wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma, double out[2]);
+int oracle_laser_complex(const etraj_laser_desc* d, double t_re, double t_im, double out[8]);
+int oracle_laser_field_deriv(const etraj_laser_desc* d, double t, double out[4]);
+int oracle_spa_point(const etraj_laser_desc* ld, const etraj_target_desc* td, double tr, double kd, double kz,
+                     double out[12]);
 int oracle_batch_times(const etraj_sampler_desc* sd, double* t);
 int64_t oracle_batch_max_size(const etraj_sampler_desc* sd);
 int oracle_gen_electron_batch(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,

@@ -59,6 +59,9 @@ def lib():
         L.oracle_ff_mf_euler.restype = None
         L.oracle_wfat_g00.argtypes = [P(SamplerDesc), C.c_double, C.c_double, DP]
         L.oracle_wfat_g00.restype = None
+        L.oracle_laser_complex.argtypes = [P(LaserDesc), C.c_double, C.c_double, DP]
+        L.oracle_laser_field_deriv.argtypes = [P(LaserDesc), C.c_double, DP]
+        L.oracle_spa_point.argtypes = [P(LaserDesc), P(TargetDesc), C.c_double, C.c_double, C.c_double, DP]
         L.oracle_batch_times.argtypes = [P(SamplerDesc), DP]
         L.oracle_batch_max_size.argtypes = [P(SamplerDesc)]
         L.oracle_batch_max_size.restype = C.c_int64
@@ -129,6 +132,35 @@ def wfat_g00(sd, beta, gamma):
     out = np.empty(2)
     lib().oracle_wfat_g00(sd, beta, gamma, _d(out))
     return complex(out[0], out[1])
+
+
+def laser_complex(ld, t):
+    """(Fx, Fy, Ax, Ay) at complex time t."""
+    out = np.empty(8)
+    _chk(lib().oracle_laser_complex(ld, t.real, t.imag, _d(out)))
+    return tuple(complex(out[2 * i], out[2 * i + 1]) for i in range(4))
+
+
+def laser_field_deriv(ld, t):
+    """(Fx, Fy, dFx/dt, dFy/dt) at real time t."""
+    out = np.empty(4)
+    _chk(lib().oracle_laser_field_deriv(ld, t, _d(out)))
+    return tuple(out)
+
+
+def spa_point(ld, td, tr, kd, kz=0.0):
+    """SFA-SPA saddle point of one sample: dict or None if the solve fails."""
+    out = np.empty(12)
+    rc = lib().oracle_spa_point(ld, td, tr, kd, kz, _d(out))
+    if rc == 1:
+        return None
+    _chk(rc)
+    k = ("ti", "px", "py", "x0", "y0", "kx0", "ky0")
+    d = dict(zip(k, out[:7]))
+    d["G"] = complex(out[7], out[8])
+    d["kdotF"] = complex(out[9], out[10])
+    d["jac"] = out[11]
+    return d
 
 
 def batch_times(sd):

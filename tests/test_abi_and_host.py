@@ -162,9 +162,12 @@ def test_init_sampler_dispatch():
         warnings.simplefilter("ignore")
         sp = init_sampler(init_cond_method=":ADK", **kw)
     assert sp.desc.sampling == _abi.SAMPLING_STEP and sp.rows == 6
-    for m in ("SPA", "SPANE"):
-        with pytest.raises(NotImplementedError):
-            init_sampler(init_cond_method=m, **kw)
+    for m, cls in (("SPA", "SPASampler"), ("MOSFA", "SPASampler"), ("SPANE", "SPANESampler"),
+                   ("SFAAE", "SPANESampler")):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            sp = init_sampler(init_cond_method=m, **kw)
+        assert type(sp).__name__ == cls and sp.desc.init_method in (_abi.INIT_SPA, _abi.INIT_SPANE)
     with pytest.raises(TypeError):              # WFAT only supports molecular targets
         init_sampler(init_cond_method="WFAT", **kw)
     with pytest.raises(ValueError):

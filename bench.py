@@ -144,6 +144,143 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB device-to-device")
 
 
+
+# ---- the other configurations (python bench.py --workload NAME) -------------------
+
+def _scan_descs(jobs):
+    import warnings
+    from etraj_hip.samplers import init_sampler
+    ds, keep = [], []
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for kw in jobs:
+            sp = init_sampler(**kw)
+            keep.append(sp)
+            ds.append((kw["laser"]._desc(), kw["target"]._desc(), sp.desc, run_desc(**kw)))
+    return ds, keep
+
+
+def _run_scan(ctx, ds, device=True):
+    n = len(ds)
+    ld = (_abi.LaserDesc * n)(*[d[0] for d in ds])
+    sd = (_abi.SamplerDesc * n)(*[d[2] for d in ds])
+    st = (_abi.Stats * n)()
+    rd = ds[0][3]
+    npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
+    if device:
+        ptr = _abi.C.POINTER(_abi.C.c_double)()
+        ctx.check(ctx.lib.etraj_run_scan_device(ctx.handle, n, ld, ds[0][1], sd, rd, _abi.C.byref(ptr), st))
+        return None, [st[j] for j in range(n)]
+    out = np.empty(n * npix)
+    ctx.check(ctx.lib.etraj_run_scan(ctx.handle, n, ld, ds[0][1], sd, rd, _abi.dptr(out), st))
+    return out.reshape(n, npix), [st[j] for j in range(n)]
+
+
+def _timed(ctx, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sts.append(fn())
+    return (time.perf_counter() - t0) / steps, sts
+
+
+def _roofline_rk4(st_list, steps, flops32, flops64):
+    """Roofline of the FP32 pass from summed per-call stats (RK4 mixed runs)."""
+    s0 = st_list[0]
+    n_steps = sum(s.n_steps for s in st_list) / steps
+    n64 = sum(s.n_steps_fp64 for s in st_list) / steps
+    launch_ms = s0.ms_integrate_main / max(s0.n_slabs, 1)
+    f = flops32 * (n_steps - n64) / max(s0.n_slabs, 1)
+    ach = f / (launch_ms / 1e3) / 1e12
+    return {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+            "frac": ach / PEAK_TFLOPS["mixed"], "traffic": None, "kernel": "k_rk4 FP32 pass",
+            "launches_per_job": s0.n_slabs, "avg_launch_ms": launch_ms, "flops_per_launch": f,
+            "flops_per_step": {"fp32_pass": flops32, "fp64": flops64}}
+
+
+def _cpu_sample(ds, threads, n_jobs_sample=2, stride=None):
+    """The oracle on a bounded sample: every stride-th batch of the first n_jobs_sample jobs."""
+    import oracle as O
+    n = 0
+    t = time.perf_counter()
+    for (ld, td, sd, rd) in ds[:n_jobs_sample]:
+        s = stride or max(1, sd.sample_t_num // 8)
+        _, st = O.run_job(ld, td, sd, rd, nthreads=threads, batch_begin=s // 2, batch_stride=s)
+        n += st["num_effective_traj"]
+    return n, time.perf_counter() - t
+
+
+def bench_workload(name, args, ctx):
+    threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    if name == "h-ctmc-grid":
+        kw = workloads.h_lp_ctmc_grid(n_t=2000, n_kd=50)
+        jobs = [dict(kw, precision="mixed")]
+        desc = ("H-atom ADK CTMC, 800 nm 1e14 W/cm^2 2-cycle LP cos2, 2000 t0 x 50 kd grid, RK4 dt 0.1, "
+                "PMD 500x500")
+    elif name == "ar-spa-3d":
+        kw = workloads.ar_cp_spa_qtmc_3d(n_t=args.n_t if args.n_t != 10000 else 2000,
+                                         n_k=args.n_k if args.n_k != 32000 else 158)
+        jobs = [kw]
+        desc = (f"Ar (SAE, l=0 prefactor) SFA-SPA 3D grid {kw['sample_t_num']} x {kw['ss_kd_num']} x "
+                f"{kw['ss_kz_num']}, CP 800 nm 2e14 W/cm^2 2 cycles, Tsit5 rtol 1e-8, QTMC, PMD 256^3")
+    elif name == "bichromatic-scan":
+        ints = list(np.logspace(np.log10(5e13), np.log10(5e14), 32))
+        probe, _ = _scan_descs(workloads.bichromatic_scan_jobs(ints, n_t=200, n_k=200))
+        _, pst = _run_scan(ctx, probe)
+        y = np.array([s.ion_prob for s in pst])
+        surv = np.array([max(s.num_effective_traj, 1) / (200 * 200) for s in pst])
+        want = np.clip(1e8 * y / y.sum(), 1e5, 2e7)
+        n_t = 2000
+        n_k = np.maximum(1, np.ceil(want / (n_t * surv))).astype(int)
+        jobs = workloads.bichromatic_scan_jobs(ints, n_t=n_t, n_k=n_k)
+        desc = ("32 intensities 5e13-5e14 W/cm^2, bichromatic counter-rotating CP 800+400 nm 8/16 cycles, "
+                "H ADK+SCTS, counts per intensity proportional to the ADK yield (1e5-2e7), 2D PMD 512x512 each")
+    elif name == "wfat-orientation-scan":
+        jobs = workloads.wfat_orientation_jobs(args.scan_jobs or 64, n_t=2000, n_k=2000)
+        desc = (f"{len(jobs)} orientations (seeded uniform SO(3)), single-centre Z=1 Ip=0.45 WFAT G00 8x16 table, "
+                "1300 nm 4-cycle LP 1e14 W/cm^2, SCTS, 2D PMD 400x400 each")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    ds, keep = _scan_descs(jobs)
+    rk4 = ds[0][3].integrator == _abi.INTEG_RK4
+    mixed = rk4 and ds[0][3].precision == _abi.PREC_MIXED
+    sec, sts = _timed(ctx, lambda: _run_scan(ctx, ds)[1], args.steps, args.warmup)
+    n_el = sum(s.num_effective_traj for s in sts[-1])
+    result = {
+        "metric": "trajectories/s + PMD wall time, mixed-prec vs FP64 GPU at PMD L1<=1e-3",
+        "value": n_el / sec, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if mixed else "fp64", "data": "synthetic (seeded sampling)",
+        "config": {"workload": desc, "electrons": n_el, "jobs": len(ds)},
+        "pmd_wall_ms": sec * 1e3,
+        "device_ms": {"sample": sts[-1][0].ms_sample, "integrate": sts[-1][0].ms_integrate,
+                      "integrate_main": sts[-1][0].ms_integrate_main},
+        "fp64_reintegrated_fraction": sum(s.n_restep for s in sts[-1]) / max(n_el, 1),
+        # FP32-pass FLOPs per step: 2D SCTS hydrogen-like 149 (+12 for the molecule's soft-core term);
+        # 2D CTMC 116 (no phase: 4 RHS x 6 + phase combination 5 + phase Kahan 4 fewer)
+        "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 161.0, "h-ctmc-grid": 116.0}.get(name, 149.0),
+                                   134.0) if mixed else None),
+    }
+    if mixed:
+        ds64, keep64 = _scan_descs([dict(kw, precision="fp64") for kw in jobs])
+        p64, s64 = _run_scan(ctx, ds64, device=False)
+        pmx, smx = _run_scan(ctx, ds, device=False)
+        result["fp64_gpu"] = {"value": sum(s.num_effective_traj for s in s64) / (s64[0].ms_total / 1e3),
+                              "pmd_wall_ms": s64[0].ms_total}
+        result["mixed_over_fp64"] = (s64[0].ms_total / smx[0].ms_total)
+        result["pmd_l1_mixed_vs_fp64_gpu"] = max(l1(pmx[j], p64[j]) for j in range(len(ds)))
+    if args.cpu_batches > 0:
+        cds = [(ld, td, sd, rd) for (ld, td, sd, rd) in (ds64 if mixed else ds)]
+        n, dt = _cpu_sample(cds, threads, n_jobs_sample=min(2, len(cds)),
+                            stride=max(1, cds[0][2].sample_t_num // args.cpu_batches))
+        result["cpu_baseline"] = {"value": n / dt, "unit": "trajectories/s", "cores": threads, "kind": "port",
+                                  "sample": f"{n} electrons, every {max(1, cds[0][2].sample_t_num // args.cpu_batches)}"
+                                            f"-th batch of {min(2, len(cds))} job(s), the oracle's "
+                                            f"{'Tsit5' if not rk4 else 'FP64 RK4'}, {dt:.1f} s"}
+    return result
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +292,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM-counter calibration)")
+    ap.add_argument("--workload", default="h-scts-1e8",
+                    choices=["h-scts-1e8", "h-ctmc-grid", "ar-spa-3d", "bichromatic-scan", "wfat-orientation-scan"],
+                    help="h-scts-1e8 (default) is the headline; the others are the remaining configurations")
+    ap.add_argument("--scan-jobs", type=int, default=0, help="orientations of wfat-orientation-scan (default 64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
@@ -164,6 +305,9 @@ def main():
     ctx = _abi.default_context()
     if args.calibrate_copy > 0:
         copy_calibration(args.calibrate_copy)
+    if args.workload != "h-scts-1e8":
+        print(json.dumps(bench_workload(args.workload, args, ctx)), flush=True)
+        return
     kw = workloads.h_lp_scts_stratified(n_t=args.n_t, n_k=args.n_k, precision="mixed")
     d_mixed = descs(kw)
     npix = kw["final_p_num"][0] * kw["final_p_num"][1]

@@ -51,9 +51,34 @@ struct DevBuf {
 
 }  // namespace
 
+// The buffers of one slab in flight. run_job alternates two slots so that a
+// slab's FP64 re-integration (second stream) overlaps the next slab's
+// sampling and FP32 pass (first stream).
+struct SlabSlot {
+    DevBuf<double> init, seg_t0;
+    DevBuf<int32_t> counts, counts2, seg_job, seg_b;
+    DevBuf<int64_t> seg_start, pre;
+    DevBuf<float4> ft32;
+    DevBuf<double2> ft64, ft0;
+    DevBuf<uint8_t> flags;
+    std::vector<double> h_t0;
+    std::vector<int32_t> h_job, h_b;
+    std::vector<int64_t> h_start;
+    hipEvent_t done = nullptr;   // every kernel of the slot's last slab finished
+    bool pending = false;
+    void release() {
+        init.release(); seg_t0.release(); counts.release(); counts2.release(); seg_job.release(); seg_b.release();
+        seg_start.release(); pre.release(); ft32.release(); ft64.release(); ft0.release(); flags.release();
+        if (done) (void)hipEventDestroy(done);
+        done = nullptr;
+    }
+};
+
 struct etraj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;   // FP64 re-integration passes of run_job (default priority)
+    SlabSlot slot[2];
     std::string err;
     DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat;
     DevBuf<float4> ft32;
@@ -313,7 +338,12 @@ int64_t steps_of(double tf, double t0, double dt) {
 constexpr int kTsit5PerWave = 64 * 8;  // electrons per wavefront for lane refill
 
 constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
-constexpr int64_t kFieldTabBudget = (int64_t)1 << 30;  // bytes of field table per launch
+// Bytes of field table per launch and init columns per slab. Slabs this large
+// (per buffer slot: 4.8 GB of init columns at 9 rows, 8 GiB of field table)
+// keep the FP64 re-integration pass of one slab wide enough to fill the
+// device even when only a few per cent of the electrons are flagged.
+constexpr int64_t kFieldTabBudget = (int64_t)8 << 30;
+constexpr int64_t kMaxSlabCols = (int64_t)1 << 26;
 
 // Allocates and fills the field table of nseg segments whose t0 are in
 // t0s_dev and whose lasers are lps_dev[seg_job_dev[s]] (seg_job_dev NULL: lps_dev[0]).
@@ -460,9 +490,15 @@ int etraj_ctx_create(int device, etraj_ctx** out) {
     }
     double glq[64];
     gauss_legendre_32(glq);
-    if (c->glq.ensure(64) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess) {
+    if (c->glq.ensure(64) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->slot[0].done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->slot[1].done, hipEventDisableTiming) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
         c->glq.release();
+        c->slot[0].release();
+        c->slot[1].release();
         delete c;
         return ETRAJ_EHIP;
     }
@@ -474,12 +510,16 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     c->init.release(); c->pmd.release(); c->pmd_out.release(); c->io1.release(); c->io2.release();
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
+    c->slot[0].release();
+    c->slot[1].release();
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -581,7 +621,7 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     HIP_TRY(c, hipMemcpyAsync(c->seg_b.p, &b, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, &start0, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
-    HIP_TRY(c, launch_adk_sample(c->stream, S.method, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p));
+    HIP_TRY(c, launch_adk_sample(c->stream, S.method, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p, nullptr));
     int32_t cnt = 0;
     HIP_TRY(c, hipMemcpyAsync(&cnt, c->counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -724,7 +764,7 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
 
 // The batch loop of n_jobs independent jobs (TrajectorySimulation.jl:150-159
 // per job). The batches of all jobs, job-major, are cut into slabs of at most
-// 2^25 init columns and kFieldTabBudget bytes of field table; each slab is
+// kMaxSlabCols init columns and kFieldTabBudget bytes of field table; each slab is
 // one field-table launch, one sampler launch and one integrator launch (per
 // pass) over segments of any job. Tsit5 slabs hold one job each (its kernel
 // evaluates the job's laser directly). PMDs, scalars and counters are per job.
@@ -758,7 +798,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             stride = std::max(stride, steps_of(P.tf, jobs[j].t[b], P.dt));
         }
     }
-    const int64_t kMaxCols = (int64_t)1 << 25;
+    const int64_t kMaxCols = kMaxSlabCols;
     const int64_t seg_max = std::max<int64_t>(1, kFieldTabBudget / (stride * kFieldTabBytesPerStep));
     struct Slab { size_t i0, i1; int64_t cols; int64_t maxcap; };
     std::vector<Slab> slabs;
@@ -793,13 +833,26 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     HIP_TRY(c, c->cnt.ensure((size_t)CNT_N * n_jobs));
     HIP_TRY(c, c->sps.ensure(n_jobs));
     HIP_TRY(c, c->lps.ensure(n_jobs));
-    HIP_TRY(c, c->init.ensure((size_t)max_cols * rows));
-    HIP_TRY(c, c->counts.ensure(max_nb));
-    HIP_TRY(c, c->seg_start.ensure(max_nb));
-    HIP_TRY(c, c->seg_t0.ensure(max_nb));
-    HIP_TRY(c, c->seg_job.ensure(max_nb));
-    HIP_TRY(c, c->seg_b.ensure(max_nb));
-    HIP_TRY(c, c->pre.ensure(max_nb + 1));
+    // restep_radius <= 0: no electron is re-integrated, so no flags, compaction or FP64 pass
+    const bool two_pass = rk4 && P.mixed && P.restep_radius > 0;
+    const int nslab = (int)slabs.size();
+    for (int q = 0; q < std::min(2, nslab); ++q) {
+        SlabSlot& Q = c->slot[q];
+        HIP_TRY(c, Q.init.ensure((size_t)max_cols * rows));
+        HIP_TRY(c, Q.counts.ensure(max_nb));
+        HIP_TRY(c, Q.counts2.ensure(max_nb));
+        HIP_TRY(c, Q.seg_start.ensure(max_nb));
+        HIP_TRY(c, Q.seg_t0.ensure(max_nb));
+        HIP_TRY(c, Q.seg_job.ensure(max_nb));
+        HIP_TRY(c, Q.seg_b.ensure(max_nb));
+        HIP_TRY(c, Q.pre.ensure(max_nb + 1));
+        if (rk4) {
+            HIP_TRY(c, Q.ft32.ensure((size_t)max_nb * stride));
+            HIP_TRY(c, Q.ft64.ensure((size_t)2 * max_nb * stride));
+            HIP_TRY(c, Q.ft0.ensure(max_nb));
+        }
+        if (two_pass) HIP_TRY(c, Q.flags.ensure(max_cols));
+    }
     HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->lps.p, hlp.data(), sizeof(LaserP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * n_jobs * plen, c->stream));
@@ -811,92 +864,126 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     O.scal = c->scal.p;
     O.cnt = c->cnt.p;
     O.job_stride = plen;
-    const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
-    const int nslab = (int)slabs.size();
-    // per slab: start, sampler done, main integrator kernel done, slab done
-    std::vector<hipEvent_t> ev(4 * nslab, nullptr);
+    // per slab, first stream: start, sampled, main integrator kernel done, first-stream work done;
+    // second stream: FP64 pass start and end
+    std::vector<hipEvent_t> ev(6 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
-    std::vector<int64_t> launched(n_jobs, 0);
-    std::vector<double> h_t0;
-    std::vector<int32_t> h_job, h_b, hcounts;
-    std::vector<int64_t> h_start;
+    // The FP64 pass of slab k-1 is put on the second stream once slab k has
+    // been sampled, so that it overlaps slab k's FP32 pass (a kernel that
+    // fills the device) rather than slab k's short sampling kernels, which
+    // it would starve of dispatch slots.
+    struct Fp64Pass { int k = -1, nb = 0, cps = 0; SlabSlot* Q = nullptr; FieldTab FT{}; };
+    Fp64Pass prev;
+    auto launch_fp64 = [&](const Fp64Pass& f, hipEvent_t after) -> hipError_t {
+        hipStream_t B = c->stream2;
+        hipError_t e = hipStreamWaitEvent(B, after, 0);
+        if (e == hipSuccess) e = hipEventRecord(ev[6 * f.k + 4], B);
+        OutP Os = O;
+        Os.seg_job = f.Q->seg_job.p;
+        IntegP P64 = P;
+        P64.mixed = 0;
+        if (e == hipSuccess)
+            e = launch_rk4(B, P64, Os, OUT_PMD, f.Q->init.p, f.Q->seg_start.p, f.Q->counts2.p, f.nb, f.cps, f.FT,
+                           nullptr, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(ev[6 * f.k + 5], B);
+        if (e == hipSuccess) e = hipEventRecord(f.Q->done, B);
+        return e;
+    };
     int rc = ETRAJ_OK;
     for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
         const Slab& sl = slabs[k];
         const int nb = (int)(sl.i1 - sl.i0);
-        h_t0.resize(nb); h_job.resize(nb); h_b.resize(nb); h_start.resize(nb); hcounts.resize(nb);
+        SlabSlot& Q = c->slot[k % 2];
+        hipError_t e = hipSuccess;
+        if (Q.pending) {   // the slot's previous slab (device buffers and host staging) must be finished
+            e = hipEventSynchronize(Q.done);
+            Q.pending = false;
+        }
+        Q.h_t0.resize(nb); Q.h_job.resize(nb); Q.h_b.resize(nb); Q.h_start.resize(nb);
         int64_t col = 0;
         for (int i = 0; i < nb; ++i) {
             const Item& it = items[sl.i0 + i];
-            h_t0[i] = it.t0;
-            h_job[i] = it.job;
-            h_b[i] = it.b;
-            h_start[i] = col;
+            Q.h_t0[i] = it.t0;
+            Q.h_job[i] = it.job;
+            Q.h_b[i] = it.b;
+            Q.h_start[i] = col;
             col += it.cap;
         }
-        hipError_t e = hipMemcpyAsync(c->seg_t0.p, h_t0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_job.p, h_job.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_b.p, h_b.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->seg_start.p, h_start.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[4 * k], c->stream);
-        FieldTab FT{};
-        if (e == hipSuccess && rk4) {
-            if (int frc = fill_field_table(c, P, c->lps.p, c->seg_job.p, c->seg_t0.p, nb, stride, FT)) {
-                rc = frc;
-                break;
-            }
-        }
+        hipStream_t A = c->stream;
+        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_t0.p, Q.h_t0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, A);
+        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_job.p, Q.h_job.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, A);
+        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_b.p, Q.h_b.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, A);
+        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_start.p, Q.h_start.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, A);
+        if (e == hipSuccess) e = hipEventRecord(ev[6 * k], A);
+        FieldTab FT{Q.ft32.p, Q.ft64.p, Q.ft0.p, stride};
+        if (e == hipSuccess && rk4) e = launch_field_table(A, c->lps.p, Q.seg_job.p, Q.seg_t0.p, nb, P.tf, P.dt, FT);
+        const SegTab G{Q.seg_t0.p, Q.seg_job.p, Q.seg_b.p, Q.seg_start.p};
         if (e == hipSuccess)
-            e = launch_adk_sample(c->stream, jobs[0].S.method, c->sps.p, c->lps.p, T, G, nb, c->init.p, c->counts.p);
-        if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 1], c->stream);
-        if (e == hipSuccess && rk4) {
-            OutP Os = O;
-            Os.seg_job = c->seg_job.p;
-            if (int prc = rk4_passes(c, P, Os, OUT_PMD, c->seg_start.p, c->counts.p, nb, (int)((sl.maxcap + 63) / 64),
-                                     FT, sl.cols, ev[4 * k + 2])) {
-                rc = prc;
-                break;
-            }
+            e = launch_adk_sample(A, jobs[0].S.method, c->sps.p, c->lps.p, T, G, nb, Q.init.p, Q.counts.p, c->cnt.p);
+        if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 1], A);
+        if (e == hipSuccess && prev.k >= 0) {
+            e = launch_fp64(prev, ev[6 * k + 1]);
+            prev.k = -1;
         }
-        if (e == hipSuccess && !rk4) {
-            const int32_t j = items[sl.i0].job;
+        if (rk4) {
+            OutP Os = O;
+            Os.seg_job = Q.seg_job.p;
+            const int cps = (int)((sl.maxcap + 63) / 64);
+            if (e == hipSuccess)
+                e = launch_rk4(A, P, Os, OUT_PMD, Q.init.p, Q.seg_start.p, Q.counts.p, nb, cps, FT,
+                               two_pass ? Q.flags.p : nullptr, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
+            if (two_pass) {
+                // the electrons flagged by the FP32 pass, integrated again in FP64 on the second
+                // stream (launched after the next slab's sampling, or after the loop)
+                if (e == hipSuccess)
+                    e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p, nullptr,
+                                               Q.counts2.p, nullptr);
+                if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);
+                prev.k = k;
+                prev.nb = nb;
+                prev.cps = cps;
+                prev.Q = &Q;
+                prev.FT = FT;
+            }
+        } else {
+            const int32_t j = items[sl.i0].job;   // a Tsit5 slab holds one job
             IntegP Pj = P;
             Pj.L = jobs[j].L;
             OutP Oj = O;
             Oj.pmd = O.pmd + j * plen;
             Oj.scal = O.scal + 2 * j;
             Oj.cnt = O.cnt + (int64_t)j * CNT_N;
-            e = launch_scan_counts(c->stream, c->counts.p, nb, c->pre.p);
+            if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(c->stream, Pj, Oj, OUT_PMD, c->init.p, c->seg_start.p, c->pre.p, nb, sl.cols,
-                                 kTsit5PerWave);
-            if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 2], c->stream);
+                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave);
+            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
         }
-        if (e == hipSuccess) e = hipEventRecord(ev[4 * k + 3], c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hcounts.data(), c->counts.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            rc = hip_fail(c, e, "run_job slab");
-            break;
+        if (!two_pass) {
+            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);
+            if (e == hipSuccess) e = hipEventRecord(Q.done, A);
         }
-        for (int i = 0; i < nb; ++i) launched[items[sl.i0 + i].job] += hcounts[i];
+        Q.pending = true;
+        if (e != hipSuccess) rc = hip_fail(c, e, "run_job slab");
     }
-    double ms_s = 0, ms_i = 0, ms_m = 0;
-    if (rc == ETRAJ_OK) {
-        for (int k = 0; k < nslab; ++k) {
-            float a = 0, b = 0, m = 0;
-            (void)hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]);
-            (void)hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 3]);
-            (void)hipEventElapsedTime(&m, ev[4 * k + 1], ev[4 * k + 2]);
-            ms_s += a;
-            ms_i += b;
-            ms_m += m;
-        }
+    if (rc == ETRAJ_OK && prev.k >= 0) {
+        hipError_t e = launch_fp64(prev, ev[6 * prev.k + 3]);
+        if (e != hipSuccess) rc = hip_fail(c, e, "run_job FP64 pass");
     }
-    for (auto& e : ev)
-        if (e) (void)hipEventDestroy(e);
-    if (rc) return rc;
+    // the first stream finishes the job after every FP64 pass
+    for (int q = 0; q < 2; ++q)
+        if (c->slot[q].pending && rc == ETRAJ_OK) {
+            hipError_t e = hipStreamWaitEvent(c->stream, c->slot[q].done, 0);
+            if (e != hipSuccess) rc = hip_fail(c, e, "run_job join");
+        }
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream2);
+        (void)hipStreamSynchronize(c->stream);
+        c->slot[0].pending = c->slot[1].pending = false;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        return rc;
+    }
 
     const double* result = c->pmd.p;
     if (coh) {
@@ -911,6 +998,20 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     if (host_out)
         HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix * n_jobs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->slot[0].pending = c->slot[1].pending = false;
+    double ms_s = 0, ms_i = 0, ms_m = 0;
+    for (int k = 0; k < nslab; ++k) {
+        float a = 0, b = 0, m = 0, f = 0;
+        (void)hipEventElapsedTime(&a, ev[6 * k], ev[6 * k + 1]);
+        (void)hipEventElapsedTime(&b, ev[6 * k + 1], ev[6 * k + 3]);
+        (void)hipEventElapsedTime(&m, ev[6 * k + 1], ev[6 * k + 2]);
+        if (two_pass) (void)hipEventElapsedTime(&f, ev[6 * k + 4], ev[6 * k + 5]);
+        ms_s += a;
+        ms_i += b + f;
+        ms_m += m;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (dev_out) *dev_out = result;
     if (stats) {
         const double ms_total =
@@ -920,7 +1021,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             const int64_t* cj = cnt.data() + (size_t)CNT_N * j;
             s.ion_prob = scal[2 * j];
             s.ion_prob_uncollected = scal[2 * j + 1];
-            s.num_effective_traj = launched[j];
+            s.num_effective_traj = cj[CNT_LAUNCHED];
             s.n_failed = cj[CNT_FAILED];
             s.n_bound = cj[CNT_BOUND];
             s.n_restep = cj[CNT_RESTEP];

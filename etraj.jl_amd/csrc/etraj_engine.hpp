@@ -55,7 +55,8 @@ struct Contrib {
 
 enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
 // CNT_RESTEP counts the electrons of the mixed pass re-integrated in FP64
-enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_N = 5 };
+// CNT_LAUNCHED counts the sampled electrons (the sampler kernel adds each batch's survivors)
+enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_LAUNCHED = 5, CNT_N = 6 };
 
 // Per-segment laser field table (k_field_table): entry s of a segment holds
 // the field at half-steps 2s+1 and 2s+2 of that segment's step grid.
@@ -95,8 +96,9 @@ hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, 
                               double* Ax, double* Ay);
 hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F);
 // sps / lps: device arrays of per-job sampler and laser parameters
+// cnt: per-job counters (CNT_N per job) receiving CNT_LAUNCHED, or NULL
 hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
-                             const SegTab& G, int nb, double* out, int32_t* counts);
+                             const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt);
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
                       const FieldTab& FT, uint8_t* flags, const int64_t* out_index);

@@ -389,7 +389,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
 template <int M>
 __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__ sps, const LaserP* __restrict__ lps,
                                                     TargetP T, SegTab G, double* __restrict__ out,
-                                                    int32_t* __restrict__ counts) {
+                                                    int32_t* __restrict__ counts, int64_t* __restrict__ cnt) {
     __shared__ int wave_tot[4];
     __shared__ int base_sh;
     const int g = blockIdx.x;
@@ -421,7 +421,11 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
         if (tid == 0) base_sh += tot;
         __syncthreads();
     }
-    if (tid == 0) counts[g] = base_sh;
+    if (tid == 0) {
+        counts[g] = base_sh;
+        if (cnt && base_sh) atomicAdd((unsigned long long*)(cnt + (int64_t)job * CNT_N + CNT_LAUNCHED),
+                                      (unsigned long long)base_sh);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -641,16 +645,11 @@ __device__ inline void rk4_fp32_pass(const IntegP& P, bool saeq, const Lane<DIM,
 // amplifies are accumulated before it, so the whole trajectory is redone.
 // out_index (may be NULL) maps a column to the index of its record/contrib.
 template <int DIM, int PHASE, bool MIXED, int OUT, int TGT>
-__global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
-                                             const int64_t* __restrict__ seg_start,
-                                             const int32_t* __restrict__ seg_count, int64_t n_seg,
-                                             int chunks_per_seg, FieldTab FT, uint8_t* __restrict__ flags,
-                                             const int64_t* __restrict__ out_index) {
+__device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* __restrict__ init,
+                                 const int64_t* __restrict__ seg_start, const int32_t* __restrict__ seg_count,
+                                 int64_t seg, int chunk, const FieldTab& FT, uint8_t* __restrict__ flags,
+                                 const int64_t* __restrict__ out_index) {
     const int lane = lane_id();
-    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int64_t seg = gw / chunks_per_seg;
-    const int chunk = (int)(gw - seg * chunks_per_seg);
-    if (seg >= n_seg) return;
     const int32_t cnt = seg_count[seg];
     const int first = chunk * 64;
     if (first >= cnt) return;
@@ -707,7 +706,7 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
             rk4_fp32_pass<DIM, PHASE, kTargetHSoft>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
         else
             rk4_fp32_pass<DIM, PHASE, TGT>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
-        if (active) flags[col_idx] = close ? 1 : 0;
+        if (active && flags) flags[col_idx] = close ? 1 : 0;
     }
 #pragma unroll
     for (int i = 0; i < DIM; ++i) bad = bad || isnan(rf[i]) || isnan(pf[i]);
@@ -725,6 +724,21 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
             if (nclose) atomicAdd((unsigned long long*)(Oj.cnt + CNT_RESTEP), (unsigned long long)nclose);
         }
     }
+}
+
+// Static launch: wave w takes chunk w % chunks_per_seg of segment
+// w / chunks_per_seg (waves of empty chunks leave at once).
+template <int DIM, int PHASE, bool MIXED, int OUT, int TGT>
+__global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __restrict__ init,
+                                             const int64_t* __restrict__ seg_start,
+                                             const int32_t* __restrict__ seg_count, int64_t n_seg,
+                                             int chunks_per_seg, FieldTab FT, uint8_t* __restrict__ flags,
+                                             const int64_t* __restrict__ out_index) {
+    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int64_t seg = gw / chunks_per_seg;
+    if (seg >= n_seg) return;
+    rk4_chunk<DIM, PHASE, MIXED, OUT, TGT>(P, O, init, seg_start, seg_count, seg, (int)(gw - seg * chunks_per_seg), FT,
+                                            flags, out_index);
 }
 
 // In-place, order-preserving compaction of the flagged columns of each
@@ -985,20 +999,20 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
 }
 
 hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
-                             const SegTab& G, int nb, double* out, int32_t* counts) {
+                             const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt) {
     if (nb <= 0) return hipSuccess;
     switch (method) {
         case ETRAJ_INIT_SPA:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPA>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPA>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
             break;
         case ETRAJ_INIT_SPANE:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPANE>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPANE>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
             break;
         case ETRAJ_INIT_WFAT:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_WFAT>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_WFAT>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
             break;
         default:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_ADK>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts);
+            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_ADK>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
     }
     return hipGetLastError();
 }
@@ -1007,8 +1021,7 @@ template <int DIM, int PHASE, bool MIXED, int OUT>
 static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
                                const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
                                const FieldTab& FT, uint8_t* flags, const int64_t* out_index) {
-    int64_t waves = n_seg * (int64_t)cps;
-    unsigned blocks = (unsigned)((waves + 3) / 4);
+    const unsigned blocks = (unsigned)((n_seg * (int64_t)cps + 3) / 4);
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
         hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st,
                            P, O, init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
@@ -1024,7 +1037,8 @@ static hipError_t launch_rk4_o(hipStream_t st, const IntegP& P, const OutP& O, i
                                uint8_t* fl, const int64_t* oi) {
     switch (out) {
         case OUT_PMD: return launch_rk4_t<DIM, PHASE, MIXED, OUT_PMD>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
-        case OUT_RECORDS: return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
+        case OUT_RECORDS:
+            return launch_rk4_t<DIM, PHASE, MIXED, OUT_RECORDS>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
         default: return launch_rk4_t<DIM, PHASE, MIXED, OUT_CONTRIB>(st, P, O, init, ss, sc, ns, cps, FT, fl, oi);
     }
 }
@@ -1042,8 +1056,10 @@ static hipError_t launch_rk4_p(hipStream_t st, const IntegP& P, const OutP& O, i
                                const int64_t* ss, const int32_t* sc, int64_t ns, int cps, const FieldTab& FT,
                                uint8_t* fl, const int64_t* oi) {
     switch (P.phase) {
-        case ETRAJ_PHASE_CTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
-        case ETRAJ_PHASE_QTMC: return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
+        case ETRAJ_PHASE_CTMC:
+            return launch_rk4_m<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
+        case ETRAJ_PHASE_QTMC:
+            return launch_rk4_m<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
         default: return launch_rk4_m<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, sc, ns, cps, FT, fl, oi);
     }
 }

@@ -138,13 +138,24 @@ prefix_bits(p::Symbol) = p == :Exp ? 0 : p == :Full ? 6 : p == :Pre ? 1 : p == :
                          error("[ADKSampler] Undefined tunneling rate prefix [$p].")
 prefix_bits(p) = isempty(p) ? 0 : reduce(|, prefix_bits.(collect(p)))
 
+# init_cond_method -> ETRAJ_INIT_* (ElectronSamplers.jl:11-24 aliases)
+init_method(m::Symbol) = m in (:ADK, :MOADK) ? Int32(1) : m in (:SPA, :MOSPA, :SFA, :MOSFA) ? Int32(2) :
+                         m in (:SPANE, :MOSPANE, :SFAAE, :MOSFAAE) ? Int32(3) : m == :WFAT ? Int32(4) :
+                         error("[ElectronSamplers] Undefined initial condition method [$m].")
+
+sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_t_num, ss_kd_max=0.0, ss_kd_num=0,
+             ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC, sample_cutoff_limit=1e-16, rate_prefix=:Full,
+             kwargs...) =
+    SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0,
+                sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
+                ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, (0.0, 0.0, 0.0), 0, 0, C_NULL)
+
 function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max, final_p_num,
                      ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC,
                      sample_cutoff_limit=1e-16, rate_prefix=:Full, dt=0.1, precision=:mixed, restep_radius=2.0,
-                     device=0)
-    sd = Ref(SamplerDesc(1, dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0, sample_t_num,
-                         ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
-                         ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, (0.0, 0.0, 0.0), 0, 0, C_NULL))
+                     init_cond_method=:ADK, device=0)
+    sd = Ref(sampler_desc(; init_cond_method, dimension, sample_t_intv, sample_t_num, ss_kd_max, ss_kd_num, ss_kz_max,
+                          ss_kz_num, traj_phase_method, sample_cutoff_limit, rate_prefix))
     rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
     spec = zeros(Float64, final_p_num...)
     st = Stats()
@@ -159,6 +170,30 @@ function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, tr
             num_effective_traj = st.num_effective_traj)
 end
 
-export launch_and_collect_hip!, run_job_hip
+# ---- parameter scans: the reference's loops of perform_traj_simulation calls --------
+# (examples/test_Bichromatic_CCP.jl:4-27) as one etraj_run_scan; `lasers` and the
+# sampler keyword sets differ per point, target and run settings are shared.
+function run_scan_hip(lasers::AbstractVector; target, dimension, sample_t_intv, sample_t_num, traj_t_final,
+                      final_p_max, final_p_num, traj_phase_method=:CTMC, dt=0.1, precision=:mixed,
+                      restep_radius=2.0, device=0, kwargs...)
+    n = length(lasers)
+    lds = [laser_desc(l) for l in lasers]
+    sds = [sampler_desc(; dimension, sample_t_intv, sample_t_num, traj_phase_method, kwargs...) for _ in 1:n]
+    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
+    npix = prod(final_p_num)
+    specs = zeros(Float64, npix, n)
+    stv = Vector{NTuple{13,Float64}}(undef, n)   # etraj_stats is 13 eight-byte fields
+    c = Ctx(device)
+    GC.@preserve lds sds specs stv begin
+        rc = ccall((:etraj_run_scan, libetraj), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{LaserDesc}, Ref{TargetDesc}, Ptr{SamplerDesc}, Ref{RunDesc}, Ptr{Float64},
+                    Ptr{Cvoid}),
+                   c.h, n, lds, Ref(target_desc(target)), sds, rd, specs, stv)
+        check(c, rc)
+    end
+    return [reshape(specs[:, j], final_p_num...) for j in 1:n]
+end
+
+export launch_and_collect_hip!, run_job_hip, run_scan_hip
 
 end # module

@@ -89,8 +89,8 @@ def test_scan_jobs_of_different_sizes(ctx):
 
 @pytest.mark.gpu
 def test_scan_slabs_cross_job_boundaries(ctx):
-    # 2 jobs x 600 batches x 32000 columns = 3.8e7 columns > 2^25: two slabs, the first ending inside job 1
-    kws = _intensity_jobs("mixed", n_t=600, n_k=32000, ints=(1e14, 1.5e14))
+    # 2 jobs x 1100 batches x 32000 columns = 7.0e7 columns > 2^26: two slabs, the first ending inside job 1
+    kws = _intensity_jobs("mixed", n_t=1100, n_k=32000, ints=(1e14, 1.5e14))
     gst = _assert_scan_equals_singles(ctx, kws)
     assert gst[0]["n_slabs"] == 2
 

@@ -201,8 +201,21 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     const etraj_sampler_desc& s = *sd;
     if (s.init_method < ETRAJ_INIT_ADK || s.init_method > ETRAJ_INIT_WFAT)
         return fail(c, ETRAJ_EUNSUPPORTED, "init_cond_method: the engine implements :ADK, :SPA, :SPANE and :WFAT");
-    if (s.init_method == ETRAJ_INIT_WFAT && (s.wfat_na < 1 || s.wfat_nb < 1 || !s.wfat_coeff))
-        return fail(c, ETRAJ_EINVAL, "[WFATSampler] the structure-factor table (wfat_na, wfat_nb, wfat_coeff) is missing");
+    if (s.init_method == ETRAJ_INIT_WFAT) {
+        if (s.wfat_na < 1 || s.wfat_nb < 1 || !s.wfat_coeff)
+            return fail(c, ETRAJ_EINVAL, "[WFATSampler] the structure-factor table (wfat_na, wfat_nb, wfat_coeff) is missing");
+        if (s.wfat_form != ETRAJ_WFAT_TABLE && s.wfat_form != ETRAJ_WFAT_INTDATA)
+            return fail(c, ETRAJ_EINVAL, "[WFATSampler] undefined structure-factor form (wfat_form)");
+        if (s.wfat_form == ETRAJ_WFAT_INTDATA) {
+            const int32_t* d = s.wfat_dims;
+            if (d[0] < 1 || d[1] < 1 || (d[1] & 1) == 0 || d[2] < 1 || d[3] != 2 * d[2] - 1 || d[0] > kWfatMaxDim ||
+                d[1] > kWfatMaxDim || d[2] > kWfatMaxDim || d[0] * d[1] > kWfatMaxChannels)
+                return fail(c, ETRAJ_EINVAL, "[WFATSampler] wfat_dims must be (nxi_max+1, 2 m_max+1, l_max+1, 2 l_max+1), "
+                                             "each <= 31, with at most 512 channels");
+            if (!(std::isfinite(s.wfat_mu[0]) && std::isfinite(s.wfat_mu[1]) && std::isfinite(s.wfat_mu[2])))
+                return fail(c, ETRAJ_EINVAL, "[WFATSampler] the orbital dipole wfat_mu is not finite");
+        }
+    }
     if (s.phase_method < 0 || s.phase_method > 2) return fail(c, ETRAJ_EINVAL, "[ADKSampler] Undefined phase method");
     if (s.dimension != 2 && s.dimension != 3) return fail(c, ETRAJ_EINVAL, "`dimension` must be either 2 or 3");
     if (s.rate_prefix & ~(ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC | ETRAJ_PREFIX_JAC))
@@ -270,7 +283,9 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     } else {
         S.dkdt_base = (s.sample_t_hi - s.sample_t_lo) / s.sample_t_num / s.mc_kt_num;
     }
-    const double kappa = std::sqrt(2 * T.Ip), nn = T.Zd / kappa;
+    // the sampler's Ip: IonPotential(mol, ridx) for WFAT (WFATSampler.jl:139)
+    S.Ip = (s.init_method == ETRAJ_INIT_WFAT && s.wfat_ip > 0) ? s.wfat_ip : T.Ip;
+    const double kappa = std::sqrt(2 * S.Ip), nn = T.Zd / kappa;
     S.nstar = nn;
     S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
     S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
@@ -280,7 +295,12 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     if (s.init_method == ETRAJ_INIT_WFAT) {   // the table is uploaded by the caller (S.wfat_coeff)
         S.wfat_na = s.wfat_na;
         S.wfat_nb = s.wfat_nb;
-        for (int k = 0; k < 3; ++k) S.mol_rot[k] = s.mol_rot[k];
+        S.wfat_form = s.wfat_form;
+        for (int k = 0; k < 4; ++k) S.wfat_dims[k] = s.wfat_dims[k];
+        for (int k = 0; k < 3; ++k) {
+            S.mol_rot[k] = s.mol_rot[k];
+            S.wfat_mu[k] = s.wfat_mu[k];
+        }
     }
     return ETRAJ_OK;
 }
@@ -598,7 +618,7 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     if (!count || (capacity > 0 && !out)) return fail(c, ETRAJ_EINVAL, "NULL output");
     HIP_TRY(c, hipSetDevice(c->device));
     if (S.method == ETRAJ_INIT_WFAT) {
-        const size_t nc = (size_t)2 * S.wfat_na * S.wfat_nb;
+        const size_t nc = wfat_doubles(S);
         HIP_TRY(c, c->wfat.ensure(nc));
         HIP_TRY(c, hipMemcpyAsync(c->wfat.p, sampler->wfat_coeff, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
         S.wfat_coeff = c->wfat.p;
@@ -744,7 +764,7 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
             if (n_jobs > 1) c->err = "job " + std::to_string(j) + ": " + c->err;
             return rc;
         }
-        if (J.S.method == ETRAJ_INIT_WFAT) ncoef += (size_t)2 * J.S.wfat_na * J.S.wfat_nb;
+        ncoef += wfat_doubles(J.S);
     }
     if (ncoef) {
         HIP_TRY(c, c->wfat.ensure(ncoef));
@@ -752,7 +772,7 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
         for (int32_t j = 0; j < n_jobs; ++j) {
             SamplerP& S = jobs[j].S;
             if (S.method != ETRAJ_INIT_WFAT) continue;
-            const size_t nc = (size_t)2 * S.wfat_na * S.wfat_nb;
+            const size_t nc = wfat_doubles(S);
             HIP_TRY(c, hipMemcpyAsync(c->wfat.p + off, samplers[j].wfat_coeff, sizeof(double) * nc,
                                       hipMemcpyHostToDevice, c->stream));
             S.wfat_coeff = c->wfat.p + off;

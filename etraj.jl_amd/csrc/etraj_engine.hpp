@@ -11,6 +11,10 @@ namespace etraj {
 
 // Resolved ADK sampler parameters for one job (ADKSampler.jl:19-150 + the
 // per-job constants of gen_electron_batch).
+// WFAT integral tables (ETRAJ_WFAT_INTDATA): each dimension <= kWfatMaxDim,
+// at most kWfatMaxChannels channels (nxi, m), |m| <= kWfatMaxM
+constexpr int kWfatMaxDim = 31, kWfatMaxChannels = 512, kWfatMaxM = 15;
+
 struct SamplerP {
     int dim, phase, sampling, prefix;
     int t_num, kd_num, kz_num, rows;
@@ -29,7 +33,11 @@ struct SamplerP {
     const double* glq; // SPA: 32 Gauss-Legendre (node, weight) pairs on [-1, 1]
     int wfat_na, wfat_nb;
     double mol_rot[3];
-    const double* wfat_coeff;  // device copy of the G00 table (re, im), a-major
+    const double* wfat_coeff;  // device copy of the structure-factor data (re, im), layout per wfat_form
+    int wfat_form;             // ETRAJ_WFAT_*
+    int wfat_dims[4];          // ETRAJ_WFAT_INTDATA: (nxi_max+1, 2 m_max+1, l_max+1, 2 l_max+1)
+    double wfat_mu[3];
+    double Ip;                 // the sampler's Ip: the ionizing orbital's for WFAT, else the target's
 };
 
 struct IntegP {
@@ -99,6 +107,13 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
 // cnt: per-job counters (CNT_N per job) receiving CNT_LAUNCHED, or NULL
 hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
                              const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt);
+// Doubles of the WFAT structure-factor data of a sampler (0 for other samplers)
+inline size_t wfat_doubles(const SamplerP& S) {
+    if (S.method != ETRAJ_INIT_WFAT) return 0;
+    if (S.wfat_form == ETRAJ_WFAT_INTDATA)
+        return (size_t)2 * S.wfat_dims[0] * S.wfat_dims[1] * S.wfat_dims[2] * S.wfat_dims[3];
+    return (size_t)2 * S.wfat_na * S.wfat_nb;
+}
 hipError_t launch_rk4(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                       const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int chunks_per_seg,
                       const FieldTab& FT, uint8_t* flags, const int64_t* out_index);

@@ -97,7 +97,10 @@ __global__ void k_target_eval(TargetP T, const double* __restrict__ xyz, int64_t
 // ---------------------------------------------------------------------------
 struct AdkBatch {
     double tr, F, cphi, sphi, c_cc, dkdt;
-    double g2;          // WFAT: |G00(beta, gamma)|^2 of this batch's field direction
+    double g2;          // WFAT table form: |G00(beta, gamma)|^2 of this batch's field direction
+    double beta, gamma; // WFAT: Euler angles FF -> MF of this batch (WFATSampler.jl:143-145)
+    const double* wB;   // WFAT: B_|m| (LDS), rate = dkdt E(k) sum_|m| B_|m| k^(2|m|)
+    int wM;             // WFAT: m_max of wB
     double dFx, dFy;    // SPANE: dF/dt at tr
     double cross;       // SPANE: Fx dFy - dFx Fy (SPANESampler.jl:238)
     bool skip;          // SPA: F(tr) <= 1e-3 empties the batch (SPASampler.jl:167-169)
@@ -126,6 +129,58 @@ __device__ inline void ff_mf_euler(const double* mr, double Fx, double Fy, doubl
     sincos(t1, &st, &ct);
     beta = atan2(ct * R[0][2] + st * R[1][2], R[2][2]);
     gamma = atan2(-st * R[0][0] + ct * R[1][0], -st * R[0][1] + ct * R[1][1]);
+}
+
+// Wigner small-d d^l_{m m'}(beta) = <l m| exp(-i beta J_y) |l m'> by Wigner's
+// formula: sum_s (-1)^(m-m'+s) sqrt((l+m)!(l-m)!(l+m')!(l-m')!) /
+// ((l+m'-s)! s! (m-m'+s)! (l-m-s)!) cos(beta/2)^(2l+m'-m-2s) sin(beta/2)^(m-m'+2s).
+// l <= 30 (factorials to 60! stay finite in FP64).
+__device__ inline double wigner_d(int l, int m, int mp, double beta) {
+    double c, sn;
+    sincos(0.5 * beta, &sn, &c);
+    auto fact = [](int n) {
+        double f = 1.0;
+        for (int i = 2; i <= n; ++i) f *= i;
+        return f;
+    };
+    const double pre = sqrt(fact(l + m) * fact(l - m) * fact(l + mp) * fact(l - mp));
+    const int s0 = max(0, mp - m), s1 = min(l + mp, l - m);
+    double sum = 0.0;
+    for (int k = s0; k <= s1; ++k) {
+        const double den = fact(l + mp - k) * fact(k) * fact(m - mp + k) * fact(l - m - k);
+        const double t = pre / den * pow(c, 2 * l + mp - m - 2 * k) * pow(sn, m - mp + 2 * k);
+        sum += ((m - mp + k) & 1) ? -t : t;
+    }
+    return sum;
+}
+
+// |G_{nxi m}(beta, gamma)|^2 exp(-2 kappa mu_z) of one channel from the WFAT
+// integrals (MolWFATStructureFactor_G, GenericMolecule.jl:236-260). The
+// integrals are Julia's Array{ComplexF64,4} I[nxi, m + m_max, l, m' + l]
+// (0-based), column-major, (re, im).
+__device__ inline double wfat_channel_g2(const SamplerP& S, double kappa, double beta, double gamma, int nxi, int m) {
+    const int D0 = S.wfat_dims[0], D1 = S.wfat_dims[1], D2 = S.wfat_dims[2];
+    const int mM = (D1 - 1) / 2, L = D2 - 1;
+    double gre = 0.0, gim = 0.0;
+    for (int l = abs(m); l <= L; ++l)
+        for (int mp = -l; mp <= l; ++mp) {
+            const double* I = S.wfat_coeff + 2 * (nxi + (int64_t)D0 * ((m + mM) + (int64_t)D1 * (l + (int64_t)D2 * (mp + l))));
+            const double d = wigner_d(l, m, mp, beta);
+            double sg, cg;
+            sincos(mp * gamma, &sg, &cg);   // e^{-i m' gamma} = cg - i sg
+            gre += d * (I[0] * cg + I[1] * sg);
+            gim += d * (I[1] * cg - I[0] * sg);
+        }
+    const double muz = -sin(beta) * S.wfat_mu[0] + cos(beta) * S.wfat_mu[2];   // (RotZYZ(gamma, beta, 0) mu)_z
+    return (gre * gre + gim * gim) * exp(-2.0 * kappa * muz);
+}
+
+// Field-factor prefactor of channel (nxi, |m|) (WFATSampler.jl:157 without
+// its k-dependent factors): kappa^(|m|+2) / (2 F^(|m|+1) |m|!) (4 kappa^2/F)^(2n - 2nxi - |m| - 1).
+__device__ inline double wfat_field_prefactor(double kappa, double F, double nstar, int nxi, int am) {
+    double f = 1.0;
+    for (int i = 2; i <= am; ++i) f *= i;
+    return pow(kappa, am + 2) / (2.0 * pow(F, am + 1) * f) * pow(4 * kappa * kappa / F, 2 * nstar - 2 * nxi - am - 1);
 }
 
 // |G00|^2 with G00 = sum_{a,b} c_ab P_a(cos beta) e^{-i b gamma}
@@ -166,7 +221,7 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     B.F = hypot(Fx, Fy);                                            // :172
     double phi = atan2(-Fy, -Fx);                                   // :173
     sincos(phi, &B.sphi, &B.cphi);
-    double Ip = T.Ip;
+    double Ip = S.Ip;
     double gam = 0.0;                                               // :183-187
     if (L.n == 1) gam = L.c[0].w * sqrt(2 * Ip) / (L.c[0].F0 * mono_envelope(L.c[0], tr));
     double kappa = sqrt(2 * Ip), nn = T.Zd / kappa;
@@ -174,11 +229,13 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     B.c_cc = pow(2.0, 3 * nn / 2 + 1) * pow(kappa, 5 * nn + 0.5) * pow(B.F, -nn) * pow(1 + 2 * gam / e0, -nn);  // :235
     B.dkdt = S.dkdt_base;
     B.g2 = 0.0;
+    B.beta = B.gamma = 0.0;
+    B.wB = nullptr;
+    B.wM = 0;
     B.skip = (M == ETRAJ_INIT_SPA) && !(B.F > 1e-3);
     if (M == ETRAJ_INIT_WFAT) {
-        double beta, gamma;
-        ff_mf_euler(S.mol_rot, Fx, Fy, beta, gamma);   // WFATSampler.jl:143-145
-        B.g2 = wfat_g2(S, beta, gamma);                // :151-155
+        ff_mf_euler(S.mol_rot, Fx, Fy, B.beta, B.gamma);   // WFATSampler.jl:143-145
+        if (S.wfat_form == ETRAJ_WFAT_TABLE) B.g2 = wfat_g2(S, B.beta, B.gamma);   // :151-155, channel (0, 0)
     }
     return B;
 }
@@ -223,7 +280,7 @@ __device__ inline void spa_p(double kd, Cx Ax, Cx Ay, double& px, double& py) {
 // (:280-292). Returns false if the solve fails.
 __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const TargetP& T, const AdkBatch& B, double kd,
                                 double kz, double w, double& rate, double& phase, double col[6]) {
-    const double Ip = T.Ip, tr = B.tr;
+    const double Ip = S.Ip, tr = B.tr;
     const double ti = spa_solve(L, tr, B.F, S.omega, Ip, kd, kz);
     if (ti == 0.0) return false;
     Cx Fx, Fy, Ax, Ay;
@@ -316,7 +373,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
             }
         }
     }
-    const double Ip = T.Ip;
+    const double Ip = S.Ip;
     double rate, phase = 0.0;
     double x0, y0, kx0, ky0;
     if (M == ETRAJ_INIT_SPA) {
@@ -345,11 +402,12 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         if (isnan(rate) || rate / w < S.cutoff) return false;        // :337: rate/dkdt (Q7)
     } else if (M == ETRAJ_INIT_WFAT) {
         if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) return false;    // WFATSampler.jl:188
-        // W(0, 0, kd, kz) (WFATSampler.jl:157)
-        const double kappa = sqrt(2 * Ip), k2 = kappa * kappa;
-        const double W = (k2 / 2 / B.F / 1.0) * pow(4 * k2 / B.F, 2 * S.nstar - 1) *
-                         exp(-2 * pow(k2 + kd0 * kd0 + kz0 * kz0, 1.5) / (3 * B.F));
-        rate = B.g2 * W * w;
+        // sum over channels of |G|^2 W(nxi, |m|, kd, kz) dkdt (WFATSampler.jl:157,:169),
+        // W = prefactor (kd^2+kz^2)^|m| exp(-2 (kappa^2+kd^2+kz^2)^1.5 / 3F)
+        const double k2 = 2 * Ip, q2 = kd0 * kd0 + kz0 * kz0;
+        double poly = 0.0;
+        for (int am = B.wM; am >= 0; --am) poly = poly * q2 + B.wB[am];
+        rate = poly * exp(-2 * pow(k2 + q2, 1.5) / (3 * B.F)) * w;
     } else {
         if (fabs(kd0) < 1e-4) return false;                           // ADKSampler.jl:312,:321
         const double kx = kd0 * -B.sphi, ky = kd0 * B.cphi;
@@ -397,10 +455,39 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
     const int b = G.batch[g];
     const SamplerP S = sps[job];
     const LaserP& L = lps[job];
-    const AdkBatch B = adk_batch_setup<M>(S, L, T, G.t0[g]);
+    AdkBatch B = adk_batch_setup<M>(S, L, T, G.t0[g]);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double* bout = out + G.start[g] * S.rows;
     if (tid == 0) base_sh = 0;
+    if (M == ETRAJ_INIT_WFAT) {
+        // B_|m| = sum_nxi sum_{m = +-|m|} |G_{nxi m}|^2 prefactor(nxi, |m|): one
+        // channel per thread, then summed in a fixed order (deterministic)
+        __shared__ double wfat_v[kWfatMaxChannels];
+        __shared__ double wfat_B[kWfatMaxM + 1];
+        const double kappa = sqrt(2 * S.Ip);
+        if (S.wfat_form == ETRAJ_WFAT_INTDATA) {
+            const int D0 = S.wfat_dims[0], D1 = S.wfat_dims[1], mM = (D1 - 1) / 2;
+            for (int ch = tid; ch < D0 * D1; ch += blockDim.x) {
+                const int nxi = ch / D1, m = ch % D1 - mM;
+                wfat_v[ch] = wfat_channel_g2(S, kappa, B.beta, B.gamma, nxi, m) *
+                             wfat_field_prefactor(kappa, B.F, S.nstar, nxi, abs(m));
+            }
+            __syncthreads();
+            if (tid <= mM) {
+                double acc = 0.0;
+                for (int nxi = 0; nxi < D0; ++nxi) {
+                    acc += wfat_v[nxi * D1 + mM + tid];
+                    if (tid > 0) acc += wfat_v[nxi * D1 + mM - tid];
+                }
+                wfat_B[tid] = acc;
+            }
+            B.wM = mM;
+        } else {
+            if (tid == 0) wfat_B[0] = B.g2 * wfat_field_prefactor(kappa, B.F, S.nstar, 0, 0);
+            B.wM = 0;
+        }
+        B.wB = wfat_B;
+    }
     __syncthreads();
     double col[9];
     for (int64_t j0 = 0; j0 < S.nk; j0 += 256) {

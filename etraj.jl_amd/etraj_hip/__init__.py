@@ -12,6 +12,7 @@ from .lasers import (Laser, MonochromaticLaser, Cos2Laser, Cos4Laser, GaussianLa
                      Azimuth, AngFreq, Period, CEP, TimeShift, KeldyshParameter, SpreadDuration, FWHM_Duration, Laser1,
                      Laser2, Delay21)
 from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, MoleculeBase, TabulatedMolecule, MolRotation,
+                      GenericMolecule, get_mol, get_available_mols,
                       synthetic_g00_table, get_atom, get_available_atoms,
                       hartree_asymp_coeff, IonPotential, AsympNuclCharge, AngularQuantumNumber, MagneticQuantumNumber,
                       AsympCoeff, SoftCore, TargetName, QuantizationAxisOrientaion, TargetPotential, TargetForce)

@@ -23,6 +23,7 @@ OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY = 0, -1, -2, -3, -4, -5
 LASER_COS2, LASER_COS4, LASER_GAUSSIAN = 1, 2, 3
 TARGET_HYDROGEN_LIKE, TARGET_SAE = 1, 2
 INIT_ADK, INIT_SPA, INIT_SPANE, INIT_WFAT = 1, 2, 3, 4
+WFAT_TABLE, WFAT_INTDATA = 0, 1
 PHASE_CTMC, PHASE_QTMC, PHASE_SCTS = 0, 1, 2
 PREFIX_PRE, PREFIX_PRECC, PREFIX_JAC = 1, 2, 4
 SAMPLING_STEP, SAMPLING_MONTECARLO, SAMPLING_STRATIFIED = 0, 1, 2
@@ -62,7 +63,9 @@ class SamplerDesc(C.Structure):
                 ("mc_kd_max", C.c_double), ("mc_kz_max", C.c_double),
                 ("kd_sigma", C.c_double), ("kz_sigma", C.c_double),
                 ("cutoff_limit", C.c_double), ("seed", C.c_uint64), ("mol_rot", C.c_double * 3),
-                ("wfat_na", C.c_int32), ("wfat_nb", C.c_int32), ("wfat_coeff", C.POINTER(C.c_double))]
+                ("wfat_na", C.c_int32), ("wfat_nb", C.c_int32), ("wfat_coeff", C.POINTER(C.c_double)),
+                ("wfat_form", C.c_int32), ("wfat_dims", C.c_int32 * 4), ("wfat_reserved", C.c_int32),
+                ("wfat_mu", C.c_double * 3), ("wfat_ip", C.c_double)]
 
 
 class RunDesc(C.Structure):

@@ -184,7 +184,9 @@ class WFATSampler(ADKSampler):
         pm = _sym(traj_phase_method)
         if pm not in _abi.PHASE_METHODS:
             raise ValueError(f"[WFATSampler] Unsupported phase method [{traj_phase_method}].")
-        if IonPotential(target) <= 0:
+        from .targets import GenericMolecule
+        ip_orbit = target.ion_potential(mol_orbit_ridx) if isinstance(target, GenericMolecule) else IonPotential(target)
+        if ip_orbit <= 0:                                        # WFATSampler.jl:77-79
             raise ValueError("[WFATSampler] The energy of the ionizing orbit is non-negative.")
         self.laser, self.target = laser, target
         d = _abi.SamplerDesc()
@@ -196,8 +198,22 @@ class WFATSampler(ADKSampler):
                        mc_kd_max, mc_kz_max, kd_sigma, kz_sigma, seed)
         for i, a in enumerate(MolRotation(target)):
             d.mol_rot[i] = a
-        coeff = target.coeff_table()
-        d.wfat_na, d.wfat_nb = target.g00_coeff.shape
+        if isinstance(target, GenericMolecule):     # every channel of the WFAT integrals
+            mu, intdata = target.wfat_data(mol_orbit_ridx)
+            # Julia's column-major memory of intdata[nxi, m, l, m'] as (re, im) pairs
+            flat = intdata.ravel(order="F")
+            coeff = np.ascontiguousarray(np.column_stack([flat.real, flat.imag]).ravel())
+            d.wfat_form = _abi.WFAT_INTDATA
+            for k in range(4):
+                d.wfat_dims[k] = intdata.shape[k]
+            for k in range(3):
+                d.wfat_mu[k] = mu[k]
+            d.wfat_ip = ip_orbit
+            d.wfat_na, d.wfat_nb = intdata.shape[2], intdata.shape[3]
+        else:                                       # channel (0, 0) from a G00 table
+            coeff = target.coeff_table()
+            d.wfat_form = _abi.WFAT_TABLE
+            d.wfat_na, d.wfat_nb = target.g00_coeff.shape
         d.wfat_coeff = _abi.dptr(coeff)
         d._keep_coeff = coeff                       # the descriptor points into it
         _keldysh_warning("WFATSampler", laser, IonPotential(target))

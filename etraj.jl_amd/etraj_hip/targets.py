@@ -148,6 +148,126 @@ class TabulatedMolecule(MoleculeBase):
         return f"[TabulatedMolecule] {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}, rot={self.rot}"
 
 
+class GenericMolecule(MoleculeBase):
+    """GenericMolecule.jl's molecule as far as the trajectory path reads it:
+    the data of `LoadMolecule` (GenericMolecule.jl:99-119), i.e. a molecule
+    file of eTraj.jl's database (`get_mol`) or the same fields as a dict.
+
+    * IonPotential = -max E(HOMO) over the spins (:438-443), AsympNuclCharge
+      = charge + 1 (:450); the EOM is the single-centre soft-core Coulomb
+      potential -(charge+1) (r^2 + 1)^(-1/2) (:452-454) with that Ip (:457);
+    * the WFAT sampler reads IonPotential(mol, ridx) = -E(HOMO + ridx)
+      (:444-449, :148-159) and the WFAT integrals wfat_intdata[ridx] and
+      dipole wfat_mu[ridx] (:228-233). Closed shell: ridx is an integer;
+      open shell (two rows of energies): ridx = (spin, ridx) with spin 1 or 2.
+    """
+
+    def __init__(self, data: dict, rot_α=0.0, rot_β=0.0, rot_γ=0.0, soft_core=1.0):
+        self.data = data
+        self.name = data["name"]
+        self.charge = int(data["charge"])
+        self.nucl_charge = self.charge + 1
+        E = np.asarray(data["energy_levels"], dtype=np.float64)
+        occ = np.asarray(data["orbit_occ"], dtype=np.float64)
+        self.open_shell = E.ndim == 2
+        self.energy_levels = E if self.open_shell else E[None, :]
+        occ = occ if self.open_shell else occ[None, :]
+        self.homo_idx = []
+        for row in occ:                                  # _get_HOMO_idx (:192-195), 0-based
+            nz = np.nonzero(row)[0]
+            if nz.size == 0:
+                raise ValueError("[GenericMolecule] no occupied orbital")
+            self.homo_idx.append(int(nz[-1]))
+        self.Ip = -max(float(self.energy_levels[k, h]) for k, h in enumerate(self.homo_idx))
+        self.rot = (float(rot_α), float(rot_β), float(rot_γ))
+        self.soft_core = float(soft_core)
+        self.l = self.m = 0
+        self.asymp_coeff = 1.0
+        self.quan_ax_θ = self.quan_ax_ϕ = 0.0
+
+    def _key(self, orbit_ridx):
+        if self.open_shell:
+            if not (isinstance(orbit_ridx, tuple) and len(orbit_ridx) == 2 and orbit_ridx[0] in (1, 2)):
+                raise ValueError("[GenericMolecule] For an open-shell molecule, `orbit_ridx` should be a two-element "
+                                 "tuple `(spin, ridx)`.")
+            return (int(orbit_ridx[0]), int(orbit_ridx[1]))
+        if isinstance(orbit_ridx, tuple) or int(orbit_ridx) != orbit_ridx:
+            raise ValueError("[GenericMolecule] For a closed-shell molecule, `orbit_ridx` should be an integer.")
+        return int(orbit_ridx)
+
+    def MolEnergyLevel(self, orbit_ridx) -> float:
+        """GenericMolecule.jl:148-159."""
+        k = self._key(orbit_ridx)
+        spin, r = (k[0] - 1, k[1]) if self.open_shell else (0, k)
+        i = self.homo_idx[spin] + r
+        if not 0 <= i < self.energy_levels.shape[1]:
+            raise ValueError(f"[GenericMolecule] orbit_ridx {orbit_ridx} is out of range")
+        return float(self.energy_levels[spin, i])
+
+    def ion_potential(self, orbit_ridx=0) -> float:
+        return -self.MolEnergyLevel(orbit_ridx)
+
+    def wfat_data(self, orbit_ridx=0):
+        """(mu, intdata) of MolWFATData (:228-233): the orbital's dipole and the
+        complex array intdata[nxi, m + m_max, l, m' + l] (0-based)."""
+        key = self._key(orbit_ridx)
+        norm = (lambda k: tuple(int(v) for v in k)) if self.open_shell else int
+        table = {norm(k): v for k, v in self.data.get("wfat_intdata", {}).items()}
+        mus = {norm(k): v for k, v in self.data.get("wfat_mu", {}).items()}
+        if key not in table or key not in mus:
+            raise ValueError(f"[GenericMolecule] The WFAT data of orbit_ridx={orbit_ridx} is not available "
+                             "(the PySCF calculation is not part of this engine).")
+        return np.asarray(mus[key], dtype=np.float64), np.asarray(table[key], dtype=np.complex128)
+
+    def _desc(self) -> _abi.TargetDesc:
+        d = _abi.TargetDesc()
+        d.kind = _abi.TARGET_HYDROGEN_LIKE
+        d.nucl_charge = self.nucl_charge
+        d.Ip = self.Ip
+        d.asymp_coeff = 1.0
+        d.soft_core = self.soft_core
+        return d
+
+    def with_rotation(self, rot_α, rot_β, rot_γ) -> "GenericMolecule":
+        return GenericMolecule(self.data, rot_α, rot_β, rot_γ, self.soft_core)
+
+    def __eq__(self, o):
+        return (type(o) is type(self) and self.data is o.data and self.rot == o.rot
+                and self.soft_core == o.soft_core)
+
+    def __repr__(self):
+        return f"[GenericMolecule] {self.name}, Ip={self.Ip:.4f}, Z={self.nucl_charge}, rot={self.rot}"
+
+
+_MOLECULE_FILES = {   # MoleculeDatabase.jl:54-68
+    "Hydrogen": "Molecule_Hydrogen.jld2", "Nitrogen": "Molecule_Nitrogen.jld2", "Oxygen": "Molecule_Oxygen.jld2",
+    "Carbon Monoxide": "Molecule_CarbonMonoxide.jld2", "Nitric Oxide": "Molecule_NitricOxide.jld2",
+    "Hydrochloric Acid": "Molecule_HydrochloricAcid.jld2", "Carbon Dioxide": "Molecule_CarbonDioxide.jld2",
+    "Sulfur Dioxide": "Molecule_SulfurDioxide.jld2", "Water": "Molecule_Water.jld2",
+    "Ammonia": "Molecule_Ammonia.jld2", "Acetylene": "Molecule_Acetylene.jld2", "Methane": "Molecule_Methane.jld2",
+    "Benzene": "Molecule_Benzene.jld2",
+}
+
+
+def get_available_mols():
+    """MoleculeDatabase.jl:50-52."""
+    return list(_MOLECULE_FILES)
+
+
+def get_mol(name: str, rot_α=0.0, rot_β=0.0, rot_γ=0.0, database_dir=None) -> GenericMolecule:
+    """MoleculeDatabase.jl:42-48: a database molecule, read from eTraj.jl's
+    `src/Targets/Database` directory (`database_dir`, or the environment
+    variable ETRAJ_MOL_DATABASE)."""
+    import os
+    from .moldata import read_molecule_jld2
+    if name not in _MOLECULE_FILES:
+        raise ValueError(f"[get_mol] Molecule name `{name}` not found in database.")
+    d = database_dir or os.environ.get("ETRAJ_MOL_DATABASE")
+    if not d:
+        raise FileNotFoundError("[get_mol] set database_dir or ETRAJ_MOL_DATABASE to eTraj.jl's src/Targets/Database")
+    return GenericMolecule(read_molecule_jld2(os.path.join(d, _MOLECULE_FILES[name])), rot_α, rot_β, rot_γ)
+
+
 def MolRotation(mol: MoleculeBase):
     """GenericMolecule.jl: (rot_α, rot_β, rot_γ) in radians."""
     return mol.rot

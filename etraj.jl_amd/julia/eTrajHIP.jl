@@ -46,6 +46,7 @@ struct SamplerDesc
     mc_kd_max::Float64; mc_kz_max::Float64; kd_sigma::Float64; kz_sigma::Float64
     cutoff_limit::Float64; seed::UInt64
     mol_rot::NTuple{3,Float64}; wfat_na::Int32; wfat_nb::Int32; wfat_coeff::Ptr{Float64}
+    wfat_form::Int32; wfat_dims::NTuple{4,Int32}; wfat_reserved::Int32; wfat_mu::NTuple{3,Float64}; wfat_ip::Float64
 end
 struct RunDesc
     dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
@@ -72,6 +73,10 @@ target_desc(t::HydrogenLikeAtom) = TargetDesc(1, AsympNuclCharge(t), AngularQuan
     IonPotential(t), AsympCoeff(t), QuantizationAxisOrientaion(t)..., SoftCore(t), 0, 0, 0, 0, 0, 0)
 target_desc(t::SAEAtom) = TargetDesc(2, AsympNuclCharge(t), AngularQuantumNumber(t), MagneticQuantumNumber(t),
     IonPotential(t), AsympCoeff(t), QuantizationAxisOrientaion(t)..., SoftCore(t), t.a1, t.b1, t.a2, t.b2, t.a3, t.b3)
+# GenericMolecule: the single-centre soft-core potential -(Q+1)/sqrt(r^2+1) with the HOMO Ip
+# (GenericMolecule.jl:450-457), i.e. a hydrogen-like descriptor with soft core 1
+target_desc(t::GenericMolecule) = TargetDesc(1, AsympNuclCharge(t), 0, 0, IonPotential(t), 1.0, 0.0, 0.0, 1.0,
+    0, 0, 0, 0, 0, 0)
 
 function run_desc(dimension, phase, traj_t_final, final_p_max, final_p_num; dt=0.1, precision=:mixed,
                   restep_radius=2.0, reproduce_quirks=true)
@@ -143,24 +148,42 @@ init_method(m::Symbol) = m in (:ADK, :MOADK) ? Int32(1) : m in (:SPA, :MOSPA, :S
                          m in (:SPANE, :MOSPANE, :SFAAE, :MOSFAAE) ? Int32(3) : m == :WFAT ? Int32(4) :
                          error("[ElectronSamplers] Undefined initial condition method [$m].")
 
-sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_t_num, ss_kd_max=0.0, ss_kd_num=0,
-             ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC, sample_cutoff_limit=1e-16, rate_prefix=:Full,
-             kwargs...) =
-    SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0,
-                sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
-                ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, (0.0, 0.0, 0.0), 0, 0, C_NULL)
+# The sampler descriptor and the arrays it points into (keep them under GC.@preserve).
+# WFAT: every channel of the WFAT integrals MolWFATData(mol, ridx) (GenericMolecule.jl:228-233),
+# passed as Julia's own Array{ComplexF64,4} memory (ETRAJ_WFAT_INTDATA = 1).
+function sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_t_num, ss_kd_max=0.0, ss_kd_num=0,
+                      ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC, sample_cutoff_limit=1e-16,
+                      rate_prefix=:Full, target=nothing, mol_orbit_ridx=0, kwargs...)
+    rot = (0.0, 0.0, 0.0); form = Int32(0); dims = (Int32(0), Int32(0), Int32(0), Int32(0))
+    mu = (0.0, 0.0, 0.0); ip = 0.0; coeff = C_NULL; keep = nothing; na = Int32(0); nb = Int32(0)
+    if init_cond_method == :WFAT
+        μ, intdata = MolWFATData(target, mol_orbit_ridx)
+        keep = intdata
+        coeff = Ptr{Float64}(pointer(intdata))
+        form = Int32(1); dims = Int32.(size(intdata)); mu = Tuple(Float64.(μ))
+        ip = IonPotential(target, mol_orbit_ridx); rot = Float64.(MolRotation(target))
+        na, nb = dims[3], dims[4]
+    end
+    sd = SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0,
+                     sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
+                     ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, rot, na, nb, coeff,
+                     form, dims, 0, mu, ip)
+    return sd, keep
+end
 
 function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max, final_p_num,
                      ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC,
                      sample_cutoff_limit=1e-16, rate_prefix=:Full, dt=0.1, precision=:mixed, restep_radius=2.0,
-                     init_cond_method=:ADK, device=0)
-    sd = Ref(sampler_desc(; init_cond_method, dimension, sample_t_intv, sample_t_num, ss_kd_max, ss_kd_num, ss_kz_max,
-                          ss_kz_num, traj_phase_method, sample_cutoff_limit, rate_prefix))
+                     init_cond_method=:ADK, mol_orbit_ridx=0, device=0)
+    sdv, keep = sampler_desc(; init_cond_method, dimension, sample_t_intv, sample_t_num, ss_kd_max, ss_kd_num,
+                             ss_kz_max, ss_kz_num, traj_phase_method, sample_cutoff_limit, rate_prefix, target,
+                             mol_orbit_ridx)
+    sd = Ref(sdv)
     rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
     spec = zeros(Float64, final_p_num...)
     st = Stats()
     c = Ctx(device)
-    GC.@preserve spec st begin
+    GC.@preserve spec st keep begin
         rc = ccall((:etraj_run_job, libetraj), Cint,
                    (Ptr{Cvoid}, Ref{LaserDesc}, Ref{TargetDesc}, Ref{SamplerDesc}, Ref{RunDesc}, Ptr{Float64}, Ref{Stats}),
                    c.h, Ref(laser_desc(laser)), Ref(target_desc(target)), sd, rd, spec, st)
@@ -178,13 +201,14 @@ function run_scan_hip(lasers::AbstractVector; target, dimension, sample_t_intv, 
                       restep_radius=2.0, device=0, kwargs...)
     n = length(lasers)
     lds = [laser_desc(l) for l in lasers]
-    sds = [sampler_desc(; dimension, sample_t_intv, sample_t_num, traj_phase_method, kwargs...) for _ in 1:n]
+    sdk = [sampler_desc(; dimension, sample_t_intv, sample_t_num, traj_phase_method, target, kwargs...) for _ in 1:n]
+    sds = first.(sdk); keeps = last.(sdk)
     rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
     npix = prod(final_p_num)
     specs = zeros(Float64, npix, n)
     stv = Vector{NTuple{13,Float64}}(undef, n)   # etraj_stats is 13 eight-byte fields
     c = Ctx(device)
-    GC.@preserve lds sds specs stv begin
+    GC.@preserve lds sds keeps specs stv begin
         rc = ccall((:etraj_run_scan, libetraj), Cint,
                    (Ptr{Cvoid}, Int32, Ptr{LaserDesc}, Ref{TargetDesc}, Ptr{SamplerDesc}, Ref{RunDesc}, Ptr{Float64},
                     Ptr{Cvoid}),

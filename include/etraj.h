@@ -97,10 +97,22 @@ enum {
     ETRAJ_INIT_ADK  = 1,   /* ADKSampler.jl                                                  */
     ETRAJ_INIT_SPA  = 2,   /* SPASampler.jl (SFA-SPA): complex saddle point per sample       */
     ETRAJ_INIT_SPANE = 3,  /* SPANESampler.jl (SFA-SPA with non-adiabatic expansion)         */
-    ETRAJ_INIT_WFAT = 4    /* WFATSampler.jl, channel (nxi, m) = (0, 0), structure factor
-                              G00(beta, gamma) = sum_{a<na, b<nb} c_ab P_a(cos beta) e^{-i b gamma}
-                              from the table `wfat_coeff` (complex, a-major), times
-                              exp(-kappa mu_z) with mu_z = 0                                 */
+    ETRAJ_INIT_WFAT = 4    /* WFATSampler.jl, channel (nxi, m) = (0, 0); the structure factor
+                              G00(beta, gamma) comes from `wfat_coeff` in one of the forms
+                              ETRAJ_WFAT_*                                                   */
+};
+/* WFAT structure-factor forms (etraj_sampler_desc.wfat_form) */
+enum {
+    ETRAJ_WFAT_TABLE = 0,    /* channel (0, 0) only, G00 = sum_{a<na, b<nb} c_ab P_a(cos beta)
+                                e^{-i b gamma}; c_ab at index a * nb + b                      */
+    ETRAJ_WFAT_INTDATA = 1   /* every channel (nxi, m) of the WFAT integrals I (the molecule's
+                                wfat_intdata[ridx], GenericMolecule.jl:236-260, WFATSampler.jl:
+                                151-169): G_{nxi m} = sum_{l>=|m|} sum_{|m'|<=l}
+                                I[nxi, m, l, m'] d^l_{m m'}(beta) e^{-i m' gamma}
+                                exp(-kappa mu_z(beta)), mu_z = -sin(beta) mu_x + cos(beta) mu_z;
+                                wfat_coeff is Julia's Array{ComplexF64,4} memory as is
+                                (column-major, (re, im)), wfat_dims its size
+                                (nxi_max+1, 2 m_max+1, l_max+1, 2 l_max+1), each <= 31       */
 };
 enum { ETRAJ_PHASE_CTMC = 0, ETRAJ_PHASE_QTMC = 1, ETRAJ_PHASE_SCTS = 2 };
 /* rate_prefix bit set (ADKSampler.jl:49-77): :Exp = 0, :Full = PRECC|JAC */
@@ -134,7 +146,14 @@ typedef struct etraj_sampler_desc {
      * MolRotation, GenericMolecule.jl) and the G00 coefficient table */
     double   mol_rot[3];
     int32_t  wfat_na, wfat_nb;
-    const double* wfat_coeff;  /* 2 * na * nb doubles: (re, im) of c_ab, index a * nb + b */
+    const double* wfat_coeff;  /* (re, im) pairs, layout per wfat_form                  */
+    int32_t  wfat_form;        /* ETRAJ_WFAT_*                                         */
+    int32_t  wfat_dims[4];     /* ETRAJ_WFAT_INTDATA: size(wfat_intdata[ridx])         */
+    int32_t  wfat_reserved;
+    double   wfat_mu[3];       /* ETRAJ_WFAT_INTDATA: wfat_mu[ridx], the orbital dipole */
+    double   wfat_ip;          /* Ip of the ionizing orbital, IonPotential(mol, ridx)
+                                  (WFATSampler.jl:139); <= 0: the target's Ip. The
+                                  EOM keeps the target's (HOMO) Ip, GenericMolecule.jl:457 */
 } etraj_sampler_desc;
 
 /* ---- trajectory run ---------------------------------------------------------- */

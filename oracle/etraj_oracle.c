@@ -782,13 +782,45 @@ void oracle_ff_mf_euler(const double mol_rot[3], double Fx, double Fy, double ou
     zyz_params(R, &out[0], &out[1], &out[2]);
 }
 
-/* G00(beta, gamma) = sum_{a,b} c_ab P_a(cos beta) e^{-i b gamma}, the
- * tabulated stand-in for MolWFATStructureFactor_G(mol, ridx, 0, 0, beta, gamma)
- * (GenericMolecule.jl:236-260), whose sum runs over Wigner-d functions and
- * e^{-i m' chi} of the same angles. */
-static double complex wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma) {
-    double x = cos(beta);
+/* Wigner small-d d^l_{m m'}(beta) = <l m| exp(-i beta J_y) |l m'> (Condon-Shortley
+ * J_y), by Wigner's formula summed over s:
+ * (-1)^(m-m'+s) sqrt((l+m)!(l-m)!(l+m')!(l-m')!) / ((l+m'-s)! s! (m-m'+s)! (l-m-s)!)
+ *   cos(beta/2)^(2l+m'-m-2s) sin(beta/2)^(m-m'+2s). The convention of
+ * WignerD.jl's wignerdjmn, which GenericMolecule.jl:257 calls. */
+double oracle_wigner_d(int l, int m, int mp, double beta) {
+    double c = cos(beta / 2), sn = sin(beta / 2), sum = 0;
+    double pre = sqrt(tgamma(l + m + 1.0) * tgamma(l - m + 1.0) * tgamma(l + mp + 1.0) * tgamma(l - mp + 1.0));
+    int s0 = mp - m > 0 ? mp - m : 0, s1 = (l + mp < l - m) ? l + mp : l - m;
+    for (int k = s0; k <= s1; ++k) {
+        double den = tgamma(l + mp - k + 1.0) * tgamma(k + 1.0) * tgamma(m - mp + k + 1.0) * tgamma(l - m - k + 1.0);
+        double t = pre / den * pow(c, 2 * l + mp - m - 2 * k) * pow(sn, m - mp + 2 * k);
+        sum += ((m - mp + k) & 1) ? -t : t;
+    }
+    return sum;
+}
+
+/* The structure factor G_{nxi m}(beta, gamma):
+ * ETRAJ_WFAT_TABLE: channel (0, 0) only, sum_{a,b} c_ab P_a(cos beta) e^{-i b gamma};
+ * ETRAJ_WFAT_INTDATA: MolWFATStructureFactor_G (GenericMolecule.jl:236-260),
+ *   sum_{l=|m|..L} sum_{m'=-l..l} I[nxi, m, l, m'] d^l_{m m'}(beta) e^{-i m' gamma}
+ *   times exp(-kappa mu_z), mu_z = (RotZYZ(gamma, beta, 0) mu)_z
+ *   = -sin(beta) mu_x + cos(beta) mu_z; I is Julia's column-major array. */
+static double complex wfat_G(const etraj_sampler_desc* sd, double kappa, double beta, double gamma, int nxi, int m) {
     double complex g = 0;
+    if (sd->wfat_form == ETRAJ_WFAT_INTDATA) {
+        const int D0 = sd->wfat_dims[0], D1 = sd->wfat_dims[1], D2 = sd->wfat_dims[2];
+        const int mM = (D1 - 1) / 2, L = D2 - 1;
+        if (nxi < 0 || nxi >= D0 || abs(m) > mM) return 0;
+        for (int l = abs(m); l <= L; ++l)
+            for (int mp = -l; mp <= l; ++mp) {
+                const double* c = sd->wfat_coeff + 2 * (nxi + (int64_t)D0 * ((m + mM) + (int64_t)D1 * (l + (int64_t)D2 * (mp + l))));
+                g += (c[0] + I * c[1]) * oracle_wigner_d(l, m, mp, beta) * cexp(-I * (double)mp * gamma);
+            }
+        double muz = -sin(beta) * sd->wfat_mu[0] + cos(beta) * sd->wfat_mu[2];
+        return g * exp(-kappa * muz);
+    }
+    if (nxi != 0 || m != 0) return 0;
+    double x = cos(beta);
     double pm1 = 0, p = 1;   /* P_{a-1}, P_a */
     for (int a = 0; a < sd->wfat_na; ++a) {
         if (a == 1) { pm1 = 1; p = x; }
@@ -801,8 +833,9 @@ static double complex wfat_g00(const etraj_sampler_desc* sd, double beta, double
     return g;
 }
 
-void oracle_wfat_g00(const etraj_sampler_desc* sd, double beta, double gamma, double out[2]) {
-    double complex g = wfat_g00(sd, beta, gamma);
+void oracle_wfat_G(const etraj_sampler_desc* sd, double kappa, double beta, double gamma, int nxi, int m,
+                   double out[2]) {
+    double complex g = wfat_G(sd, kappa, beta, gamma, nxi, m);
     out[0] = creal(g);
     out[1] = cimag(g);
 }
@@ -816,12 +849,19 @@ static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t ca
     laser_F(&S->L, tr, &Fxtr, &Fytr);
     const double Ftr = hypot(Fxtr, Fytr);
     const double phi = atan2(-Fytr, -Fxtr);                          /* :137 */
-    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    const double Ip = sd->wfat_ip > 0 ? sd->wfat_ip : S->T.Ip, Z = (double)S->T.Z;   /* :139 */
     double eul[3];
     oracle_ff_mf_euler(sd->mol_rot, Fxtr, Fytr, eul);                 /* :143-145 */
-    const double complex G = wfat_g00(sd, eul[1], eul[2]);     /* :151-155 */
-    const double G2 = creal(G) * creal(G) + cimag(G) * cimag(G);
     const double kappa = sqrt(2 * Ip), n = Z / kappa;
+    /* G_data[nxi, m] for every channel (:150-155); the table form has (0, 0) only */
+    const int D0 = sd->wfat_form == ETRAJ_WFAT_INTDATA ? sd->wfat_dims[0] : 1;
+    const int D1 = sd->wfat_form == ETRAJ_WFAT_INTDATA ? sd->wfat_dims[1] : 1, mM = (D1 - 1) / 2;
+    double* G2 = malloc(sizeof(double) * D0 * D1);
+    for (int nxi = 0; nxi < D0; ++nxi)
+        for (int m = -mM; m <= mM; ++m) {
+            double complex G = wfat_G(sd, kappa, eul[1], eul[2], nxi, m);
+            G2[nxi * D1 + m + mM] = creal(G) * creal(G) + cimag(G) * cimag(G);
+        }
     double dkdt;                                                      /* :159-168 */
     int64_t nk;
     if (sd->sampling == ETRAJ_SAMPLING_STEP) {
@@ -870,12 +910,23 @@ static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t ca
         if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) continue;           /* :188 */
         double kx0 = kd0 * -sin(phi), ky0 = kd0 * cos(phi);
         double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / Ftr;
-        /* W(0, 0, kd, kz) (:157) */
-        double W = (kappa * kappa / 2 / Ftr / 1.0) * pow(4 * kappa * kappa / Ftr, 2 * n - 1) *
-                   exp(-2 * pow(kappa * kappa + kd0 * kd0 + kz0 * kz0, 1.5) / (3 * Ftr));
-        double rate = G2 * W * w;
+        /* rate = sum_{nxi, m} |G|^2 W(nxi, |m|, kd, kz) dkdt (:157, :169) */
+        double rate = 0;
+        for (int nxi = 0; nxi < D0; ++nxi)
+            for (int m = -mM; m <= mM; ++m) {
+                int am = abs(m);
+                double W = (pow(kappa, am + 2) / 2 / pow(Ftr, am + 1) / tgamma(am + 1.0)) *
+                           pow(4 * kappa * kappa / Ftr, 2 * n - 2 * nxi - am - 1) *
+                           pow(kd0 * kd0 + kz0 * kz0, am) *
+                           exp(-2 * pow(kappa * kappa + kd0 * kd0 + kz0 * kz0, 1.5) / (3 * Ftr));
+                rate += G2[nxi * D1 + m + mM] * W;
+            }
+        rate *= w;
         if (isnan(rate) || rate < sd->cutoff_limit) continue;
-        if (count >= cap) return -1;
+        if (count >= cap) {
+            free(G2);
+            return -1;
+        }
         double* col = out + count * rows;
         if (dim == 3) {
             double v[8] = {r0 * cos(phi), r0 * sin(phi), 0.0, kx0, ky0, kz0, tr, rate};
@@ -888,6 +939,7 @@ static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t ca
         }
         ++count;
     }
+    free(G2);
     return count;
 }
 

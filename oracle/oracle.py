@@ -57,8 +57,10 @@ def lib():
         L.oracle_eom_rhs.argtypes = [P(LaserDesc), P(TargetDesc), C.c_int32, C.c_int32, C.c_int32, C.c_double, DP, DP]
         L.oracle_ff_mf_euler.argtypes = [DP, C.c_double, C.c_double, DP]
         L.oracle_ff_mf_euler.restype = None
-        L.oracle_wfat_g00.argtypes = [P(SamplerDesc), C.c_double, C.c_double, DP]
-        L.oracle_wfat_g00.restype = None
+        L.oracle_wfat_G.argtypes = [P(SamplerDesc), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, DP]
+        L.oracle_wfat_G.restype = None
+        L.oracle_wigner_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
+        L.oracle_wigner_d.restype = C.c_double
         L.oracle_laser_complex.argtypes = [P(LaserDesc), C.c_double, C.c_double, DP]
         L.oracle_laser_field_deriv.argtypes = [P(LaserDesc), C.c_double, DP]
         L.oracle_spa_point.argtypes = [P(LaserDesc), P(TargetDesc), C.c_double, C.c_double, C.c_double, DP]
@@ -128,10 +130,20 @@ def ff_mf_euler(mol_rot, Fx, Fy):
     return out
 
 
-def wfat_g00(sd, beta, gamma):
+def wfat_G(sd, beta, gamma, nxi=0, m=0, kappa=1.0):
+    """G_{nxi m}(beta, gamma) of the sampler descriptor's structure factor;
+    kappa = sqrt(2 Ip) enters the dipole factor of the WFAT-integral form."""
     out = np.empty(2)
-    lib().oracle_wfat_g00(sd, beta, gamma, _d(out))
+    lib().oracle_wfat_G(sd, kappa, beta, gamma, nxi, m, _d(out))
     return complex(out[0], out[1])
+
+
+def wfat_g00(sd, beta, gamma, kappa=1.0):
+    return wfat_G(sd, beta, gamma, 0, 0, kappa)
+
+
+def wigner_d(l, m, mp, beta):
+    return lib().oracle_wigner_d(l, m, mp, beta)
 
 
 def laser_complex(ld, t):

@@ -40,3 +40,19 @@ def small_wfat(n_t=24, n_k=96, precision="fp64", p_num=160, **over):
     kw = workloads.mol_lp_scts_wfat(n_t=n_t, n_k=n_k, precision=precision, p_num=p_num, rot=(0.3, 1.1, -0.7))
     kw.update(over)
     return kw
+
+
+def fixture_molecule(key, rot=(0.0, 0.0, 0.0)):
+    """GenericMolecule from tests/golden/mol_<key>.npz (fields of eTraj.jl's
+    database file, written by tests/golden/make_molecules.py)."""
+    import ast
+    import os
+    from etraj_hip.targets import GenericMolecule
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", f"mol_{key}.npz"))
+    data = {k: f[k] for k in ("energy_levels", "orbit_occ")}
+    data.update(name=str(f["name"]), charge=int(f["charge"]), spin=int(f["spin"]), wfat_intdata={}, wfat_mu={})
+    for i, k in enumerate(f["wfat_keys"]):
+        k = ast.literal_eval(str(k))
+        data["wfat_intdata"][k] = f[f"wfat_intdata_{i}"]
+        data["wfat_mu"][k] = f[f"wfat_mu_{i}"]
+    return GenericMolecule(data, *rot)

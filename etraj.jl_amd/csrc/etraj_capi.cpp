@@ -80,7 +80,7 @@ struct etraj_ctx {
     hipStream_t stream2 = nullptr;   // FP64 re-integration passes of run_job (default priority)
     SlabSlot slot[2];
     std::string err;
-    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat;
+    DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat, prec;
     DevBuf<float4> ft32;
     DevBuf<double2> ft64, ft0;
     DevBuf<int64_t> seg_start, cnt, pre;
@@ -195,6 +195,38 @@ void gauss_legendre_32(double* q) {
     }
 }
 
+// The ADK / SPA / SPANE prefactor beyond l = 0 (ADKSampler.jl:188-265): a
+// molecule's C_lm table (asymp_coeff, in the frame of mol_rot), or an atom with
+// l > 0 as the table with C at (l, m) in the frame (phi, theta, 0) of its
+// quantization axis; an atom with l = 0 keeps the scalar C Y_00 (pre_L = -1).
+// C receives the host copy of the table ((re, im), [l + (L+1)(m+l)]).
+static int make_prefactor(etraj_ctx* c, const etraj_sampler_desc& s, const etraj_target_desc& td, SamplerP& S,
+                          std::vector<double>& C) {
+    S.pre_L = -1;
+    S.pre_C = nullptr;
+    C.clear();
+    if (s.init_method == ETRAJ_INIT_WFAT) return ETRAJ_OK;
+    if (s.asymp_coeff) {
+        if (s.asymp_lmax < 0 || s.asymp_lmax > kPreMaxL)
+            return fail(c, ETRAJ_EINVAL, "asymp_lmax must be in 0..15");
+        S.pre_L = s.asymp_lmax;
+        C.assign(s.asymp_coeff, s.asymp_coeff + pre_doubles(S));
+        for (double v : C)
+            if (!std::isfinite(v)) return fail(c, ETRAJ_EINVAL, "asymp_coeff is not finite");
+        for (int k = 0; k < 3; ++k) S.pre_rot[k] = s.mol_rot[k];
+    } else if (td.l != 0) {
+        if (td.l < 0 || td.l > kPreMaxL || td.m < -td.l || td.m > td.l)
+            return fail(c, ETRAJ_EINVAL, "the target's (l, m) must satisfy 0 <= l <= 15, |m| <= l");
+        S.pre_L = td.l;
+        C.assign(pre_doubles(S), 0.0);
+        C[2 * (td.l + (size_t)(td.l + 1) * (td.m + td.l))] = td.asymp_coeff;
+        S.pre_rot[0] = td.quan_ax_phi;                                 // ADKSampler.jl:201-203
+        S.pre_rot[1] = td.quan_ax_theta;
+        S.pre_rot[2] = 0.0;
+    }
+    return ETRAJ_OK;
+}
+
 int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, const TargetP& T,
                  const std::vector<double>& t, SamplerP& S) {
     if (!sd) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
@@ -283,13 +315,15 @@ int make_sampler(etraj_ctx* c, const etraj_sampler_desc* sd, const LaserP& L, co
     } else {
         S.dkdt_base = (s.sample_t_hi - s.sample_t_lo) / s.sample_t_num / s.mc_kt_num;
     }
-    // the sampler's Ip: IonPotential(mol, ridx) for WFAT (WFATSampler.jl:139)
-    S.Ip = (s.init_method == ETRAJ_INIT_WFAT && s.wfat_ip > 0) ? s.wfat_ip : T.Ip;
+    // the sampler's Ip: IonPotential(mol, ridx) of a molecule (ADKSampler.jl:177-182, WFATSampler.jl:139)
+    S.Ip = s.orbit_ip > 0 ? s.orbit_ip : T.Ip;
     const double kappa = std::sqrt(2 * S.Ip), nn = T.Zd / kappa;
     S.nstar = nn;
     S.c_pre = std::pow(2.0, nn / 2 + 1) * std::pow(kappa, 2 * nn + 0.5) * std::tgamma(nn / 2 + 1);
     S.numer = T.C * 1.0 * std::sqrt(1.0 / (4 * kPi));
     S.method = s.init_method;
+    S.pre_L = -1;
+    S.pre_C = nullptr;
     S.omega = L.n == 1 ? L.c[0].w : (L.c[0].w + L.c[1].w) / 2;   // SPASampler.jl:160-166
     S.glq = c->glq.p;
     if (s.init_method == ETRAJ_INIT_WFAT) {   // the table is uploaded by the caller (S.wfat_coeff)
@@ -535,7 +569,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
-    c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release();
+    c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release(); c->prec.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
     c->slot[0].release();
     c->slot[1].release();
@@ -608,12 +642,11 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     if (int rc = check_laser(c, laser, L)) return rc;
     if (int rc = check_target(c, target, T)) return rc;
     if (!sampler) return fail(c, ETRAJ_EINVAL, "sampler descriptor is NULL");
-    if (target->l != 0 && sampler->init_method != ETRAJ_INIT_WFAT)
-        return fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
-    std::vector<double> t;
+    std::vector<double> t, preC;
     if (int rc = batch_times_of(c, *sampler, t)) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     if (int rc = make_sampler(c, sampler, L, T, t, S)) return rc;
+    if (int rc = make_prefactor(c, *sampler, *target, S, preC)) return rc;
     if (batch_id < 1 || batch_id > S.t_num) return fail(c, ETRAJ_EINVAL, "batch_id out of range");
     if (!count || (capacity > 0 && !out)) return fail(c, ETRAJ_EINVAL, "NULL output");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -622,6 +655,12 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
         HIP_TRY(c, c->wfat.ensure(nc));
         HIP_TRY(c, hipMemcpyAsync(c->wfat.p, sampler->wfat_coeff, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
         S.wfat_coeff = c->wfat.p;
+    }
+    if (!preC.empty()) {
+        HIP_TRY(c, c->prec.ensure(preC.size()));
+        HIP_TRY(c, hipMemcpyAsync(c->prec.p, preC.data(), sizeof(double) * preC.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        S.pre_C = c->prec.p;
     }
     HIP_TRY(c, c->init.ensure((size_t)S.max_bs * S.rows));
     HIP_TRY(c, c->counts.ensure(1));
@@ -730,6 +769,7 @@ struct JobPlan {
     LaserP L;
     SamplerP S;
     std::vector<double> t;
+    std::vector<double> preC;   // host copy of the prefactor table (make_prefactor)
 };
 
 // Validates n_jobs (laser, sampler) pairs that share the target and the run
@@ -743,17 +783,16 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
     if (!lasers || !samplers) return fail(c, ETRAJ_EINVAL, "laser / sampler descriptor array is NULL");
     if (!run) return fail(c, ETRAJ_EINVAL, "run descriptor is NULL");
     jobs.resize(n_jobs);
-    size_t ncoef = 0;
+    size_t ncoef = 0, npre = 0;
     for (int32_t j = 0; j < n_jobs; ++j) {
         JobPlan& J = jobs[j];
         const etraj_sampler_desc& sd = samplers[j];
         int rc = check_laser(c, &lasers[j], J.L);
-        if (!rc && target->l != 0 && sd.init_method != ETRAJ_INIT_WFAT)
-            rc = fail(c, ETRAJ_EUNSUPPORTED, "atoms with l > 0 need the Wigner-D prefactor, not implemented by the engine");
         if (!rc && sd.init_method != samplers[0].init_method)
             rc = fail(c, ETRAJ_EINVAL, "the jobs of a scan must share init_cond_method");
         if (!rc) rc = batch_times_of(c, sd, J.t);
         if (!rc) rc = make_sampler(c, &sd, J.L, T, J.t, J.S);
+        if (!rc) rc = make_prefactor(c, sd, *target, J.S, J.preC);
         if (!rc && (sd.dimension != run->dimension || sd.phase_method != run->phase_method))
             rc = fail(c, ETRAJ_EINVAL, "sampler and run disagree on dimension / phase method");
         if (!rc && !(run->traj_t_final > sd.sample_t_hi))
@@ -765,6 +804,19 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
             return rc;
         }
         ncoef += wfat_doubles(J.S);
+        npre += J.preC.size();
+    }
+    if (npre) {
+        HIP_TRY(c, c->prec.ensure(npre));
+        size_t off = 0;
+        for (int32_t j = 0; j < n_jobs; ++j) {
+            JobPlan& J = jobs[j];
+            if (J.preC.empty()) continue;
+            HIP_TRY(c, hipMemcpyAsync(c->prec.p + off, J.preC.data(), sizeof(double) * J.preC.size(),
+                                      hipMemcpyHostToDevice, c->stream));
+            J.S.pre_C = c->prec.p + off;
+            off += J.preC.size();
+        }
     }
     if (ncoef) {
         HIP_TRY(c, c->wfat.ensure(ncoef));

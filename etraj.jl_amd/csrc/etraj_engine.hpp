@@ -14,6 +14,9 @@ namespace etraj {
 // WFAT integral tables (ETRAJ_WFAT_INTDATA): each dimension <= kWfatMaxDim,
 // at most kWfatMaxChannels channels (nxi, m), |m| <= kWfatMaxM
 constexpr int kWfatMaxDim = 31, kWfatMaxChannels = 512, kWfatMaxM = 15;
+// Prefactor beyond l = 0 (ADK / SPA / SPANE): l_max <= kPreMaxL; the per-batch
+// table has one complex coefficient per monomial u^a v^b z^c, a+b+c <= l_max
+constexpr int kPreMaxL = 15, kPreMaxMono = (kPreMaxL + 1) * (kPreMaxL + 2) * (kPreMaxL + 3) / 6;
 
 struct SamplerP {
     int dim, phase, sampling, prefix;
@@ -37,7 +40,10 @@ struct SamplerP {
     int wfat_form;             // ETRAJ_WFAT_*
     int wfat_dims[4];          // ETRAJ_WFAT_INTDATA: (nxi_max+1, 2 m_max+1, l_max+1, 2 l_max+1)
     double wfat_mu[3];
-    double Ip;                 // the sampler's Ip: the ionizing orbital's for WFAT, else the target's
+    double Ip;                 // the sampler's Ip: the ionizing orbital's for a molecule, else the target's
+    int pre_L;                 // -1: the scalar prefactor numer (atom with l = 0); else l_max of pre_C
+    const double* pre_C;       // device: (re, im) of C_lm at [l + (pre_L+1)(m+l)] (column-major)
+    double pre_rot[3];         // frame of C_lm: MolRotation, or (phi, theta, 0) of an atom's axis
 };
 
 struct IntegP {
@@ -107,6 +113,10 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
 // cnt: per-job counters (CNT_N per job) receiving CNT_LAUNCHED, or NULL
 hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
                              const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt);
+// Doubles of the prefactor table C_lm of a sampler (0 for the scalar form)
+inline size_t pre_doubles(const SamplerP& S) {
+    return S.pre_L < 0 ? 0 : (size_t)2 * (S.pre_L + 1) * (2 * S.pre_L + 1);
+}
 // Doubles of the WFAT structure-factor data of a sampler (0 for other samplers)
 inline size_t wfat_doubles(const SamplerP& S) {
     if (S.method != ETRAJ_INIT_WFAT) return 0;

@@ -101,6 +101,10 @@ struct AdkBatch {
     double beta, gamma; // WFAT: Euler angles FF -> MF of this batch (WFATSampler.jl:143-145)
     const double* wB;   // WFAT: B_|m| (LDS), rate = dkdt E(k) sum_|m| B_|m| k^(2|m|)
     int wM;             // WFAT: m_max of wB
+    double alpha;       // prefactor beyond l = 0: Euler angles FF -> frame of C_lm (with beta, gamma)
+    double ex, ey;      // F/|F| at tr
+    double Fxr, Fyr;    // F at tr
+    const double* pT;   // prefactor beyond l = 0: the batch's monomial coefficients (LDS, (re, im))
     double dFx, dFy;    // SPANE: dF/dt at tr
     double cross;       // SPANE: Fx dFy - dFx Fy (SPANESampler.jl:238)
     bool skip;          // SPA: F(tr) <= 1e-3 empties the batch (SPASampler.jl:167-169)
@@ -108,7 +112,7 @@ struct AdkBatch {
 
 // params(RotZYZ(MF^-1 FF)) (shared_methods.jl:31-38, Rotations.jl ZYZ
 // decomposition with beta in [0, pi]); returns (beta, gamma).
-__device__ inline void ff_mf_euler(const double* mr, double Fx, double Fy, double& beta, double& gamma) {
+__device__ inline void ff_mf_euler(const double* mr, double Fx, double Fy, double& alpha, double& beta, double& gamma) {
     const double F = hypot(Fx, Fy);
     const double FF[3][3] = {{0, Fy / F, Fx / F}, {0, -Fx / F, Fy / F}, {1, 0, 0}};
     double sa, ca, sb, cb, sc, cc;
@@ -127,6 +131,7 @@ __device__ inline void ff_mf_euler(const double* mr, double Fx, double Fy, doubl
     const double t1 = atan2(R[1][2], R[0][2]);
     double st, ct;
     sincos(t1, &st, &ct);
+    alpha = t1;
     beta = atan2(ct * R[0][2] + st * R[1][2], R[2][2]);
     gamma = atan2(-st * R[0][0] + ct * R[1][0], -st * R[0][1] + ct * R[1][1]);
 }
@@ -229,15 +234,104 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     B.c_cc = pow(2.0, 3 * nn / 2 + 1) * pow(kappa, 5 * nn + 0.5) * pow(B.F, -nn) * pow(1 + 2 * gam / e0, -nn);  // :235
     B.dkdt = S.dkdt_base;
     B.g2 = 0.0;
-    B.beta = B.gamma = 0.0;
+    B.alpha = B.beta = B.gamma = 0.0;
     B.wB = nullptr;
     B.wM = 0;
+    B.pT = nullptr;
+    B.ex = Fx / B.F;
+    B.ey = Fy / B.F;
+    B.Fxr = Fx;
+    B.Fyr = Fy;
     B.skip = (M == ETRAJ_INIT_SPA) && !(B.F > 1e-3);
     if (M == ETRAJ_INIT_WFAT) {
-        ff_mf_euler(S.mol_rot, Fx, Fy, B.beta, B.gamma);   // WFATSampler.jl:143-145
+        ff_mf_euler(S.mol_rot, Fx, Fy, B.alpha, B.beta, B.gamma);   // WFATSampler.jl:143-145
         if (S.wfat_form == ETRAJ_WFAT_TABLE) B.g2 = wfat_g2(S, B.beta, B.gamma);   // :151-155, channel (0, 0)
+    } else if (S.pre_L >= 0) {
+        ff_mf_euler(S.pre_rot, Fx, Fy, B.alpha, B.beta, B.gamma);  // ADKSampler.jl:198-204
     }
     return B;
+}
+
+// ---- tunnelling prefactor beyond l = 0 (ADKSampler.jl:188-265, SPASampler.jl:
+// 204-278, SPANESampler.jl:208-266): N(k) = sum_{l,m,m'} C_lm D^l_{m' m}(alpha,
+// beta, gamma) Y_{l m'}(k'), Y of shared_methods.jl:15-24,
+//   Y_{lm'} = (-1)^m' sqrt((2l+1)/4pi (l-m')!/(l+m')!) (i/kappa)^l (l+m')!
+//             sum_k (-1)^k / (2^(2k+m') (k+m')! k! (l-m'-2k)!) u^(k+m') v^k z^(l-m'-2k),
+// u = x + iy, v = x - iy. Each (l, m', k) is one monomial u^a v^b z^c with
+// a = k+m', b = k, c = l-m'-2k (and back: l = a+b+c, m' = a-b, k = b), so per
+// batch the sum is regrouped into one coefficient T_abc per monomial, a+b+c <= L,
+// stored in the order of pre_eval's loops.
+__device__ inline double factd(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= i;
+    return f;
+}
+
+__device__ inline void pre_monomial(const SamplerP& S, const AdkBatch& B, int idx, double out[2]) {
+    const int L = S.pre_L;
+    int a = 0, b = 0, c = 0, i = idx;
+    for (a = 0; a <= L; ++a) {                       // idx -> (a, b, c) in pre_eval's order
+        const int na = (L - a + 1) * (L - a + 2) / 2;
+        if (i < na) break;
+        i -= na;
+    }
+    for (b = 0; b <= L - a; ++b) {
+        const int nb = L - a - b + 1;
+        if (i < nb) break;
+        i -= nb;
+    }
+    c = i;
+    const int l = a + b + c, m = a - b, k = b;
+    // A_{l m} = sum_{m0} C_{l m0} e^{-i m alpha} d^l_{m m0}(beta) e^{-i m0 gamma}
+    double are = 0.0, aim = 0.0;
+    for (int m0 = -l; m0 <= l; ++m0) {
+        const double* C = S.pre_C + 2 * (l + (int64_t)(L + 1) * (m0 + l));
+        if (C[0] == 0.0 && C[1] == 0.0) continue;                     // :258-259
+        const double d = wigner_d(l, m, m0, B.beta);
+        double sp, cp;
+        sincos(-(m * B.alpha + m0 * B.gamma), &sp, &cp);
+        are += d * (C[0] * cp - C[1] * sp);
+        aim += d * (C[0] * sp + C[1] * cp);
+    }
+    const double kappa = sqrt(2 * S.Ip);
+    double coef = sqrt((2 * l + 1) / (4 * kPi) * factd(l - m) / factd(l + m)) * factd(l + m) /
+                  (pow(2.0, 2 * k + m) * factd(k + m) * factd(k) * factd(l - m - 2 * k)) / pow(kappa, l);
+    if ((m + k) & 1) coef = -coef;                                   // (-1)^m' (-1)^k
+    double pre = are * coef, pim = aim * coef;                       // times i^l
+    for (int q = 0; q < (l & 3); ++q) {
+        const double t = pre;
+        pre = -pim;
+        pim = t;
+    }
+    out[0] = pre;
+    out[1] = pim;
+}
+
+// N(kts): the FF components of conj(kts) (obtain_xyz_FF_LF; LinearAlgebra.dot
+// conjugates its first argument), then sum_abc T_abc u^a v^b z^c.
+__device__ inline Cx pre_eval(const SamplerP& S, const AdkBatch& B, Cx kx, Cx ky, Cx kz) {
+    const Cx cx{kx.re, -kx.im}, cy{ky.re, -ky.im}, cz{kz.re, -kz.im};
+    const Cx nx = cz;                                                   // x' = (0, 0, 1)
+    const Cx ny = B.ey * cx - B.ex * cy;                                // y' = (Fy, -Fx, 0)/F
+    const Cx nz = B.ex * cx + B.ey * cy;                                // z' = (Fx, Fy, 0)/F
+    const Cx u{nx.re - ny.im, nx.im + ny.re}, v{nx.re + ny.im, nx.im - ny.re};
+    const int L = S.pre_L;
+    Cx sum{0.0, 0.0}, ua{1.0, 0.0};
+    int idx = 0;
+    for (int a = 0; a <= L; ++a) {
+        Cx uvb = ua;
+        for (int b = 0; b <= L - a; ++b) {
+            Cx t = uvb;
+            for (int c = 0; c <= L - a - b; ++c) {
+                const double* T = B.pT + 2 * idx++;
+                sum = sum + Cx{T[0], T[1]} * t;
+                t = t * nz;
+            }
+            uvb = uvb * v;
+        }
+        ua = ua * u;
+    }
+    return sum;
 }
 
 // SFA-SPA (SPASampler.jl:230-292). With px, py of :231-232 the saddle-point
@@ -312,7 +406,14 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
     if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
         const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
         const double p = (S.nstar + 1) / 2;
-        lmag += log(cpre * S.numer) - p * log(hypot(kdotF.re, kdotF.im));
+        if (S.pre_L < 0) {
+            lmag += log(cpre * S.numer);
+        } else {   // k_ts = (px + Ax(ts), py + Ay(ts), kz) (:250)
+            const Cx N = pre_eval(S, B, Cx{px + Ax.re, Ax.im}, Cx{py + Ay.re, Ay.im}, Cx{kz, 0.0});
+            lmag += log(cpre) + log(hypot(N.re, N.im));
+            arg += atan2(N.im, N.re);
+        }
+        lmag -= p * log(hypot(kdotF.re, kdotF.im));
         arg -= p * atan2(kdotF.im, kdotF.re);
     }
     if (S.prefix & ETRAJ_PREFIX_JAC) {
@@ -392,13 +493,28 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         y0 = r0 * B.sphi;
         if (f2 <= B.F * B.F / (S.sampling == ETRAJ_SAMPLING_STEP ? 9.0 : 25.0)) return false;   // :332, :378
         double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));
+        double aim = 0.0;
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
             const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
-            amp *= cpre * S.numer / pow((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
+            const double den = pow((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
+            if (S.pre_L < 0) {
+                amp *= cpre * S.numer / den;
+            } else {   // ti = sqrt((kappa^2+kd^2+kz^2)/F2eff), k_ts of :241-242
+                const double ti = sqrt((2 * Ip + kd0 * kd0 + kz0 * kz0) / f2);
+                const Cx N = pre_eval(S, B, Cx{kx0 + ti * ti / 2 * B.dFx, -ti * B.Fxr},
+                                      Cx{ky0 + ti * ti / 2 * B.dFy, -ti * B.Fyr}, Cx{kz0, 0.0});
+                const double f = amp * cpre / den;
+                amp = f * N.re;
+                aim = f * N.im;
+            }
         }
-        if (S.prefix & ETRAJ_PREFIX_JAC)
-            amp *= sqrt(fabs(B.F + sqrt(kd0 * kd0 + kz0 * kz0) / (B.F * B.F) * B.cross));   // :270
-        rate = amp * amp;
+        if (S.prefix & ETRAJ_PREFIX_JAC) {
+            const double jf = sqrt(fabs(B.F + sqrt(kd0 * kd0 + kz0 * kz0) / (B.F * B.F) * B.cross));   // :270
+            amp *= jf;
+            aim *= jf;
+        }
+        rate = amp * amp + aim * aim;
+        if (S.pre_L >= 0) phase = atan2(aim, amp);
         if (isnan(rate) || rate / w < S.cutoff) return false;        // :337: rate/dkdt (Q7)
     } else if (M == ETRAJ_INIT_WFAT) {
         if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) return false;    // WFATSampler.jl:188
@@ -412,12 +528,26 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         if (fabs(kd0) < 1e-4) return false;                           // ADKSampler.jl:312,:321
         const double kx = kd0 * -B.sphi, ky = kd0 * B.cphi;
         double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * B.F));  // :198
+        double aim = 0.0;                                             // Im amp (prefactor beyond l = 0)
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
             double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
-            amp *= cpre * S.numer / pow((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
+            const double den = pow((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
+            if (S.pre_L < 0) {
+                amp *= cpre * S.numer / den;
+            } else {   // k_ts at ts = tr + i ti, ti = sqrt(kappa^2+kd^2+kz^2)/F (:237-238)
+                const double ti = sqrt(2 * Ip + kd0 * kd0 + kz0 * kz0) / B.F;
+                const Cx N = pre_eval(S, B, Cx{kx, -ti * B.Fxr}, Cx{ky, -ti * B.Fyr}, Cx{kz0, 0.0});
+                const double f = amp * cpre / den;
+                amp = f * N.re;
+                aim = f * N.im;
+            }
         }
-        if (S.prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(B.F);             // :266,:283
-        rate = amp * amp;
+        if (S.prefix & ETRAJ_PREFIX_JAC) {                            // :266,:283
+            amp *= sqrt(B.F);
+            aim *= sqrt(B.F);
+        }
+        rate = amp * amp + aim * aim;
+        if (S.pre_L >= 0) phase = atan2(aim, amp);                    // angle(amp)
     }
     if (M == ETRAJ_INIT_ADK || M == ETRAJ_INIT_WFAT) {
         kx0 = kd0 * -B.sphi;                                          // :324-325
@@ -487,6 +617,11 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
             B.wM = 0;
         }
         B.wB = wfat_B;
+    } else if (S.pre_L >= 0 && (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC))) {
+        __shared__ double pre_T[2 * kPreMaxMono];
+        const int nmono = (S.pre_L + 1) * (S.pre_L + 2) * (S.pre_L + 3) / 6;
+        for (int i = tid; i < nmono; i += blockDim.x) pre_monomial(S, B, i, pre_T + 2 * i);
+        B.pT = pre_T;
     }
     __syncthreads();
     double col[9];

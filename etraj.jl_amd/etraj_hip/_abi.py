@@ -65,7 +65,8 @@ class SamplerDesc(C.Structure):
                 ("cutoff_limit", C.c_double), ("seed", C.c_uint64), ("mol_rot", C.c_double * 3),
                 ("wfat_na", C.c_int32), ("wfat_nb", C.c_int32), ("wfat_coeff", C.POINTER(C.c_double)),
                 ("wfat_form", C.c_int32), ("wfat_dims", C.c_int32 * 4), ("wfat_reserved", C.c_int32),
-                ("wfat_mu", C.c_double * 3), ("wfat_ip", C.c_double)]
+                ("wfat_mu", C.c_double * 3), ("orbit_ip", C.c_double),
+                ("asymp_lmax", C.c_int32), ("asymp_reserved", C.c_int32), ("asymp_coeff", C.POINTER(C.c_double))]
 
 
 class RunDesc(C.Structure):

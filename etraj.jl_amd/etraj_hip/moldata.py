@@ -16,8 +16,9 @@ executes nothing from the file. JLD2's encodings that occur in these files:
 * `Tuple{Int,Int}` (open-shell orbital keys): a compound {"1", "2"} of integers.
 
 Only the fields the trajectory path uses are decoded: energies, occupations,
-charge, spin, atoms, and the WFAT data (`wfat_intdata[ridx]`, a 4-index complex
-array intdata[nxi, m, l, m'], and `wfat_mu[ridx]`, a 3-vector).
+charge, spin, atoms, the WFAT data (`wfat_intdata[ridx]`, a 4-index complex
+array intdata[nxi, m, l, m'], and `wfat_mu[ridx]`, a 3-vector) and the
+asymptotic coefficients (`asymp_coeff[ridx]`, C_lm at [l, m + l]).
 """
 import ctypes
 import ctypes.util
@@ -264,16 +265,18 @@ def read_molecule_jld2(path):
     trajectory path uses, as plain Python / numpy values in Julia's index
     order (0-based): name, atoms, atom_coords (natoms x 3), charge, spin,
     energy_levels, orbit_occ, wfat_intdata {ridx: complex array
-    [nxi, m, l, m']}, wfat_mu {ridx: 3-vector}."""
+    [nxi, m, l, m']}, wfat_mu {ridx: 3-vector}, asymp_coeff {ridx: complex
+    array [l, m + l]} (MolAsympCoeff, GenericMolecule.jl:304-309)."""
     f = _File(path)
     try:
         out = {k: f.get(k) for k in ("name", "atoms", "atom_coords", "charge", "spin", "energy_levels",
                                      "orbit_occ")}
         out["wfat_intdata"] = f.get("wfat_intdata") if f.exists("wfat_intdata") else {}
         out["wfat_mu"] = f.get("wfat_μ") if f.exists("wfat_μ") else {}
+        out["asymp_coeff"] = f.get("asymp_coeff") if f.exists("asymp_coeff") else {}
     finally:
         f.close()
-    for k in ("wfat_intdata", "wfat_mu"):
+    for k in ("wfat_intdata", "wfat_mu", "asymp_coeff"):
         if not isinstance(out[k], dict):
             out[k] = {}
     out["charge"] = int(out["charge"])

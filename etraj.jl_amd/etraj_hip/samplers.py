@@ -111,7 +111,7 @@ class ElectronSampler:
 
 
 class ADKSampler(ElectronSampler):
-    """ADKSampler.jl:3-150 (atoms); device sampling."""
+    """ADKSampler.jl:3-150 (atoms and GenericMolecule); device sampling."""
     _method = _abi.INIT_ADK
     _who = "ADKSampler"
 
@@ -119,9 +119,10 @@ class ADKSampler(ElectronSampler):
                  sample_monte_carlo=False, traj_phase_method="CTMC", rate_prefix="Full", ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
                  ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, mol_orbit_ridx=0, seed=0,
                  sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, **_):
+        from .targets import GenericMolecule
         who = self._who
-        if not isinstance(target, SAEAtomBase):
-            raise NotImplementedError(f"[{who}] the engine samples atomic targets (SAEAtomBase) only")
+        if not isinstance(target, (SAEAtomBase, GenericMolecule)):
+            raise NotImplementedError(f"[{who}] the engine samples atoms (SAEAtomBase) and GenericMolecule targets")
         pm = _sym(traj_phase_method)
         if pm not in _abi.PHASE_METHODS:
             raise ValueError(f"[{who}] Undefined phase method [{traj_phase_method}].")
@@ -134,8 +135,22 @@ class ADKSampler(ElectronSampler):
         _fill_sampling(d, who, dimension, sample_t_intv, sample_t_num, sample_cutoff_limit,
                        sample_monte_carlo, sample_stratified, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num,
                        mc_kd_max, mc_kz_max, kd_sigma, kz_sigma, seed)
+        Ip = IonPotential(target)
+        if isinstance(target, GenericMolecule):     # MO-ADK / MO-SFA: the orbital's C_lm (ADKSampler.jl:188-265)
+            Ip = target.ion_potential(mol_orbit_ridx)
+            if Ip <= 0:
+                raise ValueError(f"[{who}] The energy of the ionizing orbit is non-negative.")
+            C = target.MolAsympCoeff(mol_orbit_ridx)
+            flat = C.ravel(order="F")               # Julia's column-major memory of C[l, m + l]
+            coeff = np.ascontiguousarray(np.column_stack([flat.real, flat.imag]).ravel())
+            d.asymp_lmax = C.shape[0] - 1
+            d.asymp_coeff = _abi.dptr(coeff)
+            d._keep_asymp = coeff                   # the descriptor points into it
+            d.orbit_ip = Ip
+            for i, a in enumerate(target.rot):
+                d.mol_rot[i] = a
         if self._method == _abi.INIT_ADK:
-            _keldysh_warning(who, laser, IonPotential(target))
+            _keldysh_warning(who, laser, Ip)
         self.desc = d
 
     @property
@@ -208,7 +223,7 @@ class WFATSampler(ADKSampler):
                 d.wfat_dims[k] = intdata.shape[k]
             for k in range(3):
                 d.wfat_mu[k] = mu[k]
-            d.wfat_ip = ip_orbit
+            d.orbit_ip = ip_orbit
             d.wfat_na, d.wfat_nb = intdata.shape[2], intdata.shape[3]
         else:                                       # channel (0, 0) from a G00 table
             coeff = target.coeff_table()

@@ -219,6 +219,17 @@ class GenericMolecule(MoleculeBase):
                              "(the PySCF calculation is not part of this engine).")
         return np.asarray(mus[key], dtype=np.float64), np.asarray(table[key], dtype=np.complex128)
 
+    def MolAsympCoeff(self, orbit_ridx=0) -> np.ndarray:
+        """GenericMolecule.jl:304-309: the orbital's asymptotic coefficients,
+        complex C[l, m + l] (0-based), l = 0..l_max."""
+        key = self._key(orbit_ridx)
+        norm = (lambda k: tuple(int(v) for v in k)) if self.open_shell else int
+        table = {norm(k): v for k, v in self.data.get("asymp_coeff", {}).items()}
+        if key not in table:
+            raise ValueError(f"[GenericMolecule] The asymptotic coefficients of orbit_ridx={orbit_ridx} are not "
+                             "available (the PySCF calculation is not part of this engine).")
+        return np.asarray(table[key], dtype=np.complex128)
+
     def _desc(self) -> _abi.TargetDesc:
         d = _abi.TargetDesc()
         d.kind = _abi.TARGET_HYDROGEN_LIKE

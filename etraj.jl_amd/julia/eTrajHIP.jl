@@ -46,7 +46,8 @@ struct SamplerDesc
     mc_kd_max::Float64; mc_kz_max::Float64; kd_sigma::Float64; kz_sigma::Float64
     cutoff_limit::Float64; seed::UInt64
     mol_rot::NTuple{3,Float64}; wfat_na::Int32; wfat_nb::Int32; wfat_coeff::Ptr{Float64}
-    wfat_form::Int32; wfat_dims::NTuple{4,Int32}; wfat_reserved::Int32; wfat_mu::NTuple{3,Float64}; wfat_ip::Float64
+    wfat_form::Int32; wfat_dims::NTuple{4,Int32}; wfat_reserved::Int32; wfat_mu::NTuple{3,Float64}
+    orbit_ip::Float64; asymp_lmax::Int32; asymp_reserved::Int32; asymp_coeff::Ptr{Float64}
 end
 struct RunDesc
     dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
@@ -151,12 +152,20 @@ init_method(m::Symbol) = m in (:ADK, :MOADK) ? Int32(1) : m in (:SPA, :MOSPA, :S
 # The sampler descriptor and the arrays it points into (keep them under GC.@preserve).
 # WFAT: every channel of the WFAT integrals MolWFATData(mol, ridx) (GenericMolecule.jl:228-233),
 # passed as Julia's own Array{ComplexF64,4} memory (ETRAJ_WFAT_INTDATA = 1).
+# ADK / SPA / SPANE of a molecule (MO-ADK, MO-SFA): MolAsympCoeff(mol, ridx), Julia's own
+# Matrix{ComplexF64} memory. Atoms with l > 0 need nothing here (the target descriptor's l, m).
 function sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_t_num, ss_kd_max=0.0, ss_kd_num=0,
                       ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC, sample_cutoff_limit=1e-16,
                       rate_prefix=:Full, target=nothing, mol_orbit_ridx=0, kwargs...)
     rot = (0.0, 0.0, 0.0); form = Int32(0); dims = (Int32(0), Int32(0), Int32(0), Int32(0))
     mu = (0.0, 0.0, 0.0); ip = 0.0; coeff = C_NULL; keep = nothing; na = Int32(0); nb = Int32(0)
-    if init_cond_method == :WFAT
+    alm = Int32(0); acoeff = C_NULL
+    if init_cond_method != :WFAT && target isa GenericMolecule
+        C = MolAsympCoeff(target, mol_orbit_ridx)               # GenericMolecule.jl:304-309
+        keep = C
+        acoeff = Ptr{Float64}(pointer(C)); alm = Int32(size(C, 1) - 1)
+        ip = IonPotential(target, mol_orbit_ridx); rot = Float64.(MolRotation(target))
+    elseif init_cond_method == :WFAT
         μ, intdata = MolWFATData(target, mol_orbit_ridx)
         keep = intdata
         coeff = Ptr{Float64}(pointer(intdata))
@@ -167,7 +176,7 @@ function sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_
     sd = SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0,
                      sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
                      ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, rot, na, nb, coeff,
-                     form, dims, 0, mu, ip)
+                     form, dims, 0, mu, ip, alm, 0, acoeff)
     return sd, keep
 end
 

@@ -142,8 +142,8 @@ typedef struct etraj_sampler_desc {
     double   kd_sigma, kz_sigma;
     double   cutoff_limit;
     uint64_t seed;
-    /* WFAT only: orientation of the molecular frame (ZYZ Euler angles, as
-     * MolRotation, GenericMolecule.jl) and the G00 coefficient table */
+    /* molecules: orientation of the molecular frame (ZYZ Euler angles, as
+     * MolRotation, GenericMolecule.jl:315); WFAT: the structure factor */
     double   mol_rot[3];
     int32_t  wfat_na, wfat_nb;
     const double* wfat_coeff;  /* (re, im) pairs, layout per wfat_form                  */
@@ -151,9 +151,19 @@ typedef struct etraj_sampler_desc {
     int32_t  wfat_dims[4];     /* ETRAJ_WFAT_INTDATA: size(wfat_intdata[ridx])         */
     int32_t  wfat_reserved;
     double   wfat_mu[3];       /* ETRAJ_WFAT_INTDATA: wfat_mu[ridx], the orbital dipole */
-    double   wfat_ip;          /* Ip of the ionizing orbital, IonPotential(mol, ridx)
-                                  (WFATSampler.jl:139); <= 0: the target's Ip. The
-                                  EOM keeps the target's (HOMO) Ip, GenericMolecule.jl:457 */
+    double   orbit_ip;         /* molecules: Ip of the ionizing orbital, IonPotential(mol,
+                                  ridx) (ADKSampler.jl:177-182, WFATSampler.jl:139); <= 0:
+                                  the target's Ip. The EOM keeps the target's (HOMO) Ip,
+                                  GenericMolecule.jl:457                                 */
+    /* ADK / SPA / SPANE prefactor of a molecule (ADKSampler.jl:188-265, SPASampler.jl:
+     * 204-278, SPANESampler.jl:208-266): the orbital's asymptotic coefficients C_lm,
+     * MolAsympCoeff(mol, ridx), Julia's Matrix{ComplexF64} memory as is ((re, im),
+     * column-major (asymp_lmax+1) x (2 asymp_lmax+1), C_lm at [l, m+l]) in the frame of
+     * mol_rot. NULL: an atom's (l, m, asymp_coeff) and quantization axis from the
+     * target descriptor (l = 0: the scalar C Y_00).                                    */
+    int32_t  asymp_lmax;       /* 0..15 */
+    int32_t  asymp_reserved;
+    const double* asymp_coeff;
 } etraj_sampler_desc;
 
 /* ---- trajectory run ---------------------------------------------------------- */

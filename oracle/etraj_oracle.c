@@ -229,6 +229,7 @@ double oracle_unit_envelope(const etraj_laser_desc* d, double t) {
 typedef struct {
     int kind, Z, l, m;
     double Ip, C, sc, a1, b1, a2, b2, a3, b3;
+    double qth, qph;   /* QuantizationAxisOrientaion (theta, phi) */
 } target_t;
 
 static int target_init(const etraj_target_desc* d, target_t* T) {
@@ -237,6 +238,7 @@ static int target_init(const etraj_target_desc* d, target_t* T) {
     T->kind = d->kind; T->Z = d->nucl_charge; T->l = d->l; T->m = d->m;
     T->Ip = d->Ip; T->C = d->asymp_coeff; T->sc = d->soft_core;
     T->a1 = d->a1; T->b1 = d->b1; T->a2 = d->a2; T->b2 = d->b2; T->a3 = d->a3; T->b3 = d->b3;
+    T->qth = d->quan_ax_theta; T->qph = d->quan_ax_phi;
     return 0;
 }
 
@@ -545,6 +547,7 @@ typedef struct {
     double* tsamp;     /* t_samples, length sample_t_num */
     double tstep;      /* step(t_samples) = (max-min)/length  (ADKSampler.jl:267) */
     double c_pre;      /* c (ADKSampler.jl:233) */
+    double Ip;         /* the sampler's Ip: IonPotential(mol, ridx) of a molecule, else the target's */
 } sampler_t;
 
 /* i-th point (0-based) of range(lo, hi; length=n): (lo*(n-1-i) + hi*i)/(n-1),
@@ -607,7 +610,8 @@ static int sampler_init(const etraj_laser_desc* ld, const etraj_target_desc* td,
     S->sd = sd;
     if (laser_init(ld, &S->L) || target_init(td, &S->T)) return ETRAJ_EINVAL;
     if (sd->init_method < ETRAJ_INIT_ADK || sd->init_method > ETRAJ_INIT_WFAT) return ETRAJ_EUNSUPPORTED;
-    if (sd->init_method != ETRAJ_INIT_WFAT && S->T.l != 0) return ETRAJ_EUNSUPPORTED;
+    if (sd->init_method != ETRAJ_INIT_WFAT && sd->asymp_coeff && !(sd->asymp_lmax >= 0 && sd->asymp_lmax <= 15))
+        return ETRAJ_EINVAL;
     if (sd->init_method == ETRAJ_INIT_WFAT && (sd->wfat_na < 1 || sd->wfat_nb < 1 || !sd->wfat_coeff)) return ETRAJ_EINVAL;
     if (sd->dimension != 2 && sd->dimension != 3) return ETRAJ_EINVAL;
     if (sd->phase_method < 0 || sd->phase_method > 2) return ETRAJ_EINVAL;
@@ -631,7 +635,8 @@ static int sampler_init(const etraj_laser_desc* ld, const etraj_target_desc* td,
         tmax = fmax(tmax, S->tsamp[i]);
     }
     S->tstep = (tmax - tmin) / sd->sample_t_num;
-    double kappa = sqrt(2 * S->T.Ip), n = S->T.Z / kappa;
+    S->Ip = sd->orbit_ip > 0 ? sd->orbit_ip : S->T.Ip;
+    double kappa = sqrt(2 * S->Ip), n = S->T.Z / kappa;
     S->c_pre = pow(2, n / 2 + 1) * pow(kappa, 2 * n + 0.5) * tgamma(n / 2 + 1);
     return ETRAJ_OK;
 }
@@ -640,9 +645,103 @@ static void sampler_free(sampler_t* S) { free(S->tsamp); S->tsamp = NULL; }
 
 static inline double gauss_pdf(double x, double s) { return exp(-0.5 * (x / s) * (x / s)) / (s * sqrt(2 * M_PI)); }
 
-/* gen_electron_batch(sp::ADKSampler, batch_id) for an atom with l = 0
- * (ADKSampler.jl:167-396). For l = 0 the Wigner-D factor is 1 and Y_00 =
- * 1/sqrt(4 pi), so the prefactor is real and positive: angle(amp) = 0. */
+/* ---- prefactor beyond l = 0: sum over (l, m, m') of C_lm D^l_{m' m}(alpha, beta, gamma)
+ * Y_{l m'}(k') (ADKSampler.jl:188-265, SPASampler.jl:204-278, SPANESampler.jl:208-266) ---- */
+typedef struct {
+    int on;               /* 0: the scalar C Y_00 of an atom with l = 0 */
+    int L, atom_l, atom_m;
+    const double* C;      /* molecule: (re, im) of C_lm at [l, m+l], column-major (L+1) x (2L+1) */
+    double atomC;         /* atom: its asymptotic coefficient at (atom_l, atom_m) */
+    double alpha, beta, gamma, kappa, ex, ey;
+} prefac_t;
+
+/* Per batch: the Euler angles FF -> target frame (obtain_FF_MF_Euler of MolRotation(mol), or of
+ * (phi, theta, 0) of an atom's quantization axis, ADKSampler.jl:198-204) and F/|F| at tr. */
+static void prefac_init(const sampler_t* S, double Fx, double Fy, prefac_t* P) {
+    const etraj_sampler_desc* sd = S->sd;
+    memset(P, 0, sizeof *P);
+    double rot[3];
+    if (sd->asymp_coeff) {
+        P->L = sd->asymp_lmax;
+        P->C = sd->asymp_coeff;
+        for (int k = 0; k < 3; ++k) rot[k] = sd->mol_rot[k];
+    } else if (S->T.l != 0) {
+        P->L = P->atom_l = S->T.l;
+        P->atom_m = S->T.m;
+        P->atomC = S->T.C;
+        rot[0] = S->T.qph; rot[1] = S->T.qth; rot[2] = 0.0;
+    } else {
+        return;
+    }
+    P->on = 1;
+    double eul[3];
+    oracle_ff_mf_euler(rot, Fx, Fy, eul);
+    P->alpha = eul[0]; P->beta = eul[1]; P->gamma = eul[2];
+    P->kappa = sqrt(2 * S->Ip);
+    double F = hypot(Fx, Fy);
+    P->ex = Fx / F;
+    P->ey = Fy / F;
+}
+
+/* Y_{lm}(x, y, z) of gen_sph_harm_funcs (shared_methods.jl:15-24) at complex (x, y, z):
+ * (-1)^m sqrt((2l+1)/4pi (l-m)!/(l+m)!) / (-i kappa)^l (l+m)! sum_k (-1)^k /
+ * (2^(2k+m) (k+m)! k! (l-m-2k)!) (x+iy)^(k+m) (x-iy)^k z^(l-m-2k). */
+/* z^n for an integer n >= 0, as Julia's ^(::Complex, ::Integer) (z^0 = 1, also for z = 0) */
+static double complex cipow(double complex z, int n) {
+    double complex r = 1;
+    for (int i = 0; i < n; ++i) r *= z;
+    return r;
+}
+
+static double complex sph_harm_c(int l, int m, double kappa, double complex x, double complex y, double complex z) {
+    double complex s = 0;
+    for (int k = (m < 0 ? -m : 0); k <= (l - m) / 2; ++k) {
+        double c = ((k & 1) ? -1.0 : 1.0) /
+                   (pow(2.0, 2 * k + m) * tgamma(k + m + 1.0) * tgamma(k + 1.0) * tgamma(l - m - 2 * k + 1.0));
+        s += c * cipow(x + I * y, k + m) * cipow(x - I * y, k) * cipow(z, l - m - 2 * k);
+    }
+    double nrm = ((m & 1) ? -1.0 : 1.0) * sqrt((2 * l + 1) / (4 * M_PI) * tgamma(l - m + 1.0) / tgamma(l + m + 1.0));
+    return nrm / cipow(-I * kappa, l) * tgamma(l + m + 1.0) * s;
+}
+
+void oracle_sph_harm_c(int l, int m, double kappa, const double xyz[6], double out[2]) {
+    double complex v = sph_harm_c(l, m, kappa, xyz[0] + I * xyz[1], xyz[2] + I * xyz[3], xyz[4] + I * xyz[5]);
+    out[0] = creal(v);
+    out[1] = cimag(v);
+}
+
+/* D^l_{m' m}(alpha, beta, gamma) = e^{-i m' alpha} d^l_{m' m}(beta) e^{-i m gamma} (WignerD.jl's wignerDjmn) */
+static double complex wigner_D(int l, int mp, int m, double a, double b, double g) {
+    return cexp(-I * (double)mp * a) * oracle_wigner_d(l, mp, m, b) * cexp(-I * (double)m * g);
+}
+
+/* pre_numerator summed over its (l, m, m') list. k' are the FF components of kts
+ * (obtain_xyz_FF_LF: x' = z, y' = (Fy, -Fx, 0)/F, z' = (Fx, Fy, 0)/F) taken with
+ * LinearAlgebra.dot, which conjugates its first argument: k'_i = conj(kts) . e_i. */
+static double complex prefac_numer(const prefac_t* P, const double complex kts[3]) {
+    double complex c0 = conj(kts[0]), c1 = conj(kts[1]), c2 = conj(kts[2]);
+    double complex nx = c2, ny = c0 * P->ey - c1 * P->ex, nz = c0 * P->ex + c1 * P->ey;
+    double complex sum = 0;
+    for (int l = (P->C ? 0 : P->atom_l); l <= P->L; ++l)
+        for (int m = -l; m <= l; ++m) {
+            double complex C;
+            if (P->C) {
+                const double* c = P->C + 2 * (l + (int64_t)(P->L + 1) * (m + l));
+                C = c[0] + I * c[1];
+                if (C == 0) continue;                              /* :258-259 */
+            } else {
+                if (m != P->atom_m) continue;
+                C = P->atomC;
+            }
+            for (int mp = -l; mp <= l; ++mp)
+                sum += C * wigner_D(l, mp, m, P->alpha, P->beta, P->gamma) * sph_harm_c(l, mp, P->kappa, nx, ny, nz);
+        }
+    return sum;
+}
+
+/* gen_electron_batch(sp::ADKSampler, batch_id) (ADKSampler.jl:167-396). For an
+ * atom with l = 0 the Wigner-D factor is 1 and Y_00 = 1/sqrt(4 pi), so the
+ * prefactor is real and positive; otherwise the sum of prefac_numer. */
 static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out, int64_t cap) {
     const etraj_sampler_desc* sd = S->sd;
     const int dim = sd->dimension, ph = sd->phase_method;
@@ -652,7 +751,7 @@ static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out,
     laser_F(&S->L, tr, &Fxtr, &Fytr);
     const double Ftr = hypot(Fxtr, Fytr);
     const double phi = atan2(-Fytr, -Fxtr);                       /* :173 */
-    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    const double Ip = S->Ip, Z = (double)S->T.Z;
     double gam = 0.0;                                              /* :183-187 */
     if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
     const double kappa = sqrt(2 * Ip), n = Z / kappa;
@@ -660,6 +759,8 @@ static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out,
     const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n); /* :235 */
     const double Y00 = sqrt(1.0 / (4 * M_PI));                     /* shared_methods.jl:21, l = m = 0 */
     const double numer = S->T.C * 1.0 * Y00;                       /* pre_numerator :255, D^0_00 = 1 */
+    prefac_t PF;
+    prefac_init(S, Fxtr, Fytr, &PF);                               /* l > 0 or a molecule: :188-265 */
     const double jac = Ftr;                                        /* :266 */
     double dkdt;                                                   /* :268-276 */
     int64_t nk;
@@ -712,25 +813,32 @@ static int64_t adk_gen_batch(const sampler_t* S, int b /*0-based*/, double* out,
         double kx0 = kd0 * -sin(phi), ky0 = kd0 * cos(phi);           /* :324-325 */
         double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / Ftr;         /* :326 */
         double x0 = r0 * cos(phi), y0 = r0 * sin(phi);
-        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * Ftr));  /* ADKAmpExp :198 */
+        double complex amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * Ftr));  /* ADKAmpExp :198 */
         if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
-            /* pre/pre_cc take (kx,ky,kz) (:258-265, called at :283-291) */
+            /* pre/pre_cc take (kx,ky,kz) and ts = tr + i ti, ti = sqrt(kappa^2+kd^2+kz^2)/F (:238-265, :283-291) */
             double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
-            amp *= cpre * numer / pow((kx0 * kx0 + ky0 * ky0 + kz0 * kz0 + 2 * Ip) * (Ftr * Ftr), (n + 1) / 4);
+            double complex num = numer;
+            if (PF.on) {
+                double ti = sqrt(kappa * kappa + kd0 * kd0 + kz0 * kz0) / Ftr;   /* :237 */
+                double complex kts[3] = {kx0 - I * ti * Fxtr, ky0 - I * ti * Fytr, kz0};   /* k_ts :238 */
+                num = prefac_numer(&PF, kts);
+            }
+            amp *= cpre * num / pow((kx0 * kx0 + ky0 * ky0 + kz0 * kz0 + 2 * Ip) * (Ftr * Ftr), (n + 1) / 4);
         }
         if (S->prefix & ETRAJ_PREFIX_JAC) amp *= sqrt(jac);
-        double rate = amp * amp;                                             /* abs2 :331 */
+        double rate = creal(amp) * creal(amp) + cimag(amp) * cimag(amp);   /* abs2 :331 */
+        double ph0 = carg(amp);                                              /* angle(amp) :337 */
         if (isnan(rate) || rate < sd->cutoff_limit) continue;
         if (count >= cap) return -1;
         double* col = out + count * rows;
         if (dim == 3) {
             double v[8] = {x0, y0, 0.0, kx0, ky0, kz0, tr, rate};
             memcpy(col, v, sizeof v);
-            if (ph != ETRAJ_PHASE_CTMC) col[8] = 0.0;                    /* angle(amp) of a positive real */
+            if (ph != ETRAJ_PHASE_CTMC) col[8] = ph0;
         } else {
             double v[6] = {x0, y0, kx0, ky0, tr, rate};
             memcpy(col, v, sizeof v);
-            if (ph != ETRAJ_PHASE_CTMC) col[6] = 0.0;
+            if (ph != ETRAJ_PHASE_CTMC) col[6] = ph0;
         }
         ++count;
     }
@@ -849,7 +957,7 @@ static int64_t wfat_gen_batch(const sampler_t* S, int b, double* out, int64_t ca
     laser_F(&S->L, tr, &Fxtr, &Fytr);
     const double Ftr = hypot(Fxtr, Fytr);
     const double phi = atan2(-Fytr, -Fxtr);                          /* :137 */
-    const double Ip = sd->wfat_ip > 0 ? sd->wfat_ip : S->T.Ip, Z = (double)S->T.Z;   /* :139 */
+    const double Ip = S->Ip, Z = (double)S->T.Z;   /* :139 */
     double eul[3];
     oracle_ff_mf_euler(sd->mol_rot, Fxtr, Fytr, eul);                 /* :143-145 */
     const double kappa = sqrt(2 * Ip), n = Z / kappa;
@@ -1147,13 +1255,15 @@ static int64_t spane_gen_batch(const sampler_t* S, int b, double* out, int64_t c
     double Fx, Fy, dFx, dFy;
     laser_F_dual(&S->L, tr, &Fx, &Fy, &dFx, &dFy);                    /* :164-167 */
     const double Ftr = hypot(Fx, Fy), phi = atan2(-Fy, -Fx);           /* :168-169 */
-    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    const double Ip = S->Ip, Z = (double)S->T.Z;
     double gam = 0.0;                                                  /* :178-182 */
     if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
     const double kappa = sqrt(2 * Ip), n = Z / kappa;
     const double e0 = 2.71828182845904523;
     const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n);
     const double numer = S->T.C * sqrt(1.0 / (4 * M_PI));           /* C D^0_00 Y_00 */
+    prefac_t PF;
+    prefac_init(S, Fx, Fy, &PF);                                       /* l > 0 or a molecule: :188-257 */
     const double cross = Fx * dFy - dFx * Fy;                          /* :238 */
     int64_t nk;
     const double dkdt = sample_dkdt(S, &nk);
@@ -1168,16 +1278,24 @@ static int64_t spane_gen_batch(const sampler_t* S, int b, double* out, int64_t c
         const double f2 = Ftr * Ftr - (kx0 * dFx + ky0 * dFy);         /* F2eff :239 */
         const double r0 = Ftr / 2 * (k2 + 2 * Ip) / f2;                 /* :240 */
         if (f2 <= f2min) continue;
-        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));  /* :275 */
+        double complex amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));  /* :275 */
         if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
             const double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
-            amp *= cpre * numer / pow((k2 + 2 * Ip) * f2, (n + 1) / 4);   /* :262-269 */
+            double complex num = numer;
+            if (PF.on) {   /* ti = sqrt((kappa^2+kd^2+kz^2)/F2eff), k_ts of :241-242 */
+                const double ti = sqrt((kappa * kappa + kd0 * kd0 + kz0 * kz0) / f2);
+                double complex kts[3] = {kx0 - I * ti * Fx + ti * ti / 2 * dFx, ky0 - I * ti * Fy + ti * ti / 2 * dFy,
+                                         kz0};
+                num = prefac_numer(&PF, kts);
+            }
+            amp *= cpre * num / pow((k2 + 2 * Ip) * f2, (n + 1) / 4);   /* :262-269 */
         }
         if (S->prefix & ETRAJ_PREFIX_JAC)
             amp *= sqrt(fabs(Ftr + sqrt(kd0 * kd0 + kz0 * kz0) / (Ftr * Ftr) * cross));   /* :270 */
-        const double rate = amp * amp;
+        const double rate = creal(amp) * creal(amp) + cimag(amp) * cimag(amp);
         if (isnan(rate) || rate / w < sd->cutoff_limit) continue;      /* :337 (rate/dkdt, Q7) */
-        if (put_column(sd, out, &count, cap, r0 * cos(phi), r0 * sin(phi), kx0, ky0, kz0, tr, rate, 0.0)) return -1;
+        if (put_column(sd, out, &count, cap, r0 * cos(phi), r0 * sin(phi), kx0, ky0, kz0, tr, rate, carg(amp)))
+            return -1;
     }
     return count;
 }
@@ -1285,6 +1403,7 @@ typedef struct {
     double ti, px, py, x0, y0, kx0, ky0;
     double complex G;      /* S_tun = i G */
     double complex kdotF;  /* k(ts) . F(ts) */
+    double complex ktx, kty;   /* k_ts = (px + Ax(ts), py + Ay(ts)) (:250) */
     double jac;
 } spa_point_t;
 
@@ -1293,7 +1412,7 @@ typedef struct {
 static int spa_point(const sampler_t* S, double tr, double Ftr, double omega, double kd, double kz, int want_jac,
                      spa_point_t* P) {
     const laser_t* L = &S->L;
-    const double Ip = S->T.Ip;
+    const double Ip = S->Ip;
     pthread_once(&gl_once, gl_init);
     const double ti = spa_solve(L, tr, Ftr, omega, Ip, kd, kz);
     if (ti == 0) return 0;
@@ -1311,6 +1430,8 @@ static int spa_point(const sampler_t* S, double tr, double Ftr, double omega, do
     P->kx0 = P->px + Axr;                                             /* :352-353 */
     P->ky0 = P->py + Ayr;
     P->kdotF = (P->px + Ax) * Fx + (P->py + Ay) * Fy;                 /* :272-277 */
+    P->ktx = P->px + Ax;
+    P->kty = P->py + Ay;
     P->jac = 0;
     if (want_jac) {                                                   /* :280-292 */
         const double dtr = 0.01, dkd = 0.01;
@@ -1346,6 +1467,7 @@ int oracle_spa_point(const etraj_laser_desc* ld, const etraj_target_desc* td, do
     sampler_t S;
     memset(&S, 0, sizeof S);
     if (laser_init(ld, &S.L) || target_init(td, &S.T)) return ETRAJ_EINVAL;
+    S.Ip = S.T.Ip;
     double Fx, Fy;
     laser_F(&S.L, tr, &Fx, &Fy);
     spa_point_t P;
@@ -1366,13 +1488,15 @@ static int64_t spa_gen_batch(const sampler_t* S, int b, double* out, int64_t cap
     const double Ftr = hypot(Fx, Fy);
     const double omega = spa_omega(&S->L);
     if (Ftr <= 1e-3) return 0;                                        /* :167-169 (Q8) */
-    const double Ip = S->T.Ip, Z = (double)S->T.Z;
+    const double Ip = S->Ip, Z = (double)S->T.Z;
     double gam = 0.0;                                                 /* :177-181 */
     if (S->L.n == 1) gam = S->L.c[0].w * sqrt(2 * Ip) / (S->L.c[0].F0 * mono_envelope(&S->L.c[0], tr));
     const double kappa = sqrt(2 * Ip), n = Z / kappa;
     const double e0 = 2.71828182845904523;
     const double c_cc = pow(2, 3 * n / 2 + 1) * pow(kappa, 5 * n + 0.5) * pow(Ftr, -n) * pow(1 + 2 * gam / e0, -n);
     const double numer = S->T.C * sqrt(1.0 / (4 * M_PI));
+    prefac_t PF;
+    prefac_init(S, Fx, Fy, &PF);                                      /* l > 0 or a molecule: :184-270 */
     int64_t nk;
     const double dkdt = sample_dkdt(S, &nk);
     const int jac = (S->prefix & ETRAJ_PREFIX_JAC) != 0;
@@ -1386,7 +1510,12 @@ static int64_t spa_gen_batch(const sampler_t* S, int b, double* out, int64_t cap
         double complex amp = sqrt(w) * cexp(-P.G);                    /* sqrt(dkdt) exp(i S_tun) */
         if (S->prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
             const double cpre = (S->prefix & ETRAJ_PREFIX_PRE) ? S->c_pre : c_cc;
-            amp *= cpre * numer / cpow(P.kdotF, (n + 1) / 2);         /* :271-278 */
+            double complex num = numer;
+            if (PF.on) {
+                double complex kts[3] = {P.ktx, P.kty, kz0};
+                num = prefac_numer(&PF, kts);
+            }
+            amp *= cpre * num / cpow(P.kdotF, (n + 1) / 2);           /* :271-278 */
         }
         if (jac) amp *= sqrt(P.jac);
         const double rate = creal(amp) * creal(amp) + cimag(amp) * cimag(amp);

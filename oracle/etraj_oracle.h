@@ -30,6 +30,7 @@ void oracle_ff_mf_euler(const double mol_rot[3], double Fx, double Fy, double ou
 void oracle_wfat_G(const etraj_sampler_desc* sd, double kappa, double beta, double gamma, int nxi, int m,
                    double out[2]);
 double oracle_wigner_d(int l, int m, int mp, double beta);
+void oracle_sph_harm_c(int l, int m, double kappa, const double xyz[6], double out[2]);
 int oracle_laser_complex(const etraj_laser_desc* d, double t_re, double t_im, double out[8]);
 int oracle_laser_field_deriv(const etraj_laser_desc* d, double t, double out[4]);
 int oracle_spa_point(const etraj_laser_desc* ld, const etraj_target_desc* td, double tr, double kd, double kz,

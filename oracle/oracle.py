@@ -61,6 +61,8 @@ def lib():
         L.oracle_wfat_G.restype = None
         L.oracle_wigner_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
         L.oracle_wigner_d.restype = C.c_double
+        L.oracle_sph_harm_c.argtypes = [C.c_int, C.c_int, C.c_double, DP, DP]
+        L.oracle_sph_harm_c.restype = None
         L.oracle_laser_complex.argtypes = [P(LaserDesc), C.c_double, C.c_double, DP]
         L.oracle_laser_field_deriv.argtypes = [P(LaserDesc), C.c_double, DP]
         L.oracle_spa_point.argtypes = [P(LaserDesc), P(TargetDesc), C.c_double, C.c_double, C.c_double, DP]
@@ -144,6 +146,15 @@ def wfat_g00(sd, beta, gamma, kappa=1.0):
 
 def wigner_d(l, m, mp, beta):
     return lib().oracle_wigner_d(l, m, mp, beta)
+
+
+def sph_harm_c(l, m, kappa, x, y, z):
+    """Y_lm of gen_sph_harm_funcs (shared_methods.jl:15-24) at complex (x, y, z)."""
+    xyz = np.array([complex(x).real, complex(x).imag, complex(y).real, complex(y).imag, complex(z).real,
+                    complex(z).imag])
+    out = np.empty(2)
+    lib().oracle_sph_harm_c(l, m, kappa, _d(xyz), _d(out))
+    return complex(out[0], out[1])
 
 
 def laser_complex(ld, t):

@@ -55,4 +55,5 @@ def fixture_molecule(key, rot=(0.0, 0.0, 0.0)):
         k = ast.literal_eval(str(k))
         data["wfat_intdata"][k] = f[f"wfat_intdata_{i}"]
         data["wfat_mu"][k] = f[f"wfat_mu_{i}"]
+    data["asymp_coeff"] = {ast.literal_eval(str(k)): f[f"asymp_coeff_{i}"] for i, k in enumerate(f["asymp_keys"])}
     return GenericMolecule(data, *rot)

@@ -26,6 +26,10 @@ def main(ref):
         for i, k in enumerate(keys):
             out[f"wfat_intdata_{i}"] = m["wfat_intdata"][k]
             out[f"wfat_mu_{i}"] = np.asarray(m["wfat_mu"][k])
+        akeys = sorted(m["asymp_coeff"], key=str)
+        out["asymp_keys"] = np.array([str(k) for k in akeys])
+        for i, k in enumerate(akeys):
+            out[f"asymp_coeff_{i}"] = m["asymp_coeff"][k]
         np.savez_compressed(os.path.join(HERE, f"mol_{key}.npz"), **out)
         print(key, keys)
 

@@ -14,7 +14,8 @@ CPU pins of the oracle:
     sinh(w ti) = sqrt(2Ip + k^2)/A and G = (Ip + k^2/2) ti - (A^2/2)(sinh(2 w ti)/(4w) - ti/2);
   * the finite-difference Jacobian of the reference (steps 0.01) agrees with a
     Richardson-extrapolated derivative of the saddle-point momenta to O(h^2);
-  * SPANE equals ADK for linear polarization (F x dF/dt = 0 and k_perp . dF/dt = 0).
+  * SPANE equals ADK for linear polarization (F x dF/dt = 0 and k_perp . dF/dt = 0);
+  * atoms with l > 0 (Ar) are sampled (their prefactor is pinned in test_mol_prefactor.py).
 GPU tests (marked gpu): device sampling and a whole SPA job against the oracle.
 """
 import math
@@ -203,10 +204,15 @@ def test_spane_equals_adk_for_linear_polarization():
     assert n > 100
 
 
-def test_sfa_samplers_reject_l_gt_0():
-    l, _, sp = _kat("SPA", {}, target=get_atom("Ar"))
-    with pytest.raises(RuntimeError):
-        O.gen_electron_batch(l._desc(), get_atom("Ar")._desc(), sp.desc, 200)
+def test_sfa_samplers_accept_l_gt_0():
+    # Ar (l = 1): the Wigner-D / Y_lm prefactor (tests/test_mol_prefactor.py pins it)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ar = get_atom("Ar")
+    for method in ("SPA", "SPANE"):
+        l, _, sp = _kat(method, {}, target=ar)
+        got = O.gen_electron_batch(l._desc(), ar._desc(), sp.desc, 200)
+        assert got is not None and np.isfinite(got).all()
 
 
 # ---- device ----------------------------------------------------------------------

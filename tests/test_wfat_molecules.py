@@ -150,7 +150,7 @@ def test_open_shell_orbital_indices():
     with pytest.raises(ValueError):
         t.ion_potential(0)
     sd = _wfat_desc(t, ridx=(1, -1))
-    assert sd.wfat_form == _abi.WFAT_INTDATA and sd.wfat_ip == pytest.approx(t.ion_potential((1, -1)))
+    assert sd.wfat_form == _abi.WFAT_INTDATA and sd.orbit_ip == pytest.approx(t.ion_potential((1, -1)))
 
 
 REF_DB = "/root/reference/src/Targets/Database"

@@ -11,8 +11,9 @@ Keyword interface of the reference, plus engine keywords:
   seed, sample_stratified, kd_sigma, kz_sigma   seeded sampling modes
   reproduce_quirks reproduce the reference's 3D SCTS / SAE 2D SCTS formulas
 Returns the fields write_output! stores (:516-543) as a dict; when
-output_path is given they are also saved as .npz (JLD2/HDF5 writing is not
-part of the engine).
+output_path is given they are also saved: output_fmt "h5" / "hdf5" writes the
+reference's HDF5 file (etraj_hip.h5out), "npz" a numpy archive of the same
+fields; JLD2 (the reference's default format) is not written.
 """
 from __future__ import annotations
 
@@ -134,7 +135,7 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
                      target=target, dimension=dimension, sample_t_intv=sample_t_intv, sample_t_num=sample_t_num,
                      traj_phase_method=traj_phase_method, traj_t_final=traj_t_final, final_p_max=final_p_max,
                      final_p_num=final_p_num, traj_integrator=traj_integrator, traj_dt=traj_dt, precision=precision)
-    _save(result, output_path)
+    _save(result, output_path, output_fmt, output_compress)
     return result
 
 
@@ -161,24 +162,67 @@ def _result(spec, ion_prob, ion_unc, n_eff, stats, **kw):
     return result
 
 
-def _save(result, output_path):
-    if output_path is not None:
-        path = output_path if output_path.endswith(".npz") else output_path + ".npz"
-        if os.path.exists(path):
-            warnings.warn(f"[TrajectorySimulationJob] File `{path}` already exists, will save at default filename.")
-            path = default_filename() + ".npz"
+def _params_text(params) -> str:
+    """YAML text of the run parameters (write_output!'s params_text, :507-511)."""
+    import yaml
+
+    def plain(v):
+        if isinstance(v, dict):
+            return {str(k): plain(x) for k, x in v.items()}
+        if isinstance(v, (list, tuple)):
+            return [plain(x) for x in v]
+        if isinstance(v, np.ndarray):
+            return plain(v.tolist())
+        if isinstance(v, (np.integer,)):
+            return int(v)
+        if isinstance(v, (np.floating,)):
+            return float(v)
+        if isinstance(v, (bool, int, float, str)) or v is None:
+            return v
+        return str(v)
+    return yaml.safe_dump(plain(params), sort_keys=False, allow_unicode=True)
+
+
+def _save(result, output_path, output_fmt="npz", output_compress=True):
+    """The file handling of TrajectorySimulationJob / write_output! (:195-215,
+    :502-546): a suffix per format, a default file name instead of an existing
+    file or temp file, the data written to `path!` and renamed at the end."""
+    if output_path is None:
+        return
+    fmt = _sym(output_fmt).lower()
+    if fmt in ("h5", "hdf5"):
+        ext = ".h5"
+    elif fmt == "npz":
+        ext = ".npz"
+    elif fmt == "jld2":
+        raise NotImplementedError("[TrajectorySimulationJob] JLD2 output is not written by this engine; "
+                                  "use output_fmt='h5' (the reference's HDF5 format) or 'npz'")
+    else:
+        raise ValueError(f"[TrajectorySimulationJob] Unsupported output format [{output_fmt}].")
+    path = output_path if output_path.endswith(ext) else output_path + ext
+    if os.path.exists(path):
+        warnings.warn(f"[TrajectorySimulationJob] File `{path}` already exists, will save at default filename.")
+        path = default_filename() + ext
+    if os.path.exists(path + "!"):
+        warnings.warn(f"[TrajectorySimulationJob] The temp file `{path}!` exists, will save at default filename.")
+        path = default_filename() + ext
+    if ext == ".h5":
+        from .h5out import write_h5
+        write_h5(path, result, _params_text(result["params"]), compress=bool(output_compress))
+    else:
         tmp = path + "!"
         with open(tmp, "wb") as fh:
             np.savez_compressed(fh, **{k: v for k, v in result.items() if isinstance(v, (np.ndarray, float, int))})
         os.replace(tmp, path)
-        result["output_path"] = path
+    result["output_path"] = path
 
 
 _SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0,
                       mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=1e-6, sample_monte_carlo=False,
                       sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="rk4",
                       traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
-                      kz_sigma=0.0, reproduce_quirks=True, output_path=None)
+                      kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="npz",
+                      output_compress=True)
 
 
 def perform_traj_scan(jobs):
@@ -225,6 +269,6 @@ def perform_traj_scan(jobs):
         st = stats[j]
         r = _result(out[j * npix:(j + 1) * npix].copy(), st.ion_prob, st.ion_prob_uncollected,
                     st.num_effective_traj, st.as_dict(), **kw)
-        _save(r, kw["output_path"])
+        _save(r, kw["output_path"], kw.get("output_fmt", "npz"), kw.get("output_compress", True))
         results.append(r)
     return results

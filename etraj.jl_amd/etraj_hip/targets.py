@@ -329,8 +329,14 @@ def TargetForce(t):
 
 
 def Serialize(t) -> dict:
+    """Targets' Serialize (HydrogenLikeAtom.jl, SAEAtom.jl, GenericMolecule.jl:620-633)."""
     d = {"type": type(t).__name__}
-    d.update(t.__dict__)
+    if isinstance(t, GenericMolecule):
+        d.update(atoms=list(t.data.get("atoms", [])), atom_coords=np.asarray(t.data.get("atom_coords", [])).tolist(),
+                 charge=t.charge, spin=int(t.data.get("spin", 0)), name=t.name, rot_α=t.rot[0], rot_β=t.rot[1],
+                 rot_γ=t.rot[2])
+        return d
+    d.update({k: v for k, v in t.__dict__.items() if not isinstance(v, np.ndarray)})
     return d
 
 

@@ -296,6 +296,8 @@ def main():
                     choices=["h-scts-1e8", "h-ctmc-grid", "ar-spa-3d", "bichromatic-scan", "wfat-orientation-scan"],
                     help="h-scts-1e8 (default) is the headline; the others are the remaining configurations")
     ap.add_argument("--scan-jobs", type=int, default=0, help="orientations of wfat-orientation-scan (default 64)")
+    ap.add_argument("--restep-radius", type=float, default=None,
+                    help="FP64 re-integration radius of the mixed mode (a.u.; default: the workload's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
@@ -309,6 +311,8 @@ def main():
         print(json.dumps(bench_workload(args.workload, args, ctx)), flush=True)
         return
     kw = workloads.h_lp_scts_stratified(n_t=args.n_t, n_k=args.n_k, precision="mixed")
+    if args.restep_radius is not None:
+        kw["restep_radius"] = args.restep_radius
     d_mixed = descs(kw)
     npix = kw["final_p_num"][0] * kw["final_p_num"][1]
 

@@ -680,7 +680,7 @@ int etraj_gen_electron_batch(etraj_ctx* c, const etraj_laser_desc* laser, const 
     HIP_TRY(c, hipMemcpyAsync(c->seg_b.p, &b, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, &start0, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     const SegTab G{c->seg_t0.p, c->seg_job.p, c->seg_b.p, c->seg_start.p};
-    HIP_TRY(c, launch_adk_sample(c->stream, S.method, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p, nullptr));
+    HIP_TRY(c, launch_adk_sample(c->stream, S.method, S.pre_L >= 0, c->sps.p, c->lps.p, T, G, 1, c->init.p, c->counts.p, nullptr));
     int32_t cnt = 0;
     HIP_TRY(c, hipMemcpyAsync(&cnt, c->counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -793,6 +793,8 @@ static int plan_jobs(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* laser
         if (!rc) rc = batch_times_of(c, sd, J.t);
         if (!rc) rc = make_sampler(c, &sd, J.L, T, J.t, J.S);
         if (!rc) rc = make_prefactor(c, sd, *target, J.S, J.preC);
+        if (!rc && j > 0 && (J.S.pre_L >= 0) != (jobs[0].S.pre_L >= 0))
+            rc = fail(c, ETRAJ_EINVAL, "the jobs of a scan must all carry, or all omit, asymp_coeff");
         if (!rc && (sd.dimension != run->dimension || sd.phase_method != run->phase_method))
             rc = fail(c, ETRAJ_EINVAL, "sampler and run disagree on dimension / phase method");
         if (!rc && !(run->traj_t_final > sd.sample_t_hi))
@@ -991,7 +993,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         if (e == hipSuccess && rk4) e = launch_field_table(A, c->lps.p, Q.seg_job.p, Q.seg_t0.p, nb, P.tf, P.dt, FT);
         const SegTab G{Q.seg_t0.p, Q.seg_job.p, Q.seg_b.p, Q.seg_start.p};
         if (e == hipSuccess)
-            e = launch_adk_sample(A, jobs[0].S.method, c->sps.p, c->lps.p, T, G, nb, Q.init.p, Q.counts.p, c->cnt.p);
+            e = launch_adk_sample(A, jobs[0].S.method, jobs[0].S.pre_L >= 0, c->sps.p, c->lps.p, T, G, nb, Q.init.p, Q.counts.p, c->cnt.p);
         if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 1], A);
         if (e == hipSuccess && prev.k >= 0) {
             e = launch_fp64(prev, ev[6 * k + 1]);

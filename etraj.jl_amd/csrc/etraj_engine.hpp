@@ -111,8 +111,9 @@ hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, 
 hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xyz, int64_t n, double* V, double* F);
 // sps / lps: device arrays of per-job sampler and laser parameters
 // cnt: per-job counters (CNT_N per job) receiving CNT_LAUNCHED, or NULL
-hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
-                             const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt);
+// pre: the samplers carry a prefactor table (SamplerP::pre_L >= 0), the same for every job
+hipError_t launch_adk_sample(hipStream_t st, int method, bool pre, const SamplerP* sps, const LaserP* lps,
+                             const TargetP& T, const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt);
 // Doubles of the prefactor table C_lm of a sampler (0 for the scalar form)
 inline size_t pre_doubles(const SamplerP& S) {
     return S.pre_L < 0 ? 0 : (size_t)2 * (S.pre_L + 1) * (2 * S.pre_L + 1);

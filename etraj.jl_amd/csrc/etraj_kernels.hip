@@ -105,6 +105,7 @@ struct AdkBatch {
     double ex, ey;      // F/|F| at tr
     double Fxr, Fyr;    // F at tr
     const double* pT;   // prefactor beyond l = 0: the batch's monomial coefficients (LDS, (re, im))
+    const LaserTr* lt;  // SPA: the laser at tr, tr + 0.01, tr - 0.01 (LDS; complex-time evaluations)
     double dFx, dFy;    // SPANE: dF/dt at tr
     double cross;       // SPANE: Fx dFy - dFx Fy (SPANESampler.jl:238)
     bool skip;          // SPA: F(tr) <= 1e-3 empties the batch (SPASampler.jl:167-169)
@@ -211,7 +212,7 @@ __device__ inline double wfat_g2(const SamplerP& S, double beta, double gamma) {
     return gre * gre + gim * gim;
 }
 
-template <int M>
+template <int M, bool PRE>
 __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, const TargetP& T, double tr) {
     AdkBatch B;
     B.tr = tr;
@@ -238,6 +239,7 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     B.wB = nullptr;
     B.wM = 0;
     B.pT = nullptr;
+    B.lt = nullptr;
     B.ex = Fx / B.F;
     B.ey = Fy / B.F;
     B.Fxr = Fx;
@@ -246,7 +248,7 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
     if (M == ETRAJ_INIT_WFAT) {
         ff_mf_euler(S.mol_rot, Fx, Fy, B.alpha, B.beta, B.gamma);   // WFATSampler.jl:143-145
         if (S.wfat_form == ETRAJ_WFAT_TABLE) B.g2 = wfat_g2(S, B.beta, B.gamma);   // :151-155, channel (0, 0)
-    } else if (S.pre_L >= 0) {
+    } else if (PRE) {
         ff_mf_euler(S.pre_rot, Fx, Fy, B.alpha, B.beta, B.gamma);  // ADKSampler.jl:198-204
     }
     return B;
@@ -340,21 +342,21 @@ __device__ inline Cx pre_eval(const SamplerP& S, const AdkBatch& B, Cx kx, Cx ky
 // with f'(ti) = Im Ax Re Fx + Im Ay Re Fy (dA/dti = -i F). Newton from the
 // reference's initial guess (:240); accepted if |f| <= 1e-8 (NLsolve's ftol)
 // and ti > 0 (:241-246), else 0.
-__device__ inline double spa_solve(const LaserP& L, double tr, double Ftr, double omega, double Ip, double kd,
-                                   double kz) {
+__device__ inline double spa_solve(const LaserP& L, const LaserTr& LT, double tr, double Ftr, double omega, double Ip,
+                                   double kd, double kz) {
     const double k2 = kd * kd + kz * kz;
     double ti = asinh(omega / Ftr * sqrt(k2 + 2 * Ip)) / omega;
     Cx Fx, Fy, Ax, Ay;
 #pragma unroll 1
     for (int it = 0; it < 100; ++it) {
-        laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+        laser_field_pot_tau(L, LT, tr, ti, Fx, Fy, Ax, Ay);
         const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
         const double step = f / (Ax.im * Fx.re + Ay.im * Fy.re);
         if (!isfinite(step)) return 0.0;
         ti -= step;
         if (fabs(step) <= 1e-15 * fabs(ti)) break;
     }
-    laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+    laser_field_pot_tau(L, LT, tr, ti, Fx, Fy, Ax, Ay);
     const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
     return (fabs(f) <= 1e-8 && ti > 0) ? ti : 0.0;
 }
@@ -372,13 +374,14 @@ __device__ inline void spa_p(double kd, Cx Ax, Cx Ay, double& px, double& py) {
 // by 32-point Gauss-Legendre on [0, ti] (S.glq: nodes, weights); the
 // prefactor k(ts).F(ts) (:271-278) and the finite-difference Jacobian
 // (:280-292). Returns false if the solve fails.
+template <bool PRE>
 __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const TargetP& T, const AdkBatch& B, double kd,
                                 double kz, double w, double& rate, double& phase, double col[6]) {
     const double Ip = S.Ip, tr = B.tr;
-    const double ti = spa_solve(L, tr, B.F, S.omega, Ip, kd, kz);
+    const double ti = spa_solve(L, B.lt[0], tr, B.F, S.omega, Ip, kd, kz);
     if (ti == 0.0) return false;
     Cx Fx, Fy, Ax, Ay;
-    laser_field_pot_c(L, Cx{tr, ti}, Fx, Fy, Ax, Ay);
+    laser_field_pot_tau(L, B.lt[0], tr, ti, Fx, Fy, Ax, Ay);
     double px, py;
     spa_p(kd, Ax, Ay, px, py);
     const Cx kdotF = Cx{px + Ax.re, Ax.im} * Fx + Cx{py + Ay.re, Ay.im} * Fy;
@@ -388,7 +391,7 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
 #pragma unroll 1
     for (int i = 0; i < 32; ++i) {
         Cx fx, fy, ax, ay;
-        laser_field_pot_c(L, Cx{tr, h + h * S.glq[2 * i]}, fx, fy, ax, ay);
+        laser_field_pot_tau(L, B.lt[0], tr, h + h * S.glq[2 * i], fx, fy, ax, ay);
         const double wi = S.glq[2 * i + 1];
         x0 += wi * ax.im;
         y0 += wi * ay.im;
@@ -406,7 +409,7 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
     if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
         const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
         const double p = (S.nstar + 1) / 2;
-        if (S.pre_L < 0) {
+        if (!PRE) {
             lmag += log(cpre * S.numer);
         } else {   // k_ts = (px + Ax(ts), py + Ay(ts), kz) (:250)
             const Cx N = pre_eval(S, B, Cx{px + Ax.re, Ax.im}, Cx{py + Ay.re, Ay.im}, Cx{kz, 0.0});
@@ -425,9 +428,10 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
         for (int q = 0; q < 4; ++q) {
             const double trq = tr + (q == 0 ? dtr : q == 1 ? -dtr : 0.0);
             const double kdq = kd + (q == 2 ? dkd : q == 3 ? -dkd : 0.0);
-            const double tq = spa_solve(L, trq, B.F, S.omega, Ip, kdq, kz);
+            const LaserTr& LTq = B.lt[q == 0 ? 1 : q == 1 ? 2 : 0];   // laser at trq
+            const double tq = spa_solve(L, LTq, trq, B.F, S.omega, Ip, kdq, kz);
             Cx f1, f2, a1, b1;
-            laser_field_pot_c(L, Cx{trq, tq}, f1, f2, a1, b1);
+            laser_field_pot_tau(L, LTq, trq, tq, f1, f2, a1, b1);
             spa_p(kdq, a1, b1, pq[q][0], pq[q][1]);
         }
         const double dpxdtr = pq[0][0] - pq[1][0], dpydtr = pq[0][1] - pq[1][1];
@@ -444,7 +448,7 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
 }
 
 // Evaluates sample j of batch b; returns false if it is discarded.
-template <int M>
+template <int M, bool PRE>
 __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const TargetP& T, const AdkBatch& B, int b,
                                   int64_t j, double* col) {
     double kd0, kz0 = 0.0, w = B.dkdt;
@@ -480,7 +484,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
     if (M == ETRAJ_INIT_SPA) {
         if (fabs(kd0) < 1e-4) return false;                           // SPASampler.jl:345
         double v[6];
-        if (!spa_eval(S, L, T, B, kd0, kz0, w, rate, phase, v)) return false;
+        if (!spa_eval<PRE>(S, L, T, B, kd0, kz0, w, rate, phase, v)) return false;
         x0 = v[0]; y0 = v[1]; kx0 = v[2]; ky0 = v[3];
     } else if (M == ETRAJ_INIT_SPANE) {
         if (fabs(kd0) < 1e-4) return false;                           // SPANESampler.jl:323
@@ -497,7 +501,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
             const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
             const double den = pow((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
-            if (S.pre_L < 0) {
+            if (!PRE) {
                 amp *= cpre * S.numer / den;
             } else {   // ti = sqrt((kappa^2+kd^2+kz^2)/F2eff), k_ts of :241-242
                 const double ti = sqrt((2 * Ip + kd0 * kd0 + kz0 * kz0) / f2);
@@ -514,7 +518,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
             aim *= jf;
         }
         rate = amp * amp + aim * aim;
-        if (S.pre_L >= 0) phase = atan2(aim, amp);
+        if (PRE) phase = atan2(aim, amp);
         if (isnan(rate) || rate / w < S.cutoff) return false;        // :337: rate/dkdt (Q7)
     } else if (M == ETRAJ_INIT_WFAT) {
         if (kd0 * kd0 + kz0 * kz0 < 1e-4 * 1e-4) return false;    // WFATSampler.jl:188
@@ -532,7 +536,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
             double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
             const double den = pow((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
-            if (S.pre_L < 0) {
+            if (!PRE) {
                 amp *= cpre * S.numer / den;
             } else {   // k_ts at ts = tr + i ti, ti = sqrt(kappa^2+kd^2+kz^2)/F (:237-238)
                 const double ti = sqrt(2 * Ip + kd0 * kd0 + kz0 * kz0) / B.F;
@@ -547,7 +551,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
             aim *= sqrt(B.F);
         }
         rate = amp * amp + aim * aim;
-        if (S.pre_L >= 0) phase = atan2(aim, amp);                    // angle(amp)
+        if (PRE) phase = atan2(aim, amp);                    // angle(amp)
     }
     if (M == ETRAJ_INIT_ADK || M == ETRAJ_INIT_WFAT) {
         kx0 = kd0 * -B.sphi;                                          // :324-325
@@ -574,7 +578,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
 // wave ballot + LDS scan, so column order equals the reference's loop order.
 // Workgroup g samples batch G.batch[g] of job G.job[g] (per-job parameters
 // in sps / lps) into the columns from G.start[g].
-template <int M>
+template <int M, bool PRE>
 __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__ sps, const LaserP* __restrict__ lps,
                                                     TargetP T, SegTab G, double* __restrict__ out,
                                                     int32_t* __restrict__ counts, int64_t* __restrict__ cnt) {
@@ -585,10 +589,15 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
     const int b = G.batch[g];
     const SamplerP S = sps[job];
     const LaserP& L = lps[job];
-    AdkBatch B = adk_batch_setup<M>(S, L, T, G.t0[g]);
+    AdkBatch B = adk_batch_setup<M, PRE>(S, L, T, G.t0[g]);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double* bout = out + G.start[g] * S.rows;
     if (tid == 0) base_sh = 0;
+    if (M == ETRAJ_INIT_SPA) {   // the laser at the three real times of this batch's complex-time evaluations
+        __shared__ LaserTr spa_lt[3];
+        if (tid < 3) spa_lt[tid] = laser_tr(L, tid == 0 ? B.tr : tid == 1 ? B.tr + 0.01 : B.tr - 0.01);
+        B.lt = spa_lt;
+    }
     if (M == ETRAJ_INIT_WFAT) {
         // B_|m| = sum_nxi sum_{m = +-|m|} |G_{nxi m}|^2 prefactor(nxi, |m|): one
         // channel per thread, then summed in a fixed order (deterministic)
@@ -617,7 +626,7 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
             B.wM = 0;
         }
         B.wB = wfat_B;
-    } else if (S.pre_L >= 0 && (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC))) {
+    } else if (PRE && (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC))) {
         __shared__ double pre_T[2 * kPreMaxMono];
         const int nmono = (S.pre_L + 1) * (S.pre_L + 2) * (S.pre_L + 3) / 6;
         for (int i = tid; i < nmono; i += blockDim.x) pre_monomial(S, B, i, pre_T + 2 * i);
@@ -627,7 +636,7 @@ __global__ __launch_bounds__(256) void k_adk_sample(const SamplerP* __restrict__
     double col[9];
     for (int64_t j0 = 0; j0 < S.nk; j0 += 256) {
         int64_t j = j0 + tid;
-        bool keep = (j < S.nk) && !B.skip && adk_sample<M>(S, L, T, B, b, j, col);
+        bool keep = (j < S.nk) && !B.skip && adk_sample<M, PRE>(S, L, T, B, b, j, col);
         uint64_t m = __ballot(keep);
         int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
         if (lane == 0) wave_tot[wv] = __popcll(m);
@@ -1220,21 +1229,26 @@ hipError_t launch_target_eval(hipStream_t st, const TargetP& T, const double* xy
     return hipGetLastError();
 }
 
-hipError_t launch_adk_sample(hipStream_t st, int method, const SamplerP* sps, const LaserP* lps, const TargetP& T,
-                             const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt) {
+template <int M>
+static void launch_sample_m(hipStream_t st, bool pre, const SamplerP* sps, const LaserP* lps, const TargetP& T,
+                            const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt) {
+    if (pre)
+        hipLaunchKernelGGL((k_adk_sample<M, true>), dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
+    else
+        hipLaunchKernelGGL((k_adk_sample<M, false>), dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
+}
+
+hipError_t launch_adk_sample(hipStream_t st, int method, bool pre, const SamplerP* sps, const LaserP* lps,
+                             const TargetP& T, const SegTab& G, int nb, double* out, int32_t* counts, int64_t* cnt) {
     if (nb <= 0) return hipSuccess;
     switch (method) {
-        case ETRAJ_INIT_SPA:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPA>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
-            break;
-        case ETRAJ_INIT_SPANE:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_SPANE>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
-            break;
+        case ETRAJ_INIT_SPA: launch_sample_m<ETRAJ_INIT_SPA>(st, pre, sps, lps, T, G, nb, out, counts, cnt); break;
+        case ETRAJ_INIT_SPANE: launch_sample_m<ETRAJ_INIT_SPANE>(st, pre, sps, lps, T, G, nb, out, counts, cnt); break;
         case ETRAJ_INIT_WFAT:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_WFAT>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
+            hipLaunchKernelGGL((k_adk_sample<ETRAJ_INIT_WFAT, false>), dim3(nb), dim3(256), 0, st, sps, lps, T, G, out,
+                               counts, cnt);
             break;
-        default:
-            hipLaunchKernelGGL(k_adk_sample<ETRAJ_INIT_ADK>, dim3(nb), dim3(256), 0, st, sps, lps, T, G, out, counts, cnt);
+        default: launch_sample_m<ETRAJ_INIT_ADK>(st, pre, sps, lps, T, G, nb, out, counts, cnt);
     }
     return hipGetLastError();
 }

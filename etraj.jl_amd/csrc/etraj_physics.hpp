@@ -256,6 +256,83 @@ __host__ __device__ inline void mono_field_pot_c(const MonoP& m, Cx t, Cx& Fx, C
     }
 }
 
+// mono_field_pot_c at t = tr + i tau with the parts that depend on tr alone
+// computed once (one batch of the SPA sampler has one tr): sin / cos of the
+// real parts of the carrier and envelope phases and the cut-off window (which
+// takes real(t)). The arithmetic is that of mono_field_pot_c.
+struct MonoTr {
+    double sa, ca;     // sin, cos of w (tr - tshift) + cep
+    double sae, cae;   // sin, cos of w (tr - tshift) / (2N)  (cos^n envelopes)
+    double win;        // cos_window(tr - tshift)
+};
+struct LaserTr {
+    MonoTr m[2];       // m[1] at tr - delay (bichromatic)
+};
+__host__ __device__ inline MonoTr mono_tr(const MonoP& m, double tr) {
+    const double tre = tr - m.tshift;
+    MonoTr r;
+    sincos(m.w * tre + m.cep, &r.sa, &r.ca);
+    r.sae = r.cae = r.win = 0.0;
+    if (m.kind != ETRAJ_LASER_GAUSSIAN) {
+        sincos(m.w * tre / (2 * m.N), &r.sae, &r.cae);
+        r.win = cos_window(m, tre);
+    }
+    return r;
+}
+__host__ __device__ inline LaserTr laser_tr(const LaserP& L, double tr) {
+    LaserTr r;
+    r.m[0] = mono_tr(L.c[0], tr);
+    r.m[1] = L.n == 2 ? mono_tr(L.c[1], tr - L.delay) : r.m[0];
+    return r;
+}
+__host__ __device__ inline void mono_field_pot_tau(const MonoP& m, const MonoTr& r, double trm, double tau, Cx& Fx,
+                                                   Cx& Fy, Cx& Ax, Cx& Ay) {
+    const double wi = m.w * tau;
+    const double chb = cosh(wi), shb = sinh(wi);
+    const Cx sw{r.sa * chb, r.ca * shb}, cw{r.ca * chb, -r.sa * shb};
+    Cx g, gA, a, b;
+    if (m.kind == ETRAJ_LASER_GAUSSIAN) {
+        const Cx t{trm - m.tshift, tau};
+        const Cx t2 = t * t;
+        gA = m.A0 * cexp_(Cx{-t2.re / 2 / m.sigma2, -t2.im / 2 / m.sigma2});
+        g = gA;
+        const Cx ts = t / m.sigma2;
+        a = ts * cw + m.w * sw;
+        b = ts * sw - m.w * cw;
+    } else {
+        const double ei = m.w * tau / (2 * m.N);
+        const double che = cosh(ei), she = sinh(ei);
+        const Cx sh{r.sae * che, r.cae * she}, ch{r.cae * che, -r.sae * she};
+        const Cx c2 = ch * ch;
+        const bool cos2 = m.kind == ETRAJ_LASER_COS2;
+        gA = (m.A0 * r.win) * (cos2 ? c2 : c2 * c2);
+        g = (m.F0 * r.win) * (cos2 ? ch : c2 * ch);
+        a = ch * sw + m.coefN * (sh * cw);
+        b = m.coefN * (sh * sw) - ch * cw;
+    }
+    if (!m.azi_nz) {
+        Fx = g * a;
+        Fy = m.ellip * (g * b);
+        Ax = gA * cw;
+        Ay = m.ellip * (gA * sw);
+    } else {
+        Fx = g * (m.cazi * a + (m.ellip * m.sazi) * b);
+        Fy = g * ((-m.sazi) * a + (m.ellip * m.cazi) * b);
+        Ax = gA * (m.cazi * cw + (m.ellip * m.sazi) * sw);
+        Ay = gA * ((-m.sazi) * cw + (m.ellip * m.cazi) * sw);
+    }
+}
+// laser_field_pot_c(L, tr + i tau) from laser_tr(L, tr)
+__host__ __device__ inline void laser_field_pot_tau(const LaserP& L, const LaserTr& r, double tr, double tau, Cx& Fx,
+                                                    Cx& Fy, Cx& Ax, Cx& Ay) {
+    mono_field_pot_tau(L.c[0], r.m[0], tr, tau, Fx, Fy, Ax, Ay);
+    if (L.n == 2) {
+        Cx fx, fy, ax, ay;
+        mono_field_pot_tau(L.c[1], r.m[1], tr - L.delay, tau, fx, fy, ax, ay);
+        Fx = Fx + fx; Fy = Fy + fy; Ax = Ax + ax; Ay = Ay + ay;
+    }
+}
+
 // Not inlined: the SPA sampler calls it from several loops, and one shared
 // body keeps that kernel's register count within an occupancy of 2+.
 __host__ __device__ inline __attribute__((noinline)) void laser_field_pot_c(const LaserP& L, Cx t, Cx& Fx, Cx& Fy, Cx& Ax,

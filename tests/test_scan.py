@@ -139,3 +139,21 @@ def test_perform_traj_scan_rejects_different_runs():
         perform_traj_scan(kws)
     with pytest.raises(ValueError):
         perform_traj_scan([])
+
+
+@pytest.mark.gpu
+def test_wfat_orientation_scan_of_a_database_molecule(ctx):
+    # every channel of H2's WFAT integrals, four orientations in one launch; each equals its job alone
+    from cases import fixture_molecule
+    rots = [(0.0, 0.0, 0.0), (0.0, math.pi / 2, math.pi / 2), (1.0, 2.0, -0.5), (-2.0, 0.4, 2.5)]
+    kws = [dict(small_wfat(n_t=24, n_k=128), target=fixture_molecule("Hydrogen", r)) for r in rots]
+    _assert_scan_equals_singles(ctx, kws)
+
+
+@pytest.mark.gpu
+def test_mo_adk_intensity_scan(ctx):
+    # CO's asymptotic coefficients (MO-ADK), three intensities in one launch
+    from cases import fixture_molecule
+    t = fixture_molecule("CarbonMonoxide", (0.3, 1.1, -0.7))
+    kws = [dict(kw, target=t, mol_orbit_ridx=0) for kw in _intensity_jobs("mixed")]
+    _assert_scan_equals_singles(ctx, kws)

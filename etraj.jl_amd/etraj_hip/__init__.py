@@ -5,6 +5,8 @@ ElectronSamplers interface.
 Every compute entry point calls libetraj_hip.so (include/etraj.h); there is
 no CPU fallback.
 """
+__version__ = "0.1.0"
+
 from . import _abi
 from ._abi import Context, EngineError, EngineUnavailable, load_library
 from .lasers import (Laser, MonochromaticLaser, Cos2Laser, Cos4Laser, GaussianLaser, BichromaticLaser, LaserFx,

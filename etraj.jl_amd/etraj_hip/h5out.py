@@ -2,6 +2,8 @@
 written through the HDF5 C library (the loader of etraj_hip.moldata).
 
 Datasets, as HDF5.jl writes them for the reference:
+  info                  "Trajectory simulation output generated by ..." (UTF-8 string,
+                        scalar; written when the job opens the file, :239)
   params_text           the YAML text of the run parameters (UTF-8 string, scalar)
   px, py[, pz]          range(-pmax, pmax, length=num) as Float64 vectors
   momentum_spec         the PMD (Float64; |amplitude|^2 for QTMC/SCTS); Julia's
@@ -17,6 +19,7 @@ import os
 
 import numpy as np
 
+from . import __version__
 from .moldata import _g, _hdf5
 
 __all__ = ["write_h5"]
@@ -125,6 +128,7 @@ def write_h5(path, result, params_text, compress=True):
     tmp = path + "!"
     w = _Writer(tmp)
     try:
+        w.string("info", f"Trajectory simulation output generated by etraj_hip @ {__version__}")
         w.string("params_text", params_text)
         w.array("px", result["px"])
         w.array("py", result["py"])

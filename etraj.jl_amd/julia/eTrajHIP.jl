@@ -12,6 +12,9 @@ descriptors of include/etraj.h and the device does the work.
                    and bins it into the thread's `spec_collect` slot.
   * coarse path -- `run_job_hip(; kwargs...)` replaces the whole `@threads` loop
                    and the sum of `write_output!` (:154-157, :521).
+  * whole run   -- `perform_traj_simulation_hip(; kwargs...)` takes the reference's
+                   keywords, builds its TrajectorySimulationJob (validation, the
+                   .jld2 / .h5 file) and writes the same datasets as write_output!.
 
 Usage (in eTraj's module, after `include("TrajectorySimulation.jl")`):
     include("eTrajHIP.jl"); using .eTrajHIP
@@ -20,6 +23,7 @@ module eTrajHIP
 
 using ..Lasers
 using ..Targets
+using ..ElectronSamplers          # gen_electron_batch, batch_num (ElectronSamplers.jl:9)
 
 const libetraj = get(ENV, "ETRAJ_HIP_LIB", joinpath(@__DIR__, "..", "etraj_hip", "libetraj_hip.so"))
 
@@ -154,9 +158,12 @@ init_method(m::Symbol) = m in (:ADK, :MOADK) ? Int32(1) : m in (:SPA, :MOSPA, :S
 # passed as Julia's own Array{ComplexF64,4} memory (ETRAJ_WFAT_INTDATA = 1).
 # ADK / SPA / SPANE of a molecule (MO-ADK, MO-SFA): MolAsympCoeff(mol, ridx), Julia's own
 # Matrix{ComplexF64} memory. Atoms with l > 0 need nothing here (the target descriptor's l, m).
+# sample_monte_carlo = true: sorted random t0 and uniform kd [kz] as ADKSampler.jl:140-147, drawn
+# from Philox4x64-10 keyed by `seed` (the reference's unseeded rand() cannot be reproduced).
 function sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_t_num, ss_kd_max=0.0, ss_kd_num=0,
                       ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC, sample_cutoff_limit=1e-16,
-                      rate_prefix=:Full, target=nothing, mol_orbit_ridx=0, kwargs...)
+                      rate_prefix=:Full, target=nothing, mol_orbit_ridx=0, sample_monte_carlo=false,
+                      mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, seed=0, kwargs...)
     rot = (0.0, 0.0, 0.0); form = Int32(0); dims = (Int32(0), Int32(0), Int32(0), Int32(0))
     mu = (0.0, 0.0, 0.0); ip = 0.0; coeff = C_NULL; keep = nothing; na = Int32(0); nb = Int32(0)
     alm = Int32(0); acoeff = C_NULL
@@ -173,20 +180,21 @@ function sampler_desc(; init_cond_method=:ADK, dimension, sample_t_intv, sample_
         ip = IonPotential(target, mol_orbit_ridx); rot = Float64.(MolRotation(target))
         na, nb = dims[3], dims[4]
     end
-    sd = SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix), 0,
-                     sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1, 0, sample_t_intv[1], sample_t_intv[2],
-                     ss_kd_max, ss_kz_max, 0, 0, 0, 0, sample_cutoff_limit, 0, rot, na, nb, coeff,
-                     form, dims, 0, mu, ip, alm, 0, acoeff)
+    sd = SamplerDesc(init_method(init_cond_method), dimension, PHASE[traj_phase_method], prefix_bits(rate_prefix),
+                     sample_monte_carlo ? 1 : 0, sample_t_num, ss_kd_num, dimension == 3 ? ss_kz_num : 1,
+                     sample_monte_carlo ? mc_kt_num : 0, sample_t_intv[1], sample_t_intv[2],
+                     ss_kd_max, ss_kz_max, mc_kd_max, dimension == 3 ? mc_kz_max : 0.0, 0, 0, sample_cutoff_limit,
+                     UInt64(seed), rot, na, nb, coeff, form, dims, 0, mu, ip, alm, 0, acoeff)
     return sd, keep
 end
 
 function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max, final_p_num,
                      ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC,
                      sample_cutoff_limit=1e-16, rate_prefix=:Full, dt=0.1, precision=:mixed, restep_radius=2.0,
-                     init_cond_method=:ADK, mol_orbit_ridx=0, device=0)
+                     init_cond_method=:ADK, mol_orbit_ridx=0, device=0, kwargs...)
     sdv, keep = sampler_desc(; init_cond_method, dimension, sample_t_intv, sample_t_num, ss_kd_max, ss_kd_num,
                              ss_kz_max, ss_kz_num, traj_phase_method, sample_cutoff_limit, rate_prefix, target,
-                             mol_orbit_ridx)
+                             mol_orbit_ridx, kwargs...)
     sd = Ref(sdv)
     rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
     spec = zeros(Float64, final_p_num...)
@@ -227,6 +235,75 @@ function run_scan_hip(lasers::AbstractVector; target, dimension, sample_t_intv, 
     return [reshape(specs[:, j], final_p_num...) for j in 1:n]
 end
 
-export launch_and_collect_hip!, run_job_hip, run_scan_hip
+# ---- whole-run drop-in for perform_traj_simulation (TrajectorySimulation.jl:103-161) ----
+# The reference's TrajectorySimulationJob validates the keywords, opens `output_path!` as .jld2
+# or .h5 and records `params`, so the file is written by JLD2.jl / HDF5.jl themselves.
+#   coarse = true   etraj_run_job samples, integrates and bins on the device; the result is
+#                   written by write_output_hip! (the dataset set of write_output!, :502-546,
+#                   with the PMD that already holds abs2 for QTMC / SCTS).
+#   coarse = false  the reference sampler makes every batch and launch_and_collect_hip!
+#                   integrates it; the reference's own write_output! writes the file.
+# Keywords and defaults are the reference's (:105-137) plus the device settings.
+function perform_traj_simulation_hip(; init_cond_method::Symbol, laser::Laser, target::Target, dimension::Integer,
+        sample_t_intv, sample_t_num::Integer, traj_t_final, final_p_max, final_p_num,
+        ss_kd_max::Real=0.0, ss_kd_num::Integer=0, ss_kz_max::Real=0.0, ss_kz_num::Integer=0,
+        mc_kt_num::Integer=0, mc_kd_max::Real=0.0, mc_kz_max::Real=0.0,
+        traj_phase_method::Symbol=:CTMC, traj_rtol::Real=1e-6, output_fmt::Symbol=:jld2,
+        output_compress::Bool=true, output_path::String=parentmodule(@__MODULE__).default_filename(),
+        sample_monte_carlo::Bool=false, sample_cutoff_limit::Real=1e-16, rate_prefix=:Full, mol_orbit_ridx=0,
+        show_progress::Bool=false, coarse::Bool=true, dt=0.1, precision=:mixed, restep_radius=2.0, seed=0,
+        device=0)
+    ref = parentmodule(@__MODULE__)
+    job = ref.TrajectorySimulationJob(; init_cond_method, laser, target, dimension, sample_t_intv, sample_t_num,
+        traj_t_final, final_p_max, final_p_num, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num, mc_kd_max,
+        mc_kz_max, traj_phase_method, traj_rtol, output_fmt, output_compress, output_path, sample_monte_carlo,
+        sample_cutoff_limit, rate_prefix, mol_orbit_ridx)
+    if coarse
+        res = run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max,
+                          final_p_num, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, traj_phase_method,
+                          sample_cutoff_limit, rate_prefix, dt, precision, restep_radius, init_cond_method,
+                          mol_orbit_ridx, device, sample_monte_carlo, mc_kt_num, mc_kd_max, mc_kz_max, seed)
+        write_output_hip!(job, res)
+    else
+        Threads.@threads for i in 1:batch_num(job.sp)
+            launch_and_collect_hip!(job, i; dt, precision, restep_radius)
+        end
+        ref.write_output!(job)
+    end
+    @info "Task finished, data saved at `$(job.output_path)`."
+    return job.output_path
+end
+
+# write_output! (:502-546) for a PMD reduced on the device. The keys, the params / params_text
+# encoding and the compression settings are the reference's, so readers such as
+# examples/plot_1cyc_LP.jl:9-15 see the same file.
+function write_output_hip!(job, res)
+    ref = parentmodule(@__MODULE__)
+    file = job.file
+    params = copy(job.params)
+    params[:laser] = Lasers.Serialize(params[:laser])
+    params[:target] = Targets.Serialize(params[:target])
+    file["params_text"] = ref.YAML.write(params)
+    job.output_fmt == :jld2 && (file["params"] = job.params)
+    file["px"] = job.px
+    file["py"] = job.py
+    job.dimension == 3 && (file["pz"] = job.pz)
+    spec = res.momentum_spec                          # Float64, abs2 taken for QTMC / SCTS
+    if job.output_fmt == :jld2
+        write(file, "momentum_spec", spec; compress=job.output_compress)   # JLD2's method of Base.write
+    elseif job.output_compress
+        file["momentum_spec", shuffle=true, deflate=6] = spec
+    else
+        file["momentum_spec"] = spec
+    end
+    file["ion_prob"] = res.ion_prob
+    file["ion_prob_uncollected"] = res.ion_prob_uncollected
+    file["num_effective_traj"] = res.num_effective_traj
+    close(file)
+    mv(job.output_path * "!", job.output_path)
+    return job.output_path
+end
+
+export launch_and_collect_hip!, run_job_hip, run_scan_hip, perform_traj_simulation_hip
 
 end # module

@@ -52,6 +52,7 @@ def test_h5_file_has_the_reference_datasets(tmp_path, dim, compress):
         assert f.get("num_effective_traj") == 12345
         text = f.get("params_text")
         assert "init_cond_method: ADK" in text and "Cos2Laser" in text
+        assert f.get("info").startswith("Trajectory simulation output generated by")
     finally:
         f.close()
     if H5DUMP:

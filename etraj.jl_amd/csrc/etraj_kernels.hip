@@ -11,53 +11,14 @@
 //                     RK4 in FP64 or FP32+compensation with FP64 re-steps near
 //                     the core; FP64 Kepler asymptote; FP64 atomics into the PMD.
 //   k_finalize        |amplitude|^2 for coherent sums.
+// The adaptive Tsit5 kernel (k_tsit5) is in etraj_tsit5.hip.
 // The work is VALU-bound ODE arithmetic (no dense contraction): no MFMA.
 #include "etraj_engine.hpp"
+#include "etraj_device.hpp"
 
 #include <hip/hip_runtime.h>
 
 namespace etraj {
-
-// ---------------------------------------------------------------------------
-// lane helpers
-// ---------------------------------------------------------------------------
-__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
-
-__device__ inline double readlane_f64(double v, int lane) {
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline float readlane_f32(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-__device__ inline double uniform_f64(double v) {
-    int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
-    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
-    return __hiloint2double(hi, lo);
-}
-// A wave-uniform value moved into a vector register (v_mov_b32 from an SGPR).
-__device__ inline float sgpr_to_vgpr(float x) {
-    float v;
-    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
-    return v;
-}
-
-__device__ inline int64_t uniform_i64(int64_t v) {
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ inline double wave_sum_f64(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline int64_t wave_sum_i64(int64_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // field / target evaluation (LaserFx/Fy/Ax/Ay, TargetPotential/TargetForce)
@@ -1017,89 +978,6 @@ __global__ __launch_bounds__(256) void k_compact_flagged(double* __restrict__ in
     if (tid == 0) count_out[seg] = base_sh;
 }
 
-// Per-lane output of one finished electron for kernels whose lanes finish at
-// different times (no wave-wide operations inside). Accumulates the scalar
-// counters into `acc` for one reduction per wave at the end.
-struct LaneAcc {
-    double ip = 0.0, iu = 0.0;
-    int64_t failed = 0, bound = 0, steps = 0;
-};
-
-template <int DIM, int PHASE, int OUT>
-__device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx, bool failed, const double* r,
-                                 const double* p, double phase, double t0, double prob, LaneAcc& acc) {
-    Final f = kepler_final<DIM, PHASE>(P.T, P.quirks, r, p, phase, t0);
-    if (failed) f.ionized = false;
-    int64_t lin = -1;
-    bool inb = false;
-    double cre = 0.0, cim = 0.0;
-    if (f.ionized) {
-        int64_t idx[3] = {1, 1, 1};
-        inb = true;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            idx[d] = bin_index(f.pinf[d], P.pmax[d], P.pnum[d]);
-            if (idx[d] < 1 || idx[d] > P.pnum[d]) inb = false;
-        }
-        if (inb) {
-            lin = (idx[0] - 1) + (int64_t)P.pnum[0] * ((idx[1] - 1) + (DIM == 3 ? (int64_t)P.pnum[1] * (idx[2] - 1) : 0));
-            if (PHASE == ETRAJ_PHASE_CTMC) {
-                cre = prob;
-            } else {
-                double sq = sqrt(prob), sp, cp;
-                sincos(f.phase, &sp, &cp);
-                cre = sq * cp;
-                cim = sq * sp;
-            }
-        }
-    }
-    if (failed) acc.failed++;
-    else if (!f.ionized) acc.bound++;
-    else {
-        acc.ip += prob;
-        if (!inb) acc.iu += prob;
-    }
-    if (OUT == OUT_PMD) {
-        if (inb) {
-            if (PHASE == ETRAJ_PHASE_CTMC) {
-                unsafeAtomicAdd(O.pmd + lin, cre);
-            } else {
-                unsafeAtomicAdd(O.pmd + 2 * lin, cre);
-                unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
-            }
-        }
-    } else if (OUT == OUT_RECORDS) {
-        double* rec = O.records + 12 * col_idx;
-        rec[0] = r[0]; rec[1] = r[1]; rec[2] = DIM == 3 ? r[2] : 0.0;
-        rec[3] = p[0]; rec[4] = p[1]; rec[5] = DIM == 3 ? p[2] : 0.0;
-        rec[6] = phase; rec[7] = f.E;
-        rec[8] = f.pinf[0]; rec[9] = f.pinf[1]; rec[10] = f.pinf[2];
-        rec[11] = f.phase;
-        O.status[col_idx] = failed ? 1 : (f.ionized ? 0 : 2);
-    } else {
-        Contrib c;
-        c.lin = lin;
-        c.re = cre;
-        c.im = cim;
-        c.prob = prob;
-        c.status = failed ? 1 : (f.ionized ? (inb ? 0 : 3) : 2);
-        c.pad = 0;
-        O.contrib[col_idx] = c;
-    }
-}
-
-__device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
-    const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
-    const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
-    if (lane_id() == 0) {
-        if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
-        if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
-        if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
-        if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
-        if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
-    }
-}
-
 // Exclusive prefix sum of the segment counts (one workgroup):
 // pre[k] = sum_{j<k} count[j], pre[n] = total.
 __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* __restrict__ count, int64_t n,
@@ -1124,84 +1002,6 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* __restrict_
         if (threadIdx.x == 0) carry += part[1023];
         __syncthreads();
     }
-}
-
-// item (0-based over all electrons of all segments) -> column index
-__device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t n_seg, const int64_t* seg_start) {
-    int64_t lo = 0, hi = n_seg;  // pre[lo] <= item < pre[hi]
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= item) lo = mid; else hi = mid;
-    }
-    return seg_start[lo] + (item - pre[lo]);
-}
-
-// Adaptive Tsit5, one electron per lane. A wavefront owns `per_wave`
-// consecutive items; a lane whose electron has finished takes the next item
-// of its wavefront (ballot + mbcnt), so short and long trajectories share
-// the wave without idling until the range is exhausted.
-template <int DIM, int PHASE, int OUT, int TGT>
-__global__ __launch_bounds__(256) void k_tsit5(IntegP P, OutP O, const double* __restrict__ init,
-                                               const int64_t* __restrict__ seg_start, const int64_t* __restrict__ pre,
-                                               int64_t n_seg, int per_wave) {
-    const int lane = lane_id();
-    const int64_t total = pre[n_seg];
-    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int64_t wbeg = gw * per_wave;
-    if (wbeg >= total) return;
-    const int64_t wend = (wbeg + per_wave < total) ? wbeg + per_wave : total;
-    const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
-    const double atol = P.atol;
-    LaneAcc acc;
-    Ts5Lane<DIM, PHASE> st;
-    int64_t item = wbeg + lane, next = wbeg + 64, col = 0;
-    bool valid = item < wend;
-    double t0 = 0.0, prob = 0.0;
-    auto start = [&]() {
-        col = item_column(item, pre, n_seg, seg_start);
-        const double* c = init + col * P.rows;
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            st.u.v[i] = c[i];
-            st.u.v[DIM + i] = c[DIM + i];
-        }
-        t0 = c[2 * DIM];
-        prob = c[2 * DIM + 1];
-        if (PHASE != ETRAJ_PHASE_CTMC) st.u.v[2 * DIM] = c[2 * DIM + 2];
-        ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
-    };
-    if (valid) start();
-    while (__ballot(valid) != 0) {
-        bool done = false;
-        if (valid) {
-            const int rc = ts5_attempt<DIM, PHASE, TGT>(P.L, P.T, saeq, P.rtol, atol, P.maxiters, st);
-            if (rc != 0) {
-                done = true;
-                bool failed = rc < 0;
-                double r[DIM], p[DIM];
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) {
-                    r[i] = st.u.v[i];
-                    p[i] = st.u.v[DIM + i];
-                    failed = failed || isnan(r[i]) || isnan(p[i]);
-                }
-                const double ph = (PHASE != ETRAJ_PHASE_CTMC) ? st.u.v[2 * DIM] : 0.0;
-                acc.steps += st.accepted;
-                emit_lane<DIM, PHASE, OUT>(P, O, col, failed, r, p, ph, t0, prob, acc);
-            }
-        }
-        const uint64_t m = __ballot(done);
-        if (m) {
-            const int pre_l = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            if (done) {
-                item = next + pre_l;
-                valid = item < wend;
-                if (valid) start();
-            }
-            next += __popcll(m);
-        }
-    }
-    if (OUT == OUT_PMD) flush_lane_acc(O, acc);
 }
 
 __global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, double* __restrict__ out) {
@@ -1315,49 +1115,6 @@ hipError_t launch_compact_flagged(hipStream_t st, double* init, int rows, const 
     hipLaunchKernelGGL(k_compact_flagged, dim3((unsigned)n_seg), dim3(256), 0, st, init, rows, seg_start, seg_count,
                        flags, out_index, count_out, orig);
     return hipGetLastError();
-}
-
-template <int DIM, int PHASE, int OUT>
-static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                                 const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
-                                 int per_wave) {
-    const int64_t waves = (max_items + per_wave - 1) / per_wave;
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    if (blocks == 0) return hipSuccess;
-    if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st, P, O,
-                           init, seg_start, pre, n_seg, per_wave);
-    else
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O, init,
-                           seg_start, pre, n_seg, per_wave);
-    return hipGetLastError();
-}
-
-template <int DIM, int PHASE>
-static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
-    switch (out) {
-        case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD>(st, P, O, init, ss, pre, ns, mi, pw);
-        case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS>(st, P, O, init, ss, pre, ns, mi, pw);
-        default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB>(st, P, O, init, ss, pre, ns, mi, pw);
-    }
-}
-
-template <int DIM>
-static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
-    switch (P.phase) {
-        case ETRAJ_PHASE_CTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
-        case ETRAJ_PHASE_QTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
-        default: return launch_tsit5_o<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, pre, ns, mi, pw);
-    }
-}
-
-hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave) {
-    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    return P.dim == 3 ? launch_tsit5_p<3>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave)
-                      : launch_tsit5_p<2>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave);
 }
 
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre) {

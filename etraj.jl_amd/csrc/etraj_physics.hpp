@@ -695,147 +695,234 @@ constexpr double b1 = -0.00178001105222577714, b2 = -0.0008164344596567469, b3 =
                  b7 = 0.015151515151515152;
 }  // namespace ts5
 
+// Per-electron Tsit5 state, advanced one attempted step at a time so that a
+// wavefront can refill lanes whose electron has finished. The EOM is second
+// order (dr/dt = p, and no derivative depends on the phase), so the stages are
+// kept in the Runge-Kutta-Nystrom arrangement of the same tableau: only the p
+// and phase parts of k_j are stored; the r part of k_j is the stage's own p,
+// whose weighted sums fold into C_s p + dt sum_l (A^2)_sl kp_l. The result is
+// the Tsit5 step up to rounding, with about half the live stage registers.
 template <int DIM, int PHASE>
-struct Vec {
-    static constexpr int N = 2 * DIM + (PHASE != ETRAJ_PHASE_CTMC ? 1 : 0);
-    double v[N];
+struct Ts5Lane {
+    double r[DIM], p[DIM], ph;
+    double k1p[DIM], k1ph;
+    double t, dt, qold, q11;
+    int32_t accepted, attempted;
 };
 
 template <int DIM, int PHASE, int TGT>
-__host__ __device__ inline void rhs_f(const TargetP& T, bool saeq, double Fx, double Fy, const Vec<DIM, PHASE>& u,
-                                      Vec<DIM, PHASE>& du) {
+__host__ __device__ __forceinline__ void ts5_deriv(const TargetP& T, bool saeq, double Fx, double Fy,
+                                                   const double* yr, const double* yp, double* kp, double& kph) {
     State<DIM, double> s, ds;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        s.r[i] = u.v[i];
-        s.p[i] = u.v[DIM + i];
+        s.r[i] = yr[i];
+        s.p[i] = yp[i];
     }
     double dphi, r2;
     eom<DIM, PHASE, TGT, double>(T, saeq, s, Fx, Fy, ds, dphi, r2);
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-        du.v[i] = ds.r[i];
-        du.v[DIM + i] = ds.p[i];
-    }
-    if (PHASE != ETRAJ_PHASE_CTMC) du.v[2 * DIM] = dphi;
+    for (int i = 0; i < DIM; ++i) kp[i] = ds.p[i];
+    kph = dphi;
 }
 
 template <int DIM, int PHASE, int TGT>
-__host__ __device__ inline void rhs_full(const LaserP& L, const TargetP& T, bool saeq, double t,
-                                         const Vec<DIM, PHASE>& u, Vec<DIM, PHASE>& du) {
-    double Fx, Fy;
-    laser_field(L, t, Fx, Fy);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx, Fy, u, du);
-}
-
-template <int N>
-__host__ __device__ inline double rms(const double* x) {
-    double s = 0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) s += x[i] * x[i];
-    return sqrt(s / N);
-}
-
-// Per-electron Tsit5 integration state, advanced one attempted step at a
-// time so that a wavefront can refill lanes whose electron has finished.
-template <int DIM, int PHASE>
-struct Ts5Lane {
-    Vec<DIM, PHASE> u, k1;
-    double t, tf, dt, dtmax, dtmin, qold, q11;
-    int64_t accepted, attempted;
-};
-
-template <int DIM, int PHASE, int TGT>
-__host__ __device__ inline void ts5_init(const LaserP& L, const TargetP& T, bool saeq, double t0, double tf, double rtol,
-                                         double atol, Ts5Lane<DIM, PHASE>& s) {
-    using V = Vec<DIM, PHASE>;
-    constexpr int N = V::N;
+__host__ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool saeq, double t0, double tf,
+                                                  double rtol, double atol, Ts5Lane<DIM, PHASE>& s) {
+    constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
+    constexpr int N = 2 * DIM + (PH ? 1 : 0);
     s.t = t0;
-    s.tf = tf;
-    s.dtmax = tf - t0;
-    s.dtmin = fmax(fabs(t0), fabs(tf)) * 0x1.0p-52;
     s.qold = 1e-4;
     s.q11 = 1.0;
     s.accepted = 0;
     s.attempted = 0;
-    // initial step (Hairer & Wanner, Solving ODEs I, sec. II.4)
-    double sk[N], tmp[N];
+    const double dtmax = tf - t0;
     const double dtmin0 = fmax(0.0, fabs(t0) * 0x1.0p-52);
     const double smalldt = fmax(dtmin0, 1e-6);
+    double u[N], sk[N], d0s = 0, d1s = 0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        u[i] = s.r[i];
+        u[DIM + i] = s.p[i];
+    }
+    if (PH) u[N - 1] = s.ph;
+    double Fx, Fy;
+    laser_field(L, t0, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, s.r, s.p, s.k1p, s.k1ph);
+    double f0[N];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        f0[i] = s.p[i];
+        f0[DIM + i] = s.k1p[i];
+    }
+    if (PH) f0[N - 1] = s.k1ph;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        sk[i] = atol + fabs(s.u.v[i]) * rtol;
-        tmp[i] = s.u.v[i] / sk[i];
+        sk[i] = atol + fabs(u[i]) * rtol;
+        const double a = u[i] / sk[i], b = f0[i] / sk[i];
+        d0s += a * a;
+        d1s += b * b;
     }
-    const double d0 = rms<N>(tmp);
-    rhs_full<DIM, PHASE, TGT>(L, T, saeq, t0, s.u, s.k1);
-#pragma unroll
-    for (int i = 0; i < N; ++i) tmp[i] = s.k1.v[i] / sk[i];
-    const double d1 = rms<N>(tmp);
+    const double d0 = sqrt(d0s / N), d1 = sqrt(d1s / N);
     double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
-    dt0 = fmin(dt0, s.dtmax);
-    V y, k2;
+    dt0 = fmin(dt0, dtmax);
+    double yr[DIM], yp[DIM], k2p[DIM], k2ph;
 #pragma unroll
-    for (int i = 0; i < N; ++i) y.v[i] = s.u.v[i] + dt0 * s.k1.v[i];
-    rhs_full<DIM, PHASE, TGT>(L, T, saeq, t0 + dt0, y, k2);
+    for (int i = 0; i < DIM; ++i) {
+        yr[i] = u[i] + dt0 * f0[i];
+        yp[i] = u[DIM + i] + dt0 * f0[DIM + i];
+    }
+    laser_field(L, t0 + dt0, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, k2p, k2ph);
+    double d2s = 0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) tmp[i] = (k2.v[i] - s.k1.v[i]) / sk[i];
-    const double d2 = rms<N>(tmp) / dt0;
+    for (int i = 0; i < N; ++i) {
+        const double f1 = i < DIM ? yp[i] : (i < 2 * DIM ? k2p[i - DIM] : k2ph);
+        const double a = (f1 - f0[i]) / sk[i];
+        d2s += a * a;
+    }
+    const double d2 = sqrt(d2s / N) / dt0;
     const double mx = fmax(d1, d2);
     const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(10.0, -(2 + log10(mx)) / 5.0);
-    s.dt = fmax(dtmin0, fmin(fmin(100 * dt0, dt1), s.dtmax));
+    s.dt = fmax(dtmin0, fmin(fmin(100 * dt0, dt1), dtmax));
 }
 
-// One attempted step. Returns 0 while running, 1 when t reached tf, -1 on
-// failure (maxiters, NaN, dt below dtmin).
+namespace ts5n {
+// Tsit5 tableau as arrays (row s = stage s+1, 0-based), and the products the
+// second-order (r' = p) form needs: r-stage sums use sum_j a_sj yp_j with
+// yp_j = p + dt sum_l a_jl kp_l, i.e. C_s p + dt sum_l A2_sl kp_l.
+struct Tab {
+    double a[7][7], c[7], A2[7][7], b[7], BA[7], B6;
+};
+constexpr Tab make() {
+    Tab t{};
+    using namespace ts5;
+    const double A[7][7] = {{0, 0, 0, 0, 0, 0, 0},
+                            {a21, 0, 0, 0, 0, 0, 0},
+                            {a31, a32, 0, 0, 0, 0, 0},
+                            {a41, a42, a43, 0, 0, 0, 0},
+                            {a51, a52, a53, a54, 0, 0, 0},
+                            {a61, a62, a63, a64, a65, 0, 0},
+                            {a71, a72, a73, a74, a75, a76, 0}};
+    const double B[7] = {b1, b2, b3, b4, b5, b6, b7};
+    for (int s = 0; s < 7; ++s) {
+        double cs = 0;
+        for (int j = 0; j < 7; ++j) {
+            t.a[s][j] = A[s][j];
+            cs += A[s][j];
+        }
+        t.c[s] = cs;
+        for (int l = 0; l < 7; ++l) {
+            double v = 0;
+            for (int j = l + 1; j < s; ++j) v += A[s][j] * A[j][l];
+            t.A2[s][l] = v;
+        }
+        t.b[s] = B[s];
+    }
+    t.B6 = 0;
+    for (int j = 0; j < 6; ++j) t.B6 += B[j];
+    for (int l = 0; l < 7; ++l) {
+        double v = 0;
+        for (int j = l + 1; j < 6; ++j) v += B[j] * A[j][l];
+        t.BA[l] = v;
+    }
+    return t;
+}
+constexpr Tab tab = make();
+}  // namespace ts5n
+
+// Inputs of stage S (1-based, 2..7): yp = p + dt sum_{l<S} a_Sl kp_l,
+// yr = r + dt (C_S p + dt sum_{l<S-1} A2_Sl kp_l).
+template <int S, int DIM>
+__host__ __device__ __forceinline__ void ts5_inputs(const double* r, const double* p, const double (*kp)[DIM],
+                                                    double dt, double* yr, double* yp) {
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        double sp = ts5n::tab.a[S - 1][0] * kp[0][i];
+#pragma unroll
+        for (int l = 1; l < S - 1; ++l) sp += ts5n::tab.a[S - 1][l] * kp[l][i];
+        yp[i] = p[i] + dt * sp;
+        double sr = 0;
+        if (S > 2) {
+            sr = ts5n::tab.A2[S - 1][0] * kp[0][i];
+#pragma unroll
+            for (int l = 1; l < S - 2; ++l) sr += ts5n::tab.A2[S - 1][l] * kp[l][i];
+        }
+        yr[i] = r[i] + dt * (ts5n::tab.c[S - 1] * p[i] + dt * sr);
+    }
+}
+
 template <int DIM, int PHASE, int TGT>
-__host__ __device__ inline int ts5_attempt(const LaserP& L, const TargetP& T, bool saeq, double rtol, double atol,
-                                           int64_t maxiters, Ts5Lane<DIM, PHASE>& s) {
-    using V = Vec<DIM, PHASE>;
-    constexpr int N = V::N;
+__host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bool saeq, double t0,
+                                                    double tf, double rtol, double atol, int64_t maxiters,
+                                                    Ts5Lane<DIM, PHASE>& s) {
+    constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
+    constexpr int N = 2 * DIM + (PH ? 1 : 0);
     double dt = s.dt;
     const double t = s.t;
-    if (s.tf - t < dt) dt = s.tf - t;
+    if (tf - t < dt) dt = tf - t;
     if (++s.attempted > maxiters) return -1;
-    // the five distinct stage times t + c_i dt (c6 = c7 = 1): one field loop
-    double Fx[5], Fy[5];
-    const double cs[5] = {ts5::c2, ts5::c3, ts5::c4, ts5::c5, 1.0};
-#pragma unroll 1
-    for (int j = 0; j < 5; ++j) laser_field(L, t + cs[j] * dt, Fx[j], Fy[j]);
-    V y, k2, k3, k4, k5, k6, k7, un;
-    const V& u = s.u;
-    const V& k1 = s.k1;
+    double kp[6][DIM], kh[6];
 #pragma unroll
-    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a21 * k1.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[0], Fy[0], y, k2);
+    for (int i = 0; i < DIM; ++i) kp[0][i] = s.k1p[i];
+    kh[0] = s.k1ph;
+    double yr[DIM], yp[DIM];
+    ts5_inputs<2, DIM>(s.r, s.p, kp, dt, yr, yp);
+    double Fx, Fy;
+    laser_field(L, t + ts5::c2 * dt, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[1], kh[1]);
+    ts5_inputs<3, DIM>(s.r, s.p, kp, dt, yr, yp);
+    laser_field(L, t + ts5::c3 * dt, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[2], kh[2]);
+    ts5_inputs<4, DIM>(s.r, s.p, kp, dt, yr, yp);
+    laser_field(L, t + ts5::c4 * dt, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[3], kh[3]);
+    ts5_inputs<5, DIM>(s.r, s.p, kp, dt, yr, yp);
+    laser_field(L, t + ts5::c5 * dt, Fx, Fy);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[4], kh[4]);
+    ts5_inputs<6, DIM>(s.r, s.p, kp, dt, yr, yp);
+    double F1x, F1y;                     // stages 6 and 7 share t + dt (c6 = c7 = 1)
+    laser_field(L, t + dt, F1x, F1y);
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, F1x, F1y, yr, yp, kp[5], kh[5]);
+    double unr[DIM], unp[DIM], er[DIM], ep[DIM], unh = 0, eh = 0;
+    ts5_inputs<7, DIM>(s.r, s.p, kp, dt, unr, unp);
 #pragma unroll
-    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a31 * k1.v[i] + ts5::a32 * k2.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[1], Fy[1], y, k3);
+    for (int i = 0; i < DIM; ++i) {
+        double sr = ts5n::tab.BA[0] * kp[0][i], sp = ts5n::tab.b[0] * kp[0][i];
 #pragma unroll
-    for (int i = 0; i < N; ++i) y.v[i] = u.v[i] + dt * (ts5::a41 * k1.v[i] + ts5::a42 * k2.v[i] + ts5::a43 * k3.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[2], Fy[2], y, k4);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-        y.v[i] = u.v[i] + dt * (ts5::a51 * k1.v[i] + ts5::a52 * k2.v[i] + ts5::a53 * k3.v[i] + ts5::a54 * k4.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[3], Fy[3], y, k5);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-        y.v[i] = u.v[i] + dt * (ts5::a61 * k1.v[i] + ts5::a62 * k2.v[i] + ts5::a63 * k3.v[i] + ts5::a64 * k4.v[i] +
-                                ts5::a65 * k5.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[4], Fy[4], y, k6);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-        un.v[i] = u.v[i] + dt * (ts5::a71 * k1.v[i] + ts5::a72 * k2.v[i] + ts5::a73 * k3.v[i] + ts5::a74 * k4.v[i] +
-                                 ts5::a75 * k5.v[i] + ts5::a76 * k6.v[i]);
-    rhs_f<DIM, PHASE, TGT>(T, saeq, Fx[4], Fy[4], un, k7);
-    double tmp[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double ut = dt * (ts5::b1 * k1.v[i] + ts5::b2 * k2.v[i] + ts5::b3 * k3.v[i] + ts5::b4 * k4.v[i] +
-                                ts5::b5 * k5.v[i] + ts5::b6 * k6.v[i] + ts5::b7 * k7.v[i]);
-        tmp[i] = ut / (atol + fmax(fabs(u.v[i]), fabs(un.v[i])) * rtol);
+        for (int l = 1; l < 6; ++l) {
+            if (l < 5) sr += ts5n::tab.BA[l] * kp[l][i];
+            sp += ts5n::tab.b[l] * kp[l][i];
+        }
+        er[i] = ts5n::tab.B6 * s.p[i] + dt * sr;
+        ep[i] = sp;
     }
-    const double EEst = rms<N>(tmp);
+    if (PH) {
+        double sa = ts5n::tab.a[6][0] * kh[0], sb = ts5n::tab.b[0] * kh[0];
+#pragma unroll
+        for (int l = 1; l < 6; ++l) {
+            sa += ts5n::tab.a[6][l] * kh[l];
+            sb += ts5n::tab.b[l] * kh[l];
+        }
+        unh = s.ph + dt * sa;
+        eh = sb;
+    }
+    double k7p[DIM], k7h;
+    ts5_deriv<DIM, PHASE, TGT>(T, saeq, F1x, F1y, unr, unp, k7p, k7h);
+    const double b7 = ts5::b7;
+    double ss = 0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        const double a = dt * (er[i] + b7 * unp[i]) / (atol + fmax(fabs(s.r[i]), fabs(unr[i])) * rtol);
+        const double b = dt * (ep[i] + b7 * k7p[i]) / (atol + fmax(fabs(s.p[i]), fabs(unp[i])) * rtol);
+        ss += a * a;
+        ss += b * b;
+    }
+    if (PH) {
+        const double a = dt * (eh + b7 * k7h) / (atol + fmax(fabs(s.ph), fabs(unh)) * rtol);
+        ss += a * a;
+    }
+    const double EEst = sqrt(ss / N);
     if (isnan(EEst)) return -1;
     double q;
     if (EEst == 0) {
@@ -845,18 +932,25 @@ __host__ __device__ inline int ts5_attempt(const LaserP& L, const TargetP& T, bo
         q = s.q11 / pow(s.qold, 2.0 / 25.0);
         q = fmax(1.0 / 10.0, fmin(1.0 / 0.2, q / 0.9));
     }
+    const double dtmax = tf - t0;
     if (EEst <= 1.0) {
-        s.t = (s.tf - t == dt) ? s.tf : t + dt;
-        s.u = un;
-        s.k1 = k7;
+        s.t = (tf - t == dt) ? tf : t + dt;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            s.r[i] = unr[i];
+            s.p[i] = unp[i];
+            s.k1p[i] = k7p[i];
+        }
+        s.ph = unh;
+        s.k1ph = k7h;
         ++s.accepted;
         s.qold = fmax(EEst, 1e-4);
-        s.dt = fmin(dt / q, s.dtmax);
+        s.dt = fmin(dt / q, dtmax);
     } else {
         s.dt = dt / fmin(1.0 / 0.2, s.q11 / 0.9);
     }
-    if (!(s.t < s.tf)) return 1;
-    if (fabs(s.dt) < s.dtmin) return -1;
+    if (!(s.t < tf)) return 1;
+    if (fabs(s.dt) < fmax(fabs(t0), fabs(tf)) * 0x1.0p-52) return -1;
     return 0;
 }
 

@@ -1,0 +1,218 @@
+// etraj_tsit5.hip -- the adaptive Tsit5 integrator kernel (the reference's
+// solve(EnsembleProblem, Tsit5(), adaptive = true, reltol = traj_rtol),
+// TrajectorySimulation.jl:332-342) and its launchers.
+//
+// A file of its own because it is built with machine-level loop-invariant code
+// motion off (-mllvm -disable-machine-licm): hoisting the FP64 constants of the
+// laser's sin/cos/tanh and of pow out of the step loop pins them in about 80
+// VGPRs for the whole kernel. Rematerialised inside the loop they cost scalar
+// moves only, and the kernel needs 170 VGPRs instead of 251 (three waves per
+// SIMD instead of two).
+#include "etraj_engine.hpp"
+#include "etraj_device.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace etraj {
+
+// Per-lane output of one finished electron for kernels whose lanes finish at
+// different times (no wave-wide operations inside). Accumulates the scalar
+// counters into `acc` for one reduction per wave at the end.
+struct LaneAcc {
+    double ip = 0.0, iu = 0.0;
+    int64_t failed = 0, bound = 0, steps = 0;
+};
+
+template <int DIM, int PHASE, int OUT>
+__device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx, bool failed, const double* r,
+                                 const double* p, double phase, double t0, double prob, LaneAcc& acc) {
+    Final f = kepler_final<DIM, PHASE>(P.T, P.quirks, r, p, phase, t0);
+    if (failed) f.ionized = false;
+    int64_t lin = -1;
+    bool inb = false;
+    double cre = 0.0, cim = 0.0;
+    if (f.ionized) {
+        int64_t idx[3] = {1, 1, 1};
+        inb = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            idx[d] = bin_index(f.pinf[d], P.pmax[d], P.pnum[d]);
+            if (idx[d] < 1 || idx[d] > P.pnum[d]) inb = false;
+        }
+        if (inb) {
+            lin = (idx[0] - 1) + (int64_t)P.pnum[0] * ((idx[1] - 1) + (DIM == 3 ? (int64_t)P.pnum[1] * (idx[2] - 1) : 0));
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                cre = prob;
+            } else {
+                double sq = sqrt(prob), sp, cp;
+                sincos(f.phase, &sp, &cp);
+                cre = sq * cp;
+                cim = sq * sp;
+            }
+        }
+    }
+    if (failed) acc.failed++;
+    else if (!f.ionized) acc.bound++;
+    else {
+        acc.ip += prob;
+        if (!inb) acc.iu += prob;
+    }
+    if (OUT == OUT_PMD) {
+        if (inb) {
+            if (PHASE == ETRAJ_PHASE_CTMC) {
+                unsafeAtomicAdd(O.pmd + lin, cre);
+            } else {
+                unsafeAtomicAdd(O.pmd + 2 * lin, cre);
+                unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
+            }
+        }
+    } else if (OUT == OUT_RECORDS) {
+        double* rec = O.records + 12 * col_idx;
+        rec[0] = r[0]; rec[1] = r[1]; rec[2] = DIM == 3 ? r[2] : 0.0;
+        rec[3] = p[0]; rec[4] = p[1]; rec[5] = DIM == 3 ? p[2] : 0.0;
+        rec[6] = phase; rec[7] = f.E;
+        rec[8] = f.pinf[0]; rec[9] = f.pinf[1]; rec[10] = f.pinf[2];
+        rec[11] = f.phase;
+        O.status[col_idx] = failed ? 1 : (f.ionized ? 0 : 2);
+    } else {
+        Contrib c;
+        c.lin = lin;
+        c.re = cre;
+        c.im = cim;
+        c.prob = prob;
+        c.status = failed ? 1 : (f.ionized ? (inb ? 0 : 3) : 2);
+        c.pad = 0;
+        O.contrib[col_idx] = c;
+    }
+}
+
+__device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
+    const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
+    const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
+    if (lane_id() == 0) {
+        if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
+        if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+        if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
+        if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
+        if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
+    }
+}
+
+// item (0-based over all electrons of all segments) -> column index
+__device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t n_seg, const int64_t* seg_start) {
+    int64_t lo = 0, hi = n_seg;  // pre[lo] <= item < pre[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= item) lo = mid; else hi = mid;
+    }
+    return seg_start[lo] + (item - pre[lo]);
+}
+
+// Adaptive Tsit5, one electron per lane. A wavefront owns `per_wave`
+// consecutive items; a lane whose electron has finished takes the next item
+// of its wavefront (ballot + mbcnt), so short and long trajectories share
+// the wave without idling until the range is exhausted. A fresh item is set
+// up at the top of the loop, so the initial-step code has one call site and
+// is inlined. Three waves per SIMD (at most 168 VGPRs).
+template <int DIM, int PHASE, int OUT, int TGT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
+    IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
+    const int64_t* __restrict__ pre, int64_t n_seg, int per_wave) {
+    const int lane = lane_id();
+    const int64_t total = pre[n_seg];
+    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int64_t wbeg = gw * per_wave;
+    if (wbeg >= total) return;
+    const int64_t wend = (wbeg + per_wave < total) ? wbeg + per_wave : total;
+    const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
+    const double atol = P.atol;
+    LaneAcc acc;
+    Ts5Lane<DIM, PHASE> st;
+    int64_t item = wbeg + lane, next = wbeg + 64, col = 0;
+    bool valid = item < wend, fresh = valid;
+    double t0 = 0.0, prob = 0.0;
+    while (__ballot(valid) != 0) {
+        bool done = false;
+        if (fresh) {
+            fresh = false;
+            col = item_column(item, pre, n_seg, seg_start);
+            const double* c = init + col * P.rows;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                st.r[i] = c[i];
+                st.p[i] = c[DIM + i];
+            }
+            t0 = c[2 * DIM];
+            prob = c[2 * DIM + 1];
+            st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
+            ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+        }
+        if (valid) {
+            const int rc = ts5_attempt<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters, st);
+            if (rc != 0) {
+                done = true;
+                bool failed = rc < 0;
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) failed = failed || isnan(st.r[i]) || isnan(st.p[i]);
+                acc.steps += st.accepted;
+                emit_lane<DIM, PHASE, OUT>(P, O, col, failed, st.r, st.p, st.ph, t0, prob, acc);
+            }
+        }
+        const uint64_t m = __ballot(done);
+        if (m) {
+            const int pre_l = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (done) {
+                item = next + pre_l;
+                valid = item < wend;
+                fresh = valid;
+            }
+            next += __popcll(m);
+        }
+    }
+    if (OUT == OUT_PMD) flush_lane_acc(O, acc);
+}
+
+template <int DIM, int PHASE, int OUT>
+static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                                 const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                 int per_wave) {
+    const int64_t waves = (max_items + per_wave - 1) / per_wave;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st, P, O,
+                           init, seg_start, pre, n_seg, per_wave);
+    else
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O, init,
+                           seg_start, pre, n_seg, per_wave);
+    return hipGetLastError();
+}
+
+template <int DIM, int PHASE>
+static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+    switch (out) {
+        case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD>(st, P, O, init, ss, pre, ns, mi, pw);
+        case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS>(st, P, O, init, ss, pre, ns, mi, pw);
+        default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB>(st, P, O, init, ss, pre, ns, mi, pw);
+    }
+}
+
+template <int DIM>
+static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+    switch (P.phase) {
+        case ETRAJ_PHASE_CTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
+        case ETRAJ_PHASE_QTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
+        default: return launch_tsit5_o<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, pre, ns, mi, pw);
+    }
+}
+
+hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave) {
+    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
+    return P.dim == 3 ? launch_tsit5_p<3>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave)
+                      : launch_tsit5_p<2>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave);
+}
+
+}  // namespace etraj

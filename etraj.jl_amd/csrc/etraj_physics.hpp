@@ -400,10 +400,32 @@ __host__ __device__ inline LaserTr laser_tr(const LaserP& L, double tr) {
     r.m[1] = L.n == 2 ? mono_tr(L.c[1], tr - L.delay) : r.m[0];
     return r;
 }
+// cosh and sinh of x from one exp: cosh = (e + 1/e)/2, and sinh = (e - 1/e)/2
+// for |x| >= 1/2, where the difference loses at most a bit and a half; below
+// that the Taylor series of sinh through x^15 (first omitted term < 1e-19).
+__host__ __device__ inline void cosh_sinh(double x, double& ch, double& sh) {
+    const double ax = fabs(x);
+    const double e = exp(ax);
+    const double ie = 1.0 / e;
+    ch = 0.5 * (e + ie);
+    const double x2 = x * x;
+    double ps = 1.0 / 1307674368000.0;
+    ps = fma(ps, x2, 1.0 / 6227020800.0);
+    ps = fma(ps, x2, 1.0 / 39916800.0);
+    ps = fma(ps, x2, 1.0 / 362880.0);
+    ps = fma(ps, x2, 1.0 / 5040.0);
+    ps = fma(ps, x2, 1.0 / 120.0);
+    ps = fma(ps, x2, 1.0 / 6.0);
+    const double small = fma(ps * x2, x, x);
+    const double big = copysign(0.5 * (e - ie), x);
+    sh = ax < 0.5 ? small : big;
+}
+
 __host__ __device__ inline void mono_field_pot_tau(const MonoP& m, const MonoTr& r, double trm, double tau, Cx& Fx,
                                                    Cx& Fy, Cx& Ax, Cx& Ay) {
     const double wi = m.w * tau;
-    const double chb = cosh(wi), shb = sinh(wi);
+    double chb, shb;
+    cosh_sinh(wi, chb, shb);
     const Cx sw{r.sa * chb, r.ca * shb}, cw{r.ca * chb, -r.sa * shb};
     Cx g, gA, a, b;
     if (m.kind == ETRAJ_LASER_GAUSSIAN) {
@@ -416,7 +438,8 @@ __host__ __device__ inline void mono_field_pot_tau(const MonoP& m, const MonoTr&
         b = ts * sw - m.w * cw;
     } else {
         const double ei = m.w * tau / (2 * m.N);
-        const double che = cosh(ei), she = sinh(ei);
+        double che, she;
+        cosh_sinh(ei, che, she);
         const Cx sh{r.sae * che, r.cae * she}, ch{r.cae * che, -r.sae * she};
         const Cx c2 = ch * ch;
         const bool cos2 = m.kind == ETRAJ_LASER_COS2;

@@ -86,7 +86,7 @@ struct etraj_ctx {
     DevBuf<int64_t> seg_start, cnt, pre;
     DevBuf<int32_t> counts, counts2, status, seg_job, seg_b;
     DevBuf<SamplerP> sps;
-    DevBuf<double> glq;   // 32-point Gauss-Legendre (node, weight) pairs for the SPA sampler
+    DevBuf<double> glq;   // kSpaGlPoints Gauss-Legendre (node, weight) pairs for the SPA sampler
     DevBuf<LaserP> lps;
     DevBuf<uint8_t> flags;
     DevBuf<int64_t> orig;
@@ -169,10 +169,9 @@ int batch_times_of(etraj_ctx* c, const etraj_sampler_desc& s, std::vector<double
 
 // ADKSampler constructor checks (ADKSampler.jl:43-105) and the per-job
 // constants of gen_electron_batch.
-// 32-point Gauss-Legendre rule on [-1, 1]: Newton iteration on P_32 from
+// n-point Gauss-Legendre rule on [-1, 1]: Newton iteration on P_n from
 // the Chebyshev-like initial guesses; (node, weight) pairs.
-void gauss_legendre_32(double* q) {
-    const int n = 32;
+static void gauss_legendre(int n, double* q) {
     for (int i = 0; i < n / 2; ++i) {
         double z = std::cos(kPi * (i + 0.75) / (n + 0.5)), dp = 0.0;
         for (int it = 0; it < 100; ++it) {
@@ -542,9 +541,9 @@ int etraj_ctx_create(int device, etraj_ctx** out) {
         delete c;
         return ETRAJ_EHIP;
     }
-    double glq[64];
-    gauss_legendre_32(glq);
-    if (c->glq.ensure(64) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess ||
+    double glq[2 * kSpaGlPoints];
+    gauss_legendre(kSpaGlPoints, glq);
+    if (c->glq.ensure(2 * kSpaGlPoints) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->slot[0].done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->slot[1].done, hipEventDisableTiming) != hipSuccess) {

@@ -16,6 +16,13 @@ namespace etraj {
 constexpr int kWfatMaxDim = 31, kWfatMaxChannels = 512, kWfatMaxM = 15;
 // Prefactor beyond l = 0 (ADK / SPA / SPANE): l_max <= kPreMaxL; the per-batch
 // table has one complex coefficient per monomial u^a v^b z^c, a+b+c <= l_max
+// SFA-SPA tunnelling integrals: Gauss-Legendre points on [0, ti]. The
+// integrands are sums of cosh/sinh(omega tau) products (derivatives grow like
+// omega^n), and omega ti ~ asinh(omega sqrt(k^2 + 2 Ip) / F) stays below about
+// 7 for the batches the sampler keeps (F(tr) > 1e-3, wavelengths down to
+// 400 nm); there the 16-point rule's relative error bound
+// (omega ti)^33 (16!)^4 / (33 (32!)^3) is below 1e-25.
+constexpr int kSpaGlPoints = 16;
 constexpr int kPreMaxL = 15, kPreMaxMono = (kPreMaxL + 1) * (kPreMaxL + 2) * (kPreMaxL + 3) / 6;
 
 struct SamplerP {
@@ -33,7 +40,7 @@ struct SamplerP {
     uint64_t seed;
     int method;        // ETRAJ_INIT_*
     double omega;      // SPA: initial-guess frequency (SPASampler.jl:160-166)
-    const double* glq; // SPA: 32 Gauss-Legendre (node, weight) pairs on [-1, 1]
+    const double* glq; // SPA: kSpaGlPoints Gauss-Legendre (node, weight) pairs on [-1, 1]
     int wfat_na, wfat_nb;
     double mol_rot[3];
     const double* wfat_coeff;  // device copy of the structure-factor data (re, im), layout per wfat_form

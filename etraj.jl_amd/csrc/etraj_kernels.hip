@@ -332,7 +332,7 @@ __device__ inline void spa_p(double kd, Cx Ax, Cx Ay, double& px, double& py) {
 // One SPA sample: saddle point, exit position x0 = int_0^ti Im Ax(tr + i tau)
 // (:350-351), the tunnelling integral G = int_0^ti g(tr + i tau) dtau with
 // g = ((px+Ax)^2 + (py+Ay)^2 + kz^2)/2 + Ip, so exp(i S_tun) = exp(-G) (:279),
-// by 32-point Gauss-Legendre on [0, ti] (S.glq: nodes, weights); the
+// by kSpaGlPoints-point Gauss-Legendre on [0, ti] (S.glq: nodes, weights); the
 // prefactor k(ts).F(ts) (:271-278) and the finite-difference Jacobian
 // (:280-292). Returns false if the solve fails.
 template <bool PRE>
@@ -350,7 +350,7 @@ __device__ inline bool spa_eval(const SamplerP& S, const LaserP& L, const Target
     Cx G{0.0, 0.0};
     const double h = ti / 2;
 #pragma unroll 1
-    for (int i = 0; i < 32; ++i) {
+    for (int i = 0; i < kSpaGlPoints; ++i) {
         Cx fx, fy, ax, ay;
         laser_field_pot_tau(L, B.lt[0], tr, h + h * S.glq[2 * i], fx, fy, ax, ay);
         const double wi = S.glq[2 * i + 1];

@@ -844,7 +844,7 @@ template <int DIM, int PHASE>
 struct Ts5Lane {
     double r[DIM], p[DIM], ph;
     double k1p[DIM], k1ph;
-    double t, dt, qold, q11;
+    double t, dt, lqold, q11;   // lqold = log(qold) of the PI controller
     int32_t accepted, attempted;
 };
 
@@ -870,7 +870,7 @@ __host__ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP
     constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
     constexpr int N = 2 * DIM + (PH ? 1 : 0);
     s.t = t0;
-    s.qold = 1e-4;
+    s.lqold = log(1e-4);
     s.q11 = 1.0;
     s.accepted = 0;
     s.attempted = 0;
@@ -1063,12 +1063,15 @@ __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const Targe
     }
     const double EEst = sqrt(ss / N);
     if (isnan(EEst)) return -1;
-    double q;
+    // q11 = EEst^(7/50), q = q11 / qold^(2/25), both as exp of logs (the
+    // logarithm of qold is kept from the accepting step)
+    double q, lE = log(1e-4);
     if (EEst == 0) {
         q = 1.0 / 10.0;
     } else {
-        s.q11 = pow(EEst, 7.0 / 50.0);
-        q = s.q11 / pow(s.qold, 2.0 / 25.0);
+        lE = log(EEst);
+        s.q11 = exp((7.0 / 50.0) * lE);
+        q = exp((7.0 / 50.0) * lE - (2.0 / 25.0) * s.lqold);
         q = fmax(1.0 / 10.0, fmin(1.0 / 0.2, q / 0.9));
     }
     const double dtmax = tf - t0;
@@ -1083,7 +1086,7 @@ __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const Targe
         s.ph = unh;
         s.k1ph = k7h;
         ++s.accepted;
-        s.qold = fmax(EEst, 1e-4);
+        s.lqold = fmax(lE, log(1e-4));   // log(max(EEst, 1e-4))
         s.dt = fmin(dt / q, dtmax);
     } else {
         s.dt = dt / fmin(1.0 / 0.2, s.q11 / 0.9);

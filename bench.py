@@ -216,9 +216,9 @@ def bench_workload(name, args, ctx):
     threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
     if name == "h-ctmc-grid":
         kw = workloads.h_lp_ctmc_grid(n_t=2000, n_kd=50)
-        jobs = [dict(kw, precision="mixed")]
+        jobs = [dict(kw, precision="mixed", pmd_accumulator="fixed")]
         desc = ("H-atom ADK CTMC, 800 nm 1e14 W/cm^2 2-cycle LP cos2, 2000 t0 x 50 kd grid, RK4 dt 0.1, "
-                "PMD 500x500")
+                "PMD 500x500: fixed-point weight sums + per-cell hit counts")
     elif name == "ar-spa-3d":
         kw = workloads.ar_cp_spa_qtmc_3d(n_t=args.n_t if args.n_t != 10000 else 2000,
                                          n_k=args.n_k if args.n_k != 32000 else 158)
@@ -244,6 +244,10 @@ def bench_workload(name, args, ctx):
     else:
         raise SystemExit(f"unknown workload {name}")
     ds, keep = _scan_descs(jobs)
+    hits = None
+    if name == "h-ctmc-grid":   # the integer-count PMD: electrons binned per cell, accumulated over the runs
+        hits = np.zeros(int(np.prod(jobs[0]["final_p_num"])), dtype=np.uint64)
+        ds[0][3].hit_counts = hits.ctypes.data_as(_abi.C.POINTER(_abi.C.c_uint64))
     rk4 = ds[0][3].integrator == _abi.INTEG_RK4
     mixed = rk4 and ds[0][3].precision == _abi.PREC_MIXED
     sec, sts = _timed(ctx, lambda: _run_scan(ctx, ds)[1], args.steps, args.warmup)
@@ -258,11 +262,15 @@ def bench_workload(name, args, ctx):
         "device_ms": {"sample": sts[-1][0].ms_sample, "integrate": sts[-1][0].ms_integrate,
                       "integrate_main": sts[-1][0].ms_integrate_main},
         "fp64_reintegrated_fraction": sum(s.n_restep for s in sts[-1]) / max(n_el, 1),
+        "pmd_accumulator": "fixed" if ds[0][3].pmd_mode == _abi.PMD_FIXED else "float",
         # FP32-pass FLOPs per step: 2D SCTS hydrogen-like 149 (+12 for the molecule's soft-core term);
         # 2D CTMC 116 (no phase: 4 RHS x 6 + phase combination 5 + phase Kahan 4 fewer)
         "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 161.0, "h-ctmc-grid": 116.0}.get(name, 149.0),
                                    134.0) if mixed else None),
     }
+    if hits is not None:
+        result["hit_counts_per_run"] = int(hits.sum()) // (args.steps + args.warmup)
+        ds[0][3].hit_counts = None
     if mixed:
         ds64, keep64 = _scan_descs([dict(kw, precision="fp64") for kw in jobs])
         p64, s64 = _run_scan(ctx, ds64, device=False)
@@ -353,7 +361,7 @@ def main():
                                f"({n_el} electrons above the 1e-16 cutoff), RK4 dt 0.1, coherent PMD 1000x1000",
                    "electrons": n_el,
                    "precision": "mixed: FP32 EOM + Kahan, FP32/FP64 Kahan phase, FP64 re-integration of electrons "
-                                "within 2 a.u. of the core, FP64 Kepler and PMD"},
+                                f"within {kw['restep_radius']:g} a.u. of the core, FP64 Kepler and PMD"},
         "pmd_wall_ms": ms_step,
         "integrate_kernel_ms": float(np.mean(ms_int)),
         "fp64_reintegrated_fraction": restep / max(n_el * args.steps, 1),

@@ -91,6 +91,7 @@ struct etraj_ctx {
     DevBuf<uint8_t> flags;
     DevBuf<int64_t> orig;
     DevBuf<Contrib> contrib;
+    DevBuf<uint64_t> pmdq, scalq, hits;   // ETRAJ_PMD_FIXED sums, hit counts
 };
 
 namespace {
@@ -354,6 +355,7 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     }
     if (r.precision != ETRAJ_PREC_FP64 && r.precision != ETRAJ_PREC_MIXED) return fail(c, ETRAJ_EINVAL, "unknown precision");
     if (r.precision == ETRAJ_PREC_MIXED && !(r.restep_radius >= 0)) return fail(c, ETRAJ_EINVAL, "restep_radius must be >= 0");
+    if (r.pmd_mode != ETRAJ_PMD_FLOAT && r.pmd_mode != ETRAJ_PMD_FIXED) return fail(c, ETRAJ_EINVAL, "unknown pmd_mode");
     for (int d = 0; d < r.dimension; ++d)
         if (r.final_p_num[d] < 1 || !(r.final_p_max[d] > 0))
             return fail(c, ETRAJ_EINVAL, "final_p_num must be >= 1 and final_p_max > 0 in every dimension");
@@ -567,6 +569,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->init.release(); c->pmd.release(); c->pmd_out.release(); c->io1.release(); c->io2.release();
     c->io3.release(); c->io4.release(); c->scal.release(); c->t0s.release(); c->seg_start.release();
     c->cnt.release(); c->counts.release(); c->status.release(); c->contrib.release();
+    c->pmdq.release(); c->scalq.release(); c->hits.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release(); c->prec.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
@@ -752,6 +755,7 @@ int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const e
         if (k.status == 2) continue;
         *ion_prob += k.prob;
         if (k.status == 3) { *ion_prob_uncollected += k.prob; continue; }
+        if (run->hit_counts) run->hit_counts[k.lin] += 1;
         if (P.phase == ETRAJ_PHASE_CTMC) {
             out[k.lin] += k.re;
         } else {
@@ -901,9 +905,15 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         hsp[j] = jobs[j].S;
         hlp[j] = jobs[j].L;
     }
+    const bool fixed = run->pmd_mode == ETRAJ_PMD_FIXED;
     HIP_TRY(c, c->pmd.ensure((size_t)n_jobs * plen));
     HIP_TRY(c, c->scal.ensure((size_t)2 * n_jobs));
     HIP_TRY(c, c->cnt.ensure((size_t)CNT_N * n_jobs));
+    if (fixed) {
+        HIP_TRY(c, c->pmdq.ensure((size_t)2 * n_jobs * plen));
+        HIP_TRY(c, c->scalq.ensure((size_t)4 * n_jobs));
+    }
+    if (run->hit_counts) HIP_TRY(c, c->hits.ensure((size_t)n_jobs * npix));
     HIP_TRY(c, c->sps.ensure(n_jobs));
     HIP_TRY(c, c->lps.ensure(n_jobs));
     // restep_radius <= 0: no electron is re-integrated, so no flags, compaction or FP64 pass
@@ -931,12 +941,21 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * n_jobs * plen, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2 * n_jobs, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N * n_jobs, c->stream));
+    if (fixed) {
+        HIP_TRY(c, hipMemsetAsync(c->pmdq.p, 0, sizeof(uint64_t) * 2 * n_jobs * plen, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->scalq.p, 0, sizeof(uint64_t) * 4 * n_jobs, c->stream));
+    }
+    if (run->hit_counts) HIP_TRY(c, hipMemsetAsync(c->hits.p, 0, sizeof(uint64_t) * n_jobs * npix, c->stream));
 
     OutP O{};
     O.pmd = c->pmd.p;
     O.scal = c->scal.p;
     O.cnt = c->cnt.p;
     O.job_stride = plen;
+    O.npix = npix;
+    O.pmdq = fixed ? c->pmdq.p : nullptr;
+    O.scalq = fixed ? c->scalq.p : nullptr;
+    O.hits = run->hit_counts ? c->hits.p : nullptr;
     // per slab, first stream: start, sampled, main integrator kernel done, first-stream work done;
     // second stream: FP64 pass start and end
     std::vector<hipEvent_t> ev(6 * nslab, nullptr);
@@ -1027,6 +1046,11 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             Oj.pmd = O.pmd + j * plen;
             Oj.scal = O.scal + 2 * j;
             Oj.cnt = O.cnt + (int64_t)j * CNT_N;
+            if (fixed) {
+                Oj.pmdq = O.pmdq + 2 * j * plen;
+                Oj.scalq = O.scalq + 4 * j;
+            }
+            if (O.hits) Oj.hits = O.hits + j * npix;
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
                 e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave);
@@ -1058,6 +1082,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         return rc;
     }
 
+    if (fixed) HIP_TRY(c, launch_fix_to_f64(c->stream, c->pmdq.p, (int64_t)n_jobs * plen, c->pmd.p));
     const double* result = c->pmd.p;
     if (coh) {
         HIP_TRY(c, c->pmd_out.ensure((size_t)n_jobs * npix));
@@ -1068,9 +1093,34 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     std::vector<int64_t> cnt((size_t)CNT_N * n_jobs);
     HIP_TRY(c, hipMemcpyAsync(scal.data(), c->scal.p, sizeof(double) * 2 * n_jobs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(cnt.data(), c->cnt.p, sizeof(int64_t) * CNT_N * n_jobs, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint64_t> scalq(fixed ? 4 * n_jobs : 0), hits(run->hit_counts ? n_jobs * npix : 0);
+    if (fixed)
+        HIP_TRY(c, hipMemcpyAsync(scalq.data(), c->scalq.p, sizeof(uint64_t) * 4 * n_jobs, hipMemcpyDeviceToHost,
+                                  c->stream));
+    if (run->hit_counts)
+        HIP_TRY(c, hipMemcpyAsync(hits.data(), c->hits.p, sizeof(uint64_t) * n_jobs * npix, hipMemcpyDeviceToHost,
+                                  c->stream));
     if (host_out)
         HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix * n_jobs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (fixed) {
+        // every partial sum of a cell is bounded by the job's sum of |values|: the
+        // weights' sum (CTMC) or, for amplitudes sqrt(W), sqrt(n * sum W)
+        for (int32_t j = 0; j < n_jobs; ++j) {
+            scal[2 * j] = fix_to_double(scalq[4 * j], scalq[4 * j + 1]);
+            scal[2 * j + 1] = fix_to_double(scalq[4 * j + 2], scalq[4 * j + 3]);
+            const int64_t* cj = cnt.data() + (size_t)CNT_N * j;
+            const double bound = coh ? std::sqrt((double)cj[CNT_LAUNCHED] * scal[2 * j]) : scal[2 * j];
+            if (cj[CNT_FIXRANGE] != 0 || !(bound < 0x1.0p26)) {
+                c->slot[0].pending = c->slot[1].pending = false;
+                for (auto& e : ev)
+                    if (e) (void)hipEventDestroy(e);
+                return fail(c, ETRAJ_ERANGE, "fixed-point PMD: a sum reaches 2^26 (use pmd_mode = ETRAJ_PMD_FLOAT)");
+            }
+        }
+    }
+    if (run->hit_counts)
+        for (int64_t i = 0; i < n_jobs * npix; ++i) run->hit_counts[i] += hits[i];
     c->slot[0].pending = c->slot[1].pending = false;
     double ms_s = 0, ms_i = 0, ms_m = 0;
     for (int k = 0; k < nslab; ++k) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "etraj_engine.hpp"
+
 namespace etraj {
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
@@ -41,6 +43,29 @@ __device__ inline double wave_sum_f64(double v) {
 __device__ inline int64_t wave_sum_i64(int64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds a fixed-point value to the two words slot[0] (lo), slot[1] (hi): the
+// carry out of the low word is known from the value the atomic returns, so
+// the 128-bit sum is exact whatever the order of the additions.
+__device__ inline void fix_atomic_add(uint64_t* slot, Fix128 v) {
+    if ((v.lo | v.hi) == 0) return;
+    const uint64_t old = atomicAdd((unsigned long long*)slot, (unsigned long long)v.lo);
+    const uint64_t hi = v.hi + ((old + v.lo < old) ? 1 : 0);
+    if (hi) atomicAdd((unsigned long long*)(slot + 1), (unsigned long long)hi);
+}
+
+// 128-bit integer sum over the wavefront (exact)
+__device__ inline Fix128 fix_wave_sum(Fix128 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t lo = (uint64_t)__shfl_xor((long long)v.lo, o, 64);
+        const uint64_t hi = (uint64_t)__shfl_xor((long long)v.hi, o, 64);
+        const uint64_t s = v.lo + lo;
+        v.hi = v.hi + hi + (s < v.lo ? 1 : 0);
+        v.lo = s;
+    }
     return v;
 }
 

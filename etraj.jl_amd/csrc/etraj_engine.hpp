@@ -77,7 +77,75 @@ struct Contrib {
 enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
 // CNT_RESTEP counts the electrons of the mixed pass re-integrated in FP64
 // CNT_LAUNCHED counts the sampled electrons (the sampler kernel adds each batch's survivors)
-enum { CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_LAUNCHED = 5, CNT_N = 6 };
+// CNT_FIXRANGE counts values too large for the fixed-point accumulator
+enum {
+    CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_LAUNCHED = 5,
+    CNT_FIXRANGE = 6, CNT_N = 7
+};
+
+// Order-independent sums (ETRAJ_PMD_FIXED): a value v is held as the 128-bit
+// two's-complement integer round(v * 2^kFixFrac) in two 64-bit words (lo,
+// hi). Integer addition is associative, so the sum of any set of values is
+// the same bits in any order. |v| < 2^kFixRange for each value and for every
+// partial sum keeps the integer below 2^126.
+constexpr int kFixFrac = 100, kFixRange = 26;
+struct Fix128 {
+    uint64_t lo, hi;
+};
+
+// round(v 2^kFixFrac), ties to even; ok = false if |v| >= 2^kFixRange or v is not finite
+__host__ __device__ inline Fix128 fix_from_double(double v, bool& ok) {
+    Fix128 r{0, 0};
+    ok = true;
+    if (v == 0.0) return r;
+    if (!(fabs(v) < 0x1.0p26)) {
+        ok = false;
+        return r;
+    }
+    int e;
+    const double f = frexp(fabs(v), &e);                 // |v| = f 2^e, f in [1/2, 1)
+    const uint64_t m = (uint64_t)ldexp(f, 53);            // the 53-bit significand, exact
+    const int sh = e - 53 + kFixFrac;                     // |v| 2^kFixFrac = m 2^sh, sh <= 73
+    if (sh >= 64) {
+        r.hi = m << (sh - 64);
+    } else if (sh > 0) {
+        r.lo = m << sh;
+        r.hi = m >> (64 - sh);
+    } else if (sh == 0) {
+        r.lo = m;
+    } else if (sh > -54) {                                // m 2^sh >= 1/2 is possible
+        const int rs = -sh;
+        uint64_t q = m >> rs;
+        const uint64_t rem = m & ((1ull << rs) - 1), half = 1ull << (rs - 1);
+        if (rem > half || (rem == half && (q & 1))) ++q;
+        r.lo = q;
+    }
+    if (v < 0) {
+        r.lo = ~r.lo + 1;
+        r.hi = ~r.hi + (r.lo == 0 ? 1 : 0);
+    }
+    return r;
+}
+
+// the double nearest to the fixed-point value (round to nearest even)
+__host__ __device__ inline double fix_to_double(uint64_t lo, uint64_t hi) {
+    const bool neg = (int64_t)hi < 0;
+    if (neg) {
+        lo = ~lo + 1;
+        hi = ~hi + (lo == 0 ? 1 : 0);
+    }
+    double d;
+    if (hi == 0) {
+        d = (double)lo;
+    } else {
+        const int n = 64 - __builtin_clzll(hi);           // bits to drop so that the value fits 64
+        const uint64_t low_mask = (n == 64) ? ~0ull : ((1ull << n) - 1);
+        const uint64_t mant = (n == 64 ? hi : ((hi << (64 - n)) | (lo >> n))) | ((lo & low_mask) ? 1 : 0);
+        d = ldexp((double)mant, n);
+    }
+    d = ldexp(d, -kFixFrac);
+    return neg ? -d : d;
+}
 
 // Per-segment laser field table (k_field_table): entry s of a segment holds
 // the field at half-steps 2s+1 and 2s+2 of that segment's step grid.
@@ -92,10 +160,18 @@ struct FieldTab {
 // one launch) seg_job maps a segment to its job, whose PMD starts at
 // pmd + job * job_stride and whose scalars and counters are scal + 2 * job and
 // cnt + job * CNT_N; seg_job == nullptr means one job.
+// With pmdq != nullptr (ETRAJ_PMD_FIXED) the PMD and the two scalars go to
+// pmdq (two words per value, job stride 2 * job_stride) and scalq (4 words per
+// job) instead of pmd and scal. hits (may be null): one counter per cell, job
+// stride npix.
 struct OutP {
     double* pmd;       // CTMC: npix; coherent: 2*npix (re, im) per job
     double* scal;      // [ion_prob, ion_prob_uncollected] per job
     int64_t* cnt;      // CNT_* per job
+    uint64_t* pmdq;    // fixed-point PMD, 2 words per value
+    uint64_t* scalq;   // fixed-point [ion_prob, ion_prob_uncollected], 4 words per job
+    uint64_t* hits;    // electrons binned per cell
+    int64_t npix;
     double* records;   // 12 per electron
     int32_t* status;
     Contrib* contrib;
@@ -145,5 +221,7 @@ hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
+// out[i] = fix_to_double(q[2i], q[2i+1]), i < n
+hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out);
 
 }  // namespace etraj

@@ -672,23 +672,47 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
         }
     }
     if (OUT == OUT_PMD) {
-        if (inb) {
-            if (PHASE == ETRAJ_PHASE_CTMC) {
-                unsafeAtomicAdd(O.pmd + lin, cre);
-            } else {
-                unsafeAtomicAdd(O.pmd + 2 * lin, cre);
-                unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
+        int64_t nr = 0;
+        if (O.pmdq) {   // fixed point: exact integer sums (wave-uniform branch)
+            bool ok1 = true, ok2 = true, ok3, ok4;
+            if (inb) {
+                if (PHASE == ETRAJ_PHASE_CTMC) {
+                    fix_atomic_add(O.pmdq + 2 * lin, fix_from_double(cre, ok1));
+                } else {
+                    fix_atomic_add(O.pmdq + 4 * lin, fix_from_double(cre, ok1));
+                    fix_atomic_add(O.pmdq + 4 * lin + 2, fix_from_double(cim, ok2));
+                }
+            }
+            const Fix128 ip = fix_wave_sum(fix_from_double((active && f.ionized) ? prob : 0.0, ok3));
+            const Fix128 iu = fix_wave_sum(fix_from_double((active && f.ionized && !inb) ? prob : 0.0, ok4));
+            nr = wave_sum_i64((ok1 && ok2 && ok3 && ok4) ? 0 : 1);
+            if (lane_id() == 0) {
+                fix_atomic_add(O.scalq + 0, ip);
+                fix_atomic_add(O.scalq + 2, iu);
+            }
+        } else {
+            if (inb) {
+                if (PHASE == ETRAJ_PHASE_CTMC) {
+                    unsafeAtomicAdd(O.pmd + lin, cre);
+                } else {
+                    unsafeAtomicAdd(O.pmd + 2 * lin, cre);
+                    unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
+                }
+            }
+            double ip = wave_sum_f64((active && f.ionized) ? prob : 0.0);
+            double iu = wave_sum_f64((active && f.ionized && !inb) ? prob : 0.0);
+            if (lane_id() == 0) {
+                if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
+                if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
             }
         }
-        double ip = wave_sum_f64((active && f.ionized) ? prob : 0.0);
-        double iu = wave_sum_f64((active && f.ionized && !inb) ? prob : 0.0);
+        if (O.hits && inb) atomicAdd((unsigned long long*)(O.hits + lin), 1ull);
         int64_t nf = wave_sum_i64((active && failed) ? 1 : 0);
         int64_t nb = wave_sum_i64(bound ? 1 : 0);
         if (lane_id() == 0) {
-            if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
-            if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
             if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
             if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
+            if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
         }
     } else if (OUT == OUT_RECORDS) {
         if (active) {
@@ -854,6 +878,11 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
         Oj.pmd = O.pmd + job * O.job_stride;
         Oj.scal = O.scal + 2 * job;
         Oj.cnt = O.cnt + (int64_t)job * CNT_N;
+        if (O.pmdq) {
+            Oj.pmdq = O.pmdq + 2 * job * O.job_stride;
+            Oj.scalq = O.scalq + 4 * job;
+        }
+        if (O.hits) Oj.hits = O.hits + job * O.npix;
     }
 
     Lane<DIM, PHASE> e;
@@ -1004,6 +1033,12 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* __restrict_
     }
 }
 
+__global__ void k_fix_to_f64(const uint64_t* __restrict__ q, int64_t n, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = fix_to_double(q[2 * i], q[2 * i + 1]);
+}
+
 __global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, double* __restrict__ out) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1128,6 +1163,12 @@ hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* 
     hipLaunchKernelGGL(k_field_table, dim3(grid_for(nseg * FT.stride, 256)), dim3(256), 0, st, lps, seg_job, t0s, nseg,
                        tf, dt, FT.stride, const_cast<float4*>(FT.f32), const_cast<double2*>(FT.f64),
                        const_cast<double2*>(FT.f0));
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_to_f64, dim3(grid_for(n, 256)), dim3(256), 0, st, q, n, out);
     return hipGetLastError();
 }
 

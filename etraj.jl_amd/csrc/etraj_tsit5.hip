@@ -20,8 +20,15 @@ namespace etraj {
 // counters into `acc` for one reduction per wave at the end.
 struct LaneAcc {
     double ip = 0.0, iu = 0.0;
-    int64_t failed = 0, bound = 0, steps = 0;
+    Fix128 ipq{0, 0}, iuq{0, 0};   // ETRAJ_PMD_FIXED
+    int64_t failed = 0, bound = 0, steps = 0, range = 0;
 };
+
+__device__ inline void fix_add(Fix128& a, Fix128 b) {
+    const uint64_t lo = a.lo + b.lo;
+    a.hi = a.hi + b.hi + (lo < a.lo ? 1 : 0);
+    a.lo = lo;
+}
 
 template <int DIM, int PHASE, int OUT>
 __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx, bool failed, const double* r,
@@ -53,18 +60,34 @@ __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx
     }
     if (failed) acc.failed++;
     else if (!f.ionized) acc.bound++;
-    else {
+    else if (OUT == OUT_PMD && O.pmdq) {
+        bool ok;
+        const Fix128 q = fix_from_double(prob, ok);
+        acc.range += ok ? 0 : 1;
+        fix_add(acc.ipq, q);
+        if (!inb) fix_add(acc.iuq, q);
+    } else {
         acc.ip += prob;
         if (!inb) acc.iu += prob;
     }
     if (OUT == OUT_PMD) {
         if (inb) {
-            if (PHASE == ETRAJ_PHASE_CTMC) {
+            if (O.pmdq) {
+                bool ok1 = true, ok2 = true;
+                if (PHASE == ETRAJ_PHASE_CTMC) {
+                    fix_atomic_add(O.pmdq + 2 * lin, fix_from_double(cre, ok1));
+                } else {
+                    fix_atomic_add(O.pmdq + 4 * lin, fix_from_double(cre, ok1));
+                    fix_atomic_add(O.pmdq + 4 * lin + 2, fix_from_double(cim, ok2));
+                }
+                acc.range += (ok1 && ok2) ? 0 : 1;
+            } else if (PHASE == ETRAJ_PHASE_CTMC) {
                 unsafeAtomicAdd(O.pmd + lin, cre);
             } else {
                 unsafeAtomicAdd(O.pmd + 2 * lin, cre);
                 unsafeAtomicAdd(O.pmd + 2 * lin + 1, cim);
             }
+            if (O.hits) atomicAdd((unsigned long long*)(O.hits + lin), 1ull);
         }
     } else if (OUT == OUT_RECORDS) {
         double* rec = O.records + 12 * col_idx;
@@ -87,14 +110,26 @@ __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx
 }
 
 __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
-    const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
     const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
+    const int64_t nr = wave_sum_i64(a.range);
+    if (O.pmdq) {
+        const Fix128 ip = fix_wave_sum(a.ipq), iu = fix_wave_sum(a.iuq);
+        if (lane_id() == 0) {
+            fix_atomic_add(O.scalq + 0, ip);
+            fix_atomic_add(O.scalq + 2, iu);
+        }
+    } else {
+        const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
+        if (lane_id() == 0) {
+            if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
+            if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+        }
+    }
     if (lane_id() == 0) {
-        if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
-        if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
         if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
         if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
         if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
+        if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
     }
 }
 

@@ -16,9 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # ETRAJ_HIP_LIB names another build of the same library (the Julia binding reads the same variable)
 LIB_PATH = os.environ.get("ETRAJ_HIP_LIB", os.path.join(HERE, "libetraj_hip.so"))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
-OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY = 0, -1, -2, -3, -4, -5
+OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY, ERANGE = 0, -1, -2, -3, -4, -5, -6
 
 LASER_COS2, LASER_COS4, LASER_GAUSSIAN = 1, 2, 3
 TARGET_HYDROGEN_LIKE, TARGET_SAE = 1, 2
@@ -31,6 +31,7 @@ INTEG_RK4, INTEG_TSIT5 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 QUIRK_SCTS3D_XPY, QUIRK_SAE2D_SCTS = 1, 2
 QUIRKS_REFERENCE = QUIRK_SCTS3D_XPY | QUIRK_SAE2D_SCTS
+PMD_FLOAT, PMD_FIXED = 0, 1
 
 PHASE_METHODS = {"CTMC": PHASE_CTMC, "QTMC": PHASE_QTMC, "SCTS": PHASE_SCTS}
 
@@ -74,7 +75,8 @@ class RunDesc(C.Structure):
                 ("precision", C.c_int32), ("traj_t_final", C.c_double), ("dt", C.c_double),
                 ("rtol", C.c_double), ("atol", C.c_double), ("restep_radius", C.c_double),
                 ("final_p_max", C.c_double * 3), ("final_p_num", C.c_int32 * 3),
-                ("quirks", C.c_int32), ("maxiters", C.c_int64)]
+                ("quirks", C.c_int32), ("maxiters", C.c_int64), ("pmd_mode", C.c_int32),
+                ("reserved", C.c_int32), ("hit_counts", C.POINTER(C.c_uint64))]
 
 
 class Stats(C.Structure):

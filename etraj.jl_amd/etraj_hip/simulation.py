@@ -10,6 +10,11 @@ Keyword interface of the reference, plus engine keywords:
                    the drop-in for launch_and_collect! (:300-401)
   seed, sample_stratified, kd_sigma, kz_sigma   seeded sampling modes
   reproduce_quirks reproduce the reference's 3D SCTS / SAE 2D SCTS formulas
+  pmd_accumulator  "float": FP64 atomic sums (the reference's Float64 sums, rounded
+                   in arrival order); "fixed": 128-bit fixed-point sums (2^-100
+                   resolution), bit-identical for any order of the electrons
+  hit_counts       also return result["hit_counts"]: the number of electrons
+                   binned into each PMD cell (the integer-count PMD)
 Returns the fields write_output! stores (:516-543) as a dict; when
 output_path is given they are also saved: output_fmt "h5" / "hdf5" writes the
 reference's HDF5 file (etraj_hip.h5out), "npz" a numpy archive of the same
@@ -30,7 +35,8 @@ from .targets import Serialize as _ser_target
 
 
 def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num, traj_integrator="rk4",
-             traj_dt=0.1, traj_rtol=1e-6, precision=None, restep_radius=2.0, reproduce_quirks=True, **_):
+             traj_dt=0.1, traj_rtol=1e-6, precision=None, restep_radius=2.0, reproduce_quirks=True,
+             pmd_accumulator="float", **_):
     d = _abi.RunDesc()
     d.dimension = int(dimension)
     d.phase_method = _abi.PHASE_METHODS[_sym(traj_phase_method)]
@@ -57,6 +63,10 @@ def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p
         d.final_p_num[k] = 1
     d.quirks = _abi.QUIRKS_REFERENCE if reproduce_quirks else 0
     d.maxiters = 1000000
+    acc = _sym(pmd_accumulator).lower()
+    if acc not in ("float", "fixed"):
+        raise ValueError(f"unknown pmd_accumulator {pmd_accumulator}")
+    d.pmd_mode = _abi.PMD_FIXED if acc == "fixed" else _abi.PMD_FLOAT
     return d
 
 
@@ -76,7 +86,7 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
                             mol_orbit_ridx=0, show_progress=False, traj_integrator="rk4", traj_dt=0.1,
                             precision=None, restep_radius=2.0, granularity="job", seed=0,
                             sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, reproduce_quirks=True,
-                            backend="hip"):
+                            pmd_accumulator="float", hit_counts=False, backend="hip"):
     if _sym(backend) != "hip":
         raise ValueError("this package is the HIP backend; the CPU method is the reference itself")
     # TrajectorySimulationJob checks (:175-186)
@@ -101,9 +111,12 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
     rd = run_desc(dimension=dimension, traj_phase_method=traj_phase_method, traj_t_final=traj_t_final,
                   final_p_max=final_p_max, final_p_num=final_p_num, traj_integrator=traj_integrator,
                   traj_dt=traj_dt, traj_rtol=traj_rtol, precision=precision, restep_radius=restep_radius,
-                  reproduce_quirks=reproduce_quirks)
+                  reproduce_quirks=reproduce_quirks, pmd_accumulator=pmd_accumulator)
     ctx = _abi.default_context()
     npix = int(np.prod(final_p_num))
+    hits = np.zeros(npix, dtype=np.uint64) if hit_counts else None
+    if hits is not None:
+        rd.hit_counts = hits.ctypes.data_as(_abi.C.POINTER(_abi.C.c_uint64))
     coherent = rd.phase_method != _abi.PHASE_CTMC
     stats = {}
     if granularity == "job":
@@ -135,18 +148,24 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
                      target=target, dimension=dimension, sample_t_intv=sample_t_intv, sample_t_num=sample_t_num,
                      traj_phase_method=traj_phase_method, traj_t_final=traj_t_final, final_p_max=final_p_max,
                      final_p_num=final_p_num, traj_integrator=traj_integrator, traj_dt=traj_dt, precision=precision)
+    if hits is not None:
+        result["hit_counts"] = _julia_shape(hits, final_p_num)
     _save(result, output_path, output_fmt, output_compress)
     return result
+
+
+def _julia_shape(a, final_p_num):
+    shape = tuple(int(n) for n in final_p_num)
+    return a.reshape(shape[::-1]).T
 
 
 def _result(spec, ion_prob, ion_unc, n_eff, stats, **kw):
     """The fields write_output! stores (:516-543); PMD layout px fastest (Julia column-major)."""
     final_p_max, final_p_num = kw["final_p_max"], kw["final_p_num"]
-    shape = tuple(int(n) for n in final_p_num)
     result = {
         "px": _axis(final_p_max[0], final_p_num[0]),
         "py": _axis(final_p_max[1], final_p_num[1]),
-        "momentum_spec": spec.reshape(shape[::-1]).T,
+        "momentum_spec": _julia_shape(spec, final_p_num),
         "ion_prob": ion_prob,
         "ion_prob_uncollected": ion_unc,
         "num_effective_traj": n_eff,
@@ -222,7 +241,7 @@ _SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc
                       sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="rk4",
                       traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
                       kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="npz",
-                      output_compress=True)
+                      output_compress=True, pmd_accumulator="float", hit_counts=False)
 
 
 def perform_traj_scan(jobs):
@@ -262,6 +281,9 @@ def perform_traj_scan(jobs):
     stats = (_abi.Stats * n)()
     npix = int(np.prod(kw0["final_p_num"]))
     out = np.empty(n * npix)
+    hits = np.zeros(n * npix, dtype=np.uint64) if kw0["hit_counts"] else None
+    if hits is not None:
+        rd0.hit_counts = hits.ctypes.data_as(_abi.C.POINTER(_abi.C.c_uint64))
     ctx = _abi.default_context()
     ctx.check(ctx.lib.etraj_run_scan(ctx.handle, n, lasers, td0, samplers, rd0, _abi.dptr(out), stats))
     results = []
@@ -269,6 +291,8 @@ def perform_traj_scan(jobs):
         st = stats[j]
         r = _result(out[j * npix:(j + 1) * npix].copy(), st.ion_prob, st.ion_prob_uncollected,
                     st.num_effective_traj, st.as_dict(), **kw)
+        if hits is not None:
+            r["hit_counts"] = _julia_shape(hits[j * npix:(j + 1) * npix].copy(), kw["final_p_num"])
         _save(r, kw["output_path"], kw.get("output_fmt", "npz"), kw.get("output_compress", True))
         results.append(r)
     return results

@@ -57,6 +57,7 @@ struct RunDesc
     dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
     traj_t_final::Float64; dt::Float64; rtol::Float64; atol::Float64; restep_radius::Float64
     final_p_max::NTuple{3,Float64}; final_p_num::NTuple{3,Int32}; quirks::Int32; maxiters::Int64
+    pmd_mode::Int32; reserved::Int32; hit_counts::Ptr{UInt64}
 end
 mutable struct Stats
     ion_prob::Float64; ion_prob_uncollected::Float64
@@ -83,12 +84,15 @@ target_desc(t::SAEAtom) = TargetDesc(2, AsympNuclCharge(t), AngularQuantumNumber
 target_desc(t::GenericMolecule) = TargetDesc(1, AsympNuclCharge(t), 0, 0, IonPotential(t), 1.0, 0.0, 0.0, 1.0,
     0, 0, 0, 0, 0, 0)
 
+# pmd_mode :fixed = ETRAJ_PMD_FIXED (order-independent 128-bit fixed-point sums); hit_counts: C_NULL or
+# the pointer of a Vector{UInt64} of prod(final_p_num) cells (kept alive by the caller) that receives
+# the number of electrons binned into each cell
 function run_desc(dimension, phase, traj_t_final, final_p_max, final_p_num; dt=0.1, precision=:mixed,
-                  restep_radius=2.0, reproduce_quirks=true)
+                  restep_radius=2.0, reproduce_quirks=true, pmd_mode=:float, hit_counts=Ptr{UInt64}(C_NULL))
     pmax = ntuple(i -> i <= dimension ? Float64(final_p_max[i]) : 1.0, 3)
     pnum = ntuple(i -> i <= dimension ? Int32(final_p_num[i]) : Int32(1), 3)
     RunDesc(dimension, PHASE[phase], 0, precision == :mixed ? 1 : 0, traj_t_final, dt, 1e-6, 1e-6, restep_radius,
-            pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000)
+            pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000, pmd_mode == :fixed ? 1 : 0, 0, hit_counts)
 end
 
 # ---- context ------------------------------------------------------------------

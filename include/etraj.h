@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ETRAJ_ABI_VERSION 1
+#define ETRAJ_ABI_VERSION 2
 
 /* status codes */
 #define ETRAJ_OK             0
@@ -37,6 +37,7 @@ extern "C" {
 #define ETRAJ_ENOMEM        (-3)  /* device allocation failed */
 #define ETRAJ_EUNSUPPORTED  (-4)  /* option the engine does not implement */
 #define ETRAJ_ECAPACITY     (-5)  /* caller buffer too small */
+#define ETRAJ_ERANGE        (-6)  /* a fixed-point PMD sum left its range (ETRAJ_PMD_FIXED) */
 
 /* ---- lasers: src/Lasers/{Cos2,Cos4,Gaussian,Bichromatic}Laser.jl ---------- */
 enum {
@@ -181,6 +182,17 @@ enum {
     ETRAJ_QUIRK_SAE2D_SCTS = 2   /* SAE 2D SCTS divides by (r^2+a) (SAEAtom.jl:136)         */
 };
 #define ETRAJ_QUIRKS_REFERENCE (ETRAJ_QUIRK_SCTS3D_XPY | ETRAJ_QUIRK_SAE2D_SCTS)
+/* PMD accumulator of the device binning (etraj_run_job / etraj_run_scan) */
+enum {
+    ETRAJ_PMD_FLOAT = 0,    /* FP64 atomic additions: the reference's Float64 sums
+                               (TrajectorySimulation.jl:385-390), rounding in arrival order */
+    ETRAJ_PMD_FIXED = 1     /* every contribution (CTMC weight, amplitude re/im) and the
+                               ion_prob sums rounded to a 128-bit two's-complement fixed
+                               point with 100 fraction bits (resolution 2^-100 ~ 7.9e-31,
+                               range |sum| < 2^26) and added with exact integer atomics:
+                               order-independent, bit-identical from run to run and for any
+                               slab split; ETRAJ_ERANGE if a sum leaves the range           */
+};
 
 typedef struct etraj_run_desc {
     int32_t dimension;
@@ -196,6 +208,12 @@ typedef struct etraj_run_desc {
     int32_t final_p_num[3];
     int32_t quirks;            /* ETRAJ_QUIRK_* bits                                       */
     int64_t maxiters;          /* Tsit5 step-attempt limit (<= 0: 1000000)                  */
+    int32_t pmd_mode;          /* ETRAJ_PMD_* (the device binning of run_job / run_scan)    */
+    int32_t reserved;          /* 0                                                        */
+    uint64_t* hit_counts;      /* NULL, or host counters, one per PMD cell (px fastest; for
+                                  run_scan n_jobs * prod(final_p_num), job-major): the number
+                                  of electrons binned into each cell is ADDED to them (the
+                                  integer-count PMD of a CTMC run)                          */
 } etraj_run_desc;
 
 typedef struct etraj_stats {

@@ -69,6 +69,55 @@ def l1(p, q):
     return float(np.abs(p - q).sum() / np.abs(q).sum())
 
 
+def _device_columns(ctx, ld, td, sd, batches, rows):
+    cap = int(ctx.lib.etraj_batch_max_size(sd))
+    buf = np.empty((cap, rows))
+    cnt = _abi.C.c_int64()
+    out = []
+    for b in batches:
+        ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, ld, td, sd, b + 1, _abi.dptr(buf), cap,
+                                                   _abi.C.byref(cnt)))
+        out.append(buf[:cnt.value].copy())
+    return np.concatenate(out) if out else np.empty((0, rows))
+
+
+def _device_records(ctx, ld, td, rd, cols):
+    rec = np.empty((len(cols), 12))
+    st = np.empty(len(cols), np.int32)
+    ctx.check(ctx.lib.etraj_integrate_records(ctx.handle, ld, td, rd, _abi.dptr(np.ascontiguousarray(cols)),
+                                              len(cols), cols.shape[1], _abi.dptr(rec),
+                                              st.ctypes.data_as(_abi.C.POINTER(_abi.C.c_int32))))
+    return rec, st
+
+
+def _per_electron_errors(rec, st, rrec, rst):
+    """final-momentum error |p_inf - p_inf,ref| and phase error mod 2 pi over the electrons that
+    both runs count as ionized (status 0)."""
+    both = (st == 0) & (rst == 0)
+    dp = np.linalg.norm(rec[both, 8:11] - rrec[both, 8:11], axis=1)
+    dphi = np.angle(np.exp(1j * (rec[both, 11] - rrec[both, 11])))
+    pref = np.linalg.norm(rrec[both, 8:11], axis=1)
+    return {"electrons": int(both.sum()), "status_mismatch": int((st != rst).sum()),
+            "p_rms": float(np.sqrt(np.mean(dp ** 2))), "p_max": float(dp.max()),
+            "p_rms_relative": float(np.sqrt(np.mean(dp ** 2)) / np.sqrt(np.mean(pref ** 2))),
+            "phase_rms_mod_2pi": float(np.sqrt(np.mean(dphi ** 2))), "phase_max_mod_2pi": float(np.abs(dphi).max())}
+
+
+def accuracy(kw, ctx, n_strata=64):
+    """Per-electron errors of the mixed GPU run against the FP64 GPU run on the same electrons
+    (n_strata strata spread over the pulse, sampled on the device)."""
+    ldm, tdm, sdm, rdm = descs(kw)
+    rd64 = descs(dict(kw, precision="fp64"))[3]
+    stride = max(1, sdm.sample_t_num // n_strata)
+    rows = (6 if rdm.dimension == 2 else 8) + (1 if rdm.phase_method != _abi.PHASE_CTMC else 0)
+    cols = _device_columns(ctx, ldm, tdm, sdm, range(stride // 2, sdm.sample_t_num, stride), rows)
+    rec, st = _device_records(ctx, ldm, tdm, rdm, cols)
+    r64, s64 = _device_records(ctx, ldm, tdm, rd64, cols)
+    out = _per_electron_errors(rec, st, r64, s64)
+    out["sample"] = f"{len(cols)} electrons of every {stride}-th stratum"
+    return out
+
+
 def cpu_baseline(kw, ctx, n_batches_sample, threads):
     """Oracle (FP64 RK4, same algorithm; and Tsit5 rtol 1e-6, the reference's algorithm) on
     n_batches_sample strata spread evenly over the pulse (the middle stratum of each of
@@ -103,10 +152,20 @@ def cpu_baseline(kw, ctx, n_batches_sample, threads):
                                                       amp.ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
                                                       _abi.C.byref(iu), _abi.C.byref(nf)))
     gpu = amp[0::2] ** 2 + amp[1::2] ** 2
+    # per-electron errors against the oracle: every 4th electron of 8 of the sampled strata
+    cols = _device_columns(ctx, ldm, tdm, sdm, range(first, sdm.sample_t_num, stride * max(1, n_batches_sample // 8)),
+                           7)[::4]
+    t = time.perf_counter()
+    rref, sref = O.integrate_records(ld, td, rd, cols)
+    dt_rec = time.perf_counter() - t
+    per_e = {"sample": f"{len(cols)} electrons, oracle FP64 RK4 records ({dt_rec:.1f} s, one thread)",
+             "gpu_mixed": _per_electron_errors(*_device_records(ctx, ldm, tdm, rdm, cols), rref, sref),
+             "gpu_fp64": _per_electron_errors(*_device_records(ctx, ldm, tdm, rd, cols), rref, sref)}
     return {
         "value": n / dt_rk4, "unit": "trajectories/s", "cores": threads, "kind": "port",
         "sample": f"{n} electrons = the strata {first}, {first + stride}, ... (every {stride}-th of "
                   f"{sd.sample_t_num}), FP64 RK4 dt 0.1 (same algorithm as the GPU), {dt_rk4:.1f} s",
+        "per_electron_vs_oracle": per_e,
         "reference_algorithm_value": stt["num_effective_traj"] / dt_ts,
         "reference_algorithm_sample": f"{stt['num_effective_traj']} electrons (every {2 * stride}-th stratum), "
                                       f"Tsit5 rtol 1e-6 (the reference's integrator), {dt_ts:.1f} s",
@@ -388,6 +447,11 @@ def main():
         result["pmd_l1_mixed_vs_fp64_gpu"] = l1(pmx, p64)
         result["ion_prob"] = {"mixed": smx.ion_prob, "fp64": s64.ion_prob}
         log(f"fp64: {s64.ms_total:.1f} ms; L1(mixed, fp64) = {result['pmd_l1_mixed_vs_fp64_gpu']:.3e}")
+    if not args.no_fp64:
+        result["accuracy_mixed_vs_fp64_gpu"] = accuracy(kw, ctx)
+        a = result["accuracy_mixed_vs_fp64_gpu"]
+        log(f"per electron (mixed vs fp64 GPU, {a['electrons']}): p_inf RMS {a['p_rms']:.2e}, "
+            f"phase RMS mod 2pi {a['phase_rms_mod_2pi']:.2e}")
     if args.cpu_batches > 0:
         threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
         result["cpu_baseline"] = cpu_baseline(kw, ctx, args.cpu_batches, threads)

@@ -99,6 +99,9 @@ def _per_electron_errors(rec, st, rrec, rst):
     pref = np.linalg.norm(rrec[both, 8:11], axis=1)
     return {"electrons": int(both.sum()), "status_mismatch": int((st != rst).sum()),
             "p_rms": float(np.sqrt(np.mean(dp ** 2))), "p_max": float(dp.max()),
+            "p_median": float(np.median(dp)), "p_p99": float(np.quantile(dp, 0.99)),
+            "fraction_p_above_1e-6": float(np.mean(dp > 1e-6)),
+            "phase_median_mod_2pi": float(np.median(np.abs(dphi))),
             "p_rms_relative": float(np.sqrt(np.mean(dp ** 2)) / np.sqrt(np.mean(pref ** 2))),
             "phase_rms_mod_2pi": float(np.sqrt(np.mean(dphi ** 2))), "phase_max_mod_2pi": float(np.abs(dphi).max())}
 

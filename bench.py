@@ -286,7 +286,8 @@ def bench_workload(name, args, ctx):
                                          n_k=args.n_k if args.n_k != 32000 else 158)
         jobs = [kw]
         desc = (f"Ar (SAE, l=0 prefactor) SFA-SPA 3D grid {kw['sample_t_num']} x {kw['ss_kd_num']} x "
-                f"{kw['ss_kz_num']}, CP 800 nm 2e14 W/cm^2 2 cycles, Tsit5 rtol 1e-8, QTMC, PMD 256^3")
+                f"{kw['ss_kz_num']}, CP 800 nm 2e14 W/cm^2 2 cycles, Tsit5 rtol 1e-8 ({kw['precision']}), QTMC, "
+                "PMD 256^3")
     elif name == "bichromatic-scan":
         ints = list(np.logspace(np.log10(5e13), np.log10(5e14), 32))
         probe, _ = _scan_descs(workloads.bichromatic_scan_jobs(ints, n_t=200, n_k=200))
@@ -311,7 +312,7 @@ def bench_workload(name, args, ctx):
         hits = np.zeros(int(np.prod(jobs[0]["final_p_num"])), dtype=np.uint64)
         ds[0][3].hit_counts = hits.ctypes.data_as(_abi.C.POINTER(_abi.C.c_uint64))
     rk4 = ds[0][3].integrator == _abi.INTEG_RK4
-    mixed = rk4 and ds[0][3].precision == _abi.PREC_MIXED
+    mixed = ds[0][3].precision == _abi.PREC_MIXED
     sec, sts = _timed(ctx, lambda: _run_scan(ctx, ds)[1], args.steps, args.warmup)
     n_el = sum(s.num_effective_traj for s in sts[-1])
     result = {
@@ -328,7 +329,7 @@ def bench_workload(name, args, ctx):
         # FP32-pass FLOPs per step: 2D SCTS hydrogen-like 149 (+12 for the molecule's soft-core term);
         # 2D CTMC 116 (no phase: 4 RHS x 6 + phase combination 5 + phase Kahan 4 fewer)
         "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 161.0, "h-ctmc-grid": 116.0}.get(name, 149.0),
-                                   134.0) if mixed else None),
+                                   134.0) if (mixed and rk4) else None),
     }
     if hits is not None:
         result["hit_counts_per_run"] = int(hits.sum()) // (args.steps + args.warmup)

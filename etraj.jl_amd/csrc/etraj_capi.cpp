@@ -350,8 +350,6 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     if (r.integrator == ETRAJ_INTEG_RK4 && !(r.dt > 0)) return fail(c, ETRAJ_EINVAL, "RK4 needs dt > 0");
     if (r.integrator == ETRAJ_INTEG_TSIT5) {
         if (!(r.rtol > 0)) return fail(c, ETRAJ_EINVAL, "Tsit5 needs traj_rtol > 0");
-        if (r.precision != ETRAJ_PREC_FP64)
-            return fail(c, ETRAJ_EUNSUPPORTED, "the Tsit5 integrator runs in FP64 only (precision = fp64)");
     }
     if (r.precision != ETRAJ_PREC_FP64 && r.precision != ETRAJ_PREC_MIXED) return fail(c, ETRAJ_EINVAL, "unknown precision");
     if (r.precision == ETRAJ_PREC_MIXED && !(r.restep_radius >= 0)) return fail(c, ETRAJ_EINVAL, "restep_radius must be >= 0");
@@ -492,7 +490,27 @@ int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t
         HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(c, hipMemcpyAsync(c->pre.p, pre.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave));
+        if (!P.mixed) {
+            HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
+                                    nullptr, nullptr));
+            return ETRAJ_OK;
+        }
+        // mixed: FP32 stage derivatives; the flagged electrons are compacted per
+        // segment and integrated again in FP64, their outputs at their own columns
+        HIP_TRY(c, c->flags.ensure((size_t)n));
+        HIP_TRY(c, c->counts.ensure(nseg));
+        HIP_TRY(c, c->counts2.ensure(nseg));
+        HIP_TRY(c, c->orig.ensure((size_t)n));
+        HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
+                                c->flags.p, nullptr));
+        HIP_TRY(c, launch_compact_flagged(c->stream, c->init.p, P.rows, c->seg_start.p, c->counts.p, nseg, c->flags.p,
+                                          nullptr, c->counts2.p, c->orig.p));
+        HIP_TRY(c, launch_scan_counts(c->stream, c->counts2.p, nseg, c->pre.p));
+        IntegP P64 = P;
+        P64.mixed = 0;
+        HIP_TRY(c, launch_tsit5(c->stream, P64, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
+                                nullptr, c->orig.p));
         return ETRAJ_OK;
     }
     std::vector<double> t0(nseg);
@@ -934,7 +952,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             HIP_TRY(c, Q.ft64.ensure((size_t)2 * max_nb * stride));
             HIP_TRY(c, Q.ft0.ensure(max_nb));
         }
-        if (two_pass) HIP_TRY(c, Q.flags.ensure(max_cols));
+        if (two_pass || (!rk4 && P.mixed)) HIP_TRY(c, Q.flags.ensure(max_cols));
+        if (!rk4 && P.mixed) HIP_TRY(c, Q.counts2.ensure(max_nb));
     }
     HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->lps.p, hlp.data(), sizeof(LaserP) * n_jobs, hipMemcpyHostToDevice, c->stream));
@@ -1053,8 +1072,20 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             if (O.hits) Oj.hits = O.hits + j * npix;
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave);
+                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave,
+                                 P.mixed ? Q.flags.p : nullptr, nullptr);
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
+            if (P.mixed) {   // the flagged electrons again in FP64, on the same stream
+                IntegP P64 = Pj;
+                P64.mixed = 0;
+                if (e == hipSuccess)
+                    e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p, nullptr,
+                                               Q.counts2.p, nullptr);
+                if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
+                if (e == hipSuccess)
+                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave,
+                                     nullptr, nullptr);
+            }
         }
         if (!two_pass) {
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);

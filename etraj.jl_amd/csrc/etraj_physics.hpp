@@ -864,6 +864,41 @@ __host__ __device__ __forceinline__ void ts5_deriv(const TargetP& T, bool saeq, 
     kph = dphi;
 }
 
+// The mixed-precision form of ts5_deriv: the derivative is evaluated in FP32
+// (the FP32 EOM of the fixed-step mixed pass) from the FP64 stage values, and
+// r2 (|r|^2, plus the soft core for hydrogen-like targets) is returned for the
+// close-encounter test; stage values, sums and the error control stay FP64.
+template <int DIM, int PHASE, int TGT>
+__host__ __device__ __forceinline__ void ts5_deriv32(const TargetP& T, bool saeq, double Fx, double Fy,
+                                                     const double* yr, const double* yp, double* kp, double& kph,
+                                                     float& r2) {
+    State<DIM, float> s, ds;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        s.r[i] = (float)yr[i];
+        s.p[i] = (float)yp[i];
+    }
+    float dphi;
+    eom<DIM, PHASE, TGT, float>(T, saeq, s, (float)Fx, (float)Fy, ds, dphi, r2);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) kp[i] = (double)ds.p[i];
+    kph = (double)dphi;
+}
+
+// stage derivative in FP64, or in FP32 with the close-encounter test (MIXED)
+template <int DIM, int PHASE, int TGT, bool MIXED>
+__host__ __device__ __forceinline__ void ts5_stage_deriv(const TargetP& T, bool saeq, double Fx, double Fy,
+                                                         const double* yr, const double* yp, double* kp, double& kph,
+                                                         float rc2, bool& close) {
+    if (MIXED) {
+        float r2;
+        ts5_deriv32<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp, kph, r2);
+        close = close || (r2 < rc2);
+    } else {
+        ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp, kph);
+    }
+}
+
 template <int DIM, int PHASE, int TGT>
 __host__ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool saeq, double t0, double tf,
                                                   double rtol, double atol, Ts5Lane<DIM, PHASE>& s) {
@@ -990,10 +1025,14 @@ __host__ __device__ __forceinline__ void ts5_inputs(const double* r, const doubl
     }
 }
 
-template <int DIM, int PHASE, int TGT>
+// One attempted step. MIXED: stage derivatives in FP32 (ts5_deriv32); `close`
+// is set when a stage position comes within the re-step radius (r2 < rc2).
+template <int DIM, int PHASE, int TGT, bool MIXED = false>
 __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bool saeq, double t0,
                                                     double tf, double rtol, double atol, int64_t maxiters,
-                                                    Ts5Lane<DIM, PHASE>& s) {
+                                                    Ts5Lane<DIM, PHASE>& s, float rc2 = 0.0f,
+                                                    bool* close_out = nullptr) {
+    bool close = false;
     constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
     constexpr int N = 2 * DIM + (PH ? 1 : 0);
     double dt = s.dt;
@@ -1009,20 +1048,20 @@ __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const Targe
     const LaserBase lb = laser_base(L, t);
     double Fx, Fy;
     laser_field_at(L, lb, ts5::c2 * dt, Fx, Fy);
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[1], kh[1]);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[1], kh[1], rc2, close);
     ts5_inputs<3, DIM>(s.r, s.p, kp, dt, yr, yp);
     laser_field_at(L, lb, ts5::c3 * dt, Fx, Fy);
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[2], kh[2]);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[2], kh[2], rc2, close);
     ts5_inputs<4, DIM>(s.r, s.p, kp, dt, yr, yp);
     laser_field_at(L, lb, ts5::c4 * dt, Fx, Fy);
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[3], kh[3]);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[3], kh[3], rc2, close);
     ts5_inputs<5, DIM>(s.r, s.p, kp, dt, yr, yp);
     laser_field_at(L, lb, ts5::c5 * dt, Fx, Fy);
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, kp[4], kh[4]);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[4], kh[4], rc2, close);
     ts5_inputs<6, DIM>(s.r, s.p, kp, dt, yr, yp);
     double F1x, F1y;                     // stages 6 and 7 share t + dt (c6 = c7 = 1)
     laser_field_at(L, lb, dt, F1x, F1y);
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, F1x, F1y, yr, yp, kp[5], kh[5]);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, F1x, F1y, yr, yp, kp[5], kh[5], rc2, close);
     double unr[DIM], unp[DIM], er[DIM], ep[DIM], unh = 0, eh = 0;
     ts5_inputs<7, DIM>(s.r, s.p, kp, dt, unr, unp);
 #pragma unroll
@@ -1047,7 +1086,7 @@ __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const Targe
         eh = sb;
     }
     double k7p[DIM], k7h;
-    ts5_deriv<DIM, PHASE, TGT>(T, saeq, F1x, F1y, unr, unp, k7p, k7h);
+    ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, F1x, F1y, unr, unp, k7p, k7h, rc2, close);
     const double b7 = ts5::b7;
     double ss = 0;
 #pragma unroll
@@ -1061,6 +1100,7 @@ __host__ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const Targe
         const double a = dt * (eh + b7 * k7h) / (atol + fmax(fabs(s.ph), fabs(unh)) * rtol);
         ss += a * a;
     }
+    if (MIXED && close_out) *close_out = *close_out || close;
     const double EEst = sqrt(ss / N);
     if (isnan(EEst)) return -1;
     // q11 = EEst^(7/50), q = q11 / qold^(2/25), both as exp of logs (the

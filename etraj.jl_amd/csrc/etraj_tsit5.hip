@@ -21,7 +21,7 @@ namespace etraj {
 struct LaneAcc {
     double ip = 0.0, iu = 0.0;
     Fix128 ipq{0, 0}, iuq{0, 0};   // ETRAJ_PMD_FIXED
-    int64_t failed = 0, bound = 0, steps = 0, range = 0;
+    int64_t failed = 0, bound = 0, steps = 0, range = 0, restep = 0;
 };
 
 __device__ inline void fix_add(Fix128& a, Fix128 b) {
@@ -109,9 +109,10 @@ __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx
     }
 }
 
-__device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
+// steps64: the launch integrates in FP64 (its steps also count as FP64 steps)
+__device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool steps64) {
     const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
-    const int64_t nr = wave_sum_i64(a.range);
+    const int64_t nr = wave_sum_i64(a.range), nq = wave_sum_i64(a.restep);
     if (O.pmdq) {
         const Fix128 ip = fix_wave_sum(a.ipq), iu = fix_wave_sum(a.iuq);
         if (lane_id() == 0) {
@@ -129,7 +130,9 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a) {
         if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
         if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
         if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
+        if (ns && steps64) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS64), (unsigned long long)ns);
         if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
+        if (nq) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nq);
     }
 }
 
@@ -149,10 +152,16 @@ __device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t 
 // the wave without idling until the range is exhausted. A fresh item is set
 // up at the top of the loop, so the initial-step code has one call site and
 // is inlined. Three waves per SIMD (at most 168 VGPRs).
-template <int DIM, int PHASE, int OUT, int TGT>
+// MIXED: stage derivatives in FP32; an electron with a stage position within
+// restep_radius of the core stops at once, is not binned and gets flags[col]
+// = 1 (every other column of the launch gets 0), for the FP64 launch over the
+// compacted flagged columns. orig (may be null): the output index of a
+// compacted column (records / contributions of that FP64 launch).
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
-    const int64_t* __restrict__ pre, int64_t n_seg, int per_wave) {
+    const int64_t* __restrict__ pre, int64_t n_seg, int per_wave, uint8_t* __restrict__ flags,
+    const int64_t* __restrict__ orig) {
     const int lane = lane_id();
     const int64_t total = pre[n_seg];
     const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -161,10 +170,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int64_t wend = (wbeg + per_wave < total) ? wbeg + per_wave : total;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
     const double atol = P.atol;
+    // the FP32 hydrogen-like EOM reports |r|^2 + soft core
+    const float rc2 = (float)(P.restep_radius * P.restep_radius + (TGT != ETRAJ_TARGET_SAE ? P.T.sc : 0.0));
     LaneAcc acc;
     Ts5Lane<DIM, PHASE> st;
     int64_t item = wbeg + lane, next = wbeg + 64, col = 0;
-    bool valid = item < wend, fresh = valid;
+    bool valid = item < wend, fresh = valid, close = false;
     double t0 = 0.0, prob = 0.0;
     while (__ballot(valid) != 0) {
         bool done = false;
@@ -180,17 +191,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             t0 = c[2 * DIM];
             prob = c[2 * DIM + 1];
             st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
+            close = false;
             ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
         }
         if (valid) {
-            const int rc = ts5_attempt<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters, st);
-            if (rc != 0) {
+            const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters,
+                                                               st, rc2, &close);
+            if (MIXED && close) {   // re-integrated in FP64 by the next launch
+                done = true;
+                acc.steps += st.accepted;
+                acc.restep++;
+                flags[col] = 1;
+            } else if (rc != 0) {
                 done = true;
                 bool failed = rc < 0;
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) failed = failed || isnan(st.r[i]) || isnan(st.p[i]);
                 acc.steps += st.accepted;
-                emit_lane<DIM, PHASE, OUT>(P, O, col, failed, st.r, st.p, st.ph, t0, prob, acc);
+                if (MIXED) flags[col] = 0;
+                emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
             }
         }
         const uint64_t m = __ballot(done);
@@ -204,50 +223,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             next += __popcll(m);
         }
     }
-    if (OUT == OUT_PMD) flush_lane_acc(O, acc);
+    if (OUT == OUT_PMD) flush_lane_acc(O, acc, !MIXED);
 }
 
-template <int DIM, int PHASE, int OUT>
-static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                                 const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
-                                 int per_wave) {
-    const int64_t waves = (max_items + per_wave - 1) / per_wave;
+struct Ts5Launch {
+    const double* init;
+    const int64_t* seg_start;
+    const int64_t* pre;
+    int64_t n_seg, max_items;
+    int per_wave;
+    uint8_t* flags;
+    const int64_t* orig;
+};
+
+template <int DIM, int PHASE, int OUT, bool MIXED>
+static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
+    const int64_t waves = (a.max_items + a.per_wave - 1) / a.per_wave;
     const unsigned blocks = (unsigned)((waves + 3) / 4);
     if (blocks == 0) return hipSuccess;
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st, P, O,
-                           init, seg_start, pre, n_seg, per_wave);
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE, MIXED>), dim3(blocks), dim3(256), 0, st,
+                           P, O, a.init, a.seg_start, a.pre, a.n_seg, a.per_wave, a.flags, a.orig);
     else
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O, init,
-                           seg_start, pre, n_seg, per_wave);
+        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE, MIXED>), dim3(blocks), dim3(256), 0, st, P, O,
+                           a.init, a.seg_start, a.pre, a.n_seg, a.per_wave, a.flags, a.orig);
     return hipGetLastError();
 }
 
-template <int DIM, int PHASE>
-static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+template <int DIM, int PHASE, bool MIXED>
+static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O, int out, const Ts5Launch& a) {
     switch (out) {
-        case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD>(st, P, O, init, ss, pre, ns, mi, pw);
-        case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS>(st, P, O, init, ss, pre, ns, mi, pw);
-        default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB>(st, P, O, init, ss, pre, ns, mi, pw);
+        case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD, MIXED>(st, P, O, a);
+        case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS, MIXED>(st, P, O, a);
+        default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB, MIXED>(st, P, O, a);
     }
 }
 
-template <int DIM>
-static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                                 const int64_t* ss, const int64_t* pre, int64_t ns, int64_t mi, int pw) {
+template <int DIM, bool MIXED>
+static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O, int out, const Ts5Launch& a) {
     switch (P.phase) {
-        case ETRAJ_PHASE_CTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_CTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
-        case ETRAJ_PHASE_QTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_QTMC>(st, P, O, out, init, ss, pre, ns, mi, pw);
-        default: return launch_tsit5_o<DIM, ETRAJ_PHASE_SCTS>(st, P, O, out, init, ss, pre, ns, mi, pw);
+        case ETRAJ_PHASE_CTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_CTMC, MIXED>(st, P, O, out, a);
+        case ETRAJ_PHASE_QTMC: return launch_tsit5_o<DIM, ETRAJ_PHASE_QTMC, MIXED>(st, P, O, out, a);
+        default: return launch_tsit5_o<DIM, ETRAJ_PHASE_SCTS, MIXED>(st, P, O, out, a);
     }
 }
 
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave) {
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave,
+                        uint8_t* flags, const int64_t* orig) {
     if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    return P.dim == 3 ? launch_tsit5_p<3>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave)
-                      : launch_tsit5_p<2>(st, P, O, out, init, seg_start, pre, n_seg, max_items, per_wave);
+    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, per_wave, flags, orig};
+    if (P.mixed) {
+        if (!flags) return hipErrorInvalidValue;
+        return P.dim == 3 ? launch_tsit5_p<3, true>(st, P, O, out, a) : launch_tsit5_p<2, true>(st, P, O, out, a);
+    }
+    return P.dim == 3 ? launch_tsit5_p<3, false>(st, P, O, out, a) : launch_tsit5_p<2, false>(st, P, O, out, a);
 }
 
 }  // namespace etraj

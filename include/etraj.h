@@ -171,10 +171,13 @@ typedef struct etraj_sampler_desc {
 enum { ETRAJ_INTEG_RK4 = 0, ETRAJ_INTEG_TSIT5 = 1 };
 enum {
     ETRAJ_PREC_FP64  = 0,   /* every operation in FP64                                   */
-    ETRAJ_PREC_MIXED = 1    /* RK4 only: FP32 EOM with Kahan-compensated state, Kahan FP64
+    ETRAJ_PREC_MIXED = 1    /* RK4: FP32 EOM with Kahan-compensated state, Kahan FP64
                                phase; electrons that start any step within restep_radius of
                                the core are integrated again from t0 in FP64; FP64 Kepler
-                               asymptote and FP64 PMD sums                                 */
+                               asymptote and FP64 PMD sums.
+                               Tsit5: FP32 stage derivatives, FP64 state, stage sums and
+                               error control; an electron with a stage within
+                               restep_radius is integrated again from t0 in FP64 */
 };
 /* quirk bits: set = reproduce the reference's behaviour (default) */
 enum {

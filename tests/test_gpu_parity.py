@@ -305,11 +305,48 @@ def test_tsit5_maxiters_failures_are_counted(ctx):
     assert (st == 1).all() and (rst == 1).all()
 
 
-def test_tsit5_mixed_precision_is_rejected(ctx):
-    kw = small_ctmc(n_t=60, n_kd=40, traj_integrator="tsit5", precision="mixed")
-    ld, td, sd, rd, cols = _columns(kw, [25])
-    with pytest.raises(_abi.EngineError):
-        _device_records(ctx, ld, td, rd, cols)
+def test_tsit5_mixed_records_close_to_fp64(ctx):
+    """Mixed Tsit5 (FP32 stage derivatives, FP64 state and error control, electrons with a stage
+    within restep_radius integrated again in FP64): every column gets its record, at its own index."""
+    kw = small_ctmc(n_t=60, n_kd=40, traj_phase_method="SCTS", traj_integrator="tsit5", traj_rtol=1e-8)
+    ld, td, sd, rd, cols = _columns(dict(kw, precision="mixed"), range(8, 60, 4))
+    rd64 = descriptors(dict(kw, precision="fp64"))[3]
+    rec, st = _device_records(ctx, ld, td, rd, cols)
+    r64, s64 = _device_records(ctx, ld, td, rd64, cols)
+    assert (st != s64).mean() <= 0.01
+    err, both = _momentum_errors(rec, st, r64, s64)
+    # the FP32 derivatives change every step size slightly, so the two runs differ by the
+    # integrator's own truncation error: the same statistics for FP64 at rtol 1e-8 against 1e-10
+    r10, s10 = _device_records(ctx, ld, td, descriptors(dict(kw, precision="fp64", traj_rtol=1e-10))[3], cols)
+    trunc, _ = _momentum_errors(r64, s64, r10, s10)
+    print("mixed tsit5 median", np.median(err), "p95", np.quantile(err, 0.95),
+          "| fp64 rtol 1e-8 vs 1e-10: median", np.median(trunc), "p95", np.quantile(trunc, 0.95))
+    assert np.median(err) <= 3 * np.median(trunc) + 1e-9
+    assert np.quantile(err, 0.95) <= 3 * np.quantile(trunc, 0.95) + 1e-9
+    # re-integration radius beyond every orbit: all electrons are re-integrated, records equal FP64
+    rall = descriptors(dict(kw, precision="mixed", restep_radius=1e6))[3]
+    ra, sa = _device_records(ctx, ld, td, rall, cols)
+    np.testing.assert_array_equal(sa, s64)
+    np.testing.assert_array_equal(ra, r64)
+
+
+@pytest.mark.parametrize("phase", ["CTMC", "QTMC"])
+def test_tsit5_mixed_run_job_pmd(ctx, phase):
+    kw = small_ctmc(n_t=100, n_kd=32, traj_phase_method=phase, traj_integrator="tsit5", traj_rtol=1e-8,
+                    pmd_accumulator="fixed")
+    ld, td, sd, rd = descriptors(dict(kw, precision="mixed"))
+    got, st = _device_job(ctx, ld, td, sd, rd)
+    ref, rst = _device_job(ctx, ld, td, sd, descriptors(dict(kw, precision="fp64"))[3])
+    err = l1(got, ref)
+    print(phase, "mixed tsit5 PMD L1 vs fp64", err, "restep", st["n_restep"], "steps", st["n_steps"],
+          st["n_steps_fp64"])
+    assert st["num_effective_traj"] == rst["num_effective_traj"]
+    assert 0 < st["n_restep"] < st["num_effective_traj"]
+    assert err <= 1e-3
+    # everything re-integrated: the FP64 result, bit for bit (fixed-point sums)
+    allin, sta = _device_job(ctx, ld, td, sd, descriptors(dict(kw, precision="mixed", restep_radius=1e6))[3])
+    assert sta["n_restep"] == sta["num_effective_traj"]
+    assert allin.tobytes() == ref.tobytes()
 
 
 def test_mixed_flags_close_encounters_for_fp64(ctx):

@@ -87,11 +87,15 @@ target_desc(t::GenericMolecule) = TargetDesc(1, AsympNuclCharge(t), 0, 0, IonPot
 # pmd_mode :fixed = ETRAJ_PMD_FIXED (order-independent 128-bit fixed-point sums); hit_counts: C_NULL or
 # the pointer of a Vector{UInt64} of prod(final_p_num) cells (kept alive by the caller) that receives
 # the number of electrons binned into each cell
+# integrator :rk4 (fixed step dt) or :tsit5 (adaptive, reltol `rtol`, abstol 1e-6 as OrdinaryDiffEq's default)
 function run_desc(dimension, phase, traj_t_final, final_p_max, final_p_num; dt=0.1, precision=:mixed,
-                  restep_radius=2.0, reproduce_quirks=true, pmd_mode=:float, hit_counts=Ptr{UInt64}(C_NULL))
+                  restep_radius=2.0, reproduce_quirks=true, pmd_mode=:float, hit_counts=Ptr{UInt64}(C_NULL),
+                  integrator=:rk4, rtol=1e-6)
     pmax = ntuple(i -> i <= dimension ? Float64(final_p_max[i]) : 1.0, 3)
     pnum = ntuple(i -> i <= dimension ? Int32(final_p_num[i]) : Int32(1), 3)
-    RunDesc(dimension, PHASE[phase], 0, precision == :mixed ? 1 : 0, traj_t_final, dt, 1e-6, 1e-6, restep_radius,
+    integ = integrator == :tsit5 ? Int32(1) : integrator == :rk4 ? Int32(0) :
+            error("[eTrajHIP] unknown integrator $integrator")
+    RunDesc(dimension, PHASE[phase], integ, precision == :mixed ? 1 : 0, traj_t_final, dt, rtol, 1e-6, restep_radius,
             pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000, pmd_mode == :fixed ? 1 : 0, 0, hit_counts)
 end
 
@@ -121,13 +125,14 @@ thread_ctx() = lock(CTX_LOCK) do
 end
 
 # ---- fine path: drop-in for the per-batch body of launch_and_collect! ----------
-function launch_and_collect_hip!(job, batch_id::Integer; dt=0.1, precision=:mixed, restep_radius=2.0)
+function launch_and_collect_hip!(job, batch_id::Integer; dt=0.1, precision=:mixed, restep_radius=2.0,
+                                 integrator=:rk4)
     (job.batch_completion[batch_id] == 1) && return
     init = gen_electron_batch(job.sp, batch_id)                         # the reference sampler, unchanged
     isnothing(init) && (job.batch_completion[batch_id] = 1; return)
     tid = Threads.threadid()
     rd = Ref(run_desc(job.dimension, job.traj_phase_method, job.traj_t_final, job.final_p_max, job.final_p_num;
-                      dt, precision, restep_radius))
+                      dt, precision, restep_radius, integrator, rtol=job.traj_rtol))
     ld = Ref(laser_desc(job.laser)); td = Ref(target_desc(job.target))
     ip = Ref(0.0); iu = Ref(0.0); nf = Ref(Int64(0))
     spec = job.spec_collect[tid]                                        # Float64 or ComplexF64 array, px fastest
@@ -195,12 +200,14 @@ end
 function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max, final_p_num,
                      ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, traj_phase_method=:CTMC,
                      sample_cutoff_limit=1e-16, rate_prefix=:Full, dt=0.1, precision=:mixed, restep_radius=2.0,
-                     init_cond_method=:ADK, mol_orbit_ridx=0, device=0, kwargs...)
+                     init_cond_method=:ADK, mol_orbit_ridx=0, device=0, integrator=:rk4, traj_rtol=1e-6,
+                     pmd_mode=:float, kwargs...)
     sdv, keep = sampler_desc(; init_cond_method, dimension, sample_t_intv, sample_t_num, ss_kd_max, ss_kd_num,
                              ss_kz_max, ss_kz_num, traj_phase_method, sample_cutoff_limit, rate_prefix, target,
                              mol_orbit_ridx, kwargs...)
     sd = Ref(sdv)
-    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
+    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius,
+                      integrator, rtol=traj_rtol, pmd_mode))
     spec = zeros(Float64, final_p_num...)
     st = Stats()
     c = Ctx(device)
@@ -219,12 +226,13 @@ end
 # sampler keyword sets differ per point, target and run settings are shared.
 function run_scan_hip(lasers::AbstractVector; target, dimension, sample_t_intv, sample_t_num, traj_t_final,
                       final_p_max, final_p_num, traj_phase_method=:CTMC, dt=0.1, precision=:mixed,
-                      restep_radius=2.0, device=0, kwargs...)
+                      restep_radius=2.0, device=0, integrator=:rk4, traj_rtol=1e-6, pmd_mode=:float, kwargs...)
     n = length(lasers)
     lds = [laser_desc(l) for l in lasers]
     sdk = [sampler_desc(; dimension, sample_t_intv, sample_t_num, traj_phase_method, target, kwargs...) for _ in 1:n]
     sds = first.(sdk); keeps = last.(sdk)
-    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius))
+    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision, restep_radius,
+                      integrator, rtol=traj_rtol, pmd_mode))
     npix = prod(final_p_num)
     specs = zeros(Float64, npix, n)
     stv = Vector{NTuple{13,Float64}}(undef, n)   # etraj_stats is 13 eight-byte fields
@@ -247,7 +255,9 @@ end
 #                   with the PMD that already holds abs2 for QTMC / SCTS).
 #   coarse = false  the reference sampler makes every batch and launch_and_collect_hip!
 #                   integrates it; the reference's own write_output! writes the file.
-# Keywords and defaults are the reference's (:105-137) plus the device settings.
+# Keywords and defaults are the reference's (:105-137) plus the device settings: dt, precision
+# (:mixed / :fp64), restep_radius, seed, device, integrator (:rk4, or :tsit5 at traj_rtol, the
+# reference's integrator) and pmd_mode (:float / :fixed).
 function perform_traj_simulation_hip(; init_cond_method::Symbol, laser::Laser, target::Target, dimension::Integer,
         sample_t_intv, sample_t_num::Integer, traj_t_final, final_p_max, final_p_num,
         ss_kd_max::Real=0.0, ss_kd_num::Integer=0, ss_kz_max::Real=0.0, ss_kz_num::Integer=0,
@@ -256,7 +266,7 @@ function perform_traj_simulation_hip(; init_cond_method::Symbol, laser::Laser, t
         output_compress::Bool=true, output_path::String=parentmodule(@__MODULE__).default_filename(),
         sample_monte_carlo::Bool=false, sample_cutoff_limit::Real=1e-16, rate_prefix=:Full, mol_orbit_ridx=0,
         show_progress::Bool=false, coarse::Bool=true, dt=0.1, precision=:mixed, restep_radius=2.0, seed=0,
-        device=0)
+        device=0, integrator=:rk4, pmd_mode=:float)
     ref = parentmodule(@__MODULE__)
     job = ref.TrajectorySimulationJob(; init_cond_method, laser, target, dimension, sample_t_intv, sample_t_num,
         traj_t_final, final_p_max, final_p_num, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, mc_kt_num, mc_kd_max,
@@ -266,11 +276,12 @@ function perform_traj_simulation_hip(; init_cond_method::Symbol, laser::Laser, t
         res = run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max,
                           final_p_num, ss_kd_max, ss_kd_num, ss_kz_max, ss_kz_num, traj_phase_method,
                           sample_cutoff_limit, rate_prefix, dt, precision, restep_radius, init_cond_method,
-                          mol_orbit_ridx, device, sample_monte_carlo, mc_kt_num, mc_kd_max, mc_kz_max, seed)
+                          mol_orbit_ridx, device, integrator, traj_rtol, pmd_mode, sample_monte_carlo, mc_kt_num,
+                          mc_kd_max, mc_kz_max, seed)
         write_output_hip!(job, res)
     else
         Threads.@threads for i in 1:batch_num(job.sp)
-            launch_and_collect_hip!(job, i; dt, precision, restep_radius)
+            launch_and_collect_hip!(job, i; dt, precision, restep_radius, integrator)
         end
         ref.write_output!(job)
     end

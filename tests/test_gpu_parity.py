@@ -360,3 +360,29 @@ def test_mixed_flags_close_encounters_for_fp64(ctx):
     rec64, st64 = _device_records(ctx, ld, td, rd64, cols)
     np.testing.assert_array_equal(st, st64)
     np.testing.assert_array_equal(rec[st == 0], rec64[st64 == 0])
+
+
+def test_tsit5_mixed_integrate_and_bin_matches_records(ctx):
+    """The batch drop-in with mixed Tsit5: the flagged electrons' contributions come from the FP64
+    launch at their own columns, so the host-binned PMD equals the histogram of the records."""
+    kw = small_ctmc(n_t=60, n_kd=40, traj_integrator="tsit5", precision="mixed", restep_radius=3.0)
+    ld, td, sd, rd, cols = _columns(kw, range(10, 50, 3))
+    npix = 500 * 500
+    pmd = np.zeros(npix)
+    ip, iu, nf = _abi.C.c_double(0.0), _abi.C.c_double(0.0), _abi.C.c_int64(0)
+    ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ld, td, rd, _abi.dptr(np.ascontiguousarray(cols)),
+                                              cols.shape[0], cols.shape[1], pmd.ctypes.data_as(_abi.C.c_void_p),
+                                              _abi.C.byref(ip), _abi.C.byref(iu), _abi.C.byref(nf)))
+    rec, st = _device_records(ctx, ld, td, rd, cols)
+    ok = st == 0
+    idx = []
+    for d in range(2):
+        q = (rec[:, 8 + d] + 1.5) / (1.5 / 500 * 2)
+        i = np.where(np.abs(q) < 9.0e15, np.rint(q), 0).astype(np.int64)
+        ok &= (i >= 1) & (i <= 500)
+        idx.append(i)
+    lin = (idx[0] - 1) + 500 * (idx[1] - 1)
+    expect = np.zeros(npix)
+    np.add.at(expect, lin[ok], cols[ok, 5])          # CTMC: the weight row
+    assert nf.value == int((st == 1).sum())
+    assert l1(pmd, expect) <= 1e-13

@@ -279,3 +279,32 @@ def test_sfa_run_job_matches_oracle(ctx, method):
     ref, rst = O.run_job(ld, td, sd, rd, nthreads=4)
     assert st.num_effective_traj == rst["num_effective_traj"] > 0
     assert l1(out, ref) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,kd_num,kz_num", [(2, 3, 1), (3, 7, 5), (3, 2, 1)], ids=["2d-nk3", "3d-nk35", "3d-nk2"])
+def test_spa_small_batches_run_job_matches_oracle(ctx, dim, kd_num, kz_num):
+    """SFA-SPA batches far smaller than one wavefront (2 to 35 samples), in 2D
+    and 3D, with fixed-point PMD sums: the same electrons, ion probability and
+    PMD as the oracle."""
+    w = 45.563352525 / 800.0
+    half = 2 * math.pi / w
+    p_num = (40,) * dim
+    kw = dict(laser=Cos2Laser(peak_int=2e14, wave_len=800.0, cyc_num=2, ellip=1.0), target=get_atom("H"),
+              dimension=dim, sample_t_intv=(-half, half), sample_t_num=50, traj_t_final=half + 10.0,
+              final_p_max=(1.5,) * dim, final_p_num=p_num, ss_kd_max=1.2, ss_kd_num=kd_num,
+              traj_phase_method="QTMC", rate_prefix="Full", sample_cutoff_limit=1e-16, traj_dt=0.1,
+              precision="fp64", pmd_accumulator="fixed")
+    if dim == 3:
+        kw.update(ss_kz_max=1.0, ss_kz_num=kz_num)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sp = init_sampler(init_cond_method="SPA", **kw)
+    ld, td, sd, rd = kw["laser"]._desc(), kw["target"]._desc(), sp.desc, run_desc(**kw)
+    out = np.empty(int(np.prod(p_num)))
+    st = _abi.Stats()
+    ctx.check(ctx.lib.etraj_run_job(ctx.handle, ld, td, sd, rd, _abi.dptr(out), _abi.C.byref(st)))
+    ref, rst = O.run_job(ld, td, sd, rd, nthreads=4)
+    assert st.num_effective_traj == rst["num_effective_traj"] > 0
+    assert st.ion_prob == pytest.approx(rst["ion_prob"], rel=1e-9)
+    assert l1(out, ref) <= 1e-6

@@ -292,12 +292,9 @@ def bench_workload(name, args, ctx):
         ints = list(np.logspace(np.log10(5e13), np.log10(5e14), 32))
         probe, _ = _scan_descs(workloads.bichromatic_scan_jobs(ints, n_t=200, n_k=200))
         _, pst = _run_scan(ctx, probe)
-        y = np.array([s.ion_prob for s in pst])
-        surv = np.array([max(s.num_effective_traj, 1) / (200 * 200) for s in pst])
-        want = np.clip(1e8 * y / y.sum(), 1e5, 2e7)
-        n_t = 2000
-        n_k = np.maximum(1, np.ceil(want / (n_t * surv))).astype(int)
-        jobs = workloads.bichromatic_scan_jobs(ints, n_t=n_t, n_k=n_k)
+        n_k = workloads.bichromatic_scan_counts([s.ion_prob for s in pst], [s.num_effective_traj for s in pst],
+                                                200 * 200)
+        jobs = workloads.bichromatic_scan_jobs(ints, n_t=2000, n_k=n_k)
         desc = ("32 intensities 5e13-5e14 W/cm^2, bichromatic counter-rotating CP 800+400 nm 8/16 cycles, "
                 "H ADK+SCTS, counts per intensity proportional to the ADK yield (1e5-2e7), 2D PMD 512x512 each")
     elif name == "wfat-orientation-scan":

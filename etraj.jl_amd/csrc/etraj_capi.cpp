@@ -92,6 +92,12 @@ struct etraj_ctx {
     DevBuf<int64_t> orig;
     DevBuf<Contrib> contrib;
     DevBuf<uint64_t> pmdq, scalq, hits;   // ETRAJ_PMD_FIXED sums, hit counts
+    // host staging of integrate_columns: kept in the ctx so that every asynchronous
+    // host-to-device copy reads live memory until the call synchronises the stream
+    std::vector<int64_t> h_ss, h_pre;
+    std::vector<int32_t> h_sc;
+    std::vector<double> h_t0;
+    std::vector<Contrib> h_contrib;   // per-electron records copied back for host binning
 };
 
 namespace {
@@ -388,12 +394,6 @@ int64_t steps_of(double tf, double t0, double dt) {
     return n < 1 ? 1 : n;
 }
 
-// Electrons per Tsit5 wavefront. With more than 64 a lane whose electron has
-// finished takes the next one of its wavefront; measured on the Ar SPA 3D grid
-// (9.6M electrons), one electron per lane and the hardware's own wave
-// scheduling is fastest: 39.7 ms at 64, 40.4 at 128, 44.1 at 512, 53.9 at 2048.
-constexpr int kTsit5PerWave = 64;
-
 constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
 // Bytes of field table per launch and init columns per slab. Slabs this large
 // (per buffer slot: 4.8 GB of init columns at 9 rows, 8 GiB of field table)
@@ -402,16 +402,63 @@ constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
 constexpr int64_t kFieldTabBudget = (int64_t)8 << 30;
 constexpr int64_t kMaxSlabCols = (int64_t)1 << 26;
 
-// Allocates and fills the field table of nseg segments whose t0 are in
-// t0s_dev and whose lasers are lps_dev[seg_job_dev[s]] (seg_job_dev NULL: lps_dev[0]).
-int fill_field_table(etraj_ctx* c, const IntegP& P, const LaserP* lps_dev, const int32_t* seg_job_dev,
+// Device buffers of integrate_columns: the ctx's own (grown on demand) or
+// fixed slices of a preallocated region (`fixed`, with the field table's
+// capacity in RK4 steps).
+struct ColBufs {
+    bool fixed = false;
+    double* init = nullptr;
+    int64_t* seg_start = nullptr;
+    int32_t* counts = nullptr;
+    int32_t* counts2 = nullptr;
+    double* seg_t0 = nullptr;
+    int64_t* pre = nullptr;
+    uint8_t* flags = nullptr;
+    int64_t* orig = nullptr;
+    LaserP* lps = nullptr;
+    float4* ft32 = nullptr;
+    double2* ft64 = nullptr;
+    double2* ft0 = nullptr;
+    int64_t ft_steps = 0;   // fixed: field-table capacity (segments x stride), in steps
+    int64_t ft_segs = 0;    // fixed: capacity of ft0, in segments
+};
+
+// Grows the ctx's buffers of a call over n columns and nseg segments (a no-op
+// for fixed buffers, whose sizes the caller has checked).
+int col_bufs_ensure(etraj_ctx* c, ColBufs& B, int64_t n, int64_t nseg, int rows) {
+    if (B.fixed) return ETRAJ_OK;
+    HIP_TRY(c, c->init.ensure((size_t)n * rows));
+    HIP_TRY(c, c->seg_start.ensure(nseg));
+    HIP_TRY(c, c->counts.ensure(nseg));
+    HIP_TRY(c, c->counts2.ensure(nseg));
+    HIP_TRY(c, c->seg_t0.ensure(nseg));
+    HIP_TRY(c, c->pre.ensure(nseg + 1));
+    HIP_TRY(c, c->flags.ensure((size_t)n));
+    HIP_TRY(c, c->orig.ensure((size_t)n));
+    HIP_TRY(c, c->lps.ensure(1));
+    B.init = c->init.p; B.seg_start = c->seg_start.p; B.counts = c->counts.p; B.counts2 = c->counts2.p;
+    B.seg_t0 = c->seg_t0.p; B.pre = c->pre.p; B.flags = c->flags.p; B.orig = c->orig.p; B.lps = c->lps.p;
+    return ETRAJ_OK;
+}
+
+// Fills the field table of nseg segments whose t0 are in t0s_dev and whose
+// lasers are lps_dev[seg_job_dev[s]] (seg_job_dev NULL: lps_dev[0]); the ctx's
+// table buffers are grown to fit unless B is fixed.
+int fill_field_table(etraj_ctx* c, ColBufs& B, const IntegP& P, const LaserP* lps_dev, const int32_t* seg_job_dev,
                      const double* t0s_dev, int64_t nseg, int64_t stride, FieldTab& FT) {
-    HIP_TRY(c, c->ft32.ensure((size_t)nseg * stride));
-    HIP_TRY(c, c->ft64.ensure((size_t)2 * nseg * stride));
-    HIP_TRY(c, c->ft0.ensure((size_t)nseg));
-    FT.f32 = c->ft32.p;
-    FT.f64 = c->ft64.p;
-    FT.f0 = c->ft0.p;
+    if (!B.fixed) {
+        HIP_TRY(c, c->ft32.ensure((size_t)nseg * stride));
+        HIP_TRY(c, c->ft64.ensure((size_t)2 * nseg * stride));
+        HIP_TRY(c, c->ft0.ensure((size_t)nseg));
+        B.ft32 = c->ft32.p;
+        B.ft64 = c->ft64.p;
+        B.ft0 = c->ft0.p;
+    } else if (nseg * stride > B.ft_steps || nseg > B.ft_segs) {
+        return fail(c, ETRAJ_ECAPACITY, "field table of the segment group exceeds the fixed buffers");
+    }
+    FT.f32 = B.ft32;
+    FT.f64 = B.ft64;
+    FT.f0 = B.ft0;
     FT.stride = stride;
     HIP_TRY(c, launch_field_table(c->stream, lps_dev, seg_job_dev, t0s_dev, nseg, P.tf, P.dt, FT));
     return ETRAJ_OK;
@@ -419,27 +466,22 @@ int fill_field_table(etraj_ctx* c, const IntegP& P, const LaserP* lps_dev, const
 
 // Fixed-step RK4 over nseg segments. FP64: one pass. MIXED: the FP32 pass
 // flags the electrons that come within restep_radius of the core, which are
-// compacted in place per segment and integrated again in FP64. `ncols` is the
-// number of columns the init buffer spans (for the flag array).
-int rk4_passes(etraj_ctx* c, const IntegP& P, const OutP& O, int out, const int64_t* seg_start, int32_t* counts,
-               int64_t nseg, int cps, const FieldTab& FT, int64_t ncols, hipEvent_t after_main = nullptr) {
+// compacted in place per segment and integrated again in FP64 (their outputs
+// at their own indices through B.orig unless out is OUT_PMD).
+int rk4_passes(etraj_ctx* c, const ColBufs& B, const IntegP& P, const OutP& O, int out, const int64_t* seg_start,
+               int32_t* counts, int32_t* counts2, int64_t nseg, int cps, const FieldTab& FT) {
     if (!P.mixed) {
-        HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, nullptr, nullptr));
-        if (after_main) HIP_TRY(c, hipEventRecord(after_main, c->stream));
+        HIP_TRY(c, launch_rk4(c->stream, P, O, out, B.init, seg_start, counts, nseg, cps, FT, nullptr, nullptr));
         return ETRAJ_OK;
     }
-    HIP_TRY(c, c->flags.ensure((size_t)ncols));
-    HIP_TRY(c, c->counts2.ensure((size_t)nseg));
     const bool need_orig = out != OUT_PMD;
-    if (need_orig) HIP_TRY(c, c->orig.ensure((size_t)ncols));
-    HIP_TRY(c, launch_rk4(c->stream, P, O, out, c->init.p, seg_start, counts, nseg, cps, FT, c->flags.p, nullptr));
-    if (after_main) HIP_TRY(c, hipEventRecord(after_main, c->stream));
-    HIP_TRY(c, launch_compact_flagged(c->stream, c->init.p, P.rows, seg_start, counts, nseg, c->flags.p, nullptr,
-                                      c->counts2.p, need_orig ? c->orig.p : nullptr));
+    HIP_TRY(c, launch_rk4(c->stream, P, O, out, B.init, seg_start, counts, nseg, cps, FT, B.flags, nullptr));
+    HIP_TRY(c, launch_compact_flagged(c->stream, B.init, P.rows, seg_start, counts, nseg, B.flags, nullptr, counts2,
+                                      need_orig ? B.orig : nullptr));
     IntegP P64 = P;
     P64.mixed = 0;
-    HIP_TRY(c, launch_rk4(c->stream, P64, O, out, c->init.p, seg_start, c->counts2.p, nseg, cps, FT, nullptr,
-                          need_orig ? c->orig.p : nullptr));
+    HIP_TRY(c, launch_rk4(c->stream, P64, O, out, B.init, seg_start, counts2, nseg, cps, FT, nullptr,
+                          need_orig ? B.orig : nullptr));
     return ETRAJ_OK;
 }
 
@@ -447,22 +489,23 @@ int64_t npix_of(const IntegP& P) { return (int64_t)P.pnum[0] * P.pnum[1] * (P.di
 
 // Segments of equal t0 in a Julia init matrix (columns of one
 // gen_electron_batch share tr), so that each wavefront shares its field.
-void build_segments(const double* init, int64_t n, int rows, int dim, std::vector<int64_t>& start,
-                    std::vector<int32_t>& count, int& cps) {
-    start.clear();
-    count.clear();
-    int64_t maxlen = 0;
+// start / count receive the segments (capacity >= n); returns their number.
+int64_t build_segments(const double* init, int64_t n, int rows, int dim, int64_t* start, int32_t* count,
+                       int64_t& maxlen) {
+    int64_t nseg = 0;
+    maxlen = 0;
     int64_t i = 0;
     while (i < n) {
         int64_t j = i + 1;
         const double t0 = init[i * rows + 2 * dim];
         while (j < n && init[j * rows + 2 * dim] == t0 && j - i < ((int64_t)1 << 30)) ++j;
-        start.push_back(i);
-        count.push_back((int32_t)(j - i));
+        start[nseg] = i;
+        count[nseg] = (int32_t)(j - i);
+        ++nseg;
         maxlen = std::max(maxlen, j - i);
         i = j;
     }
-    cps = (int)((maxlen + 63) / 64);
+    return nseg;
 }
 
 int check_init(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int32_t rows) {
@@ -477,73 +520,77 @@ int check_init(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int
     return ETRAJ_OK;
 }
 
-// integrate n columns; out = OUT_RECORDS or OUT_CONTRIB. Segments are
-// processed in groups whose field tables fit kFieldTabBudget.
-int integrate_columns(etraj_ctx* c, const IntegP& P, const double* init, int64_t n, int out, OutP& O) {
-    std::vector<int64_t> ss;
-    std::vector<int32_t> sc;
-    int cps_all = 0;
-    build_segments(init, n, P.rows, P.dim, ss, sc, cps_all);
-    const int64_t nseg = (int64_t)ss.size();
+// Integrates n columns; out = OUT_RECORDS or OUT_CONTRIB. The host staging
+// arrays are the ctx's (h_ss, h_sc, h_pre, h_t0: capacity >= n + 1 on entry).
+// RK4 segments are processed in groups whose field tables fit the budget
+// (kFieldTabBudget, or the fixed buffers' capacity).
+int integrate_columns(etraj_ctx* c, ColBufs& B, const IntegP& P, const double* init, int64_t n, int out, OutP& O) {
+    int64_t* ss = c->h_ss.data();
+    int32_t* sc = c->h_sc.data();
+    int64_t maxlen = 0;
+    const int64_t nseg = build_segments(init, n, P.rows, P.dim, ss, sc, maxlen);
+    if (int rc = col_bufs_ensure(c, B, n, nseg, P.rows)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(B.init, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.seg_start, ss, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.counts, sc, sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
     if (P.integrator == ETRAJ_INTEG_TSIT5) {
-        std::vector<int64_t> pre(nseg + 1, 0);
+        int64_t* pre = c->h_pre.data();
+        pre[0] = 0;
         for (int64_t k = 0; k < nseg; ++k) pre[k + 1] = pre[k] + sc[k];
-        HIP_TRY(c, c->init.ensure((size_t)n * P.rows));
-        HIP_TRY(c, c->seg_start.ensure(nseg));
-        HIP_TRY(c, c->pre.ensure(nseg + 1));
-        HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(c->pre.p, pre.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(B.pre, pre, sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
         if (!P.mixed) {
-            HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
-                                    nullptr, nullptr));
+            HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, nullptr, nullptr));
             return ETRAJ_OK;
         }
         // mixed: FP32 stage derivatives; the flagged electrons are compacted per
         // segment and integrated again in FP64, their outputs at their own columns
-        HIP_TRY(c, c->flags.ensure((size_t)n));
-        HIP_TRY(c, c->counts.ensure(nseg));
-        HIP_TRY(c, c->counts2.ensure(nseg));
-        HIP_TRY(c, c->orig.ensure((size_t)n));
-        HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
-                                c->flags.p, nullptr));
-        HIP_TRY(c, launch_compact_flagged(c->stream, c->init.p, P.rows, c->seg_start.p, c->counts.p, nseg, c->flags.p,
-                                          nullptr, c->counts2.p, c->orig.p));
-        HIP_TRY(c, launch_scan_counts(c->stream, c->counts2.p, nseg, c->pre.p));
+        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, B.flags, nullptr));
+        HIP_TRY(c, launch_compact_flagged(c->stream, B.init, P.rows, B.seg_start, B.counts, nseg, B.flags, nullptr,
+                                          B.counts2, B.orig));
+        HIP_TRY(c, launch_scan_counts(c->stream, B.counts2, nseg, B.pre));
         IntegP P64 = P;
         P64.mixed = 0;
-        HIP_TRY(c, launch_tsit5(c->stream, P64, O, out, c->init.p, c->seg_start.p, c->pre.p, nseg, n, kTsit5PerWave,
-                                nullptr, c->orig.p));
+        HIP_TRY(c, launch_tsit5(c->stream, P64, O, out, B.init, B.seg_start, B.pre, nseg, n, nullptr, B.orig));
         return ETRAJ_OK;
     }
-    std::vector<double> t0(nseg);
+    double* t0 = c->h_t0.data();
     int64_t stride = 1;
     for (int64_t k = 0; k < nseg; ++k) {
         t0[k] = init[ss[k] * P.rows + 2 * P.dim];
         stride = std::max(stride, steps_of(P.tf, t0[k], P.dt));
     }
-    const int64_t group = std::max<int64_t>(1, std::min<int64_t>(nseg, kFieldTabBudget / (stride * kFieldTabBytesPerStep)));
-    HIP_TRY(c, c->init.ensure((size_t)n * P.rows));
-    HIP_TRY(c, c->seg_start.ensure(nseg));
-    HIP_TRY(c, c->counts.ensure(nseg));
-    HIP_TRY(c, c->seg_t0.ensure(nseg));
-    HIP_TRY(c, hipMemcpyAsync(c->init.p, init, sizeof(double) * n * P.rows, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->seg_start.p, ss.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->counts.p, sc.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->seg_t0.p, t0.data(), sizeof(double) * nseg, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, c->lps.ensure(1));
-    HIP_TRY(c, hipMemcpyAsync(c->lps.p, &P.L, sizeof(LaserP), hipMemcpyHostToDevice, c->stream));
+    const int64_t budget = B.fixed ? B.ft_steps : kFieldTabBudget / kFieldTabBytesPerStep;
+    int64_t group = std::min<int64_t>(nseg, budget / stride);
+    if (B.fixed) group = std::min(group, B.ft_segs);
+    if (group < 1) {
+        if (B.fixed)
+            return fail(c, ETRAJ_ECAPACITY, "one segment's field table (" + std::to_string(stride) +
+                                                 " steps) exceeds the fixed buffers");
+        group = 1;
+    }
+    HIP_TRY(c, hipMemcpyAsync(B.seg_t0, t0, sizeof(double) * nseg, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.lps, &P.L, sizeof(LaserP), hipMemcpyHostToDevice, c->stream));
     for (int64_t g0 = 0; g0 < nseg; g0 += group) {
         const int64_t ng = std::min(group, nseg - g0);
         int32_t maxc = 0;
         for (int64_t k = g0; k < g0 + ng; ++k) maxc = std::max(maxc, sc[k]);
         FieldTab FT{};
-        if (int rc = fill_field_table(c, P, c->lps.p, nullptr, c->seg_t0.p + g0, ng, stride, FT)) return rc;
-        if (int rc = rk4_passes(c, P, O, out, c->seg_start.p + g0, c->counts.p + g0, ng, (int)((maxc + 63) / 64), FT, n))
+        if (int rc = fill_field_table(c, B, P, B.lps, nullptr, B.seg_t0 + g0, ng, stride, FT)) return rc;
+        if (int rc = rk4_passes(c, B, P, O, out, B.seg_start + g0, B.counts + g0, B.counts2 + g0, ng,
+                                (int)((maxc + 63) / 64), FT))
             return rc;
     }
     return ETRAJ_OK;
+}
+
+// Grows the ctx's host staging of integrate_columns to n columns (no-op within capacity).
+void host_staging_reserve(etraj_ctx* c, int64_t n) {
+    if ((int64_t)c->h_ss.size() < n + 1) {
+        c->h_ss.resize(n + 1);
+        c->h_sc.resize(n + 1);
+        c->h_pre.resize(n + 1);
+        c->h_t0.resize(n + 1);
+    }
 }
 
 }  // namespace
@@ -737,11 +784,36 @@ int etraj_integrate_records(etraj_ctx* c, const etraj_laser_desc* laser, const e
     OutP O{};
     O.records = c->io1.p;
     O.status = c->status.p;
-    if (int rc = integrate_columns(c, P, init, n, OUT_RECORDS, O)) return rc;
+    ColBufs B;
+    host_staging_reserve(c, n);
+    if (int rc = integrate_columns(c, B, P, init, n, OUT_RECORDS, O)) return rc;
     HIP_TRY(c, hipMemcpyAsync(records, c->io1.p, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(status, c->status.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ETRAJ_OK;
+}
+
+// Adds the per-electron records h[0..n) into the caller's PMD and counters in
+// column order, as the reference's per-electron loop (TrajectorySimulation.jl:344-397).
+static void bin_contribs_host(const IntegP& P, const etraj_run_desc* run, const Contrib* h, int64_t n, void* pmd,
+                              double* ion_prob, double* ion_prob_uncollected, int64_t* n_failed) {
+    double* out = static_cast<double*>(pmd);
+    int64_t nf = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const Contrib& k = h[i];
+        if (k.status == 1) { ++nf; continue; }
+        if (k.status == 2) continue;
+        *ion_prob += k.prob;
+        if (k.status == 3) { *ion_prob_uncollected += k.prob; continue; }
+        if (run->hit_counts) run->hit_counts[k.lin] += 1;
+        if (P.phase == ETRAJ_PHASE_CTMC) {
+            out[k.lin] += k.re;
+        } else {
+            out[2 * k.lin] += k.re;
+            out[2 * k.lin + 1] += k.im;
+        }
+    }
+    *n_failed = nf;
 }
 
 int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
@@ -763,29 +835,13 @@ int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const e
     HIP_TRY(c, c->contrib.ensure(n));
     OutP O{};
     O.contrib = c->contrib.p;
-    if (int rc = integrate_columns(c, P, init, n, OUT_CONTRIB, O)) return rc;
-    std::vector<Contrib> h(n);
-    HIP_TRY(c, hipMemcpyAsync(h.data(), c->contrib.p, sizeof(Contrib) * n, hipMemcpyDeviceToHost, c->stream));
+    ColBufs B;
+    host_staging_reserve(c, n);
+    if (int rc = integrate_columns(c, B, P, init, n, OUT_CONTRIB, O)) return rc;
+    if ((int64_t)c->h_contrib.size() < n) c->h_contrib.resize(n);
+    HIP_TRY(c, hipMemcpyAsync(c->h_contrib.data(), c->contrib.p, sizeof(Contrib) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    // accumulate in column order, as the reference's per-electron loop
-    // (TrajectorySimulation.jl:344-397)
-    double* out = static_cast<double*>(pmd);
-    int64_t nf = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        const Contrib& k = h[i];
-        if (k.status == 1) { ++nf; continue; }
-        if (k.status == 2) continue;
-        *ion_prob += k.prob;
-        if (k.status == 3) { *ion_prob_uncollected += k.prob; continue; }
-        if (run->hit_counts) run->hit_counts[k.lin] += 1;
-        if (P.phase == ETRAJ_PHASE_CTMC) {
-            out[k.lin] += k.re;
-        } else {
-            out[2 * k.lin] += k.re;
-            out[2 * k.lin + 1] += k.im;
-        }
-    }
-    *n_failed = nf;
+    bin_contribs_host(P, run, c->h_contrib.data(), n, pmd, ion_prob, ion_prob_uncollected, n_failed);
     return ETRAJ_OK;
 }
 
@@ -1076,7 +1132,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             if (O.hits) Oj.hits = O.hits + j * npix;
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave,
+                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols,
                                  P.mixed ? Q.flags.p : nullptr, nullptr);
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
             if (P.mixed) {   // the flagged electrons again in FP64, on the same stream
@@ -1087,7 +1143,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                                                Q.counts2.p, nullptr);
                 if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
                 if (e == hipSuccess)
-                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, kTsit5PerWave,
+                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols,
                                      nullptr, nullptr);
             }
         }
@@ -1190,6 +1246,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             s.ms_integrate_main = ms_m;
             s.ms_total = ms_total;
             s.n_slabs = nslab;
+            s.n_attempts = cj[CNT_ATTEMPTS];
         }
     }
     return ETRAJ_OK;

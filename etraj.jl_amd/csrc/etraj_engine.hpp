@@ -78,9 +78,10 @@ enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
 // CNT_RESTEP counts the electrons of the mixed pass re-integrated in FP64
 // CNT_LAUNCHED counts the sampled electrons (the sampler kernel adds each batch's survivors)
 // CNT_FIXRANGE counts values too large for the fixed-point accumulator
+// CNT_ATTEMPTS counts integrator step attempts (Tsit5: accepted + rejected; RK4: = CNT_STEPS)
 enum {
     CNT_FAILED = 0, CNT_BOUND = 1, CNT_RESTEP = 2, CNT_STEPS = 3, CNT_STEPS64 = 4, CNT_LAUNCHED = 5,
-    CNT_FIXRANGE = 6, CNT_N = 7
+    CNT_FIXRANGE = 6, CNT_ATTEMPTS = 7, CNT_N = 8
 };
 
 // Order-independent sums (ETRAJ_PMD_FIXED): a value v is held as the 128-bit
@@ -221,11 +222,12 @@ hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* 
 // to integrate again in FP64 and 0 for the others. orig (may be null) maps a
 // column to its output index.
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave,
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
                         uint8_t* flags, const int64_t* orig);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
 // out[i] = fix_to_double(q[2i], q[2i+1]), i < n
 hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out);
+
 
 }  // namespace etraj

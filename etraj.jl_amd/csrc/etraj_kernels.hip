@@ -943,6 +943,7 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
         const int64_t nclose = wave_sum_i64((active && close) ? 1 : 0);
         if (lane == 0) {
             atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS), (unsigned long long)(n * nact));
+            atomicAdd((unsigned long long*)(Oj.cnt + CNT_ATTEMPTS), (unsigned long long)(n * nact));
             if (!MIXED) atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS64), (unsigned long long)(n * nact));
             if (nclose) atomicAdd((unsigned long long*)(Oj.cnt + CNT_RESTEP), (unsigned long long)nclose);
         }

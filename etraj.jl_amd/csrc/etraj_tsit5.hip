@@ -21,7 +21,7 @@ namespace etraj {
 struct LaneAcc {
     double ip = 0.0, iu = 0.0;
     Fix128 ipq{0, 0}, iuq{0, 0};   // ETRAJ_PMD_FIXED
-    int64_t failed = 0, bound = 0, steps = 0, range = 0, restep = 0;
+    int64_t failed = 0, bound = 0, steps = 0, attempts = 0, range = 0, restep = 0;
 };
 
 __device__ inline void fix_add(Fix128& a, Fix128 b) {
@@ -112,7 +112,7 @@ __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx
 // steps64: the launch integrates in FP64 (its steps also count as FP64 steps)
 __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool steps64) {
     const int64_t nf = wave_sum_i64(a.failed), nb = wave_sum_i64(a.bound), ns = wave_sum_i64(a.steps);
-    const int64_t nr = wave_sum_i64(a.range), nq = wave_sum_i64(a.restep);
+    const int64_t nr = wave_sum_i64(a.range), nq = wave_sum_i64(a.restep), na = wave_sum_i64(a.attempts);
     if (O.pmdq) {
         const Fix128 ip = fix_wave_sum(a.ipq), iu = fix_wave_sum(a.iuq);
         if (lane_id() == 0) {
@@ -133,6 +133,7 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
         if (ns && steps64) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS64), (unsigned long long)ns);
         if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
         if (nq) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nq);
+        if (na) atomicAdd((unsigned long long*)(O.cnt + CNT_ATTEMPTS), (unsigned long long)na);
     }
 }
 
@@ -146,12 +147,13 @@ __device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t 
     return seg_start[lo] + (item - pre[lo]);
 }
 
-// Adaptive Tsit5, one electron per lane. A wavefront owns `per_wave`
-// consecutive items; a lane whose electron has finished takes the next item
-// of its wavefront (ballot + mbcnt), so short and long trajectories share
-// the wave without idling until the range is exhausted. A fresh item is set
-// up at the top of the loop, so the initial-step code has one call site and
-// is inlined. Three waves per SIMD (at most 168 VGPRs).
+// Adaptive Tsit5, one electron per lane: wavefront w integrates items
+// [64 w, 64 w + 64) (0-based over all electrons of all segments), each lane
+// advancing its electron one attempted step per loop trip until every lane of
+// the wave is done. (A form in which a finished lane took the next electron
+// of a longer per-wave range measured slower at every range length: 40.4 ms
+// at 128 electrons per wave, 44.1 at 512, 53.9 at 2048, against 39.7 for one
+// electron per lane, on the Ar SPA 3D grid of 9.6M electrons.)
 // MIXED: stage derivatives in FP32; an electron with a stage position within
 // restep_radius of the core stops at once, is not binned and gets flags[col]
 // = 1 (every other column of the launch gets 0), for the FP64 launch over the
@@ -160,67 +162,54 @@ __device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t 
 template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
-    const int64_t* __restrict__ pre, int64_t n_seg, int per_wave, uint8_t* __restrict__ flags,
-    const int64_t* __restrict__ orig) {
+    const int64_t* __restrict__ pre, int64_t n_seg, uint8_t* __restrict__ flags, const int64_t* __restrict__ orig) {
     const int lane = lane_id();
     const int64_t total = pre[n_seg];
     const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int64_t wbeg = gw * per_wave;
-    if (wbeg >= total) return;
-    const int64_t wend = (wbeg + per_wave < total) ? wbeg + per_wave : total;
+    const int64_t item = gw * 64 + lane;
+    if (gw * 64 >= total) return;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
     const double atol = P.atol;
     // the FP32 hydrogen-like EOM reports |r|^2 + soft core
     const float rc2 = (float)(P.restep_radius * P.restep_radius + (TGT != ETRAJ_TARGET_SAE ? P.T.sc : 0.0));
     LaneAcc acc;
     Ts5Lane<DIM, PHASE> st;
-    int64_t item = wbeg + lane, next = wbeg + 64, col = 0;
-    bool valid = item < wend, fresh = valid, close = false;
+    bool valid = item < total, close = false;
+    int64_t col = 0;
     double t0 = 0.0, prob = 0.0;
-    while (__ballot(valid) != 0) {
-        bool done = false;
-        if (fresh) {
-            fresh = false;
-            col = item_column(item, pre, n_seg, seg_start);
-            const double* c = init + col * P.rows;
+    if (valid) {
+        col = item_column(item, pre, n_seg, seg_start);
+        const double* c = init + col * P.rows;
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                st.r[i] = c[i];
-                st.p[i] = c[DIM + i];
-            }
-            t0 = c[2 * DIM];
-            prob = c[2 * DIM + 1];
-            st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
-            close = false;
-            ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+        for (int i = 0; i < DIM; ++i) {
+            st.r[i] = c[i];
+            st.p[i] = c[DIM + i];
         }
+        t0 = c[2 * DIM];
+        prob = c[2 * DIM + 1];
+        st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
+        ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+    }
+    while (__ballot(valid) != 0) {
         if (valid) {
             const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters,
                                                                st, rc2, &close);
             if (MIXED && close) {   // re-integrated in FP64 by the next launch
-                done = true;
+                valid = false;
                 acc.steps += st.accepted;
+                acc.attempts += st.attempted;
                 acc.restep++;
                 flags[col] = 1;
             } else if (rc != 0) {
-                done = true;
+                valid = false;
                 bool failed = rc < 0;
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) failed = failed || isnan(st.r[i]) || isnan(st.p[i]);
                 acc.steps += st.accepted;
+                acc.attempts += st.attempted;
                 if (MIXED) flags[col] = 0;
                 emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
             }
-        }
-        const uint64_t m = __ballot(done);
-        if (m) {
-            const int pre_l = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            if (done) {
-                item = next + pre_l;
-                valid = item < wend;
-                fresh = valid;
-            }
-            next += __popcll(m);
         }
     }
     if (OUT == OUT_PMD) flush_lane_acc(O, acc, !MIXED);
@@ -231,22 +220,30 @@ struct Ts5Launch {
     const int64_t* seg_start;
     const int64_t* pre;
     int64_t n_seg, max_items;
-    int per_wave;
     uint8_t* flags;
     const int64_t* orig;
 };
 
 template <int DIM, int PHASE, int OUT, bool MIXED>
 static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
-    const int64_t waves = (a.max_items + a.per_wave - 1) / a.per_wave;
+    const int64_t waves = (a.max_items + 63) / 64;
     const unsigned blocks = (unsigned)((waves + 3) / 4);
     if (blocks == 0) return hipSuccess;
+    // The FP32 hydrogen-like EOM drops the soft-core term -Z a / r2^(3/2) of the SCTS phase rate;
+    // a soft core that FP32 resolves (the molecular a = 1) takes the kTargetHSoft form, which keeps it.
+    if constexpr (MIXED) {
+        if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE && P.T.sc > 1e-6) {
+            hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, kTargetHSoft, MIXED>), dim3(blocks), dim3(256), 0, st, P, O,
+                               a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
+            return hipGetLastError();
+        }
+    }
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
         hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE, MIXED>), dim3(blocks), dim3(256), 0, st,
-                           P, O, a.init, a.seg_start, a.pre, a.n_seg, a.per_wave, a.flags, a.orig);
+                           P, O, a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
     else
         hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE, MIXED>), dim3(blocks), dim3(256), 0, st, P, O,
-                           a.init, a.seg_start, a.pre, a.n_seg, a.per_wave, a.flags, a.orig);
+                           a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
     return hipGetLastError();
 }
 
@@ -269,10 +266,10 @@ static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O,
 }
 
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
-                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items, int per_wave,
+                        const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
                         uint8_t* flags, const int64_t* orig) {
     if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, per_wave, flags, orig};
+    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, flags, orig};
     if (P.mixed) {
         if (!flags) return hipErrorInvalidValue;
         return P.dim == 3 ? launch_tsit5_p<3, true>(st, P, O, out, a) : launch_tsit5_p<2, true>(st, P, O, out, a);

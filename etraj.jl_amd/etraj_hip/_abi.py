@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # ETRAJ_HIP_LIB names another build of the same library (the Julia binding reads the same variable)
 LIB_PATH = os.environ.get("ETRAJ_HIP_LIB", os.path.join(HERE, "libetraj_hip.so"))
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY, ERANGE = 0, -1, -2, -3, -4, -5, -6
 
@@ -85,7 +85,7 @@ class Stats(C.Structure):
                 ("n_restep", C.c_int64), ("n_steps", C.c_int64), ("n_steps_fp64", C.c_int64),
                 ("ms_sample", C.c_double),
                 ("ms_integrate", C.c_double), ("ms_integrate_main", C.c_double), ("ms_total", C.c_double),
-                ("n_slabs", C.c_int64)]
+                ("n_slabs", C.c_int64), ("n_attempts", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

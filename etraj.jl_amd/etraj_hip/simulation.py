@@ -1,7 +1,12 @@
 """perform_traj_simulation with the HIP engine (TrajectorySimulation.jl:103-161).
 
 Keyword interface of the reference, plus engine keywords:
-  traj_integrator  "rk4" (fixed step, default; the engine's algorithm)
+  traj_integrator  "tsit5" (default): the reference's adaptive Tsit5 at traj_rtol
+                   (TrajectorySimulation.jl:342), in FP64 unless precision says
+                   otherwise; "rk4": fixed-step RK4 at traj_dt, mixed precision by
+                   default (the engine's throughput form; its PMD differs from the
+                   reference algorithm's by its truncation error, and traj_rtol
+                   does not apply to it -- a warning says so if it is given)
   traj_dt          RK4 nominal step (a.u., default 0.1)
   precision        "mixed" (FP32 EOM, FP64 phase/Kepler/re-step) or "fp64"
   restep_radius    FP64 re-step radius of the mixed mode (a.u., default 2)
@@ -70,6 +75,17 @@ def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p
     return d
 
 
+def _check_rtol(traj_integrator, traj_rtol):
+    """traj_rtol of the reference (default 1e-6); a warning when it is given to the fixed-step RK4,
+    which does not use it."""
+    if traj_rtol is None:
+        return 1e-6
+    if _sym(traj_integrator).lower() == "rk4":
+        warnings.warn("[perform_traj_simulation] traj_rtol applies to the adaptive Tsit5 integrator (the reference's "
+                      "algorithm); the fixed-step RK4 integrates at traj_dt and ignores it.")
+    return traj_rtol
+
+
 def _axis(pmax, num):
     return np.linspace(-pmax, pmax, num) if num > 1 else np.array([0.0])
 
@@ -81,14 +97,15 @@ def default_filename():
 def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sample_t_intv, sample_t_num,
                             traj_t_final, final_p_max, final_p_num, ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
                             ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, traj_phase_method="CTMC",
-                            traj_rtol=1e-6, output_fmt="npz", output_compress=True, output_path=None,
+                            traj_rtol=None, output_fmt="npz", output_compress=True, output_path=None,
                             sample_monte_carlo=False, sample_cutoff_limit=1e-16, rate_prefix="Full",
-                            mol_orbit_ridx=0, show_progress=False, traj_integrator="rk4", traj_dt=0.1,
+                            mol_orbit_ridx=0, show_progress=False, traj_integrator="tsit5", traj_dt=0.1,
                             precision=None, restep_radius=2.0, granularity="job", seed=0,
                             sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, reproduce_quirks=True,
                             pmd_accumulator="float", hit_counts=False, backend="hip"):
     if _sym(backend) != "hip":
         raise ValueError("this package is the HIP backend; the CPU method is the reference itself")
+    traj_rtol = _check_rtol(traj_integrator, traj_rtol)
     # TrajectorySimulationJob checks (:175-186)
     assert sample_t_intv[0] < sample_t_intv[1], "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval."
     assert traj_t_final > sample_t_intv[1], \
@@ -147,7 +164,8 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
     result = _result(spec, ion_prob, ion_unc, n_eff, stats, init_cond_method=init_cond_method, laser=laser,
                      target=target, dimension=dimension, sample_t_intv=sample_t_intv, sample_t_num=sample_t_num,
                      traj_phase_method=traj_phase_method, traj_t_final=traj_t_final, final_p_max=final_p_max,
-                     final_p_num=final_p_num, traj_integrator=traj_integrator, traj_dt=traj_dt, precision=precision)
+                     final_p_num=final_p_num, traj_integrator=traj_integrator, traj_dt=traj_dt, traj_rtol=traj_rtol,
+                     precision=precision)
     if hits is not None:
         result["hit_counts"] = _julia_shape(hits, final_p_num)
     _save(result, output_path, output_fmt, output_compress)
@@ -169,9 +187,9 @@ def _result(spec, ion_prob, ion_unc, n_eff, stats, **kw):
         "ion_prob": ion_prob,
         "ion_prob_uncollected": ion_unc,
         "num_effective_traj": n_eff,
-        "params": {k: kw[k] for k in ("init_cond_method", "sample_t_intv", "sample_t_num", "traj_phase_method",
-                                      "traj_t_final", "final_p_max", "final_p_num", "traj_integrator", "traj_dt",
-                                      "precision")},
+        "params": {k: kw.get(k) for k in ("init_cond_method", "sample_t_intv", "sample_t_num", "traj_phase_method",
+                                          "traj_t_final", "final_p_max", "final_p_num", "traj_integrator", "traj_dt",
+                                          "traj_rtol", "precision")},
         "stats": stats,
     }
     result["params"]["laser"] = _ser_laser(kw["laser"])
@@ -237,8 +255,8 @@ def _save(result, output_path, output_fmt="npz", output_compress=True):
 
 
 _SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0,
-                      mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=1e-6, sample_monte_carlo=False,
-                      sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="rk4",
+                      mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=None, sample_monte_carlo=False,
+                      sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="tsit5",
                       traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
                       kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="npz",
                       output_compress=True, pmd_accumulator="float", hit_counts=False)
@@ -257,6 +275,7 @@ def perform_traj_scan(jobs):
     prep = []
     for kw in jobs:
         kw = dict(_SCAN_DEFAULTS, **kw)
+        kw["traj_rtol"] = _check_rtol(kw["traj_integrator"], kw["traj_rtol"])
         assert kw["sample_t_intv"][0] < kw["sample_t_intv"][1], \
             "[TrajectorySimulationJob] `sample_t_intv` must be a valid interval."
         assert kw["traj_t_final"] > kw["sample_t_intv"][1], \

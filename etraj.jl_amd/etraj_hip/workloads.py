@@ -6,8 +6,9 @@ h_lp_ctmc_grid        H atom (Ip 0.5, Z 1), ADK :Full, Cos2 800 nm 1e14 W/cm^2 2
                       over +-1.5 a.u.  (n_t = 2000, n_kd = 50: 1e5 trajectories)
 h_lp_scts_stratified  same atom and pulse, SCTS phase, n_t strata with one seeded jitter each and
                       n_k Gaussian k_perp (sigma 0.4 a.u.) per stratum, weights divided by the
-                      sampling pdf; coherent PMD 1000 x 1000 over +-1.5 a.u.
-                      (n_t = 10000, n_k = 32000: 3.2e8 samples of which ~1.0e8 electrons carry a
+                      sampling pdf; coherent PMD 1000 x 1000 over +-1.5 a.u.; mixed precision with
+                      FP64 re-integration within 2 a.u. of the core
+                      (n_t = 10000, n_k = 34400: 3.44e8 samples of which 1.0e8 electrons carry a
                       weight above the 1e-16 cutoff and are integrated; the rest sit at the pulse
                       edges and field zeros, where the ADK rate underflows the cutoff)
 mol_lp_scts_wfat      single-centre molecule (Ip 0.45, Z 1, soft core 1.0) with a seeded 8 x 16 G00
@@ -48,11 +49,11 @@ def h_lp_ctmc_grid(n_t: int = 2000, n_kd: int = 50, precision: str = "fp64") -> 
                 traj_integrator="rk4", traj_dt=0.1, precision=precision)
 
 
-def h_lp_scts_stratified(n_t: int = 10000, n_k: int = 32000, precision: str = "mixed", seed: int = 0xE7A10002,
-                         p_num: int = 1000, restep_radius: float = 0.5) -> dict:
-    """The 1e8-electron H-atom ADK+SCTS job. Its FP64 re-integration radius is 0.5 a.u.: at full
-    scale the mixed PMD's L1 against FP64 is 4.8e-5 (2 a.u.: 4.5e-5 with 2.3x the re-integrated
-    electrons; 0.2 a.u.: 5.5e-5; none: 9.7e-4)."""
+def h_lp_scts_stratified(n_t: int = 10000, n_k: int = 34400, precision: str = "mixed", seed: int = 0xE7A10002,
+                         p_num: int = 1000, restep_radius: float = 2.0) -> dict:
+    """The 1e8-electron H-atom ADK+SCTS job: 10000 strata x 34400 samples, of which just over 1e8
+    electrons carry a weight above the cutoff; FP64 re-integration of the electrons that come
+    within 2 a.u. of the core."""
     laser, half = _pulse()
     return dict(init_cond_method="ADK", laser=laser, target=get_atom("H"), dimension=2,
                 sample_t_intv=(-half, half), sample_t_num=n_t, traj_t_final=half + 10.0,
@@ -112,6 +113,17 @@ def bichromatic_scan_jobs(intensities, n_t: int = 2000, n_k=1000, precision: str
                  sample_cutoff_limit=1e-16, traj_integrator="rk4", traj_dt=0.1, precision=precision,
                  restep_radius=2.0)
             for j, (I, nk) in enumerate(zip(intensities, n_ks))]
+
+
+def bichromatic_scan_counts(ion_prob, survivors, n_probe, total=1e8, lo=1e5, hi=2e7, n_t=2000):
+    """Samples per stratum (n_k) of each intensity of the scan, so that job j integrates about
+    clip(total * Y_j / sum Y, lo, hi) electrons: Y_j the job's ADK yield (ion_prob of a probe
+    scan), and survivors_j / n_probe the fraction of its samples above the weight cutoff."""
+    import numpy as np
+    y = np.asarray(ion_prob, float)
+    surv = np.maximum(np.asarray(survivors, float), 1.0) / n_probe
+    want = np.clip(total * y / y.sum(), lo, hi)
+    return [int(v) for v in np.maximum(1, np.ceil(want / (n_t * surv)))]
 
 
 def so3_uniform(n: int, seed: int = 0xE7A10005):

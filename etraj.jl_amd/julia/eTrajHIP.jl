@@ -63,7 +63,8 @@ mutable struct Stats
     ion_prob::Float64; ion_prob_uncollected::Float64
     num_effective_traj::Int64; n_failed::Int64; n_bound::Int64; n_restep::Int64; n_steps::Int64; n_steps_fp64::Int64
     ms_sample::Float64; ms_integrate::Float64; ms_integrate_main::Float64; ms_total::Float64; n_slabs::Int64
-    Stats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    n_attempts::Int64
+    Stats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
 end
 
 const PHASE = Dict(:CTMC => Int32(0), :QTMC => Int32(1), :SCTS => Int32(2))

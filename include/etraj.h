@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ETRAJ_ABI_VERSION 2
+#define ETRAJ_ABI_VERSION 3
 
 /* status codes */
 #define ETRAJ_OK             0
@@ -233,6 +233,8 @@ typedef struct etraj_stats {
     double  ms_integrate_main;      /* of which the main integrator kernel (MIXED: the FP32 pass) */
     double  ms_total;               /* wall time of the call                 */
     int64_t n_slabs;                /* run_job: slabs = launches of the main integrator kernel */
+    int64_t n_attempts;             /* integrator step attempts, all passes (Tsit5: accepted +
+                                       rejected; RK4: n_steps)                                */
 } etraj_stats;
 
 typedef struct etraj_ctx etraj_ctx;

@@ -1,9 +1,9 @@
 """The workloads at their full sizes, checked through properties that do not
 need the oracle to run at that size:
 
-  * the 1e8-electron H-atom ADK+SCTS job (10000 t0 strata x 32000 Gaussian
+  * the 1e8-electron H-atom ADK+SCTS job (10000 t0 strata x 34400 Gaussian
     k_perp, coherent 1000^2 PMD): the mixed and FP64 runs sample the same
-    93 463 129 electrons; with fixed-point sums two mixed runs are the same bits;
+    >= 1e8 electrons; with fixed-point sums two mixed runs are the same bits;
     the mixed PMD is within relative L1 1e-3 of the FP64 PMD (the stated bound),
     and both runs give the same ion probability;
   * the reference's CPU-runnable case (2000 t0 x 50 kd CTMC grid, 500^2 PMD) in
@@ -38,13 +38,71 @@ def test_headline_job_at_full_size(ctx):
     mixed, sm, _, _ = _job(ctx, kw)
     again, sa, _, _ = _job(ctx, kw)
     fp64, s64, _, _ = _job(ctx, dict(kw, precision="fp64"))
-    assert sm["num_effective_traj"] == sa["num_effective_traj"] == s64["num_effective_traj"] == 93463129
+    assert sm["num_effective_traj"] == sa["num_effective_traj"] == s64["num_effective_traj"] >= 1e8
     assert mixed.tobytes() == again.tobytes() and sm["ion_prob"] == sa["ion_prob"]
     err = l1(mixed, fp64)
     print("full-size PMD L1 mixed vs FP64", err, "re-integrated", sm["n_restep"])
     assert err <= 1e-3
     assert sm["ion_prob"] == pytest.approx(s64["ion_prob"], rel=1e-9)
     assert 0 < sm["n_restep"] < 0.1 * sm["num_effective_traj"]
+
+
+def _scan(ctx, jobs):
+    ds = [descriptors(kw) for kw in jobs]
+    n = len(ds)
+    ld = (_abi.LaserDesc * n)(*[d[0] for d in ds])
+    sd = (_abi.SamplerDesc * n)(*[d[2] for d in ds])
+    st = (_abi.Stats * n)()
+    rd = ds[0][3]
+    npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
+    out = np.empty(n * npix)
+    ctx.check(ctx.lib.etraj_run_scan(ctx.handle, n, ld, ds[0][1], sd, rd, _abi.dptr(out), st))
+    return out.reshape(n, npix), [st[j].as_dict() for j in range(n)]
+
+
+def test_bichromatic_scan_at_full_size(ctx):
+    """The 32-intensity counter-rotating bichromatic scan (~1e8 electrons, per-intensity counts from
+    the ADK yield): every job of the scan equals the same job run alone (same counters, PMD L1
+    <= 1e-12: only the order of the atomic sums differs), and each job's mixed PMD is within the
+    stated bound of its FP64 PMD."""
+    ints = list(np.logspace(np.log10(5e13), np.log10(5e14), 32))
+    _, pst = _scan(ctx, workloads.bichromatic_scan_jobs(ints, n_t=200, n_k=200))
+    n_k = workloads.bichromatic_scan_counts([s["ion_prob"] for s in pst], [s["num_effective_traj"] for s in pst],
+                                            200 * 200)
+    jobs = workloads.bichromatic_scan_jobs(ints, n_t=2000, n_k=n_k)
+    mixed, sm = _scan(ctx, jobs)
+    total = sum(s["num_effective_traj"] for s in sm)
+    print(f"bichromatic scan: {total} electrons")
+    assert 0.8e8 < total < 1.3e8
+    for j, kw in enumerate(jobs):
+        alone, sa, _, _ = _job(ctx, kw)
+        for k in ("num_effective_traj", "n_failed", "n_bound", "n_restep", "n_steps", "n_steps_fp64"):
+            assert sm[j][k] == sa[k], (j, k)
+        assert l1(mixed[j], alone) <= 1e-12, j
+    fp64, s64 = _scan(ctx, [dict(kw, precision="fp64") for kw in jobs])
+    errs = [l1(mixed[j], fp64[j]) for j in range(len(jobs))]
+    print("per-intensity PMD L1 mixed vs FP64: max", max(errs), "median", float(np.median(errs)))
+    assert max(errs) <= 1e-3
+
+
+def test_ar_spa_3d_at_full_size(ctx):
+    """The Ar SFA-SPA 3D grid (2000 x 158 x 158, Tsit5 rtol 1e-8, QTMC, 256^3 PMD): with fixed-point
+    sums two FP64 runs are the same bits; mixed and FP64 sample the same electrons with the same
+    ion probability; and the mixed PMD is as close to an FP64 run at rtol 1e-10 as the FP64 run
+    at rtol 1e-8 is (at 9.6M electrons over 16.7M cells the L1 of either is set by the
+    integrator's tolerance moving electrons between neighbouring cells)."""
+    kw = dict(workloads.ar_cp_spa_qtmc_3d(), pmd_accumulator="fixed")
+    a, sa, _, _ = _job(ctx, kw)
+    b, sb, _, _ = _job(ctx, kw)
+    assert a.tobytes() == b.tobytes() and sa["num_effective_traj"] == sb["num_effective_traj"] > 9e6
+    m, smx, _, _ = _job(ctx, dict(kw, precision="mixed"))
+    ref, sr, _, _ = _job(ctx, dict(kw, traj_rtol=1e-10))
+    assert smx["num_effective_traj"] == sa["num_effective_traj"]
+    assert smx["ion_prob"] == pytest.approx(sa["ion_prob"], rel=1e-6)
+    trunc, mix = l1(a, ref), l1(m, ref)
+    print(f"Ar SPA 3D full: L1 vs FP64 rtol 1e-10: FP64 rtol 1e-8 {trunc:.2e}, mixed {mix:.2e}; "
+          f"mixed vs FP64 {l1(m, a):.2e}, re-integrated {smx['n_restep']}")
+    assert mix <= 1.5 * trunc
 
 
 def test_ctmc_grid_at_full_size_matches_oracle(ctx):

@@ -2,8 +2,10 @@
 
 Tolerances:
   fields / forces / sampler columns          1e-13 relative (FP64, different libm)
-  FP64 RK4 per-electron asymptotic momentum  median 1e-12, <= 1 % of electrons above 1e-8
-                                             (chaotic rescattering amplifies ulp differences)
+  FP64 RK4 per-electron asymptotic momentum  median 1e-12; 1e-10 for every electron whose
+                                             result a one-ulp change of its initial state moves
+                                             by less than 1e-11 (the ill-conditioned rest, close
+                                             rescattering on the bare core, may differ more)
   FP64 PMD                                   relative L1 <= 1e-6
   mixed FP32/FP64 PMD                        relative L1 <= 1e-3 (the engine's stated bound)
 """
@@ -142,6 +144,31 @@ def _momentum_errors(rec, st, rrec, rst):
     return np.linalg.norm(p - q, axis=1) / np.maximum(np.linalg.norm(q, axis=1), 1e-300), both
 
 
+def _ulp_sensitivity(ld, td, rd, cols, rec, st):
+    """Per electron, the largest relative change of the oracle's asymptotic momentum when one of
+    x, y, px, py of its initial state, or the step size, moves by one ulp up or down (inf if its
+    status changes): the electron's conditioning in FP64."""
+    sens = np.zeros(len(cols))
+    for k in range(4):
+        for direction in (np.inf, -np.inf):
+            pc = cols.copy()
+            pc[:, k] = np.nextafter(pc[:, k], direction)
+            r2, s2 = O.integrate_records(ld, td, rd, np.ascontiguousarray(pc))
+            d = np.linalg.norm(r2[:, 8:11] - rec[:, 8:11], axis=1) / np.maximum(np.linalg.norm(rec[:, 8:11], axis=1),
+                                                                                1e-300)
+            d[s2 != st] = np.inf
+            sens = np.maximum(sens, d)
+    # and every step's arithmetic: t_final one ulp up or down changes the step size's last bits
+    for direction in (np.inf, -np.inf):
+        rd2 = O.RunDesc.from_buffer_copy(rd)
+        rd2.traj_t_final = float(np.nextafter(rd.traj_t_final, direction))
+        r2, s2 = O.integrate_records(ld, td, rd2, cols)
+        d = np.linalg.norm(r2[:, 8:11] - rec[:, 8:11], axis=1) / np.maximum(np.linalg.norm(rec[:, 8:11], axis=1), 1e-300)
+        d[s2 != st] = np.inf
+        sens = np.maximum(sens, d)
+    return sens
+
+
 @pytest.mark.parametrize("phase", ["CTMC", "QTMC", "SCTS"])
 def test_records_fp64_match_oracle(ctx, phase):
     kw = small_ctmc(n_t=60, n_kd=40, traj_phase_method=phase)
@@ -151,10 +178,41 @@ def test_records_fp64_match_oracle(ctx, phase):
     assert (st != rst).mean() <= 0.01
     err, both = _momentum_errors(rec, st, rrec, rst)
     assert np.median(err) < 1e-12
-    assert (err > 1e-8).mean() <= 0.01
+    # these grid electrons are all well conditioned (one-ulp changes move none by 1e-13)
+    assert err.max() < 1e-10
     if phase != "CTMC":
         dphi = np.angle(np.exp(1j * (rec[both, 11] - rrec[both, 11])))
         assert np.median(np.abs(dphi)) < 1e-9
+
+
+def test_fp64_outliers_are_the_ill_conditioned_electrons(ctx):
+    """FP64 RK4 on the device against the oracle over ~8000 electrons of the stratified SCTS
+    workload. The device's libm (rsqrt, sin, exp ...) and FMA contraction differ from the oracle's
+    in the last bits; a fraction of a per cent of the electrons (close rescattering on the bare
+    Coulomb core, where the fixed step dt 0.1 meets r -> 0 and some leave with |p| of 5-45 a.u.)
+    amplify such differences by up to 1e11. Every electron whose device result is
+    off by more than 1e-8 must be one of those: one whose oracle result moves by more than 1e-11
+    when its initial state or its step size moves by one ulp. The well-conditioned rest agree to
+    1e-10."""
+    kw = small_scts(n_t=40, n_k=512, precision="fp64")
+    ld, td, sd, rd, cols = _columns(kw, range(1, 40))
+    rec, st = _device_records(ctx, ld, td, rd, cols)
+    rrec, rst = O.integrate_records(ld, td, rd, cols)
+    sens = _ulp_sensitivity(ld, td, rd, cols, rrec, rst)
+    # a status change (ionized <-> bound) is either an ill-conditioned electron or one whose final
+    # energy E = p^2/2 + V is so close to zero that the last bits of its two terms decide its sign
+    flipped = st != rst
+    marginal = np.abs(rrec[:, 7]) < 1e-8
+    print(f"{flipped.sum()} status changes: sensitivity {np.sort(sens[flipped])}, |E| {np.sort(np.abs(rrec[flipped, 7]))}")
+    assert np.all((sens[flipped] > 1e-11) | marginal[flipped])
+    err, both = _momentum_errors(rec, st, rrec, rst)
+    s = sens[both]
+    off = err > 1e-8
+    print(f"{both.sum()} electrons: {off.sum()} off by > 1e-8 (sensitivity of those: min {s[off].min() if off.any() else 0:.1e}); "
+          f"{(s > 1e-11).sum()} ill-conditioned; well-conditioned max error {err[s <= 1e-11].max():.1e}")
+    assert np.all(s[off] > 1e-11)
+    assert err[s <= 1e-11].max() < 1e-10
+    assert np.median(err) < 1e-13
 
 
 def test_records_mixed_close_to_oracle(ctx):

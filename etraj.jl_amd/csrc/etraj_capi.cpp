@@ -98,6 +98,7 @@ struct etraj_ctx {
     std::vector<int32_t> h_sc;
     std::vector<double> h_t0;
     std::vector<Contrib> h_contrib;   // per-electron records copied back for host binning
+    void* scratch = nullptr;          // this device's no-allocation scratch array (noalloc_scratch)
 };
 
 namespace {
@@ -403,8 +404,8 @@ constexpr int64_t kFieldTabBudget = (int64_t)8 << 30;
 constexpr int64_t kMaxSlabCols = (int64_t)1 << 26;
 
 // Device buffers of integrate_columns: the ctx's own (grown on demand) or
-// fixed slices of a preallocated region (`fixed`, with the field table's
-// capacity in RK4 steps).
+// fixed slices of the no-allocation scratch array (`fixed`, with the field
+// table's capacity in RK4 steps).
 struct ColBufs {
     bool fixed = false;
     double* init = nullptr;
@@ -454,7 +455,7 @@ int fill_field_table(etraj_ctx* c, ColBufs& B, const IntegP& P, const LaserP* lp
         B.ft64 = c->ft64.p;
         B.ft0 = c->ft0.p;
     } else if (nseg * stride > B.ft_steps || nseg > B.ft_segs) {
-        return fail(c, ETRAJ_ECAPACITY, "field table of the segment group exceeds the fixed buffers");
+        return fail(c, ETRAJ_ECAPACITY, "field table of the segment group exceeds the no-allocation scratch");
     }
     FT.f32 = B.ft32;
     FT.f64 = B.ft64;
@@ -565,7 +566,7 @@ int integrate_columns(etraj_ctx* c, ColBufs& B, const IntegP& P, const double* i
     if (group < 1) {
         if (B.fixed)
             return fail(c, ETRAJ_ECAPACITY, "one segment's field table (" + std::to_string(stride) +
-                                                 " steps) exceeds the fixed buffers");
+                                                 " steps) exceeds the no-allocation scratch");
         group = 1;
     }
     HIP_TRY(c, hipMemcpyAsync(B.seg_t0, t0, sizeof(double) * nseg, hipMemcpyHostToDevice, c->stream));
@@ -626,6 +627,14 @@ int etraj_ctx_create(int device, etraj_ctx** out) {
         delete c;
         return ETRAJ_EHIP;
     }
+    // the no-allocation entry point's scratch (this device's copy of the static
+    // array) and host staging, set up once here so that its calls allocate nothing
+    if (noalloc_scratch(device, &c->scratch) != hipSuccess) {
+        etraj_ctx_destroy(c);
+        return ETRAJ_EHIP;
+    }
+    host_staging_reserve(c, kNoAllocMaxCols);
+    c->h_contrib.resize(kNoAllocMaxCols);
     *out = c;
     return ETRAJ_OK;
 }
@@ -840,6 +849,72 @@ int etraj_integrate_and_bin(etraj_ctx* c, const etraj_laser_desc* laser, const e
     if (int rc = integrate_columns(c, B, P, init, n, OUT_CONTRIB, O)) return rc;
     if ((int64_t)c->h_contrib.size() < n) c->h_contrib.resize(n);
     HIP_TRY(c, hipMemcpyAsync(c->h_contrib.data(), c->contrib.p, sizeof(Contrib) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    bin_contribs_host(P, run, c->h_contrib.data(), n, pmd, ion_prob, ion_prob_uncollected, n_failed);
+    return ETRAJ_OK;
+}
+
+// etraj_integrate_and_bin with every device buffer a slice of the library's
+// static scratch array (etraj_noalloc.hip) and the host staging the ctx's
+// (sized at ctx creation): no device or host allocation on the way. Calls on
+// one device share that device's array, so calls that overlap on one device
+// (two host threads with two ctxs) race; consecutive calls need no clearing,
+// since every slice a call reads is first written by that call.
+int etraj_integrate_and_bin_noalloc(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                                    const etraj_run_desc* run, const double* init, int64_t n, int32_t rows,
+                                    void* pmd, double* ion_prob, double* ion_prob_uncollected, int64_t* n_failed) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    LaserP L;
+    TargetP T;
+    IntegP P;
+    if (int rc = check_laser(c, laser, L)) return rc;
+    if (int rc = check_target(c, target, T)) return rc;
+    if (int rc = make_integ(c, run, L, T, P)) return rc;
+    if (n > kNoAllocMaxCols)
+        return fail(c, ETRAJ_ECAPACITY, "etraj_integrate_and_bin_noalloc takes at most 65536 columns per call");
+    if (int rc = check_init(c, P, init, n, rows)) return rc;
+    if (!pmd || !ion_prob || !ion_prob_uncollected || !n_failed) return fail(c, ETRAJ_EINVAL, "NULL output");
+    *n_failed = 0;
+    if (n == 0) return ETRAJ_OK;
+    if (!c->scratch) return fail(c, ETRAJ_EHIP, "the no-allocation scratch of this device is not available");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // scratch layout: columns, records, flags, original indices, segment tables, laser, field table
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const int64_t M = kNoAllocMaxCols;
+    ColBufs B;
+    B.fixed = true;
+    B.init = reinterpret_cast<double*>(take(sizeof(double) * 9 * M));
+    Contrib* contrib = reinterpret_cast<Contrib*>(take(sizeof(Contrib) * M));
+    B.flags = take(M);
+    B.orig = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * M));
+    B.seg_start = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * M));
+    B.counts = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * M));
+    B.counts2 = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * M));
+    B.seg_t0 = reinterpret_cast<double*>(take(sizeof(double) * M));
+    B.pre = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (M + 1)));
+    B.lps = reinterpret_cast<LaserP*>(take(sizeof(LaserP)));
+    // field table of a segment group: F(t0) per segment, then the rest of the array as
+    // (float4 + 2 double2) per step; each slice above is rounded up to 256 bytes
+    B.ft_segs = M;
+    B.ft0 = reinterpret_cast<double2*>(take(sizeof(double2) * M));
+    B.ft_steps = (int64_t)(((size_t)kNoAllocScratchBytes - off - 512) / kFieldTabBytesPerStep);
+    B.ft32 = reinterpret_cast<float4*>(take(sizeof(float4) * B.ft_steps));
+    B.ft64 = reinterpret_cast<double2*>(take(sizeof(double2) * 2 * B.ft_steps));
+    if (off > (size_t)kNoAllocScratchBytes) return fail(c, ETRAJ_EHIP, "no-allocation scratch layout overflow");
+    OutP O{};
+    O.contrib = contrib;
+    if (int rc = integrate_columns(c, B, P, init, n, OUT_CONTRIB, O)) {
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->h_contrib.data(), contrib, sizeof(Contrib) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     bin_contribs_host(P, run, c->h_contrib.data(), n, pmd, ion_prob, ion_prob_uncollected, n_failed);
     return ETRAJ_OK;

@@ -229,5 +229,14 @@ hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, do
 // out[i] = fix_to_double(q[2i], q[2i+1]), i < n
 hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out);
 
+// The no-allocation entry point (etraj_integrate_and_bin_noalloc): its device
+// buffers are slices of a static device array of kNoAllocScratchBytes per
+// device. Limits: kNoAllocMaxCols columns per call, and the RK4 field table of
+// a group of segments within what the array leaves after the column buffers.
+constexpr int64_t kNoAllocMaxCols = 65536;
+constexpr int64_t kNoAllocScratchBytes = (int64_t)32 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+// The device address of `device`'s copy of the scratch array (that device must be current).
+hipError_t noalloc_scratch(int device, void** out);
 
 }  // namespace etraj

@@ -108,6 +108,8 @@ SIGNATURES = {
                                            DP, C.c_int64, P(C.c_int64)]),
     "etraj_integrate_and_bin": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(RunDesc), DP, C.c_int64,
                                           C.c_int32, C.c_void_p, DP, DP, P(C.c_int64)]),
+    "etraj_integrate_and_bin_noalloc": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(RunDesc), DP,
+                                                  C.c_int64, C.c_int32, C.c_void_p, DP, DP, P(C.c_int64)]),
     "etraj_integrate_records": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(RunDesc), DP, C.c_int64,
                                           C.c_int32, DP, P(C.c_int32)]),
     "etraj_run_job": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), DP,

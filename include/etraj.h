@@ -336,6 +336,22 @@ int etraj_run_scan_device(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc
                           const etraj_target_desc* target, const etraj_sampler_desc* samplers,
                           const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and errors as etraj_integrate_and_bin, with every device buffer a
+ * slice of a static 32 MiB device array that each GPU the library runs on
+ * holds until the process ends, and the host staging the ctx set up at
+ * creation: the call allocates no device or host memory. Limits: n <= 65536
+ * columns (ETRAJ_ECAPACITY beyond), and for RK4 one segment's field table
+ * (ceil((traj_t_final - t0)/dt) steps, 48 B each) within about 22 MB. Calls
+ * on one device share its array: calls that overlap on one device (two host
+ * threads, two ctxs) race; consecutive calls need no clearing. */
+int etraj_integrate_and_bin_noalloc(etraj_ctx* ctx, const etraj_laser_desc* laser,
+                                    const etraj_target_desc* target, const etraj_run_desc* run,
+                                    const double* init, int64_t n, int32_t rows,
+                                    void* pmd, double* ion_prob, double* ion_prob_uncollected,
+                                    int64_t* n_failed);
+
 #ifdef __cplusplus
 }
 #endif

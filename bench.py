@@ -355,7 +355,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-t", type=int, default=10000)
-    ap.add_argument("--n-k", type=int, default=32000)
+    ap.add_argument("--n-k", type=int, default=34400)
     ap.add_argument("--cpu-batches", type=int, default=24, help="strata in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

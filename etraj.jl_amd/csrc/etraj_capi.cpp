@@ -51,9 +51,11 @@ struct DevBuf {
 
 }  // namespace
 
-// The buffers of one slab in flight. run_job alternates two slots so that a
-// slab's FP64 re-integration (second stream) overlaps the next slab's
-// sampling and FP32 pass (first stream).
+// The buffers of one slab in flight. run_job cycles through kSlabSlots slots
+// so that a slab's FP64 re-integration (second stream) overlaps the next
+// slab's sampling and FP32 pass (first stream). (Three slots, and slabs
+// ordered by expected work, measured no faster on the 1e8-electron job.)
+constexpr int kSlabSlots = 2;
 struct SlabSlot {
     DevBuf<double> init, seg_t0;
     DevBuf<int32_t> counts, counts2, seg_job, seg_b;
@@ -78,7 +80,7 @@ struct etraj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;   // FP64 re-integration passes of run_job (default priority)
-    SlabSlot slot[2];
+    SlabSlot slot[kSlabSlots];
     std::string err;
     DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat, prec;
     DevBuf<float4> ft32;
@@ -596,6 +598,12 @@ void host_staging_reserve(etraj_ctx* c, int64_t n) {
 
 }  // namespace
 
+bool create_slot_events(etraj_ctx* c) {
+    for (SlabSlot& Q : c->slot)
+        if (hipEventCreateWithFlags(&Q.done, hipEventDisableTiming) != hipSuccess) return false;
+    return true;
+}
+
 extern "C" {
 
 int etraj_abi_version(void) { return ETRAJ_ABI_VERSION; }
@@ -617,13 +625,11 @@ int etraj_ctx_create(int device, etraj_ctx** out) {
     gauss_legendre(kSpaGlPoints, glq);
     if (c->glq.ensure(2 * kSpaGlPoints) != hipSuccess || hipMemcpy(c->glq.p, glq, sizeof glq, hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->slot[0].done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->slot[1].done, hipEventDisableTiming) != hipSuccess) {
+        !create_slot_events(c)) {
         (void)hipStreamDestroy(c->stream);
         if (c->stream2) (void)hipStreamDestroy(c->stream2);
         c->glq.release();
-        c->slot[0].release();
-        c->slot[1].release();
+        for (SlabSlot& Q : c->slot) Q.release();
         delete c;
         return ETRAJ_EHIP;
     }
@@ -651,8 +657,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release(); c->prec.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
-    c->slot[0].release();
-    c->slot[1].release();
+    for (SlabSlot& Q : c->slot) Q.release();
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1072,7 +1077,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     // restep_radius <= 0: no electron is re-integrated, so no flags, compaction or FP64 pass
     const bool two_pass = rk4 && P.mixed && P.restep_radius > 0;
     const int nslab = (int)slabs.size();
-    for (int q = 0; q < std::min(2, nslab); ++q) {
+    for (int q = 0; q < std::min(kSlabSlots, nslab); ++q) {
         SlabSlot& Q = c->slot[q];
         HIP_TRY(c, Q.init.ensure((size_t)max_cols * rows));
         HIP_TRY(c, Q.counts.ensure(max_nb));
@@ -1139,7 +1144,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
         const Slab& sl = slabs[k];
         const int nb = (int)(sl.i1 - sl.i0);
-        SlabSlot& Q = c->slot[k % 2];
+        SlabSlot& Q = c->slot[k % kSlabSlots];
         hipError_t e = hipSuccess;
         if (Q.pending) {   // the slot's previous slab (device buffers and host staging) must be finished
             e = hipEventSynchronize(Q.done);
@@ -1234,7 +1239,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         if (e != hipSuccess) rc = hip_fail(c, e, "run_job FP64 pass");
     }
     // the first stream finishes the job after every FP64 pass
-    for (int q = 0; q < 2; ++q)
+    for (int q = 0; q < kSlabSlots; ++q)
         if (c->slot[q].pending && rc == ETRAJ_OK) {
             hipError_t e = hipStreamWaitEvent(c->stream, c->slot[q].done, 0);
             if (e != hipSuccess) rc = hip_fail(c, e, "run_job join");
@@ -1242,7 +1247,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     if (rc) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamSynchronize(c->stream);
-        c->slot[0].pending = c->slot[1].pending = false;
+        for (SlabSlot& Q : c->slot) Q.pending = false;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         return rc;
@@ -1278,7 +1283,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             const int64_t* cj = cnt.data() + (size_t)CNT_N * j;
             const double bound = coh ? std::sqrt((double)cj[CNT_LAUNCHED] * scal[2 * j]) : scal[2 * j];
             if (cj[CNT_FIXRANGE] != 0 || !(bound < 0x1.0p26)) {
-                c->slot[0].pending = c->slot[1].pending = false;
+                for (SlabSlot& Q : c->slot) Q.pending = false;
                 for (auto& e : ev)
                     if (e) (void)hipEventDestroy(e);
                 return fail(c, ETRAJ_ERANGE, "fixed-point PMD: a sum reaches 2^26 (use pmd_mode = ETRAJ_PMD_FLOAT)");
@@ -1287,7 +1292,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     }
     if (run->hit_counts)
         for (int64_t i = 0; i < n_jobs * npix; ++i) run->hit_counts[i] += hits[i];
-    c->slot[0].pending = c->slot[1].pending = false;
+    for (SlabSlot& Q : c->slot) Q.pending = false;
     double ms_s = 0, ms_i = 0, ms_m = 0;
     for (int k = 0; k < nslab; ++k) {
         float a = 0, b = 0, m = 0, f = 0;

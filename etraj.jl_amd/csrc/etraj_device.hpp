@@ -23,11 +23,20 @@ __device__ inline double uniform_f64(double v) {
     int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
     return __hiloint2double(hi, lo);
 }
-// A wave-uniform value moved into a vector register (v_mov_b32 from an SGPR).
-__device__ inline float sgpr_to_vgpr(float x) {
-    float v;
-    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
-    return v;
+// A per-segment row of the field table as a buffer resource: a load through
+// it takes a wave-uniform byte offset in a scalar register and returns the
+// values in vector registers, with no vector address arithmetic (a plain load
+// of a uniform address would land in scalar registers, and every VALU operand
+// would then need a v_mov). Reads at offsets of `bytes` or more return zeros.
+// Word 3 = 0x00020000: the gfx9 untyped-buffer setting (32-bit data format).
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float4 row_load_f4(__amdgpu_buffer_rsrc_t rs, int byte_off) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, 0, byte_off, 0));
+}
+__device__ inline double2 row_load_d2(__amdgpu_buffer_rsrc_t rs, int byte_off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, 0, byte_off, 0));
 }
 
 __device__ inline int64_t uniform_i64(int64_t v) {

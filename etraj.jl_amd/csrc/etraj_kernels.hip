@@ -773,80 +773,160 @@ __global__ __launch_bounds__(256) void k_field_table(const LaserP* __restrict__ 
 
 // The FP32 pass of one electron over n steps from t0 (see k_rk4). TGT2 is
 // the target kind, kTargetHSoft for a hydrogen-like target whose soft core is
-// not negligible in FP32 (decided once per wave, outside the step loop).
-template <int DIM, int PHASE, int TGT2>
+// not negligible in FP32 (decided once per wave, outside the step loop);
+// UNITZ: the charge Z is 1. T32 is the segment's row of the FP32 field table.
+//
+// The state is an FP64 base plus an FP32 increment: r = rb + dr, rb the base
+// rounded to FP32 and dr the sum of the step increments h/6 Sr since the base
+// was last updated; every 32 steps dr is added to the FP64 base. The EOM sees
+// the FP32 sum rb + dr, and the increments are summed at the resolution of
+// |dr| (at most 32 steps' worth) rather than of |r| -- two FP32 operations
+// per component and step, where a Kahan-compensated FP32 state takes four.
+// The phase increments are summed the same way over the 32 steps and the
+// partial sums Kahan-summed in FP64 (the SCTS phase reaches 1e3 rad).
+template <int DIM, int PHASE, int TGT2, bool UNITZ>
 __device__ inline void rk4_fp32_pass(const IntegP& P, bool saeq, const Lane<DIM, PHASE>& e, int64_t n, double h,
-                                     const float4* __restrict__ T32, double2 F0, bool& close, double* rf, double* pf,
+                                     __amdgpu_buffer_rsrc_t T32, double2 F0, bool& close, double* rf, double* pf,
                                      double& phase) {
-    // FP32 state with a Kahan compensation term per component (value =
-    // us - uc). Without it the PMD error against FP64 grows ~5x (chaotic
-    // electrons passing just outside the re-step radius amplify the position
-    // rounding).
-    State<DIM, float> us, uc;
+    constexpr bool HL = is_hydrogen_like(TGT2);
+    constexpr bool SOFT = TGT2 == kTargetHSoft;
+    double r64[DIM], p64[DIM];
+    float rb[DIM], pb[DIM], dr[DIM], dp[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        us.r[i] = (float)e.r[i];
-        us.p[i] = (float)e.p[i];
-        uc.r[i] = -(float)(e.r[i] - (double)us.r[i]);
-        uc.p[i] = -(float)(e.p[i] - (double)us.p[i]);
+        r64[i] = e.r[i];
+        p64[i] = e.p[i];
+        rb[i] = (float)r64[i];
+        pb[i] = (float)p64[i];
+        dr[i] = 0.0f;
+        dp[i] = 0.0f;
     }
     phase = e.phase0;
     double phc = 0.0;
-    const float hf = (float)h;
-    // the FP32 hydrogen-like EOM reports |r|^2 + soft core
-    const float rc2 = (float)(P.restep_radius * P.restep_radius +
-                              (TGT2 != ETRAJ_TARGET_SAE ? P.T.sc : 0.0));
+    const float hf = (float)h, h6 = (float)(h / 6);
+    const HlPhase<float> pc = hl_phase<float, PHASE, SOFT>(h, P.T.Ip);
+    const double c0 = (HL && PHASE == ETRAJ_PHASE_QTMC) ? -h * P.T.Ip : 0.0;   // QTMC's -Ip per step (HL form)
+    const float Zf = (float)P.T.Zd, af = (float)P.T.sc;
+    // the hydrogen-like EOM reports |r|^2 + soft core
+    const float rc2 = (float)(P.restep_radius * P.restep_radius + (HL ? P.T.sc : 0.0));
+    // the smallest r2 over the step starts, kept as the bit pattern of a positive float (the
+    // integer order of such patterns is their numeric order: one v_min_u32 per step)
+    uint32_t r2min = 0x7f7fffffu;
     float Fpx = (float)F0.x, Fpy = (float)F0.y;
     for (int64_t s0 = 0; s0 < n; s0 += 32) {
-        // the phase: FP32 Kahan partial sum over <= 32 steps, then one
-        // FP64 Kahan addition per chunk (a wave-uniform loop nest, so the
-        // flush is not executed per step)
-        const int64_t s1 = s0 + 32 < n ? s0 + 32 : n;
-        float qs = 0.0f, qc = 0.0f;
-        for (int64_t s = s0; s < s1; ++s) {
-            // The table entry arrives in scalar registers (a wave-uniform
-            // load); it is copied to vector registers once, because VALU
-            // instructions that read an SGPR operand issue at a lower rate
-            // than VGPR-only ones (6 % of the whole pass on gfx950).
-            const float4 f = T32[s];
-            const float fmx = sgpr_to_vgpr(f.x), fmy = sgpr_to_vgpr(f.y);
-            const float f1x = sgpr_to_vgpr(f.z), f1y = sgpr_to_vgpr(f.w);
-            State<DIM, float> d;
-            float r2m;
-            const float dph =
-                rk4_incr<DIM, PHASE, TGT2, float>(P.T, saeq, us, hf, Fpx, Fpy, fmx, fmy, f1x, f1y, d, r2m);
-            close = close || (r2m < rc2);
+        const int cnt = (int)(s0 + 32 < n ? 32 : n - s0);   // wave-uniform trip count
+        float qs = 0.0f;
+        // one RK4 step with the table entry f = (F at t + h/2, F at t + h)
+        auto step = [&](const float4 f) {
+            float r[DIM], p[DIM], Sr[DIM], Sp[DIM], r2;
 #pragma unroll
             for (int i = 0; i < DIM; ++i) {
-                float y = d.r[i] - uc.r[i];
-                float t = us.r[i] + y;
-                uc.r[i] = (t - us.r[i]) - y;
-                us.r[i] = t;
-                y = d.p[i] - uc.p[i];
-                t = us.p[i] + y;
-                uc.p[i] = (t - us.p[i]) - y;
-                us.p[i] = t;
+                r[i] = rb[i] + dr[i];
+                p[i] = pb[i] + dp[i];
             }
-            if (PHASE != ETRAJ_PHASE_CTMC) {
-                const float y = dph - qc;
-                const float t = qs + y;
-                qc = (t - qs) - y;
-                qs = t;
+            if constexpr (HL) {
+                float SA, SQ;
+                hl_rk4_sums<float, DIM, PHASE, SOFT, UNITZ>(Zf, af, r, p, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, Sr, Sp, SA,
+                                                            SQ, r2);
+                if (PHASE != ETRAJ_PHASE_CTMC) qs = fma(pc.cA, SA, fma(pc.cQ, SQ, qs));
+            } else {
+                float SPh;
+                gen_rk4_sums<float, DIM, PHASE, TGT2>(P.T, saeq, r, p, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, Sr, Sp, SPh,
+                                                      r2);
+                if (PHASE != ETRAJ_PHASE_CTMC) qs = fma(h6, SPh, qs);
             }
-            Fpx = f1x;
-            Fpy = f1y;
+            r2min = min(r2min, __float_as_uint(r2));
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                dr[i] = fma(h6, Sr[i], dr[i]);
+                dp[i] = fma(h6, Sp[i], dp[i]);
+            }
+            Fpx = f.z;
+            Fpy = f.w;
+        };
+        // steps in pairs: the entry of the next step is loaded while this one computes
+        // (a read past the row's end returns zeros and is not used)
+        float4 fa = row_load_f4(T32, (int)(s0 * 16));
+        for (int k = 0; k < cnt; k += 2) {
+            const float4 fb = row_load_f4(T32, (int)((s0 + k + 1) * 16));
+            step(fa);
+            if (k + 1 < cnt) {
+                fa = row_load_f4(T32, (int)((s0 + k + 2) * 16));
+                step(fb);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            r64[i] += (double)dr[i];
+            p64[i] += (double)dp[i];
+            rb[i] = (float)r64[i];
+            pb[i] = (float)p64[i];
+            dr[i] = 0.0f;
+            dp[i] = 0.0f;
         }
         if (PHASE != ETRAJ_PHASE_CTMC) {
-            const double y = ((double)qs - (double)qc) - phc;
+            const double y = ((double)qs + c0 * (double)cnt) - phc;
             const double t = phase + y;
             phc = (t - phase) - y;
             phase = t;
         }
     }
+    close = __uint_as_float(r2min) < rc2;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        rf[i] = (double)us.r[i] - (double)uc.r[i];
-        pf[i] = (double)us.p[i] - (double)uc.p[i];
+        rf[i] = r64[i];
+        pf[i] = p64[i];
+    }
+}
+
+// The FP64 pass of one electron over n steps from t0: the same RK4 step in
+// FP64, with the exact hydrogen-like phase rate (the soft-core term kept).
+// T64 is the segment's row of the FP64 field table (32 bytes per step).
+template <int DIM, int PHASE, int TGT, bool UNITZ>
+__device__ inline void rk4_fp64_pass(const IntegP& P, bool saeq, const Lane<DIM, PHASE>& e, int64_t n, double h,
+                                     __amdgpu_buffer_rsrc_t T64, double2 F0, double* r, double* p, double& phase) {
+    constexpr bool HL = is_hydrogen_like(TGT);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        r[i] = e.r[i];
+        p[i] = e.p[i];
+    }
+    phase = e.phase0;
+    const double h6 = h / 6;
+    const HlPhase<double> pc = hl_phase<double, PHASE, true>(h, P.T.Ip);
+    double Fpx = F0.x, Fpy = F0.y;
+    auto step = [&](const double2 Fm, const double2 F1) {
+        double Sr[DIM], Sp[DIM], r2;
+        if constexpr (HL) {
+            double SA, SQ;
+            hl_rk4_sums<double, DIM, PHASE, true, UNITZ>(P.T.Zd, P.T.sc, r, p, h, Fpx, Fpy, Fm.x, Fm.y, F1.x, F1.y, Sr,
+                                                         Sp, SA, SQ, r2);
+            if (PHASE != ETRAJ_PHASE_CTMC) phase = fma(pc.cA, SA, fma(pc.cQ, SQ, phase));
+            if (PHASE == ETRAJ_PHASE_QTMC) phase += pc.c0;
+        } else {
+            double SPh;
+            gen_rk4_sums<double, DIM, PHASE, TGT>(P.T, saeq, r, p, h, Fpx, Fpy, Fm.x, Fm.y, F1.x, F1.y, Sr, Sp, SPh,
+                                                  r2);
+            if (PHASE != ETRAJ_PHASE_CTMC) phase = fma(h6, SPh, phase);
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            r[i] = fma(h6, Sr[i], r[i]);
+            p[i] = fma(h6, Sp[i], p[i]);
+        }
+        Fpx = F1.x;
+        Fpy = F1.y;
+    };
+    // steps in pairs, the next step's entries loaded while this one computes
+    double2 ma = row_load_d2(T64, 0), ea = row_load_d2(T64, 16);
+    for (int64_t s = 0; s < n; s += 2) {
+        const double2 mb = row_load_d2(T64, (int)((s + 1) * 32)), eb = row_load_d2(T64, (int)((s + 1) * 32 + 16));
+        step(ma, ea);
+        if (s + 1 < n) {
+            ma = row_load_d2(T64, (int)((s + 2) * 32));
+            ea = row_load_d2(T64, (int)((s + 2) * 32 + 16));
+            step(mb, eb);
+        }
     }
 }
 
@@ -896,39 +976,32 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
     n = uniform_i64(n);   // wave-uniform: loop counters stay scalar
     const double h = span / (double)n;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
-    const float4* __restrict__ T32 = FT.f32 + seg * FT.stride;
-    const double2* __restrict__ T64 = FT.f64 + 2 * seg * FT.stride;
+    const __amdgpu_buffer_rsrc_t T32 = row_rsrc(FT.f32 + seg * FT.stride, FT.stride * (int64_t)sizeof(float4));
+    const __amdgpu_buffer_rsrc_t T64 = row_rsrc(FT.f64 + 2 * seg * FT.stride, FT.stride * (int64_t)(2 * sizeof(double2)));
     const double2 F0 = FT.f0[seg];
+    const bool unitz = P.T.Zd == 1.0;
     bool bad = false, close = false;
 
     double rf[DIM], pf[DIM], phase;
-    if (!MIXED) {
-        State<DIM, double> u;
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) { u.r[i] = e.r[i]; u.p[i] = e.p[i]; }
-        phase = e.phase0;
-        double Fpx = F0.x, Fpy = F0.y;
-        for (int64_t s = 0; s < n; ++s) {
-            const double2 Fm = T64[2 * s], F1 = T64[2 * s + 1];
-            State<DIM, double> d;
-            double r2m;
-            double dph = rk4_incr<DIM, PHASE, TGT, double>(P.T, saeq, u, h, Fpx, Fpy, Fm.x, Fm.y, F1.x, F1.y, d, r2m);
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                u.r[i] = u.r[i] + d.r[i];
-                u.p[i] = u.p[i] + d.p[i];
-            }
-            phase = phase + dph;
-            Fpx = F1.x;
-            Fpy = F1.y;
+    if constexpr (!MIXED) {
+        if constexpr (is_hydrogen_like(TGT)) {
+            if (unitz) rk4_fp64_pass<DIM, PHASE, TGT, true>(P, saeq, e, n, h, T64, F0, rf, pf, phase);
+            else rk4_fp64_pass<DIM, PHASE, TGT, false>(P, saeq, e, n, h, T64, F0, rf, pf, phase);
+        } else {
+            rk4_fp64_pass<DIM, PHASE, TGT, false>(P, saeq, e, n, h, T64, F0, rf, pf, phase);
         }
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) { rf[i] = u.r[i]; pf[i] = u.p[i]; }
     } else {
-        if (TGT == ETRAJ_TARGET_HYDROGEN_LIKE && P.T.sc > 1e-6)
-            rk4_fp32_pass<DIM, PHASE, kTargetHSoft>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
-        else
-            rk4_fp32_pass<DIM, PHASE, TGT>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+        if constexpr (is_hydrogen_like(TGT)) {
+            if (P.T.sc > 1e-6) {
+                if (unitz) rk4_fp32_pass<DIM, PHASE, kTargetHSoft, true>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+                else rk4_fp32_pass<DIM, PHASE, kTargetHSoft, false>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+            } else {
+                if (unitz) rk4_fp32_pass<DIM, PHASE, TGT, true>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+                else rk4_fp32_pass<DIM, PHASE, TGT, false>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+            }
+        } else {
+            rk4_fp32_pass<DIM, PHASE, TGT, false>(P, saeq, e, n, h, T32, F0, close, rf, pf, phase);
+        }
         if (active && flags) flags[col_idx] = close ? 1 : 0;
     }
 #pragma unroll

@@ -695,50 +695,164 @@ __host__ __device__ inline void eom(const TargetP& T, bool sae_quirk, const Stat
     }
 }
 
-// One classical RK4 step of size h from state u (stage fields F0 at t,
-// Fm at t+h/2, F1 at t+h). Returns the state increment d, so that
-// u_next = u + d with d = h/6 (k1 + 2 (k2 + k3) + k4), the phase increment
-// and |r|^2 at the start of the step (r2min).
-template <int DIM, int PHASE, int TGT, typename R>
-__host__ __device__ inline R rk4_incr(const TargetP& T, bool sae_quirk, const State<DIM, R>& u, R h,
-                                      R F0x, R F0y, R Fmx, R Fmy, R F1x, R F1y,
-                                      State<DIM, R>& d, R& r2min) {
+// ---------------------------------------------------------------------------
+// One classical RK4 step of size h from (r, p), stage fields F0 at t, Fm at
+// t + h/2 (stages 2 and 3) and F1 at t + h, in the form the step loops use.
+// Since r' = p, the r-slope of stage j is the stage's momentum p_j, and the
+// step is
+//   r += h/6 Sr,  Sr = p_1 + 2 p_2 + 2 p_3 + p_4      (p_1 = p),
+//   p += h/6 Sp,  Sp = k_1 + 2 k_2 + 2 k_3 + k_4      (k_j the stage forces),
+// with stage states (r + h/2 p_1, p + h/2 k_1), (r + h/2 p_2, p + h/2 k_2),
+// (r + h p_3, p + h k_3). The sums are formed stage by stage (progressive
+// FMAs), so the caller adds h/6 Sr to its state with one FMA per component.
+// ---------------------------------------------------------------------------
+
+// Force and phase-rate parts of the hydrogen-like EOM (HydrogenLikeAtom.jl:78-159,
+// V = -Z/sqrt(|r|^2 + a)) at one stage: k = -Z r/r2^(3/2) - F (z: no field),
+// and pa, with the phase rate = cA pa - |p|^2/2 + c0:
+//   QTMC: pa = Z/sqrt(r2), cA = 1, c0 = -Ip;
+//   SCTS: pa = Z/sqrt(r2) (2 - a/r2), cA = 1 (SOFT), or pa = Z/sqrt(r2), cA = 2:
+//         -(p^2/2 + V + r.F_target) with r.F_target = -Z (r2 - a)/r2^(3/2);
+//         the a/r2 term is dropped (not SOFT) only where it is below the
+//         arithmetic's resolution (FP32 with an atomic a = 1e-10).
+// UNITZ: Z = 1 (Z/sqrt(r2) is 1/sqrt(r2)). r2 = |r|^2 + a is returned.
+template <typename R, int DIM, int PHASE, bool SOFT, bool UNITZ>
+__host__ __device__ __forceinline__ void hl_stage(R Z, R a, const R* yr, R Fx, R Fy, R* k, R& pa, R& r2) {
+    R s2 = fma(yr[1], yr[1], a);
+    if (DIM == 3) s2 = fma(yr[DIM - 1], yr[DIM - 1], s2);
+    r2 = fma(yr[0], yr[0], s2);
+    const R ir = rsqrt_t<R>(r2);
+    const R zr = UNITZ ? ir : Z * ir;
+    const R ir2 = ir * ir;
+    const R sm = zr * ir2;
+    k[0] = fma(-sm, yr[0], -Fx);
+    k[1] = fma(-sm, yr[1], -Fy);
+    if (DIM == 3) k[DIM - 1] = -(sm * yr[DIM - 1]);
+    if (PHASE != ETRAJ_PHASE_CTMC) pa = (PHASE == ETRAJ_PHASE_SCTS && SOFT) ? zr * fma(-a, ir2, (R)2) : zr;
+}
+
+// acc + |p|^2 (one FMA per component)
+template <typename R, int DIM>
+__host__ __device__ __forceinline__ R add_p2(const R* p, R acc) {
+#pragma unroll
+    for (int i = DIM - 1; i >= 0; --i) acc = fma(p[i], p[i], acc);
+    return acc;
+}
+
+// The step sums of the hydrogen-like EOM: Sr, Sp as above; SA = sum_j w_j pa_j
+// and SQ = sum_j w_j q_j (w = 1, 2, 2, 1), so that the phase increment over
+// the step is h/6 (cA SA - SQ/2) + h c0. r2first: |r|^2 + a at the step's start.
+template <typename R, int DIM, int PHASE, bool SOFT, bool UNITZ>
+__host__ __device__ __forceinline__ void hl_rk4_sums(R Z, R a, const R* r, const R* p, R h, R F0x, R F0y, R Fmx,
+                                                     R Fmy, R F1x, R F1y, R* Sr, R* Sp, R& SA, R& SQ, R& r2first) {
     const R hh = h * (R)0.5;
-    const R h6 = h / (R)6;
-    State<DIM, R> k, y, k2;
-    R ph, ph2, phacc, r2;
-    eom<DIM, PHASE, TGT, R>(T, sae_quirk, u, F0x, F0y, k, ph, r2);
-    r2min = r2;
-    phacc = ph;
+    constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
+    R k[DIM], yr[DIM], yp[DIM], pa = 0, r2, Q14 = 0, Q23 = 0;   // |p|^2 sums of stages 1 + 4 and 2 + 3
+    hl_stage<R, DIM, PHASE, SOFT, UNITZ>(Z, a, r, F0x, F0y, k, pa, r2first);
+    if (PH) Q14 = add_p2<R, DIM>(p, (R)0);
+    SA = pa;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        d.r[i] = k.r[i];
-        d.p[i] = k.p[i];
-        y.r[i] = u.r[i] + hh * k.r[i];
-        y.p[i] = u.p[i] + hh * k.p[i];
+        Sr[i] = p[i];
+        Sp[i] = k[i];
+        yr[i] = fma(hh, p[i], r[i]);
+        yp[i] = fma(hh, k[i], p[i]);
     }
-    eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k2, ph2, r2);
+    hl_stage<R, DIM, PHASE, SOFT, UNITZ>(Z, a, yr, Fmx, Fmy, k, pa, r2);
+    if (PH) Q23 = add_p2<R, DIM>(yp, (R)0);
+    SA = fma((R)2, pa, SA);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        y.r[i] = u.r[i] + hh * k2.r[i];
-        y.p[i] = u.p[i] + hh * k2.p[i];
+        Sr[i] = fma((R)2, yp[i], Sr[i]);
+        Sp[i] = fma((R)2, k[i], Sp[i]);
+        yr[i] = fma(hh, yp[i], r[i]);
+        yp[i] = fma(hh, k[i], p[i]);
+    }
+    hl_stage<R, DIM, PHASE, SOFT, UNITZ>(Z, a, yr, Fmx, Fmy, k, pa, r2);
+    if (PH) Q23 = add_p2<R, DIM>(yp, Q23);
+    SA = fma((R)2, pa, SA);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        Sr[i] = fma((R)2, yp[i], Sr[i]);
+        Sp[i] = fma((R)2, k[i], Sp[i]);
+        yr[i] = fma(h, yp[i], r[i]);
+        yp[i] = fma(h, k[i], p[i]);
+    }
+    hl_stage<R, DIM, PHASE, SOFT, UNITZ>(Z, a, yr, F1x, F1y, k, pa, r2);
+    if (PH) Q14 = add_p2<R, DIM>(yp, Q14);
+    SA = SA + pa;
+    SQ = fma((R)2, Q23, Q14);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        Sr[i] = Sr[i] + yp[i];
+        Sp[i] = Sp[i] + k[i];
+    }
+}
+
+// The phase-increment constants of hl_rk4_sums for a step of size h:
+// increment = cA SA + cQ SQ + c0 (cA = h/6 or h/3, cQ = -h/12, c0 = -h Ip for QTMC).
+template <typename R>
+struct HlPhase {
+    R cA, cQ, c0;
+};
+template <typename R, int PHASE, bool SOFT>
+__host__ __device__ inline HlPhase<R> hl_phase(double h, double Ip) {
+    HlPhase<R> c;
+    const bool twice = PHASE == ETRAJ_PHASE_SCTS && !SOFT;
+    c.cA = (R)(twice ? h / 3 : h / 6);
+    c.cQ = (R)(-h / 12);
+    c.c0 = (R)(PHASE == ETRAJ_PHASE_QTMC ? -h * Ip : 0.0);
+    return c;
+}
+
+// The same sums for any target through its full EOM (eom): Sr, Sp as above and
+// SPh = sum_j w_j phase'_j (phase increment h/6 SPh).
+template <typename R, int DIM, int PHASE, int TGT>
+__host__ __device__ __forceinline__ void gen_rk4_sums(const TargetP& T, bool sae_quirk, const R* r, const R* p, R h,
+                                                      R F0x, R F0y, R Fmx, R Fmy, R F1x, R F1y, R* Sr, R* Sp,
+                                                      R& SPh, R& r2first) {
+    const R hh = h * (R)0.5;
+    State<DIM, R> u, y, k;
+    R ph, r2;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        u.r[i] = r[i];
+        u.p[i] = p[i];
+    }
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, u, F0x, F0y, k, ph, r2first);
+    SPh = ph;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        Sr[i] = p[i];
+        Sp[i] = k.p[i];
+        y.r[i] = fma(hh, p[i], r[i]);
+        y.p[i] = fma(hh, k.p[i], p[i]);
     }
     eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
+    SPh = fma((R)2, ph, SPh);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        d.r[i] += (R)2 * (k2.r[i] + k.r[i]);
-        d.p[i] += (R)2 * (k2.p[i] + k.p[i]);
-        y.r[i] = u.r[i] + h * k.r[i];
-        y.p[i] = u.p[i] + h * k.p[i];
+        Sr[i] = fma((R)2, y.p[i], Sr[i]);
+        Sp[i] = fma((R)2, k.p[i], Sp[i]);
+        y.r[i] = fma(hh, y.p[i], r[i]);
+        y.p[i] = fma(hh, k.p[i], p[i]);
     }
-    phacc += (R)2 * (ph2 + ph);
+    eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, Fmx, Fmy, k, ph, r2);
+    SPh = fma((R)2, ph, SPh);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        Sr[i] = fma((R)2, y.p[i], Sr[i]);
+        Sp[i] = fma((R)2, k.p[i], Sp[i]);
+        y.r[i] = fma(h, y.p[i], r[i]);
+        y.p[i] = fma(h, k.p[i], p[i]);
+    }
     eom<DIM, PHASE, TGT, R>(T, sae_quirk, y, F1x, F1y, k, ph, r2);
+    SPh = SPh + ph;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        d.r[i] = h6 * (d.r[i] + k.r[i]);
-        d.p[i] = h6 * (d.p[i] + k.p[i]);
+        Sr[i] = Sr[i] + y.p[i];
+        Sp[i] = Sp[i] + k.p[i];
     }
-    return h6 * (phacc + ph);
 }
 
 // ---------------------------------------------------------------------------

@@ -43,7 +43,9 @@ def test_headline_job_at_full_size(ctx):
     err = l1(mixed, fp64)
     print("full-size PMD L1 mixed vs FP64", err, "re-integrated", sm["n_restep"])
     assert err <= 1e-3
-    assert sm["ion_prob"] == pytest.approx(s64["ion_prob"], rel=1e-9)
+    # a few of the 1e8 electrons end with an energy within rounding of zero, where FP32 and FP64
+    # may disagree on ionized / bound (one electron's weight is ~1e-8 of the total)
+    assert sm["ion_prob"] == pytest.approx(s64["ion_prob"], rel=1e-6)
     assert 0 < sm["n_restep"] < 0.1 * sm["num_effective_traj"]
 
 

@@ -101,3 +101,105 @@ def test_stage_field_equals_the_direct_field(results):
 def test_cosh_sinh_from_one_exp(results):
     ulp = 2.0 ** -52
     assert results[1] <= 3 * ulp and results[2] <= 3 * ulp
+
+
+# ---- the device's RK4 step, host-compiled, against the oracle's RK4 ------------------------
+
+STEP_SRC = r"""
+#include "etraj_physics.hpp"
+#include <cstdio>
+#include <cstdlib>
+using namespace etraj;
+// Integrates one electron (x y px py phase) from t0 to tf with the device's RK4 step sums in
+// FP64: form 0 = hl_rk4_sums (hydrogen-like, exact soft-core phase rate), form 1 = gen_rk4_sums
+// (the full EOM), the field evaluated at the stage times. Prints the final state.
+int main(int argc, char** argv) {
+    const int form = atoi(argv[1]);
+    const double t0 = atof(argv[2]), tf = atof(argv[3]), dt = atof(argv[4]);
+    double r[2] = {atof(argv[5]), atof(argv[6])}, p[2] = {atof(argv[7]), atof(argv[8])}, ph = atof(argv[9]);
+    etraj_laser_desc d{};
+    d.n_colors = 1;
+    d.color[0].kind = ETRAJ_LASER_COS2; d.color[0].peak_int = 1e14; d.color[0].wave_len = 800; d.color[0].cyc_num = 2;
+    etraj_target_desc td{};
+    td.kind = ETRAJ_TARGET_HYDROGEN_LIKE; td.nucl_charge = 1; td.Ip = 0.5; td.asymp_coeff = 1; td.soft_core = 1e-10;
+    LaserP L; TargetP T;
+    if (make_laser(d, L) || make_target(td, T)) return 2;
+    int64_t n = (int64_t)ceil((tf - t0) / dt); if (n < 1) n = 1;
+    const double h = (tf - t0) / n, h6 = h / 6;
+    const HlPhase<double> pc = hl_phase<double, ETRAJ_PHASE_SCTS, true>(h, T.Ip);
+    for (int64_t s = 0; s < n; ++s) {
+        const double t = t0 + s * h;
+        double F0x, F0y, Fmx, Fmy, F1x, F1y, Sr[2], Sp[2], r2;
+        laser_field(L, t, F0x, F0y);
+        laser_field(L, t0 + (2 * s + 1) * (h / 2), Fmx, Fmy);
+        laser_field(L, t0 + (2 * s + 2) * (h / 2), F1x, F1y);
+        if (form == 0) {
+            double SA, SQ;
+            hl_rk4_sums<double, 2, ETRAJ_PHASE_SCTS, true, true>(T.Zd, T.sc, r, p, h, F0x, F0y, Fmx, Fmy, F1x, F1y,
+                                                                   Sr, Sp, SA, SQ, r2);
+            ph = fma(pc.cA, SA, fma(pc.cQ, SQ, ph));
+        } else {
+            double SPh;
+            gen_rk4_sums<double, 2, ETRAJ_PHASE_SCTS, ETRAJ_TARGET_HYDROGEN_LIKE>(T, false, r, p, h, F0x, F0y, Fmx, Fmy,
+                                                                              F1x, F1y, Sr, Sp, SPh, r2);
+            ph = fma(h6, SPh, ph);
+        }
+        for (int i = 0; i < 2; ++i) { r[i] = fma(h6, Sr[i], r[i]); p[i] = fma(h6, Sp[i], p[i]); }
+    }
+    printf("%.17g %.17g %.17g %.17g %.17g\n", r[0], r[1], p[0], p[1], ph);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def step_exe(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    d = tmp_path_factory.mktemp("step")
+    (d / "step.cpp").write_text(STEP_SRC)
+    exe = d / "step"
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    # FMA contraction on, as the device compiler does
+    subprocess.run([HIPCC, "-x", "c++", "-O2", "-std=c++17", "-mfma", "-ffp-contract=fast", "-D__HIP_PLATFORM_AMD__",
+                    "-I", os.path.join(rocm, "include"), "-I", os.path.join(ROOT, "etraj.jl_amd", "csrc"),
+                    str(d / "step.cpp"), "-o", str(exe)], check=True, capture_output=True)
+    return str(exe)
+
+
+def test_device_rk4_step_forms_match_the_oracle(step_exe):
+    """The RK4 step in the arrangement the kernels use (stage sums Sr, Sp formed by FMAs; the
+    hydrogen-like phase rate split into Z/sqrt(r2) (2 - a/r2) and |p|^2 parts), host-compiled in
+    FP64 with FMA contraction, against the oracle's RK4 on the full (r, p, phase) system
+    (TrajectorySimulation.jl:332-342 with the fixed step): tunnel exits of the H-atom LP pulse,
+    integrated to t_final. Both forms agree with the oracle to 1e-11 relative in momentum and
+    1e-9 rad in the SCTS phase (rounding differences over ~2000 steps)."""
+    import math
+    import sys
+    sys.path[:0] = [os.path.join(ROOT, "etraj.jl_amd"), os.path.join(ROOT, "oracle")]
+    import numpy as np
+    import oracle as O
+    from etraj_hip import Cos2Laser, get_atom
+    from etraj_hip.simulation import run_desc
+    laser = Cos2Laser(peak_int=1e14, wave_len=800.0, cyc_num=2, ellip=0.0)
+    ld, td = laser._desc(), get_atom("H")._desc()
+    tf = 2 * math.pi / (45.563352525 / 800.0) + 10.0
+    rd = run_desc(dimension=2, traj_phase_method="SCTS", traj_t_final=tf, final_p_max=(1.5, 1.5),
+                  final_p_num=(100, 100), traj_integrator="rk4", traj_dt=0.1, precision="fp64")
+    rng = np.random.default_rng(7)
+    worst_p = worst_ph = 0.0
+    for t0 in (-80.0, -41.3, -3.0, 27.5, 55.0):
+        Fx = O.laser_field(ld, [t0])[0][0]
+        for kd in rng.uniform(-0.8, 0.8, 3):
+            x0 = -(0.5 + kd * kd / 2) / Fx
+            u0 = np.array([x0, 0.0, 0.0, kd, 0.0])
+            ref, _, rc = O.integrate_state(ld, td, rd, t0, u0)
+            assert rc == 0
+            for form in (0, 1):
+                out = subprocess.run([step_exe, str(form), "%.17g" % t0, "%.17g" % tf, "0.1"] + ["%.17g" % v for v in u0],
+                                     capture_output=True, text=True, check=True).stdout.split()
+                got = np.array([float(v) for v in out])
+                worst_p = max(worst_p, np.linalg.norm(got[2:4] - ref[2:4]) / np.linalg.norm(ref[2:4]))
+                worst_ph = max(worst_ph, abs(got[4] - ref[4]))
+    assert worst_p < 1e-11, worst_p
+    assert worst_ph < 1e-9, worst_ph

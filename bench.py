@@ -31,12 +31,22 @@ from etraj_hip import _abi, workloads  # noqa: E402
 from etraj_hip.samplers import ADKSampler  # noqa: E402
 from etraj_hip.simulation import run_desc  # noqa: E402
 
-# Algorithmic FLOPs of one RK4 step of the 2D SCTS hydrogen-like integrator kernel (add/mul = 1,
-# FMA = 2, rsqrt = 1): 4 RHS x 20 + stage states 24 + combination 20 + phase 5 = 129, plus the
-# state update (FP32 pass: 4 x FP32 Kahan 4 + FP32 Kahan phase 4 = 20; FP64: 5). The laser field
-# is read from the per-batch table built by a separate kernel and is not counted here.
-FLOPS_PER_STEP = {"mixed": 149.0, "fp64": 134.0}
+# Algorithmic FLOPs of one RK4 step of the 2D SCTS hydrogen-like system (r, p, phase) (add/mul = 1,
+# FMA = 2, rsqrt = 1): 4 RHS x 20 + stage states 24 + combination and update 20 + phase 5 = 129,
+# in either precision (the FP32 pass's compensation adds and the FP64 folds are implementation,
+# not counted). The laser field is read from the per-batch table built by a separate kernel and
+# is not counted here.
+FLOPS_PER_STEP = {"mixed": 129.0, "fp64": 129.0}
+# Algorithmic FLOPs of one attempted Tsit5 step of the same 2D SCTS hydrogen-like system with a
+# monochromatic cos^2 LP laser: stage inputs 202, field at 5 stage times by angle addition 545 (+ the
+# step's base sin/cos 9), 6 FP64 RHS 108, error sums 70, RMS norm 47, PI controller 14 = 996 (the
+# breakdown is in the design notes).
+TSIT5_FLOPS_PER_ATTEMPT = 996.0
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}   # MI355X vector FP32 / FP64 (published)
+# The reference's published CPU timings (docs/src/example_overview.md:12-22): ADK-CTMC, 2-cycle CP
+# cos^4 4e14 W/cm^2, 54.2M electrons, Tsit5 at traj_rtol 1e-6, AMD Ryzen 9 7950X.
+PUBLISHED_CPU = {"us_per_traj_12_cores": 4.3, "us_per_traj_1_core": 23.0, "cpu": "AMD Ryzen 9 7950X",
+                 "source": "docs/src/example_overview.md:12-22 (ADK-CTMC 2-cycle CP cos4 4e14, Tsit5 rtol 1e-6)"}
 
 
 def log(*a):
@@ -121,59 +131,139 @@ def accuracy(kw, ctx, n_strata=64):
     return out
 
 
-def cpu_baseline(kw, ctx, n_batches_sample, threads):
-    """Oracle (FP64 RK4, same algorithm; and Tsit5 rtol 1e-6, the reference's algorithm) on
-    n_batches_sample strata spread evenly over the pulse (the middle stratum of each of
-    n_batches_sample equal groups); the GPU runs the same strata through gen_electron_batch +
-    integrate_and_bin for the L1 comparison."""
+def _strata(n_total, n_pick):
+    """n_pick strata spread over n_total: the middle one of each of n_pick equal groups."""
+    stride = max(1, n_total // n_pick)
+    return stride // 2, stride
+
+
+def host_cpu():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", aff))
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": share, "model": model,
+            "threads": max(1, min(aff, share))}
+
+
+def cpu_timing(kw, threads):
+    """The CPU baseline: the oracle (the C FP64 restatement of the reference's CPU method) built
+    with -O3 -march=native on this host, batches scheduled dynamically over the threads, timed on
+    all the host cores this process may use and on one core, for the reference's algorithm (Tsit5
+    at traj_rtol 1e-6) and for the GPU's (RK4 dt 0.1). Each run covers a bounded sample of the
+    workload's strata (spread over the pulse) chosen for a few seconds of CPU time."""
+    import tempfile
     import oracle as O
-    ld, td, sd, rd = descs(dict(kw, precision="fp64"))
-    stride = max(1, sd.sample_t_num // n_batches_sample)
-    first = stride // 2
-    t = time.perf_counter()
-    ref, st = O.run_job(ld, td, sd, rd, nthreads=threads, batch_begin=first, batch_stride=stride)
-    dt_rk4 = time.perf_counter() - t
-    n = st["num_effective_traj"]
-    # reference algorithm on every second of those strata
-    ldt, tdt, sdt, rdt = descs(dict(kw, precision="fp64", traj_integrator="tsit5"))
-    t = time.perf_counter()
-    _, stt = O.run_job(ldt, tdt, sdt, rdt, nthreads=threads, batch_begin=first, batch_stride=2 * stride)
-    dt_ts = time.perf_counter() - t
-    # the GPU (mixed) on the same strata
-    ldm, tdm, sdm, rdm = descs(kw)
-    npix = rdm.final_p_num[0] * rdm.final_p_num[1]
-    amp = np.zeros(2 * npix)
-    ip, iu, nf = _abi.C.c_double(0), _abi.C.c_double(0), _abi.C.c_int64(0)
-    cap = int(ctx.lib.etraj_batch_max_size(sdm))
-    buf = np.empty((cap, 7))
+    tmp = tempfile.mkdtemp(prefix="etraj_oracle_")
+    path = O.build_native(tmp)
+    L = O.load_variant(path) if path else None
+    build = "gcc -O3 -march=native, built on this host" if L else "prebuilt -O2 (native build failed)"
+    out = {"build": build}
+    d_ts = descs(dict(kw, precision="fp64", traj_integrator="tsit5", traj_rtol=1e-6))
+    d_rk = descs(dict(kw, precision="fp64"))
+    n_t = d_ts[2].sample_t_num
+    # (threads, strata): the sampled strata are the middle ones of equal groups; the one-core RK4
+    # run takes the single stratum at 1/5 of the pulse (the flank of the first field crest)
+    plan = {("tsit5", "all"): (threads, 200), ("tsit5", "one"): (1, 12),
+            ("rk4", "all"): (threads, 10), ("rk4", "one"): (1, 0)}
+    for (alg, cores), (nth, n_strata) in plan.items():
+        ld, td, sd, rd = d_ts if alg == "tsit5" else d_rk
+        if n_strata:
+            first, stride = _strata(n_t, n_strata)
+            last = 0
+        else:
+            first, stride, last = n_t // 5, 1, n_t // 5 + 1
+        t = time.perf_counter()
+        _, st = O.run_job(ld, td, sd, rd, nthreads=nth, batch_begin=first, batch_end=last, batch_stride=stride,
+                          dynamic=True, library=L)
+        dt = time.perf_counter() - t
+        n = st["num_effective_traj"]
+        out[f"{alg}_{'all_cores' if cores == 'all' else '1_core'}"] = {"value": n / dt, "unit": "trajectories/s", "threads": nth, "seconds": dt,
+                                       "electrons": n, "us_per_traj": 1e6 * dt / n,
+                                       "us_per_traj_x_cores": 1e6 * dt * nth / n,
+                                       "sample": (f"every {stride}-th of {n_t} strata from {first}" if n_strata
+                                                  else f"stratum {first} of {n_t}")}
+        log(f"cpu {alg} on {nth} threads: {n / dt:.0f} traj/s ({n} electrons, {dt:.1f} s)")
+    return out
+
+
+def reference_comparison(kw, ctx, n_strata, threads):
+    """The GPU headline against the reference's own algorithm on the same electrons: on n_strata
+    strata spread over the pulse, the oracle's FP64 Tsit5 at traj_rtol 1e-6 (the reference's
+    integrator, TrajectorySimulation.jl:342) and FP64 RK4 dt 0.1 PMDs, against the device's mixed
+    RK4 (the headline's algorithm) and FP64 Tsit5 on the same columns (gen_electron_batch +
+    integrate_and_bin); per-electron momentum and phase errors of the headline against the
+    oracle's Tsit5 and RK4 on every 4th electron of 8 of the strata."""
+    import oracle as O
+    d64 = descs(dict(kw, precision="fp64"))
+    dts = descs(dict(kw, precision="fp64", traj_integrator="tsit5", traj_rtol=1e-6))
+    dmx = descs(kw)
+    dgts = descs(dict(kw, precision="fp64", traj_integrator="tsit5", traj_rtol=1e-6))
+    ld, td, sd, _ = d64
+    first, stride = _strata(sd.sample_t_num, n_strata)
+    ref_ts, st_ts = O.run_job(*dts, nthreads=threads, batch_begin=first, batch_stride=stride)
+    ref_rk, _ = O.run_job(*d64, nthreads=threads, batch_begin=first, batch_stride=stride)
+    npix = dmx[3].final_p_num[0] * dmx[3].final_p_num[1]
+    rows = 7
+    cap = int(ctx.lib.etraj_batch_max_size(sd))
+    buf = np.empty((cap, rows))
     cnt = _abi.C.c_int64()
-    for b in range(first, sdm.sample_t_num, stride):
-        ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, ldm, tdm, sdm, b + 1, _abi.dptr(buf), cap,
+    amp = {"mixed_rk4": np.zeros(2 * npix), "fp64_tsit5": np.zeros(2 * npix)}
+    for b in range(first, sd.sample_t_num, stride):
+        ctx.check(ctx.lib.etraj_gen_electron_batch(ctx.handle, ld, td, sd, b + 1, _abi.dptr(buf), cap,
                                                    _abi.C.byref(cnt)))
-        if cnt.value:
-            ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ldm, tdm, rdm, _abi.dptr(buf), cnt.value, 7,
-                                                      amp.ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
+        if not cnt.value:
+            continue
+        for key, rd in (("mixed_rk4", dmx[3]), ("fp64_tsit5", dgts[3])):
+            ip, iu, nf = _abi.C.c_double(0), _abi.C.c_double(0), _abi.C.c_int64(0)
+            ctx.check(ctx.lib.etraj_integrate_and_bin(ctx.handle, ld, td, rd, _abi.dptr(buf), cnt.value, rows,
+                                                      amp[key].ctypes.data_as(_abi.C.c_void_p), _abi.C.byref(ip),
                                                       _abi.C.byref(iu), _abi.C.byref(nf)))
-    gpu = amp[0::2] ** 2 + amp[1::2] ** 2
-    # per-electron errors against the oracle: every 4th electron of 8 of the sampled strata
-    cols = _device_columns(ctx, ldm, tdm, sdm, range(first, sdm.sample_t_num, stride * max(1, n_batches_sample // 8)),
-                           7)[::4]
-    t = time.perf_counter()
-    rref, sref = O.integrate_records(ld, td, rd, cols)
-    dt_rec = time.perf_counter() - t
-    per_e = {"sample": f"{len(cols)} electrons, oracle FP64 RK4 records ({dt_rec:.1f} s, one thread)",
-             "gpu_mixed": _per_electron_errors(*_device_records(ctx, ldm, tdm, rdm, cols), rref, sref),
-             "gpu_fp64": _per_electron_errors(*_device_records(ctx, ldm, tdm, rd, cols), rref, sref)}
+    gpu = {k: v[0::2] ** 2 + v[1::2] ** 2 for k, v in amp.items()}
+    cols = _device_columns(ctx, ld, td, sd, range(first, sd.sample_t_num, stride * max(1, n_strata // 8)), rows)[::4]
+    rts, sts = O.integrate_records(*dts[:2], dts[3], cols)
+    rrk, srk = O.integrate_records(ld, td, d64[3], cols)
+    rmx, smx = _device_records(ctx, ld, td, dmx[3], cols)
+    r64, s64 = _device_records(ctx, ld, td, d64[3], cols)
     return {
-        "value": n / dt_rk4, "unit": "trajectories/s", "cores": threads, "kind": "port",
-        "sample": f"{n} electrons = the strata {first}, {first + stride}, ... (every {stride}-th of "
-                  f"{sd.sample_t_num}), FP64 RK4 dt 0.1 (same algorithm as the GPU), {dt_rk4:.1f} s",
-        "per_electron_vs_oracle": per_e,
-        "reference_algorithm_value": stt["num_effective_traj"] / dt_ts,
-        "reference_algorithm_sample": f"{stt['num_effective_traj']} electrons (every {2 * stride}-th stratum), "
-                                      f"Tsit5 rtol 1e-6 (the reference's integrator), {dt_ts:.1f} s",
-        "gpu_mixed_pmd_l1_vs_cpu_fp64": l1(gpu, ref),
+        "sample": f"{st_ts['num_effective_traj']} electrons = every {stride}-th of {sd.sample_t_num} strata from {first}",
+        "l1_vs_reference_algorithm": l1(gpu["mixed_rk4"], ref_ts),
+        "l1_oracle_rk4_vs_oracle_tsit5": l1(ref_rk, ref_ts),
+        "l1_gpu_mixed_rk4_vs_oracle_rk4": l1(gpu["mixed_rk4"], ref_rk),
+        "l1_gpu_tsit5_vs_oracle_tsit5": l1(gpu["fp64_tsit5"], ref_ts),
+        "per_electron": {"sample": f"{len(cols)} electrons",
+                         "gpu_mixed_rk4_vs_oracle_tsit5": _per_electron_errors(rmx, smx, rts, sts),
+                         "gpu_mixed_rk4_vs_oracle_rk4": _per_electron_errors(rmx, smx, rrk, srk),
+                         "gpu_fp64_rk4_vs_oracle_rk4": _per_electron_errors(r64, s64, rrk, srk)},
     }
+
+
+def tsit5_entry(kw, ctx, steps, warmup):
+    """The headline workload through the device's adaptive Tsit5 at traj_rtol 1e-6 in FP64 (the
+    reference's integrator): whole-job throughput and the k_tsit5 roofline."""
+    d = descs(dict(kw, precision="fp64", traj_integrator="tsit5", traj_rtol=1e-6))
+    for _ in range(warmup):
+        run_device(ctx, d)
+    t = time.perf_counter()
+    sts = [run_device(ctx, d) for _ in range(steps)]
+    sec = (time.perf_counter() - t) / steps
+    st = sts[-1]
+    launch_ms = float(np.mean([s.ms_integrate_main for s in sts])) / max(st.n_slabs, 1)
+    f = TSIT5_FLOPS_PER_ATTEMPT * st.n_attempts / max(st.n_slabs, 1)
+    ach = f / (launch_ms / 1e3) / 1e12
+    return {"value": st.num_effective_traj / sec, "unit": "trajectories/s", "ms_per_step": sec * 1e3,
+            "electrons": st.num_effective_traj, "steps_accepted": st.n_steps, "steps_attempted": st.n_attempts,
+            "failed": st.n_failed, "dtype": "fp64",
+            "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                         "frac": ach / PEAK_TFLOPS["fp64"], "kernel": "k_tsit5 FP64",
+                         "launches_per_job": st.n_slabs, "avg_launch_ms": launch_ms, "flops_per_launch": f,
+                         "flops_per_attempt": TSIT5_FLOPS_PER_ATTEMPT, "timing": "HIP events on the ctx stream"}}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_rk4_h_lp_scts_strat_1e8_pmc.md")
@@ -263,27 +353,31 @@ def _roofline_rk4(st_list, steps, flops32, flops64):
 
 
 def _cpu_sample(ds, threads, n_jobs_sample=2, stride=None):
-    """The oracle on a bounded sample: every stride-th batch of the first n_jobs_sample jobs."""
+    """The oracle (-O3 -march=native build, dynamic schedule) on a bounded sample: every stride-th
+    batch of the first n_jobs_sample jobs."""
+    import tempfile
     import oracle as O
+    path = O.build_native(tempfile.mkdtemp(prefix="etraj_oracle_"))
+    L = O.load_variant(path) if path else None
     n = 0
     t = time.perf_counter()
     for (ld, td, sd, rd) in ds[:n_jobs_sample]:
         s = stride or max(1, sd.sample_t_num // 8)
-        _, st = O.run_job(ld, td, sd, rd, nthreads=threads, batch_begin=s // 2, batch_stride=s)
+        _, st = O.run_job(ld, td, sd, rd, nthreads=threads, batch_begin=s // 2, batch_stride=s, dynamic=True,
+                          library=L)
         n += st["num_effective_traj"]
     return n, time.perf_counter() - t
 
 
 def bench_workload(name, args, ctx):
-    threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    threads = host_cpu()["threads"]
     if name == "h-ctmc-grid":
         kw = workloads.h_lp_ctmc_grid(n_t=2000, n_kd=50)
         jobs = [dict(kw, precision="mixed", pmd_accumulator="fixed")]
         desc = ("H-atom ADK CTMC, 800 nm 1e14 W/cm^2 2-cycle LP cos2, 2000 t0 x 50 kd grid, RK4 dt 0.1, "
                 "PMD 500x500: fixed-point weight sums + per-cell hit counts")
     elif name == "ar-spa-3d":
-        kw = workloads.ar_cp_spa_qtmc_3d(n_t=args.n_t if args.n_t != 10000 else 2000,
-                                         n_k=args.n_k if args.n_k != 32000 else 158)
+        kw = workloads.ar_cp_spa_qtmc_3d(n_t=args.n_t or 2000, n_k=args.n_k or 158)
         jobs = [kw]
         desc = (f"Ar (SAE, l=0 prefactor) SFA-SPA 3D grid {kw['sample_t_num']} x {kw['ss_kd_num']} x "
                 f"{kw['ss_kz_num']}, CP 800 nm 2e14 W/cm^2 2 cycles, Tsit5 rtol 1e-8 ({kw['precision']}), QTMC, "
@@ -323,10 +417,10 @@ def bench_workload(name, args, ctx):
                       "integrate_main": sts[-1][0].ms_integrate_main},
         "fp64_reintegrated_fraction": sum(s.n_restep for s in sts[-1]) / max(n_el, 1),
         "pmd_accumulator": "fixed" if ds[0][3].pmd_mode == _abi.PMD_FIXED else "float",
-        # FP32-pass FLOPs per step: 2D SCTS hydrogen-like 149 (+12 for the molecule's soft-core term);
-        # 2D CTMC 116 (no phase: 4 RHS x 6 + phase combination 5 + phase Kahan 4 fewer)
-        "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 161.0, "h-ctmc-grid": 116.0}.get(name, 149.0),
-                                   134.0) if (mixed and rk4) else None),
+        # algorithmic FLOPs per RK4 step: 2D SCTS hydrogen-like 129 (+8 for the molecule's soft-core
+        # term, 2 per RHS); 2D CTMC 100 (no phase: 4 RHS x 6 + phase combination 5 fewer)
+        "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 137.0, "h-ctmc-grid": 100.0}.get(name, 129.0),
+                                   129.0) if (mixed and rk4) else None),
     }
     if hits is not None:
         result["hit_counts_per_run"] = int(hits.sum()) // (args.steps + args.warmup)
@@ -354,10 +448,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n-t", type=int, default=10000)
-    ap.add_argument("--n-k", type=int, default=34400)
+    ap.add_argument("--n-t", type=int, default=None, help="t0 samples (default: the workload's own)")
+    ap.add_argument("--n-k", type=int, default=None, help="momentum samples per t0 (default: the workload's own)")
     ap.add_argument("--cpu-batches", type=int, default=24, help="strata in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tsit5", action="store_true", help="skip the device Tsit5 entry")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM-counter calibration)")
     ap.add_argument("--workload", default="h-scts-1e8",
@@ -378,7 +473,7 @@ def main():
     if args.workload != "h-scts-1e8":
         print(json.dumps(bench_workload(args.workload, args, ctx)), flush=True)
         return
-    kw = workloads.h_lp_scts_stratified(n_t=args.n_t, n_k=args.n_k, precision="mixed")
+    kw = workloads.h_lp_scts_stratified(n_t=args.n_t or 10000, n_k=args.n_k or 34400, precision="mixed")
     if args.restep_radius is not None:
         kw["restep_radius"] = args.restep_radius
     d_mixed = descs(kw)
@@ -453,10 +548,20 @@ def main():
         a = result["accuracy_mixed_vs_fp64_gpu"]
         log(f"per electron (mixed vs fp64 GPU, {a['electrons']}): p_inf RMS {a['p_rms']:.2e}, "
             f"phase RMS mod 2pi {a['phase_rms_mod_2pi']:.2e}")
+    if not args.no_tsit5:
+        result["tsit5_entry"] = tsit5_entry(kw, ctx, max(1, args.steps // 2), 1)
+        log(f"tsit5 entry: {result['tsit5_entry']['ms_per_step']:.1f} ms per job")
     if args.cpu_batches > 0:
-        threads = int(os.environ.get("ETRAJ_CPU_THREADS", min(os.cpu_count() or 1, 16)))
-        result["cpu_baseline"] = cpu_baseline(kw, ctx, args.cpu_batches, threads)
-        log("cpu baseline:", result["cpu_baseline"]["sample"])
+        cpu = host_cpu()
+        result["reference_algorithm"] = reference_comparison(kw, ctx, args.cpu_batches, cpu["threads"])
+        log(f"L1 vs the reference algorithm: {result['reference_algorithm']['l1_vs_reference_algorithm']:.3e}")
+        tm = cpu_timing(kw, cpu["threads"])
+        ta = tm["tsit5_all_cores"]
+        result["cpu_baseline"] = {
+            "value": ta["value"], "unit": "trajectories/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"the oracle's FP64 Tsit5 at traj_rtol 1e-6 (the reference's algorithm), {tm['build']}, "
+                      f"{ta['electrons']} electrons ({ta['sample']}), {ta['seconds']:.1f} s on {cpu['threads']} threads",
+            "host": cpu, "timings": tm, "published": PUBLISHED_CPU}
     print(json.dumps(result), flush=True)
 
 

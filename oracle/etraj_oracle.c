@@ -1672,6 +1672,7 @@ typedef struct {
     int b_begin, b_end, b_stride;
     int tid, nthreads;
     int64_t pmd_len;
+    int64_t* next;   /* non-NULL: batches are taken one at a time from this shared counter */
     acc_t A;
     int rc;
 } worker_t;
@@ -1682,10 +1683,17 @@ static void* worker_main(void* arg) {
     int rows = W->R->rows;
     double* buf = (double*)malloc(sizeof(double) * rows * (cap > 0 ? cap : 1));
     if (!buf) { W->rc = ETRAJ_ENOMEM; return NULL; }
-    /* static, contiguous partition of the selected batches over threads */
+    /* static, contiguous partition of the selected batches over threads, or (next != NULL)
+       dynamic: each thread takes the next batch from a shared counter, as Julia 1.11's
+       @threads :dynamic schedule does */
     int64_t nsel = (W->b_end - W->b_begin + W->b_stride - 1) / W->b_stride;
     int64_t lo = nsel * W->tid / W->nthreads, hi = nsel * (W->tid + 1) / W->nthreads;
+    if (W->next) { lo = 0; hi = nsel; }
     for (int64_t s = lo; s < hi; ++s) {
+        if (W->next) {
+            s = __atomic_fetch_add(W->next, 1, __ATOMIC_RELAXED);
+            if (s >= nsel) break;
+        }
         int b = W->b_begin + (int)(s * W->b_stride);
         int64_t c = gen_batch(W->S, b, buf, cap);
         if (c <= 0) continue;
@@ -1703,9 +1711,9 @@ static void* worker_main(void* arg) {
     return NULL;
 }
 
-int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
-                   const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end, int32_t batch_stride,
-                   double* pmd_out, etraj_stats* st) {
+static int run_job_impl(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                        const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end,
+                        int32_t batch_stride, int dynamic, double* pmd_out, etraj_stats* st) {
     sampler_t S;
     runner_t R;
     int rc = sampler_init(ld, td, sd, &S);
@@ -1723,9 +1731,11 @@ int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, cons
     int64_t len = coh ? 2 * npix : npix;
     worker_t* W = (worker_t*)calloc(nthreads, sizeof(worker_t));
     pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+    int64_t next = 0;
     if (!W || !th) { free(W); free(th); sampler_free(&S); return ETRAJ_ENOMEM; }
     for (int i = 0; i < nthreads; ++i) {
         W[i].S = &S; W[i].R = &R;
+        W[i].next = dynamic ? &next : NULL;
         W[i].b_begin = batch_begin; W[i].b_end = batch_end; W[i].b_stride = batch_stride;
         W[i].tid = i; W[i].nthreads = nthreads; W[i].pmd_len = len;
         W[i].A.pmd = (double*)calloc(len, sizeof(double));
@@ -1760,6 +1770,20 @@ int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, cons
     sampler_free(&S);
     if (st) *st = s;
     return rc;
+}
+
+int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                   const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end, int32_t batch_stride,
+                   double* pmd_out, etraj_stats* st) {
+    return run_job_impl(ld, td, sd, rd, nthreads, batch_begin, batch_end, batch_stride, 0, pmd_out, st);
+}
+
+/* the same with dynamic scheduling of the batches over the threads (timing runs: the PMD's
+   summation order then depends on the thread timing) */
+int oracle_run_job_dynamic(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                           const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end,
+                           int32_t batch_stride, double* pmd_out, etraj_stats* st) {
+    return run_job_impl(ld, td, sd, rd, nthreads, batch_begin, batch_end, batch_stride, 1, pmd_out, st);
 }
 
 /* amplitude-level job output (before abs2), for coherent-sum checks */

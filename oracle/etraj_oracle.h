@@ -52,6 +52,10 @@ int oracle_integrate_and_bin(const etraj_laser_desc* ld, const etraj_target_desc
 int oracle_run_job(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
                    const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end,
                    int32_t batch_stride, double* pmd_out, etraj_stats* st);
+/* oracle_run_job with the batches scheduled dynamically over the threads (timing runs) */
+int oracle_run_job_dynamic(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
+                   const etraj_run_desc* rd, int nthreads, int32_t batch_begin, int32_t batch_end,
+                   int32_t batch_stride, double* pmd_out, etraj_stats* st);
 int oracle_run_job_amplitude(const etraj_laser_desc* ld, const etraj_target_desc* td, const etraj_sampler_desc* sd,
                              const etraj_run_desc* rd, int nthreads, double* amp_out, etraj_stats* st);
 

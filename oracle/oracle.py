@@ -80,6 +80,7 @@ def lib():
                                                DP, DP, DP, P(C.c_int64)]
         L.oracle_run_job.argtypes = [P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), C.c_int, C.c_int32,
                                      C.c_int32, C.c_int32, DP, P(Stats)]
+        L.oracle_run_job_dynamic.argtypes = L.oracle_run_job.argtypes
         L.oracle_run_job_amplitude.argtypes = [P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), C.c_int, DP,
                                                P(Stats)]
         _lib = L
@@ -238,12 +239,34 @@ def integrate_and_bin(ld, td, rd, cols, pmd):
     return ip.value, iu.value, nf.value
 
 
-def run_job(ld, td, sd, rd, nthreads=1, batch_begin=0, batch_end=0, batch_stride=1):
+def run_job(ld, td, sd, rd, nthreads=1, batch_begin=0, batch_end=0, batch_stride=1, dynamic=False, library=None):
+    """The oracle's whole job over the batches batch_begin, +batch_stride, ... < batch_end
+    (0: all). dynamic: batches taken one at a time by the threads (timing runs). library: an
+    oracle build loaded with load_variant (default: this directory's liboracle.so)."""
     npix = int(np.prod([rd.final_p_num[k] for k in range(rd.dimension)]))
     out = np.empty(npix)
     st = Stats()
-    _chk(lib().oracle_run_job(ld, td, sd, rd, nthreads, batch_begin, batch_end, batch_stride, _d(out), C.byref(st)))
+    L = library or lib()
+    fn = L.oracle_run_job_dynamic if dynamic else L.oracle_run_job
+    _chk(fn(ld, td, sd, rd, nthreads, batch_begin, batch_end, batch_stride, _d(out), C.byref(st)))
     return out, st.as_dict()
+
+
+def load_variant(path):
+    """Another build of etraj_oracle.c (e.g. the -O3 -march=native timing build), with the
+    run_job entry points bound."""
+    L = C.CDLL(path)
+    for name in ("oracle_run_job", "oracle_run_job_dynamic"):
+        getattr(L, name).argtypes = lib().oracle_run_job.argtypes
+    return L
+
+
+def build_native(outdir):
+    """Compiles etraj_oracle.c with -O3 -march=native (FMA contraction allowed) for this host's
+    CPU into outdir/liboracle_native.so; returns the path, or None if the build fails."""
+    out = os.path.join(outdir, "liboracle_native.so")
+    r = subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out}"], capture_output=True, text=True)
+    return out if r.returncode == 0 and os.path.exists(out) else None
 
 
 def run_job_amplitude(ld, td, sd, rd):

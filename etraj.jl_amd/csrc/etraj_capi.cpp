@@ -63,6 +63,7 @@ struct SlabSlot {
     DevBuf<float4> ft32;
     DevBuf<double2> ft64, ft0;
     DevBuf<uint8_t> flags;
+    DevBuf<unsigned long long> queue;   // work-queue counter of the slot's Tsit5 launches
     std::vector<double> h_t0;
     std::vector<int32_t> h_job, h_b;
     std::vector<int64_t> h_start;
@@ -71,6 +72,7 @@ struct SlabSlot {
     void release() {
         init.release(); seg_t0.release(); counts.release(); counts2.release(); seg_job.release(); seg_b.release();
         seg_start.release(); pre.release(); ft32.release(); ft64.release(); ft0.release(); flags.release();
+        queue.release();
         if (done) (void)hipEventDestroy(done);
         done = nullptr;
     }
@@ -92,6 +94,7 @@ struct etraj_ctx {
     DevBuf<LaserP> lps;
     DevBuf<uint8_t> flags;
     DevBuf<int64_t> orig;
+    DevBuf<unsigned long long> queue;   // work-queue counter of the Tsit5 launches on `stream`
     DevBuf<Contrib> contrib;
     DevBuf<uint64_t> pmdq, scalq, hits;   // ETRAJ_PMD_FIXED sums, hit counts
     // host staging of integrate_columns: kept in the ctx so that every asynchronous
@@ -418,6 +421,7 @@ struct ColBufs {
     int64_t* pre = nullptr;
     uint8_t* flags = nullptr;
     int64_t* orig = nullptr;
+    unsigned long long* queue = nullptr;
     LaserP* lps = nullptr;
     float4* ft32 = nullptr;
     double2* ft64 = nullptr;
@@ -439,6 +443,8 @@ int col_bufs_ensure(etraj_ctx* c, ColBufs& B, int64_t n, int64_t nseg, int rows)
     HIP_TRY(c, c->flags.ensure((size_t)n));
     HIP_TRY(c, c->orig.ensure((size_t)n));
     HIP_TRY(c, c->lps.ensure(1));
+    HIP_TRY(c, c->queue.ensure(1));
+    B.queue = c->queue.p;
     B.init = c->init.p; B.seg_start = c->seg_start.p; B.counts = c->counts.p; B.counts2 = c->counts2.p;
     B.seg_t0 = c->seg_t0.p; B.pre = c->pre.p; B.flags = c->flags.p; B.orig = c->orig.p; B.lps = c->lps.p;
     return ETRAJ_OK;
@@ -542,18 +548,18 @@ int integrate_columns(etraj_ctx* c, ColBufs& B, const IntegP& P, const double* i
         for (int64_t k = 0; k < nseg; ++k) pre[k + 1] = pre[k] + sc[k];
         HIP_TRY(c, hipMemcpyAsync(B.pre, pre, sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
         if (!P.mixed) {
-            HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, nullptr, nullptr));
+            HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, B.queue, nullptr, nullptr));
             return ETRAJ_OK;
         }
         // mixed: FP32 stage derivatives; the flagged electrons are compacted per
         // segment and integrated again in FP64, their outputs at their own columns
-        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, B.flags, nullptr));
+        HIP_TRY(c, launch_tsit5(c->stream, P, O, out, B.init, B.seg_start, B.pre, nseg, n, B.queue, B.flags, nullptr));
         HIP_TRY(c, launch_compact_flagged(c->stream, B.init, P.rows, B.seg_start, B.counts, nseg, B.flags, nullptr,
                                           B.counts2, B.orig));
         HIP_TRY(c, launch_scan_counts(c->stream, B.counts2, nseg, B.pre));
         IntegP P64 = P;
         P64.mixed = 0;
-        HIP_TRY(c, launch_tsit5(c->stream, P64, O, out, B.init, B.seg_start, B.pre, nseg, n, nullptr, B.orig));
+        HIP_TRY(c, launch_tsit5(c->stream, P64, O, out, B.init, B.seg_start, B.pre, nseg, n, B.queue, nullptr, B.orig));
         return ETRAJ_OK;
     }
     double* t0 = c->h_t0.data();
@@ -656,6 +662,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->pmdq.release(); c->scalq.release(); c->hits.release();
     c->seg_t0.release(); c->ft32.release(); c->ft64.release(); c->ft0.release(); c->pre.release();
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release(); c->prec.release();
+    c->queue.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
     for (SlabSlot& Q : c->slot) Q.release();
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -905,6 +912,7 @@ int etraj_integrate_and_bin_noalloc(etraj_ctx* c, const etraj_laser_desc* laser,
     B.seg_t0 = reinterpret_cast<double*>(take(sizeof(double) * M));
     B.pre = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (M + 1)));
     B.lps = reinterpret_cast<LaserP*>(take(sizeof(LaserP)));
+    B.queue = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long)));
     // field table of a segment group: F(t0) per segment, then the rest of the array as
     // (float4 + 2 double2) per step; each slice above is rounded up to 256 bytes
     B.ft_segs = M;
@@ -1087,6 +1095,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         HIP_TRY(c, Q.seg_job.ensure(max_nb));
         HIP_TRY(c, Q.seg_b.ensure(max_nb));
         HIP_TRY(c, Q.pre.ensure(max_nb + 1));
+        HIP_TRY(c, Q.queue.ensure(1));
         if (rk4) {
             HIP_TRY(c, Q.ft32.ensure((size_t)max_nb * stride));
             HIP_TRY(c, Q.ft64.ensure((size_t)2 * max_nb * stride));
@@ -1212,7 +1221,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             if (O.hits) Oj.hits = O.hits + j * npix;
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols,
+                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
                                  P.mixed ? Q.flags.p : nullptr, nullptr);
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
             if (P.mixed) {   // the flagged electrons again in FP64, on the same stream
@@ -1223,7 +1232,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                                                Q.counts2.p, nullptr);
                 if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
                 if (e == hipSuccess)
-                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols,
+                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
                                      nullptr, nullptr);
             }
         }

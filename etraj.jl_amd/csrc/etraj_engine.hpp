@@ -220,10 +220,12 @@ hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* 
                               int64_t nseg, double tf, double dt, const FieldTab& FT);
 // P.mixed: FP32 stage derivatives; flags (required) receives 1 for the columns
 // to integrate again in FP64 and 0 for the others. orig (may be null) maps a
-// column to its output index.
+// column to its output index. queue: one device counter of the launch's work
+// queue (set to zero by the launch itself; launches that may run at the same
+// time need counters of their own).
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
-                        uint8_t* flags, const int64_t* orig);
+                        unsigned long long* queue, uint8_t* flags, const int64_t* orig);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
 // out[i] = fix_to_double(q[2i], q[2i+1]), i < n

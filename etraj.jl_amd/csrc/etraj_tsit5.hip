@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 namespace etraj {
 
 // Per-lane output of one finished electron for kernels whose lanes finish at
@@ -137,60 +139,126 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
     }
 }
 
-// item (0-based over all electrons of all segments) -> column index
-__device__ inline int64_t item_column(int64_t item, const int64_t* pre, int64_t n_seg, const int64_t* seg_start) {
-    int64_t lo = 0, hi = n_seg;  // pre[lo] <= item < pre[hi]
+// The segment holding item (0-based over all electrons of all segments):
+// pre[s] <= item < pre[s + 1] (the last such s where segments are empty).
+__device__ inline int64_t item_segment(int64_t item, const int64_t* pre, int64_t n_seg) {
+    int64_t lo = 0, hi = n_seg;
     while (hi - lo > 1) {
         const int64_t mid = (lo + hi) >> 1;
         if (pre[mid] <= item) lo = mid; else hi = mid;
     }
-    return seg_start[lo] + (item - pre[lo]);
+    return lo;
 }
 
-// Adaptive Tsit5, one electron per lane: wavefront w integrates items
-// [64 w, 64 w + 64) (0-based over all electrons of all segments), each lane
-// advancing its electron one attempted step per loop trip until every lane of
-// the wave is done. (A form in which a finished lane took the next electron
-// of a longer per-wave range measured slower at every range length: 40.4 ms
-// at 128 electrons per wave, 44.1 at 512, 53.9 at 2048, against 39.7 for one
-// electron per lane, on the Ar SPA 3D grid of 9.6M electrons.)
+// Electrons a wave claims from the global queue at a time.
+constexpr int kTs5Claim = 256;
+// Idle lanes (finished or empty) a wave collects before it writes their
+// results and refills them: one pass of the divergent start-up (column look-up,
+// loads, initial step size) and finish (Kepler, binning) code serves many lanes.
+constexpr int kTs5Refill = 16;
+
+// Adaptive Tsit5, one electron per lane, each lane advancing its electron one
+// attempted step per loop trip. Two ways to hand out the electrons (items,
+// 0-based over all electrons of all segments):
+//  - QUEUE: a grid of resident waves; once kTs5Refill lanes of a wave are idle,
+//    their results are written and they take the next items of the wave's
+//    pool, and a wave whose pool runs dry claims kTs5Claim more from the global
+//    counter *queue (zero before the launch). Trajectories of very uneven cost
+//    (close encounters take tens of times the steps of the rest) then no longer
+//    hold 63 idle lanes for the slowest one. A wave leaves when the queue is
+//    drained and all its lanes are idle; the queue only grows, so every wave
+//    gets there.
+//  - static: wave w takes items [64 w, 64 w + 64) and runs until its slowest
+//    lane is done.
 // MIXED: stage derivatives in FP32; an electron with a stage position within
 // restep_radius of the core stops at once, is not binned and gets flags[col]
 // = 1 (every other column of the launch gets 0), for the FP64 launch over the
 // compacted flagged columns. orig (may be null): the output index of a
 // compacted column (records / contributions of that FP64 launch).
-template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool QUEUE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
-    const int64_t* __restrict__ pre, int64_t n_seg, uint8_t* __restrict__ flags, const int64_t* __restrict__ orig) {
+    const int64_t* __restrict__ pre, int64_t n_seg, unsigned long long* __restrict__ queue,
+    uint8_t* __restrict__ flags, const int64_t* __restrict__ orig) {
     const int lane = lane_id();
     const int64_t total = pre[n_seg];
     const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int64_t item = gw * 64 + lane;
-    if (gw * 64 >= total) return;
+    if (!QUEUE && gw * 64 >= total) return;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
     const double atol = P.atol;
     // the FP32 hydrogen-like EOM reports |r|^2 + soft core
     const float rc2 = (float)(P.restep_radius * P.restep_radius + (TGT != ETRAJ_TARGET_SAE ? P.T.sc : 0.0));
     LaneAcc acc;
     Ts5Lane<DIM, PHASE> st;
-    bool valid = item < total, close = false;
-    int64_t col = 0;
-    double t0 = 0.0, prob = 0.0;
-    if (valid) {
-        col = item_column(item, pre, n_seg, seg_start);
-        const double* c = init + col * P.rows;
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            st.r[i] = c[i];
-            st.p[i] = c[DIM + i];
-        }
-        t0 = c[2 * DIM];
-        prob = c[2 * DIM + 1];
-        st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
-        ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+    // valid: integrating; pending: finished, result not yet written
+    bool valid = false, fresh = false, pending = false, failed = false, close = false;
+    int64_t item = gw * 64 + lane, col = 0;
+    if (!QUEUE && item < total) {
+        valid = true;
+        fresh = true;
+        const int64_t sg = item_segment(item, pre, n_seg);
+        col = seg_start[sg] + (item - pre[sg]);
     }
-    while (__ballot(valid) != 0) {
+    // QUEUE, wave-uniform: the claimed, not yet started items [pool, pool_end), the
+    // segment holding item `pool`, and whether the counter has passed the last item
+    int64_t pool = 0, pool_end = 0, pool_seg = 0;
+    bool drained = false;
+    double t0 = 0.0, prob = 0.0;
+    while (true) {
+        const uint64_t idle = __ballot(!valid);
+        if (idle == ~0ull || (QUEUE && __popcll(idle) >= kTs5Refill)) {
+            if (pending) {
+                pending = false;
+                emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
+            }
+            if (QUEUE) {
+                uint64_t want = idle;
+                for (int pass = 0; pass < 2 && want != 0; ++pass) {
+                    if (pool >= pool_end) {
+                        if (drained) break;
+                        // lane 0 claims for the wave; the other lanes add 0, so the claim
+                        // is right whether or not the compiler merges the lanes' atomics
+                        const unsigned long long old = atomicAdd(queue, lane == 0 ? (unsigned long long)kTs5Claim : 0ull);
+                        const int64_t got = (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)old, 0) |
+                                                      ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(old >> 32), 0) << 32));
+                        drained = got + kTs5Claim >= total;
+                        if (got >= total) break;
+                        pool = got;
+                        pool_end = drained ? total : got + kTs5Claim;
+                        pool_seg = item_segment(pool, pre, n_seg);
+                    }
+                    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0));
+                    const int64_t n_want = __popcll(want), left = pool_end - pool;
+                    const int64_t take = n_want < left ? n_want : left;
+                    if (((want >> lane) & 1) && rank < take) {
+                        item = pool + rank;
+                        int64_t sg = pool_seg;
+                        while (pre[sg + 1] <= item) ++sg;
+                        col = seg_start[sg] + (item - pre[sg]);
+                        valid = true;
+                        fresh = true;
+                    }
+                    pool += take;
+                    while (pool_seg < n_seg - 1 && pre[pool_seg + 1] <= pool) ++pool_seg;
+                    want = __ballot(!valid);
+                }
+            }
+            if (__ballot(valid) == 0) break;
+        }
+        if (fresh) {
+            fresh = false;
+            const double* c = init + col * P.rows;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                st.r[i] = c[i];
+                st.p[i] = c[DIM + i];
+            }
+            t0 = c[2 * DIM];
+            prob = c[2 * DIM + 1];
+            st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
+            close = false;
+            ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+        }
         if (valid) {
             const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters,
                                                                st, rc2, &close);
@@ -202,49 +270,77 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 flags[col] = 1;
             } else if (rc != 0) {
                 valid = false;
-                bool failed = rc < 0;
+                pending = true;
+                failed = rc < 0;
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) failed = failed || isnan(st.r[i]) || isnan(st.p[i]);
                 acc.steps += st.accepted;
                 acc.attempts += st.attempted;
                 if (MIXED) flags[col] = 0;
-                emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
             }
         }
     }
     if (OUT == OUT_PMD) flush_lane_acc(O, acc, !MIXED);
 }
 
+__global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }
+
 struct Ts5Launch {
     const double* init;
     const int64_t* seg_start;
     const int64_t* pre;
     int64_t n_seg, max_items;
+    unsigned long long* queue;
     uint8_t* flags;
     const int64_t* orig;
 };
 
+// Blocks of the resident grid of one k_tsit5 instance: the blocks the chip
+// holds at once, looked up once per device, and no more than the items need.
+template <typename K>
+static hipError_t ts5_grid(K kernel, int64_t max_items, unsigned& blocks) {
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> resident[kMaxDev] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int r = dev < kMaxDev ? resident[dev].load(std::memory_order_relaxed) : 0;
+    if (r == 0) {
+        int cus = 0, per_cu = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+        if (e != hipSuccess) return e;
+        r = cus * (per_cu > 0 ? per_cu : 1);
+        if (dev < kMaxDev) resident[dev].store(r, std::memory_order_relaxed);
+    }
+    const int64_t need = (max_items + 255) / 256;
+    blocks = (unsigned)(need < r ? need : r);
+    return hipSuccess;
+}
+
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
+static hipError_t launch_tsit5_k(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
+    auto kernel = k_tsit5<DIM, PHASE, OUT, TGT, MIXED, true>;
+    unsigned blocks = 0;
+    hipError_t e = ts5_grid(kernel, a.max_items, blocks);
+    if (e != hipSuccess || blocks == 0) return e;
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, st, a.queue);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, st, P, O, a.init, a.seg_start, a.pre, a.n_seg, a.queue,
+                       a.flags, a.orig);
+    return hipGetLastError();
+}
+
 template <int DIM, int PHASE, int OUT, bool MIXED>
 static hipError_t launch_tsit5_t(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
-    const int64_t waves = (a.max_items + 63) / 64;
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    if (blocks == 0) return hipSuccess;
     // The FP32 hydrogen-like EOM drops the soft-core term -Z a / r2^(3/2) of the SCTS phase rate;
     // a soft core that FP32 resolves (the molecular a = 1) takes the kTargetHSoft form, which keeps it.
     if constexpr (MIXED) {
-        if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE && P.T.sc > 1e-6) {
-            hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, kTargetHSoft, MIXED>), dim3(blocks), dim3(256), 0, st, P, O,
-                               a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
-            return hipGetLastError();
-        }
+        if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE && P.T.sc > 1e-6)
+            return launch_tsit5_k<DIM, PHASE, OUT, kTargetHSoft, MIXED>(st, P, O, a);
     }
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE, MIXED>), dim3(blocks), dim3(256), 0, st,
-                           P, O, a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
-    else
-        hipLaunchKernelGGL((k_tsit5<DIM, PHASE, OUT, ETRAJ_TARGET_SAE, MIXED>), dim3(blocks), dim3(256), 0, st, P, O,
-                           a.init, a.seg_start, a.pre, a.n_seg, a.flags, a.orig);
-    return hipGetLastError();
+        return launch_tsit5_k<DIM, PHASE, OUT, ETRAJ_TARGET_HYDROGEN_LIKE, MIXED>(st, P, O, a);
+    return launch_tsit5_k<DIM, PHASE, OUT, ETRAJ_TARGET_SAE, MIXED>(st, P, O, a);
 }
 
 template <int DIM, int PHASE, bool MIXED>
@@ -267,9 +363,10 @@ static hipError_t launch_tsit5_p(hipStream_t st, const IntegP& P, const OutP& O,
 
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
-                        uint8_t* flags, const int64_t* orig) {
+                        unsigned long long* queue, uint8_t* flags, const int64_t* orig) {
     if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, flags, orig};
+    if (!queue) return hipErrorInvalidValue;
+    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, orig};
     if (P.mixed) {
         if (!flags) return hipErrorInvalidValue;
         return P.dim == 3 ? launch_tsit5_p<3, true>(st, P, O, out, a) : launch_tsit5_p<2, true>(st, P, O, out, a);

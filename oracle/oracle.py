@@ -197,13 +197,15 @@ def rows_of(dim, phase):
     return (8 if phase == _abi.PHASE_CTMC else 9) if dim == 3 else (6 if phase == _abi.PHASE_CTMC else 7)
 
 
-def gen_electron_batch(ld, td, sd, batch_id):
-    """returns (N, rows) C-order == Julia rows x N, or None"""
-    cap = max(int(lib().oracle_batch_max_size(sd)), 1)
+def gen_electron_batch(ld, td, sd, batch_id, library=None):
+    """returns (N, rows) C-order == Julia rows x N, or None. library: another build of the
+    oracle (load_variant), default liboracle.so"""
+    L = library or lib()
+    cap = max(int(L.oracle_batch_max_size(sd)), 1)
     rows = rows_of(sd.dimension, sd.phase_method)
     out = np.empty((cap, rows))
     n = C.c_int64()
-    _chk(lib().oracle_gen_electron_batch(ld, td, sd, batch_id, _d(out), cap, C.byref(n)))
+    _chk(L.oracle_gen_electron_batch(ld, td, sd, batch_id, _d(out), cap, C.byref(n)))
     return out[:n.value].copy() if n.value else None
 
 
@@ -254,10 +256,13 @@ def run_job(ld, td, sd, rd, nthreads=1, batch_begin=0, batch_end=0, batch_stride
 
 def load_variant(path):
     """Another build of etraj_oracle.c (e.g. the -O3 -march=native timing build), with the
-    run_job entry points bound."""
+    run_job and sampler entry points bound."""
     L = C.CDLL(path)
     for name in ("oracle_run_job", "oracle_run_job_dynamic"):
         getattr(L, name).argtypes = lib().oracle_run_job.argtypes
+    for name in ("oracle_batch_max_size", "oracle_gen_electron_batch"):
+        getattr(L, name).argtypes = getattr(lib(), name).argtypes
+        getattr(L, name).restype = getattr(lib(), name).restype
     return L
 
 

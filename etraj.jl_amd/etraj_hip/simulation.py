@@ -21,9 +21,11 @@ Keyword interface of the reference, plus engine keywords:
   hit_counts       also return result["hit_counts"]: the number of electrons
                    binned into each PMD cell (the integer-count PMD)
 Returns the fields write_output! stores (:516-543) as a dict; when
-output_path is given they are also saved: output_fmt "h5" / "hdf5" writes the
-reference's HDF5 file (etraj_hip.h5out), "npz" a numpy archive of the same
-fields; JLD2 (the reference's default format) is not written.
+output_path is given they are also saved: output_fmt "jld2" (the reference's
+default) writes the JLD2 file and "h5" / "hdf5" the HDF5 file of write_output!
+(etraj_hip.h5out; the JLD2 file lacks only `params`, the parameters as Julia
+objects, which params_text holds as YAML), "npz" a numpy archive of the same
+fields.
 """
 from __future__ import annotations
 
@@ -97,7 +99,7 @@ def default_filename():
 def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sample_t_intv, sample_t_num,
                             traj_t_final, final_p_max, final_p_num, ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0,
                             ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0, mc_kz_max=0.0, traj_phase_method="CTMC",
-                            traj_rtol=None, output_fmt="npz", output_compress=True, output_path=None,
+                            traj_rtol=None, output_fmt="jld2", output_compress=True, output_path=None,
                             sample_monte_carlo=False, sample_cutoff_limit=1e-16, rate_prefix="Full",
                             mol_orbit_ridx=0, show_progress=False, traj_integrator="tsit5", traj_dt=0.1,
                             precision=None, restep_radius=2.0, granularity="job", seed=0,
@@ -220,7 +222,7 @@ def _params_text(params) -> str:
     return yaml.safe_dump(plain(params), sort_keys=False, allow_unicode=True)
 
 
-def _save(result, output_path, output_fmt="npz", output_compress=True):
+def _save(result, output_path, output_fmt="jld2", output_compress=True):
     """The file handling of TrajectorySimulationJob / write_output! (:195-215,
     :502-546): a suffix per format, a default file name instead of an existing
     file or temp file, the data written to `path!` and renamed at the end."""
@@ -232,8 +234,7 @@ def _save(result, output_path, output_fmt="npz", output_compress=True):
     elif fmt == "npz":
         ext = ".npz"
     elif fmt == "jld2":
-        raise NotImplementedError("[TrajectorySimulationJob] JLD2 output is not written by this engine; "
-                                  "use output_fmt='h5' (the reference's HDF5 format) or 'npz'")
+        ext = ".jld2"
     else:
         raise ValueError(f"[TrajectorySimulationJob] Unsupported output format [{output_fmt}].")
     path = output_path if output_path.endswith(ext) else output_path + ext
@@ -243,9 +244,9 @@ def _save(result, output_path, output_fmt="npz", output_compress=True):
     if os.path.exists(path + "!"):
         warnings.warn(f"[TrajectorySimulationJob] The temp file `{path}!` exists, will save at default filename.")
         path = default_filename() + ext
-    if ext == ".h5":
+    if ext in (".h5", ".jld2"):
         from .h5out import write_h5
-        write_h5(path, result, _params_text(result["params"]), compress=bool(output_compress))
+        write_h5(path, result, _params_text(result["params"]), compress=bool(output_compress), jld2=ext == ".jld2")
     else:
         tmp = path + "!"
         with open(tmp, "wb") as fh:
@@ -258,7 +259,7 @@ _SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc
                       mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=None, sample_monte_carlo=False,
                       sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="tsit5",
                       traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
-                      kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="npz",
+                      kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="jld2",
                       output_compress=True, pmd_accumulator="float", hit_counts=False)
 
 
@@ -312,6 +313,6 @@ def perform_traj_scan(jobs):
                     st.num_effective_traj, st.as_dict(), **kw)
         if hits is not None:
             r["hit_counts"] = _julia_shape(hits[j * npix:(j + 1) * npix].copy(), kw["final_p_num"])
-        _save(r, kw["output_path"], kw.get("output_fmt", "npz"), kw.get("output_compress", True))
+        _save(r, kw["output_path"], kw.get("output_fmt", "jld2"), kw.get("output_compress", True))
         results.append(r)
     return results

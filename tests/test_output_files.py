@@ -1,8 +1,13 @@
 """Output files of perform_traj_simulation (write_output!, TrajectorySimulation.jl:
-502-546, format :h5): dataset names, types, Julia's reversed dimension order, the
-shuffle + deflate filters of output_compress, the temp-file rename and the file
-naming rules (:195-215). Read back with the HDF5 C library (etraj_hip.moldata)
-and, where the HDF5 tools are installed, with h5dump."""
+502-546, formats :h5 and :jld2): dataset names, types, Julia's reversed dimension
+order, the filters of output_compress (shuffle + deflate for :h5, deflate for
+:jld2), the temp-file rename and the file naming rules (:195-215). Read back with
+the HDF5 C library through etraj_hip.moldata, the reader of the reference's own
+.jld2 database files, and, where the HDF5 tools are installed, with h5dump. A JLD2
+file's container layout (user block and signature, superblock version, root-group
+link storage, String encoding) must equal that of the JLD2 files that ship with
+eTraj.jl (tests/golden/jld2_layout.json, written by make_jld2_layout.py)."""
+import json
 import os
 import shutil
 import subprocess
@@ -12,6 +17,7 @@ import numpy as np
 import pytest
 
 from etraj_hip import Cos2Laser, get_atom
+from etraj_hip.h5out import file_layout
 from etraj_hip.moldata import _File, HDF5Unavailable, _hdf5
 from etraj_hip.simulation import _result, _save
 
@@ -35,13 +41,15 @@ def _fake_result(dim=2):
     return _result(spec, 0.0125, 0.0025, 12345, {}, **kw), spec, num, pmax
 
 
+@pytest.mark.parametrize("fmt", ["h5", "jld2"])
 @pytest.mark.parametrize("dim,compress", [(2, True), (2, False), (3, True)])
-def test_h5_file_has_the_reference_datasets(tmp_path, dim, compress):
+def test_output_file_has_the_reference_datasets(tmp_path, fmt, dim, compress):
     r, spec, num, pmax = _fake_result(dim)
     path = str(tmp_path / "run")
-    _save(r, path, "h5", compress)
-    assert r["output_path"] == path + ".h5" and os.path.exists(path + ".h5") and not os.path.exists(path + ".h5!")
-    f = _File(path + ".h5")
+    ext = "." + fmt
+    _save(r, path, fmt, compress)
+    assert r["output_path"] == path + ext and os.path.exists(path + ext) and not os.path.exists(path + ext + "!")
+    f = _File(path + ext)
     try:
         ms = f.get("momentum_spec")
         assert ms.shape == num                                       # Julia's size, read back in Julia order
@@ -56,13 +64,31 @@ def test_h5_file_has_the_reference_datasets(tmp_path, dim, compress):
     finally:
         f.close()
     if H5DUMP:
-        out = subprocess.run([H5DUMP, "-H", "-p", "-d", "/momentum_spec", path + ".h5"], capture_output=True,
+        out = subprocess.run([H5DUMP, "-H", "-p", "-d", "/momentum_spec", path + ext], capture_output=True,
                              text=True, cwd=str(tmp_path)).stdout
         assert f"( {', '.join(str(n) for n in num[::-1])} )" in out    # HDF5 dims = Julia's reversed
-        assert ("SHUFFLE" in out and "DEFLATE" in out and "LEVEL 6" in out) == compress
-        out = subprocess.run([H5DUMP, "-H", "-d", "/num_effective_traj", path + ".h5"], capture_output=True,
+        assert ("DEFLATE" in out and "LEVEL 6" in out) == compress
+        assert ("SHUFFLE" in out) == (compress and fmt == "h5")
+        out = subprocess.run([H5DUMP, "-H", "-d", "/num_effective_traj", path + ext], capture_output=True,
                              text=True, cwd=str(tmp_path)).stdout
         assert "H5T_STD_I64LE" in out and "SCALAR" in out
+
+
+def test_jld2_container_matches_the_reference_files(tmp_path):
+    r, *_ = _fake_result(3)
+    _save(r, str(tmp_path / "run"), "jld2")
+    got = file_layout(r["output_path"], "info")
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "jld2_layout.json")))
+    assert len(ref) >= 3
+    for fn, want in ref.items():
+        assert got == want, fn
+    head = open(r["output_path"], "rb").read(512)
+    assert head.startswith(b"HDF5-based Julia Data Format, version 0.1.1\x00")
+    assert head.rstrip(b"\x00").count(b"\x00") == 1          # signature, then the writer's note
+    # the default format is the reference's (output_fmt = :jld2, TrajectorySimulation.jl:127)
+    r2, *_ = _fake_result()
+    _save(r2, str(tmp_path / "default"))
+    assert r2["output_path"].endswith(".jld2")
 
 
 def test_existing_files_are_not_overwritten(tmp_path, monkeypatch):
@@ -82,9 +108,7 @@ def test_existing_files_are_not_overwritten(tmp_path, monkeypatch):
     assert r["output_path"] != "b.h5" and any("temp file" in str(x.message) for x in w)
 
 
-def test_jld2_and_unknown_formats_are_refused(tmp_path):
+def test_unknown_formats_are_refused(tmp_path):
     r, *_ = _fake_result()
-    with pytest.raises(NotImplementedError):
-        _save(r, str(tmp_path / "x"), "jld2")
     with pytest.raises(ValueError):
         _save(r, str(tmp_path / "x"), "csv")

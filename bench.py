@@ -376,6 +376,13 @@ def bench_workload(name, args, ctx):
         jobs = [dict(kw, precision="mixed", pmd_accumulator="fixed")]
         desc = ("H-atom ADK CTMC, 800 nm 1e14 W/cm^2 2-cycle LP cos2, 2000 t0 x 50 kd grid, RK4 dt 0.1, "
                 "PMD 500x500: fixed-point weight sums + per-cell hit counts")
+    elif name == "h-scts-1e8-tsit5":
+        kw = dict(workloads.h_lp_scts_stratified(n_t=args.n_t or 10000, n_k=args.n_k or 34400, precision="fp64"),
+                  traj_integrator="tsit5", traj_rtol=1e-6)
+        jobs = [kw]
+        desc = ("H-atom ADK+SCTS, 800 nm 1e14 W/cm^2 2-cycle LP cos2, "
+                f"{kw['sample_t_num']} t0 strata x {kw['mc_kt_num']} Gaussian k_perp samples, adaptive Tsit5 "
+                "traj_rtol 1e-6 in FP64 (the reference's integrator), coherent PMD 1000x1000")
     elif name == "ar-spa-3d":
         kw = workloads.ar_cp_spa_qtmc_3d(n_t=args.n_t or 2000, n_k=args.n_k or 158)
         jobs = [kw]
@@ -397,6 +404,12 @@ def bench_workload(name, args, ctx):
                 "1300 nm 4-cycle LP 1e14 W/cm^2, SCTS, 2D PMD 400x400 each")
     else:
         raise SystemExit(f"unknown workload {name}")
+    jobs = [dict(j, pmd_binning=args.pmd_binning) for j in jobs]
+    tuned = None
+    if args.autotune and len(jobs) == 1:
+        from etraj_hip import autotune
+        tuned = autotune(**jobs[0])
+        log(f"autotune: {tuned['start_ms']:.1f} ms -> {tuned['best_ms']:.1f} ms with {tuned['best']}")
     ds, keep = _scan_descs(jobs)
     hits = None
     if name == "h-ctmc-grid":   # the integer-count PMD: electrons binned per cell, accumulated over the runs
@@ -413,6 +426,8 @@ def bench_workload(name, args, ctx):
         "dtype": "fp32+fp64" if mixed else "fp64", "data": "synthetic (seeded sampling)",
         "config": {"workload": desc, "electrons": n_el, "jobs": len(ds)},
         "pmd_wall_ms": sec * 1e3,
+        "tuning": ctx.get_tuning(),
+        "autotune": tuned,
         "device_ms": {"sample": sts[-1][0].ms_sample, "integrate": sts[-1][0].ms_integrate,
                       "integrate_main": sts[-1][0].ms_integrate_main},
         "fp64_reintegrated_fraction": sum(s.n_restep for s in sts[-1]) / max(n_el, 1),
@@ -422,6 +437,16 @@ def bench_workload(name, args, ctx):
         "roofline": (_roofline_rk4(sts[-1], 1, {"wfat-orientation-scan": 137.0, "h-ctmc-grid": 100.0}.get(name, 129.0),
                                    129.0) if (mixed and rk4) else None),
     }
+    if name == "h-scts-1e8-tsit5":
+        st = sts[-1][0]
+        launch_ms = st.ms_integrate_main / max(st.n_slabs, 1)
+        f = TSIT5_FLOPS_PER_ATTEMPT * st.n_attempts / max(st.n_slabs, 1)
+        ach = f / (launch_ms / 1e3) / 1e12
+        result["roofline"] = {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                              "frac": ach / PEAK_TFLOPS["fp64"], "traffic": None, "kernel": "k_tsit5 FP64",
+                              "launches_per_job": st.n_slabs, "avg_launch_ms": launch_ms, "flops_per_launch": f,
+                              "flops_per_attempt": TSIT5_FLOPS_PER_ATTEMPT, "steps_attempted": st.n_attempts,
+                              "steps_accepted": st.n_steps}
     if hits is not None:
         result["hit_counts_per_run"] = int(hits.sum()) // (args.steps + args.warmup)
         ds[0][3].hit_counts = None
@@ -456,11 +481,18 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM-counter calibration)")
     ap.add_argument("--workload", default="h-scts-1e8",
-                    choices=["h-scts-1e8", "h-ctmc-grid", "ar-spa-3d", "bichromatic-scan", "wfat-orientation-scan"],
+                    choices=["h-scts-1e8", "h-scts-1e8-tsit5", "h-ctmc-grid", "ar-spa-3d", "bichromatic-scan",
+                             "wfat-orientation-scan"],
                     help="h-scts-1e8 (default) is the headline; the others are the remaining configurations")
     ap.add_argument("--scan-jobs", type=int, default=0, help="orientations of wfat-orientation-scan (default 64)")
     ap.add_argument("--restep-radius", type=float, default=None,
                     help="FP64 re-integration radius of the mixed mode (a.u.; default: the workload's)")
+    ap.add_argument("--pmd-binning", default="auto", choices=["auto", "atomic", "tiles"],
+                    help="PMD binning: the context's choice, per-electron atomics, or LDS-privatised tiles")
+    ap.add_argument("--tuning", default="", metavar="K=V,...",
+                    help="launch choices set before the run (etraj_tuning fields, e.g. rk4_block=128,bin_tiles=1)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the engine's autotuner on the workload first and time the chosen launch set")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
@@ -468,6 +500,8 @@ def main():
         sys.exit(2)
 
     ctx = _abi.default_context()
+    if args.tuning:
+        ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
     if args.calibrate_copy > 0:
         copy_calibration(args.calibrate_copy)
     if args.workload != "h-scts-1e8":
@@ -476,6 +510,12 @@ def main():
     kw = workloads.h_lp_scts_stratified(n_t=args.n_t or 10000, n_k=args.n_k or 34400, precision="mixed")
     if args.restep_radius is not None:
         kw["restep_radius"] = args.restep_radius
+    kw["pmd_binning"] = args.pmd_binning
+    tuned = None
+    if args.autotune:
+        from etraj_hip import autotune
+        tuned = autotune(**kw)
+        log(f"autotune: {tuned['start_ms']:.1f} ms -> {tuned['best_ms']:.1f} ms with {tuned['best']}")
     d_mixed = descs(kw)
     npix = kw["final_p_num"][0] * kw["final_p_num"][1]
 
@@ -518,6 +558,8 @@ def main():
                    "precision": "mixed: FP32 EOM + Kahan, FP32/FP64 Kahan phase, FP64 re-integration of electrons "
                                 f"within {kw['restep_radius']:g} a.u. of the core, FP64 Kepler and PMD"},
         "pmd_wall_ms": ms_step,
+        "tuning": ctx.get_tuning(),
+        "autotune": tuned,
         "integrate_kernel_ms": float(np.mean(ms_int)),
         "fp64_reintegrated_fraction": restep / max(n_el * args.steps, 1),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",

@@ -82,6 +82,16 @@ struct etraj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;   // FP64 re-integration passes of run_job (default priority)
+    bool overlap_fp64 = true;        // run_job: FP64 passes on stream2, beside the next slab's FP32 pass
+    // tile binning (ETRAJ_BIN_TILES): records, their tile-sorted copy and the tile tables
+    DevBuf<uint32_t> bin_cell, bin_scell, tile_count, tile_off, chunk_off, cursor;
+    DevBuf<double2> bin_amp, bin_samp;
+    bool bin_auto_tiles = false;     // ETRAJ_BIN_AUTO takes the tiles
+    int bin_shift = 0;               // tile of 2^bin_shift cells (0: the largest that fits the LDS)
+    int bin_chunk = 16384;           // records per accumulating workgroup
+    int rk4_block = 256;             // RK4 threads per workgroup
+    int ts5_refill = 16;             // Tsit5: idle lanes before a wavefront refills
+    int ts5_claim = 256;             // Tsit5: electrons per queue claim
     SlabSlot slot[kSlabSlots];
     std::string err;
     DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat, prec;
@@ -123,6 +133,41 @@ int hip_fail(etraj_ctx* c, hipError_t e, const char* what) {
         hipError_t _e = (expr);                             \
         if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
     } while (0)
+
+etraj_tuning tuning_of(const etraj_ctx* c) {
+    etraj_tuning t{};
+    t.bin_tiles = c->bin_auto_tiles ? 1 : 0;
+    t.bin_tile_shift = c->bin_shift;
+    t.bin_chunk = c->bin_chunk;
+    t.overlap_fp64 = c->overlap_fp64 ? 1 : 0;
+    t.rk4_block = c->rk4_block;
+    t.tsit5_refill = c->ts5_refill;
+    t.tsit5_claim = c->ts5_claim;
+    return t;
+}
+
+const char* tuning_invalid(const etraj_tuning& t) {
+    if (t.bin_tiles != 0 && t.bin_tiles != 1) return "tuning.bin_tiles must be 0 or 1";
+    if (t.bin_tile_shift != 0 && (t.bin_tile_shift < 6 || t.bin_tile_shift > 16))
+        return "tuning.bin_tile_shift must be 0 or 6..16";
+    if (t.bin_chunk < 256 || t.bin_chunk > (1 << 20)) return "tuning.bin_chunk must be 256..1048576";
+    if (t.overlap_fp64 != 0 && t.overlap_fp64 != 1) return "tuning.overlap_fp64 must be 0 or 1";
+    if (t.rk4_block != 64 && t.rk4_block != 128 && t.rk4_block != 256) return "tuning.rk4_block must be 64, 128 or 256";
+    if (t.tsit5_refill < 1 || t.tsit5_refill > 64) return "tuning.tsit5_refill must be 1..64";
+    if (t.tsit5_claim < 64 || t.tsit5_claim > 65536) return "tuning.tsit5_claim must be 64..65536";
+    if (t.reserved != 0) return "tuning.reserved must be 0";
+    return nullptr;
+}
+
+void apply_tuning(etraj_ctx* c, const etraj_tuning& t) {
+    c->bin_auto_tiles = t.bin_tiles != 0;
+    c->bin_shift = t.bin_tile_shift;
+    c->bin_chunk = t.bin_chunk;
+    c->overlap_fp64 = t.overlap_fp64 != 0;
+    c->rk4_block = t.rk4_block;
+    c->ts5_refill = t.tsit5_refill;
+    c->ts5_claim = t.tsit5_claim;
+}
 
 int rows_for(int dim, int phase) {
     return dim == 3 ? (phase == ETRAJ_PHASE_CTMC ? 8 : 9) : (phase == ETRAJ_PHASE_CTMC ? 6 : 7);
@@ -391,6 +436,9 @@ int make_integ(etraj_ctx* c, const etraj_run_desc* rd, const LaserP& L, const Ta
     P.rtol = r.rtol;
     P.atol = r.atol > 0 ? r.atol : 1e-6;          // OrdinaryDiffEq's default abstol
     P.maxiters = r.maxiters > 0 ? r.maxiters : 1000000;
+    P.rk4_block = c->rk4_block;
+    P.ts5_refill = c->ts5_refill;
+    P.ts5_claim = c->ts5_claim;
     return ETRAJ_OK;
 }
 
@@ -664,6 +712,8 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->counts2.release(); c->flags.release(); c->orig.release(); c->wfat.release(); c->prec.release();
     c->queue.release();
     c->seg_job.release(); c->seg_b.release(); c->sps.release(); c->lps.release(); c->glq.release();
+    c->bin_cell.release(); c->bin_scell.release(); c->tile_count.release(); c->tile_off.release();
+    c->chunk_off.release(); c->cursor.release(); c->bin_amp.release(); c->bin_samp.release();
     for (SlabSlot& Q : c->slot) Q.release();
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1065,6 +1115,41 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         max_nb = std::max<int64_t>(max_nb, (int64_t)(sl.i1 - sl.i0));
     }
 
+    // PMD binning: records + LDS tiles, or atomics from the integrator
+    const bool fixed_sums = run->pmd_mode == ETRAJ_PMD_FIXED;
+    const int64_t total_cells = (int64_t)n_jobs * npix;
+    const int words = coh ? 2 : 1;
+    if (run->pmd_binning < ETRAJ_BIN_AUTO || run->pmd_binning > ETRAJ_BIN_TILES)
+        return fail(c, ETRAJ_EINVAL, "pmd_binning must be ETRAJ_BIN_AUTO, ETRAJ_BIN_ATOMIC or ETRAJ_BIN_TILES");
+    BinCfg bcfg{0, c->bin_chunk};
+    int ntile = 0;
+    bool tiles = run->pmd_binning == ETRAJ_BIN_TILES || (run->pmd_binning == ETRAJ_BIN_AUTO && c->bin_auto_tiles);
+    if (tiles) {
+        const int lds_shift = bin_tile_shift(words, fixed_sums, run->hit_counts != nullptr, 30);
+        bcfg.shift = c->bin_shift > 0 ? std::min(c->bin_shift, lds_shift) : lds_shift;
+        while (bcfg.shift < lds_shift && ((total_cells + (1ll << bcfg.shift) - 1) >> bcfg.shift) > kBinMaxTiles)
+            ++bcfg.shift;   // few enough tiles for the tile tables
+        ntile = (int)std::min<int64_t>((total_cells + (1ll << bcfg.shift) - 1) >> bcfg.shift, INT32_MAX);
+        if (ntile > kBinMaxTiles || total_cells >= (int64_t)kNoCell) {
+            if (run->pmd_binning == ETRAJ_BIN_TILES)
+                return fail(c, ETRAJ_EUNSUPPORTED, "tile binning: the PMD needs more than 8192 tiles of LDS size "
+                                                   "(or 2^32 - 1 cells); use ETRAJ_BIN_ATOMIC");
+            tiles = false;
+        }
+    }
+    if (tiles) {
+        HIP_TRY(c, c->bin_cell.ensure(max_cols));
+        HIP_TRY(c, c->bin_scell.ensure(max_cols));
+        HIP_TRY(c, c->bin_amp.ensure(max_cols));
+        HIP_TRY(c, c->bin_samp.ensure(max_cols));
+        HIP_TRY(c, c->tile_count.ensure(kBinMaxTiles));
+        HIP_TRY(c, c->tile_off.ensure(kBinMaxTiles + 1));
+        HIP_TRY(c, c->chunk_off.ensure(kBinMaxTiles + 1));
+        HIP_TRY(c, c->cursor.ensure(kBinMaxTiles));
+    }
+    const BinBufs bbuf{c->bin_cell.p, c->bin_amp.p, c->tile_count.p, c->tile_off.p, c->chunk_off.p, c->cursor.p,
+                       c->bin_scell.p, c->bin_samp.p};
+
     std::vector<SamplerP> hsp(n_jobs);
     std::vector<LaserP> hlp(n_jobs);
     for (int32_t j = 0; j < n_jobs; ++j) {
@@ -1124,18 +1209,22 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     O.pmdq = fixed ? c->pmdq.p : nullptr;
     O.scalq = fixed ? c->scalq.p : nullptr;
     O.hits = run->hit_counts ? c->hits.p : nullptr;
+    const BinOut bout{O.pmd, O.pmdq, O.hits, O.cnt, npix};
     // per slab, first stream: start, sampled, main integrator kernel done, first-stream work done;
     // second stream: FP64 pass start and end
     std::vector<hipEvent_t> ev(6 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
-    // The FP64 pass of slab k-1 is put on the second stream once slab k has
-    // been sampled, so that it overlaps slab k's FP32 pass (a kernel that
-    // fills the device) rather than slab k's short sampling kernels, which
-    // it would starve of dispatch slots.
+    // The FP64 pass of a slab runs on the second stream (c->overlap_fp64, the
+    // default), put there once the next slab has been sampled so that it
+    // overlaps the next slab's FP32 pass rather than that slab's short
+    // sampling kernels, which it would starve of dispatch slots; or on the
+    // first stream right after the slab's FP32 pass and compaction. On the
+    // 1e8-electron job the overlap saves about 2 % (A/B on one box).
     struct Fp64Pass { int k = -1, nb = 0, cps = 0; SlabSlot* Q = nullptr; FieldTab FT{}; };
     Fp64Pass prev;
+    const bool overlap = c->overlap_fp64;
     auto launch_fp64 = [&](const Fp64Pass& f, hipEvent_t after) -> hipError_t {
-        hipStream_t B = c->stream2;
+        hipStream_t B = overlap ? c->stream2 : c->stream;
         hipError_t e = hipStreamWaitEvent(B, after, 0);
         if (e == hipSuccess) e = hipEventRecord(ev[6 * f.k + 4], B);
         OutP Os = O;
@@ -1185,26 +1274,30 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             e = launch_fp64(prev, ev[6 * k + 1]);
             prev.k = -1;
         }
+        if (tiles && e == hipSuccess) e = launch_bin_reset(A, bbuf.cell, sl.cols, bbuf.tile_count, ntile);
         if (rk4) {
             OutP Os = O;
             Os.seg_job = Q.seg_job.p;
+            if (tiles) {
+                Os.bin_cell = bbuf.cell;
+                Os.bin_amp = bbuf.amp;
+            }
             const int cps = (int)((sl.maxcap + 63) / 64);
             if (e == hipSuccess)
                 e = launch_rk4(A, P, Os, OUT_PMD, Q.init.p, Q.seg_start.p, Q.counts.p, nb, cps, FT,
                                two_pass ? Q.flags.p : nullptr, nullptr);
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
+            if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
             if (two_pass) {
-                // the electrons flagged by the FP32 pass, integrated again in FP64 on the second
-                // stream (launched after the next slab's sampling, or after the loop)
+                // the electrons flagged by the FP32 pass, integrated again in FP64: now, or (overlap)
+                // on the second stream after the next slab's sampling or after the loop
                 if (e == hipSuccess)
                     e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p, nullptr,
                                                Q.counts2.p, nullptr);
                 if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);
-                prev.k = k;
-                prev.nb = nb;
-                prev.cps = cps;
-                prev.Q = &Q;
-                prev.FT = FT;
+                const Fp64Pass cur{k, nb, cps, &Q, FT};
+                if (overlap) prev = cur;
+                else if (e == hipSuccess) e = launch_fp64(cur, ev[6 * k + 3]);
             }
         } else {
             const int32_t j = items[sl.i0].job;   // a Tsit5 slab holds one job
@@ -1219,11 +1312,18 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                 Oj.scalq = O.scalq + 4 * j;
             }
             if (O.hits) Oj.hits = O.hits + j * npix;
+            OutP Om = Oj;   // the main launch: records for the tiles, or atomics
+            if (tiles) {
+                Om.bin_cell = bbuf.cell;
+                Om.bin_amp = bbuf.amp;
+                Om.cell0 = j * npix;
+            }
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(A, Pj, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
+                e = launch_tsit5(A, Pj, Om, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
                                  P.mixed ? Q.flags.p : nullptr, nullptr);
             if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
+            if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
             if (P.mixed) {   // the flagged electrons again in FP64, on the same stream
                 IntegP P64 = Pj;
                 P64.mixed = 0;
@@ -1368,6 +1468,85 @@ int etraj_run_scan_device(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* 
                           const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats) {
     if (c && !pmd_dev) return fail(c, ETRAJ_EINVAL, "pmd_dev is NULL");
     return run_scan_impl(c, n_jobs, lasers, target, samplers, run, nullptr, pmd_dev, stats);
+}
+
+int etraj_get_tuning(const etraj_ctx* c, etraj_tuning* out) {
+    if (!c || !out) return ETRAJ_EINVAL;
+    *out = tuning_of(c);
+    return ETRAJ_OK;
+}
+
+int etraj_set_tuning(etraj_ctx* c, const etraj_tuning* t) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    if (!t) return fail(c, ETRAJ_EINVAL, "tuning is NULL");
+    if (const char* why = tuning_invalid(*t)) return fail(c, ETRAJ_EINVAL, why);
+    apply_tuning(c, *t);
+    return ETRAJ_OK;
+}
+
+int etraj_autotune(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                   const etraj_sampler_desc* sampler, const etraj_run_desc* run, etraj_tuning* best,
+                   double* best_ms, double* start_ms) {
+    if (!c) return ETRAJ_EINVAL;
+    c->err.clear();
+    if (!laser || !target || !sampler || !run) return fail(c, ETRAJ_EINVAL, "descriptor is NULL");
+    const etraj_tuning start = tuning_of(c);
+    // one candidate: two runs of the job (the first also allocates what the choice needs), the faster counts
+    auto time_of = [&](const etraj_tuning& t, double& ms) -> int {
+        apply_tuning(c, t);
+        ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            const double* dev = nullptr;
+            etraj_stats st{};
+            if (int rc = run_scan_impl(c, 1, laser, target, sampler, run, nullptr, &dev, &st)) return rc;
+            ms = std::min(ms, st.ms_total);
+        }
+        return ETRAJ_OK;
+    };
+    etraj_tuning top = start;
+    double top_ms = 0;
+    int rc = time_of(start, top_ms);
+    const double first_ms = top_ms;
+    // coordinate descent over the choices that matter for the job's integrator: a candidate
+    // replaces the best set only if it is faster by more than 1 %
+    auto try_field = [&](int32_t etraj_tuning::*field, std::initializer_list<int32_t> values) {
+        for (int32_t v : values) {
+            if (rc || top.*field == v) continue;
+            etraj_tuning cand = top;
+            cand.*field = v;
+            double ms = 0;
+            rc = time_of(cand, ms);
+            if (!rc && ms < 0.99 * top_ms) {
+                top = cand;
+                top_ms = ms;
+            }
+        }
+    };
+    const bool rk4 = run->integrator == ETRAJ_INTEG_RK4;
+    if (rk4) {
+        try_field(&etraj_tuning::rk4_block, {64, 128, 256});
+        if (run->precision == ETRAJ_PREC_MIXED && run->restep_radius > 0) try_field(&etraj_tuning::overlap_fp64, {0, 1});
+    } else {
+        try_field(&etraj_tuning::tsit5_refill, {4, 8, 16, 32, 64});
+        try_field(&etraj_tuning::tsit5_claim, {64, 256, 1024, 4096});
+    }
+    if (run->pmd_binning == ETRAJ_BIN_AUTO) {
+        try_field(&etraj_tuning::bin_tiles, {0, 1});
+        if (top.bin_tiles) {
+            try_field(&etraj_tuning::bin_tile_shift, {0, 10});
+            try_field(&etraj_tuning::bin_chunk, {4096, 16384, 65536});
+        }
+    }
+    if (rc) {
+        apply_tuning(c, start);
+        return rc;
+    }
+    apply_tuning(c, top);
+    if (best) *best = top;
+    if (best_ms) *best_ms = top_ms;
+    if (start_ms) *start_ms = first_ms;
+    return ETRAJ_OK;
 }
 
 }  // extern "C"

@@ -18,6 +18,9 @@ __device__ inline double readlane_f64(double v, int lane) {
 __device__ inline float readlane_f32(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
+__device__ inline float uniform_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 __device__ inline double uniform_f64(double v) {
     int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
     int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));

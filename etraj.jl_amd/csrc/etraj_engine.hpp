@@ -63,6 +63,9 @@ struct IntegP {
     double rtol, atol;
     int64_t maxiters;
     int integrator;
+    // launch choices (etraj_tuning): RK4 threads per workgroup; Tsit5 idle lanes
+    // before a refill and electrons per queue claim
+    int rk4_block, ts5_refill, ts5_claim;
 };
 
 // per-electron binning record for host-side accumulation in column order
@@ -165,6 +168,11 @@ struct FieldTab {
 // pmdq (two words per value, job stride 2 * job_stride) and scalq (4 words per
 // job) instead of pmd and scal. hits (may be null): one counter per cell, job
 // stride npix.
+// bin_cell (may be null): the PMD contributions of an ETRAJ_PMD_* launch are
+// written as records for the tile binning (etraj_bin.hip) instead of being
+// added to pmd / pmdq / hits: bin_cell[col] = cell0 + cell (kNoCell if the
+// electron is not binned) and bin_amp[col] = (re, im) of its contribution;
+// cell0 = job * npix.
 struct OutP {
     double* pmd;       // CTMC: npix; coherent: 2*npix (re, im) per job
     double* scal;      // [ion_prob, ion_prob_uncollected] per job
@@ -178,7 +186,44 @@ struct OutP {
     Contrib* contrib;
     const int32_t* seg_job;
     int64_t job_stride;
+    uint32_t* bin_cell;
+    double2* bin_amp;
+    int64_t cell0;
 };
+
+// ---- tile binning (etraj_bin.hip) ----------------------------------------
+constexpr uint32_t kNoCell = 0xffffffffu;
+constexpr int kBinMaxTiles = 8192;            // tiles of one binning (all jobs of a slab)
+constexpr int kBinTileBytes = 64 * 1024;      // LDS of one tile's accumulator
+// Tile 2^shift cells; chunk: records per accumulating workgroup
+struct BinCfg {
+    int shift, chunk;
+};
+struct BinBufs {
+    uint32_t* cell;        // records: n (written by the integrator)
+    double2* amp;
+    uint32_t* tile_count;  // kBinMaxTiles, zero between binnings
+    uint32_t* tile_off;    // kBinMaxTiles + 1
+    uint32_t* chunk_off;   // kBinMaxTiles + 1
+    uint32_t* cursor;      // kBinMaxTiles
+    uint32_t* scell;       // records sorted by tile: n
+    double2* samp;
+};
+struct BinOut {
+    double* pmd;       // ETRAJ_PMD_FLOAT: words per cell, all jobs (cell g at words * g)
+    uint64_t* pmdq;    // ETRAJ_PMD_FIXED: 2 words per value
+    uint64_t* hits;    // may be null
+    int64_t* cnt;      // per-job counters (CNT_FIXRANGE)
+    int64_t npix;
+};
+// The largest tile (2^shift cells, shift <= max_shift) whose accumulator fits kBinTileBytes.
+int bin_tile_shift(int words, bool fixed, bool hits, int max_shift);
+// cell[0, n) = kNoCell, tile_count[0, ntile) = 0 (before the integrator writes the records)
+hipError_t launch_bin_reset(hipStream_t st, uint32_t* cell, int64_t n, uint32_t* tile_count, int ntile);
+// Adds the records cell/amp[0, n) of Bf into out; total_cells = jobs * npix < 2^32 - 1,
+// at most kBinMaxTiles tiles. tile_count must be zero on entry and is zero on return.
+hipError_t launch_bin_tiles(hipStream_t st, const BinCfg& cfg, const BinBufs& Bf, int64_t n, int64_t total_cells,
+                            int words, bool fixed, const BinOut& out);
 
 // Per-segment (one t0 batch) description for the sampler and the field table:
 // the batch's t0, its job, its index within the job (the Philox counter) and

@@ -104,9 +104,14 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
     }
     if (OUT == OUT_PMD) {
         int64_t nr = 0;
+        const bool rec = O.bin_cell != nullptr;   // records for the tile binning (wave-uniform)
+        if (rec && active) {
+            O.bin_cell[col_idx] = inb ? (uint32_t)(O.cell0 + lin) : kNoCell;
+            O.bin_amp[col_idx] = make_double2(cre, cim);
+        }
         if (O.pmdq) {   // fixed point: exact integer sums (wave-uniform branch)
             bool ok1 = true, ok2 = true, ok3, ok4;
-            if (inb) {
+            if (inb && !rec) {
                 if (PHASE == ETRAJ_PHASE_CTMC) {
                     fix_atomic_add(O.pmdq + 2 * lin, fix_from_double(cre, ok1));
                 } else {
@@ -122,7 +127,7 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
                 fix_atomic_add(O.scalq + 2, iu);
             }
         } else {
-            if (inb) {
+            if (inb && !rec) {
                 if (PHASE == ETRAJ_PHASE_CTMC) {
                     unsafeAtomicAdd(O.pmd + lin, cre);
                 } else {
@@ -137,7 +142,7 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
                 if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
             }
         }
-        if (O.hits && inb) atomicAdd((unsigned long long*)(O.hits + lin), 1ull);
+        if (O.hits && inb && !rec) atomicAdd((unsigned long long*)(O.hits + lin), 1ull);
         int64_t nf = wave_sum_i64((active && failed) ? 1 : 0);
         int64_t nb = wave_sum_i64(bound ? 1 : 0);
         if (lane_id() == 0) {
@@ -255,8 +260,8 @@ __device__ inline void rk4_fp32_pass(const IntegP& P, bool saeq, const Lane<DIM,
             }
             if constexpr (HL) {
                 float SA, SQ;
-                hl_rk4_sums<float, DIM, PHASE, SOFT, UNITZ>(Zf, af, r, p, hf, Fpx, Fpy, f.x, f.y, f.z, f.w, Sr, Sp, SA,
-                                                            SQ, r2);
+                hl_rk4_sums<float, DIM, PHASE, SOFT, UNITZ>(Zf, af, r, p, hf, 0.5f * hf, Fpx, Fpy, f.x, f.y, f.z, f.w, Sr,
+                                                            Sp, SA, SQ, r2);
                 if (PHASE != ETRAJ_PHASE_CTMC) qs = fma(pc.cA, SA, fma(pc.cQ, SQ, qs));
             } else {
                 float SPh;
@@ -321,15 +326,17 @@ __device__ inline void rk4_fp64_pass(const IntegP& P, bool saeq, const Lane<DIM,
         p[i] = e.p[i];
     }
     phase = e.phase0;
-    const double h6 = h / 6;
-    const HlPhase<double> pc = hl_phase<double, PHASE, true>(h, P.T.Ip);
+    h = uniform_f64(h);
+    const double hh = uniform_f64(0.5 * h), h6 = uniform_f64(h / 6);
+    const HlPhase<double> pc0 = hl_phase<double, PHASE, true>(h, P.T.Ip);
+    const HlPhase<double> pc{uniform_f64(pc0.cA), uniform_f64(pc0.cQ), uniform_f64(pc0.c0)};
     double Fpx = F0.x, Fpy = F0.y;
     auto step = [&](const double2 Fm, const double2 F1) {
         double Sr[DIM], Sp[DIM], r2;
         if constexpr (HL) {
             double SA, SQ;
-            hl_rk4_sums<double, DIM, PHASE, true, UNITZ>(P.T.Zd, P.T.sc, r, p, h, Fpx, Fpy, Fm.x, Fm.y, F1.x, F1.y, Sr,
-                                                         Sp, SA, SQ, r2);
+            hl_rk4_sums<double, DIM, PHASE, true, UNITZ>(P.T.Zd, P.T.sc, r, p, h, hh, Fpx, Fpy, Fm.x, Fm.y, F1.x,
+                                                         F1.y, Sr, Sp, SA, SQ, r2);
             if (PHASE != ETRAJ_PHASE_CTMC) phase = fma(pc.cA, SA, fma(pc.cQ, SQ, phase));
             if (PHASE == ETRAJ_PHASE_QTMC) phase += pc.c0;
         } else {
@@ -394,6 +401,7 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
             Oj.scalq = O.scalq + 4 * job;
         }
         if (O.hits) Oj.hits = O.hits + job * O.npix;
+        Oj.cell0 = job * O.npix;
     }
 
     Lane<DIM, PHASE> e;
@@ -573,12 +581,14 @@ template <int DIM, int PHASE, bool MIXED, int OUT>
 static hipError_t launch_rk4_t(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
                                const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
                                const FieldTab& FT, uint8_t* flags, const int64_t* out_index) {
-    const unsigned blocks = (unsigned)((n_seg * (int64_t)cps + 3) / 4);
+    const int block = (P.rk4_block == 64 || P.rk4_block == 128) ? P.rk4_block : 256;
+    const int waves = block / 64;
+    const unsigned blocks = (unsigned)((n_seg * (int64_t)cps + waves - 1) / waves);
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
-        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_HYDROGEN_LIKE>), dim3(blocks), dim3(block), 0, st,
                            P, O, init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
     else
-        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(256), 0, st, P, O,
+        hipLaunchKernelGGL((k_rk4<DIM, PHASE, MIXED, OUT, ETRAJ_TARGET_SAE>), dim3(blocks), dim3(block), 0, st, P, O,
                            init, seg_start, seg_count, n_seg, cps, FT, flags, out_index);
     return hipGetLastError();
 }

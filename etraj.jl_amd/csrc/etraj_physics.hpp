@@ -743,9 +743,9 @@ __host__ __device__ __forceinline__ R add_p2(const R* p, R acc) {
 // and SQ = sum_j w_j q_j (w = 1, 2, 2, 1), so that the phase increment over
 // the step is h/6 (cA SA - SQ/2) + h c0. r2first: |r|^2 + a at the step's start.
 template <typename R, int DIM, int PHASE, bool SOFT, bool UNITZ>
-__host__ __device__ __forceinline__ void hl_rk4_sums(R Z, R a, const R* r, const R* p, R h, R F0x, R F0y, R Fmx,
+// hh = h / 2 (passed in, so that a wave-uniform caller can keep it in a scalar register).
+__host__ __device__ __forceinline__ void hl_rk4_sums(R Z, R a, const R* r, const R* p, R h, R hh, R F0x, R F0y, R Fmx,
                                                      R Fmy, R F1x, R F1y, R* Sr, R* Sp, R& SA, R& SQ, R& r2first) {
-    const R hh = h * (R)0.5;
     constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
     R k[DIM], yr[DIM], yp[DIM], pa = 0, r2, Q14 = 0, Q23 = 0;   // |p|^2 sums of stages 1 + 4 and 2 + 3
     hl_stage<R, DIM, PHASE, SOFT, UNITZ>(Z, a, r, F0x, F0y, k, pa, r2first);

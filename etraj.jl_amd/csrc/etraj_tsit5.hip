@@ -73,7 +73,10 @@ __device__ inline void emit_lane(const IntegP& P, const OutP& O, int64_t col_idx
         if (!inb) acc.iu += prob;
     }
     if (OUT == OUT_PMD) {
-        if (inb) {
+        if (O.bin_cell) {   // records for the tile binning
+            O.bin_cell[col_idx] = inb ? (uint32_t)(O.cell0 + lin) : kNoCell;
+            O.bin_amp[col_idx] = make_double2(cre, cim);
+        } else if (inb) {
             if (O.pmdq) {
                 bool ok1 = true, ok2 = true;
                 if (PHASE == ETRAJ_PHASE_CTMC) {
@@ -150,19 +153,18 @@ __device__ inline int64_t item_segment(int64_t item, const int64_t* pre, int64_t
     return lo;
 }
 
-// Electrons a wave claims from the global queue at a time.
-constexpr int kTs5Claim = 256;
-// Idle lanes (finished or empty) a wave collects before it writes their
-// results and refills them: one pass of the divergent start-up (column look-up,
-// loads, initial step size) and finish (Kepler, binning) code serves many lanes.
-constexpr int kTs5Refill = 16;
+// P.ts5_claim: electrons a wave claims from the global queue at a time.
+// P.ts5_refill: idle lanes (finished or empty) a wave collects before it writes
+// their results and refills them: one pass of the divergent start-up (column
+// look-up, loads, initial step size) and finish (Kepler, binning) code serves
+// many lanes.
 
 // Adaptive Tsit5, one electron per lane, each lane advancing its electron one
 // attempted step per loop trip. Two ways to hand out the electrons (items,
 // 0-based over all electrons of all segments):
-//  - QUEUE: a grid of resident waves; once kTs5Refill lanes of a wave are idle,
+//  - QUEUE: a grid of resident waves; once P.ts5_refill lanes of a wave are idle,
 //    their results are written and they take the next items of the wave's
-//    pool, and a wave whose pool runs dry claims kTs5Claim more from the global
+//    pool, and a wave whose pool runs dry claims P.ts5_claim more from the global
 //    counter *queue (zero before the launch). Trajectories of very uneven cost
 //    (close encounters take tens of times the steps of the rest) then no longer
 //    hold 63 idle lanes for the slowest one. A wave leaves when the queue is
@@ -203,10 +205,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // segment holding item `pool`, and whether the counter has passed the last item
     int64_t pool = 0, pool_end = 0, pool_seg = 0;
     bool drained = false;
+    const int refill = P.ts5_refill;
+    const int64_t claim = P.ts5_claim;
     double t0 = 0.0, prob = 0.0;
     while (true) {
         const uint64_t idle = __ballot(!valid);
-        if (idle == ~0ull || (QUEUE && __popcll(idle) >= kTs5Refill)) {
+        if (idle == ~0ull || (QUEUE && __popcll(idle) >= refill)) {
             if (pending) {
                 pending = false;
                 emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
@@ -218,13 +222,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         if (drained) break;
                         // lane 0 claims for the wave; the other lanes add 0, so the claim
                         // is right whether or not the compiler merges the lanes' atomics
-                        const unsigned long long old = atomicAdd(queue, lane == 0 ? (unsigned long long)kTs5Claim : 0ull);
+                        const unsigned long long old = atomicAdd(queue, lane == 0 ? (unsigned long long)claim : 0ull);
                         const int64_t got = (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)old, 0) |
                                                       ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(old >> 32), 0) << 32));
-                        drained = got + kTs5Claim >= total;
+                        drained = got + claim >= total;
                         if (got >= total) break;
                         pool = got;
-                        pool_end = drained ? total : got + kTs5Claim;
+                        pool_end = drained ? total : got + claim;
                         pool_seg = item_segment(pool, pre, n_seg);
                     }
                     const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0));

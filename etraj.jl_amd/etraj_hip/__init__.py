@@ -19,6 +19,6 @@ from .targets import (Target, SAEAtomBase, HydrogenLikeAtom, SAEAtom, MoleculeBa
                       hartree_asymp_coeff, IonPotential, AsympNuclCharge, AngularQuantumNumber, MagneticQuantumNumber,
                       AsympCoeff, SoftCore, TargetName, QuantizationAxisOrientaion, TargetPotential, TargetForce)
 from .samplers import ElectronSampler, ADKSampler, SPASampler, SPANESampler, WFATSampler, init_sampler, batch_num, batch_max_size, gen_electron_batch
-from .simulation import perform_traj_simulation, perform_traj_scan, run_desc
+from .simulation import perform_traj_simulation, perform_traj_scan, run_desc, autotune
 
 __all__ = [n for n in dir() if not n.startswith("_")]

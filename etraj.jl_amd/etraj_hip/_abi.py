@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # ETRAJ_HIP_LIB names another build of the same library (the Julia binding reads the same variable)
 LIB_PATH = os.environ.get("ETRAJ_HIP_LIB", os.path.join(HERE, "libetraj_hip.so"))
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ECAPACITY, ERANGE = 0, -1, -2, -3, -4, -5, -6
 
@@ -32,6 +32,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 QUIRK_SCTS3D_XPY, QUIRK_SAE2D_SCTS = 1, 2
 QUIRKS_REFERENCE = QUIRK_SCTS3D_XPY | QUIRK_SAE2D_SCTS
 PMD_FLOAT, PMD_FIXED = 0, 1
+BIN_AUTO, BIN_ATOMIC, BIN_TILES = 0, 1, 2
 
 PHASE_METHODS = {"CTMC": PHASE_CTMC, "QTMC": PHASE_QTMC, "SCTS": PHASE_SCTS}
 
@@ -76,7 +77,7 @@ class RunDesc(C.Structure):
                 ("rtol", C.c_double), ("atol", C.c_double), ("restep_radius", C.c_double),
                 ("final_p_max", C.c_double * 3), ("final_p_num", C.c_int32 * 3),
                 ("quirks", C.c_int32), ("maxiters", C.c_int64), ("pmd_mode", C.c_int32),
-                ("reserved", C.c_int32), ("hit_counts", C.POINTER(C.c_uint64))]
+                ("pmd_binning", C.c_int32), ("hit_counts", C.POINTER(C.c_uint64))]
 
 
 class Stats(C.Structure):
@@ -89,6 +90,15 @@ class Stats(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("bin_tiles", C.c_int32), ("bin_tile_shift", C.c_int32), ("bin_chunk", C.c_int32),
+                ("overlap_fp64", C.c_int32), ("rk4_block", C.c_int32), ("tsit5_refill", C.c_int32),
+                ("tsit5_claim", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 P = C.POINTER
@@ -120,6 +130,10 @@ SIGNATURES = {
                                  P(Stats)]),
     "etraj_run_scan_device": (C.c_int, [C.c_void_p, C.c_int32, P(LaserDesc), P(TargetDesc), P(SamplerDesc),
                                         P(RunDesc), P(DP), P(Stats)]),
+    "etraj_get_tuning": (C.c_int, [C.c_void_p, P(Tuning)]),
+    "etraj_set_tuning": (C.c_int, [C.c_void_p, P(Tuning)]),
+    "etraj_autotune": (C.c_int, [C.c_void_p, P(LaserDesc), P(TargetDesc), P(SamplerDesc), P(RunDesc), P(Tuning),
+                                 DP, DP]),
 }
 
 
@@ -185,6 +199,22 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self.check(self.lib.etraj_get_tuning(self.handle, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **choices) -> dict:
+        """Changes the named launch choices (etraj_tuning fields) and returns the full set."""
+        t = Tuning()
+        self.check(self.lib.etraj_get_tuning(self.handle, C.byref(t)))
+        for k, v in choices.items():
+            if k not in t.as_dict():
+                raise ValueError(f"unknown tuning field {k}")
+            setattr(t, k, int(v))
+        self.check(self.lib.etraj_set_tuning(self.handle, C.byref(t)))
+        return t.as_dict()
 
     def check(self, rc: int):
         if rc != OK:

@@ -18,6 +18,9 @@ Keyword interface of the reference, plus engine keywords:
   pmd_accumulator  "float": FP64 atomic sums (the reference's Float64 sums, rounded
                    in arrival order); "fixed": 128-bit fixed-point sums (2^-100
                    resolution), bit-identical for any order of the electrons
+  pmd_binning      "auto" (the context's choice), "atomic" (one atomic addition per
+                   electron from the integrator) or "tiles" (electrons sorted by PMD
+                   tile, each tile accumulated in LDS and added to the PMD once)
   hit_counts       also return result["hit_counts"]: the number of electrons
                    binned into each PMD cell (the integer-count PMD)
 Returns the fields write_output! stores (:516-543) as a dict; when
@@ -43,7 +46,7 @@ from .targets import Serialize as _ser_target
 
 def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num, traj_integrator="rk4",
              traj_dt=0.1, traj_rtol=1e-6, precision=None, restep_radius=2.0, reproduce_quirks=True,
-             pmd_accumulator="float", **_):
+             pmd_accumulator="float", pmd_binning="auto", **_):
     d = _abi.RunDesc()
     d.dimension = int(dimension)
     d.phase_method = _abi.PHASE_METHODS[_sym(traj_phase_method)]
@@ -74,6 +77,10 @@ def run_desc(*, dimension, traj_phase_method, traj_t_final, final_p_max, final_p
     if acc not in ("float", "fixed"):
         raise ValueError(f"unknown pmd_accumulator {pmd_accumulator}")
     d.pmd_mode = _abi.PMD_FIXED if acc == "fixed" else _abi.PMD_FLOAT
+    binning = {"auto": _abi.BIN_AUTO, "atomic": _abi.BIN_ATOMIC, "tiles": _abi.BIN_TILES}
+    if _sym(pmd_binning).lower() not in binning:
+        raise ValueError(f"unknown pmd_binning {pmd_binning}")
+    d.pmd_binning = binning[_sym(pmd_binning).lower()]
     return d
 
 
@@ -104,7 +111,7 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
                             mol_orbit_ridx=0, show_progress=False, traj_integrator="tsit5", traj_dt=0.1,
                             precision=None, restep_radius=2.0, granularity="job", seed=0,
                             sample_stratified=False, kd_sigma=0.0, kz_sigma=0.0, reproduce_quirks=True,
-                            pmd_accumulator="float", hit_counts=False, backend="hip"):
+                            pmd_accumulator="float", pmd_binning="auto", hit_counts=False, backend="hip"):
     if _sym(backend) != "hip":
         raise ValueError("this package is the HIP backend; the CPU method is the reference itself")
     traj_rtol = _check_rtol(traj_integrator, traj_rtol)
@@ -130,7 +137,7 @@ def perform_traj_simulation(*, init_cond_method, laser, target, dimension, sampl
     rd = run_desc(dimension=dimension, traj_phase_method=traj_phase_method, traj_t_final=traj_t_final,
                   final_p_max=final_p_max, final_p_num=final_p_num, traj_integrator=traj_integrator,
                   traj_dt=traj_dt, traj_rtol=traj_rtol, precision=precision, restep_radius=restep_radius,
-                  reproduce_quirks=reproduce_quirks, pmd_accumulator=pmd_accumulator)
+                  reproduce_quirks=reproduce_quirks, pmd_accumulator=pmd_accumulator, pmd_binning=pmd_binning)
     ctx = _abi.default_context()
     npix = int(np.prod(final_p_num))
     hits = np.zeros(npix, dtype=np.uint64) if hit_counts else None
@@ -255,12 +262,32 @@ def _save(result, output_path, output_fmt="jld2", output_compress=True):
     result["output_path"] = path
 
 
+def autotune(**kwargs) -> dict:
+    """The engine's autotuner (etraj_autotune) on the job that perform_traj_simulation(**kwargs) would run:
+    each candidate launch choice (workgroup size, binning, tile size, FP64-pass overlap for RK4; refill
+    threshold, queue claim, binning for Tsit5) is timed on the whole job, and the fastest set stays in the
+    default context for the calls that follow. Returns {"best": choices, "best_ms": ms per job, "start_ms": ms
+    per job with the choices it started from}."""
+    kw = dict(_SCAN_DEFAULTS)
+    kw.update(kwargs)
+    kw["traj_rtol"] = _check_rtol(kw["traj_integrator"], kw["traj_rtol"])
+    sp = init_sampler(**{k: v for k, v in kw.items() if k not in ("output_path", "output_fmt", "output_compress",
+                                                                   "hit_counts")})
+    rd = run_desc(**kw)
+    ctx = _abi.default_context()
+    best = _abi.Tuning()
+    best_ms, start_ms = _abi.C.c_double(0), _abi.C.c_double(0)
+    ctx.check(ctx.lib.etraj_autotune(ctx.handle, kw["laser"]._desc(), kw["target"]._desc(), sp.desc, rd,
+                                     _abi.C.byref(best), _abi.C.byref(best_ms), _abi.C.byref(start_ms)))
+    return {"best": best.as_dict(), "best_ms": best_ms.value, "start_ms": start_ms.value}
+
+
 _SCAN_DEFAULTS = dict(ss_kd_max=0.0, ss_kd_num=0, ss_kz_max=0.0, ss_kz_num=0, mc_kt_num=0, mc_kd_max=0.0,
                       mc_kz_max=0.0, traj_phase_method="CTMC", traj_rtol=None, sample_monte_carlo=False,
                       sample_cutoff_limit=1e-16, rate_prefix="Full", mol_orbit_ridx=0, traj_integrator="tsit5",
                       traj_dt=0.1, precision=None, restep_radius=2.0, seed=0, sample_stratified=False, kd_sigma=0.0,
                       kz_sigma=0.0, reproduce_quirks=True, output_path=None, output_fmt="jld2",
-                      output_compress=True, pmd_accumulator="float", hit_counts=False)
+                      output_compress=True, pmd_accumulator="float", pmd_binning="auto", hit_counts=False)
 
 
 def perform_traj_scan(jobs):

@@ -57,7 +57,7 @@ struct RunDesc
     dimension::Int32; phase_method::Int32; integrator::Int32; precision::Int32
     traj_t_final::Float64; dt::Float64; rtol::Float64; atol::Float64; restep_radius::Float64
     final_p_max::NTuple{3,Float64}; final_p_num::NTuple{3,Int32}; quirks::Int32; maxiters::Int64
-    pmd_mode::Int32; reserved::Int32; hit_counts::Ptr{UInt64}
+    pmd_mode::Int32; pmd_binning::Int32; hit_counts::Ptr{UInt64}
 end
 mutable struct Stats
     ion_prob::Float64; ion_prob_uncollected::Float64
@@ -68,6 +68,7 @@ mutable struct Stats
 end
 
 const PHASE = Dict(:CTMC => Int32(0), :QTMC => Int32(1), :SCTS => Int32(2))
+const BINNING = Dict(:auto => Int32(0), :atomic => Int32(1), :tiles => Int32(2))
 
 mono(l::Cos2Laser) = MonoLaser(1, 0, PeakInt(l), WaveLen(l), CycNum(l), Ellipticity(l), Azimuth(l), CEP(l), TimeShift(l))
 mono(l::Cos4Laser) = MonoLaser(2, 0, PeakInt(l), WaveLen(l), CycNum(l), Ellipticity(l), Azimuth(l), CEP(l), TimeShift(l))
@@ -89,15 +90,17 @@ target_desc(t::GenericMolecule) = TargetDesc(1, AsympNuclCharge(t), 0, 0, IonPot
 # the pointer of a Vector{UInt64} of prod(final_p_num) cells (kept alive by the caller) that receives
 # the number of electrons binned into each cell
 # integrator :rk4 (fixed step dt) or :tsit5 (adaptive, reltol `rtol`, abstol 1e-6 as OrdinaryDiffEq's default)
+# pmd_binning :auto, :atomic or :tiles (ETRAJ_BIN_*: LDS-privatised PMD tiles)
 function run_desc(dimension, phase, traj_t_final, final_p_max, final_p_num; dt=0.1, precision=:mixed,
                   restep_radius=2.0, reproduce_quirks=true, pmd_mode=:float, hit_counts=Ptr{UInt64}(C_NULL),
-                  integrator=:rk4, rtol=1e-6)
+                  integrator=:rk4, rtol=1e-6, pmd_binning=:auto)
     pmax = ntuple(i -> i <= dimension ? Float64(final_p_max[i]) : 1.0, 3)
     pnum = ntuple(i -> i <= dimension ? Int32(final_p_num[i]) : Int32(1), 3)
     integ = integrator == :tsit5 ? Int32(1) : integrator == :rk4 ? Int32(0) :
             error("[eTrajHIP] unknown integrator $integrator")
     RunDesc(dimension, PHASE[phase], integ, precision == :mixed ? 1 : 0, traj_t_final, dt, rtol, 1e-6, restep_radius,
-            pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000, pmd_mode == :fixed ? 1 : 0, 0, hit_counts)
+            pmax, pnum, reproduce_quirks ? 3 : 0, 1_000_000, pmd_mode == :fixed ? 1 : 0,
+            Int32(BINNING[pmd_binning]), hit_counts)
 end
 
 # ---- context ------------------------------------------------------------------

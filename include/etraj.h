@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ETRAJ_ABI_VERSION 3
+#define ETRAJ_ABI_VERSION 4
 
 /* status codes */
 #define ETRAJ_OK             0
@@ -196,6 +196,20 @@ enum {
                                order-independent, bit-identical from run to run and for any
                                slab split; ETRAJ_ERANGE if a sum leaves the range           */
 };
+/* How the device binning of etraj_run_job / etraj_run_scan adds the electrons to the PMD */
+enum {
+    ETRAJ_BIN_AUTO = 0,     /* the context's choice (ETRAJ_BIN_ATOMIC unless etraj_autotune
+                               measured the tiles faster for the workload)                 */
+    ETRAJ_BIN_ATOMIC = 1,   /* one global atomic addition per electron and value, from the
+                               integrator kernel                                            */
+    ETRAJ_BIN_TILES = 2     /* the integrator writes (cell, contribution) records; they are
+                               sorted by PMD tile and each tile is accumulated in LDS
+                               (privatised per workgroup) and added to the PMD in HBM once
+                               per non-zero cell. Same sums (ETRAJ_PMD_FIXED: bit for bit);
+                               the FP64 re-integration pass of MIXED runs adds its electrons
+                               with atomics. ETRAJ_EUNSUPPORTED if the PMD needs more than
+                               8192 tiles or 2^32 - 1 cells                                 */
+};
 
 typedef struct etraj_run_desc {
     int32_t dimension;
@@ -212,7 +226,7 @@ typedef struct etraj_run_desc {
     int32_t quirks;            /* ETRAJ_QUIRK_* bits                                       */
     int64_t maxiters;          /* Tsit5 step-attempt limit (<= 0: 1000000)                  */
     int32_t pmd_mode;          /* ETRAJ_PMD_* (the device binning of run_job / run_scan)    */
-    int32_t reserved;          /* 0                                                        */
+    int32_t pmd_binning;       /* ETRAJ_BIN_* (the device binning of run_job / run_scan)   */
     uint64_t* hit_counts;      /* NULL, or host counters, one per PMD cell (px fastest; for
                                   run_scan n_jobs * prod(final_p_num), job-major): the number
                                   of electrons binned into each cell is ADDED to them (the
@@ -335,6 +349,37 @@ int etraj_run_scan(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc* laser
 int etraj_run_scan_device(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc* lasers,
                           const etraj_target_desc* target, const etraj_sampler_desc* samplers,
                           const etraj_run_desc* run, const double** pmd_dev, etraj_stats* stats);
+
+/* ---- launch tuning --------------------------------------------------------------
+ * No reference counterpart (the reference runs on CPU threads): the launch
+ * choices of a ctx. Each choice changes the speed only; the results are the same
+ * up to the order of FP64 sums (bit for bit with ETRAJ_PMD_FIXED). */
+typedef struct etraj_tuning {
+    int32_t bin_tiles;       /* what ETRAJ_BIN_AUTO means: 0 atomics, 1 LDS tiles              */
+    int32_t bin_tile_shift;  /* tiles of 2^shift PMD cells, 6..16; 0: the largest that fits the
+                                64 KiB LDS accumulator                                        */
+    int32_t bin_chunk;       /* records per accumulating workgroup, 256..1048576               */
+    int32_t overlap_fp64;    /* MIXED RK4: 1 runs each slab's FP64 re-integration on a second
+                                stream beside the next slab's FP32 pass, 0 right after it      */
+    int32_t rk4_block;       /* RK4 threads per workgroup: 64, 128 or 256                      */
+    int32_t tsit5_refill;    /* Tsit5: idle lanes of a wavefront before it refills them, 1..64 */
+    int32_t tsit5_claim;     /* Tsit5: electrons a wavefront claims from the queue, 64..65536  */
+    int32_t reserved;        /* 0 */
+} etraj_tuning;
+
+/* The ctx's current choices. */
+int etraj_get_tuning(const etraj_ctx* ctx, etraj_tuning* out);
+/* Sets the ctx's choices (ETRAJ_EINVAL if a field is out of its range). */
+int etraj_set_tuning(etraj_ctx* ctx, const etraj_tuning* tuning);
+/* The autotuner: runs the job (as etraj_run_job_device) with the ctx's choices
+ * and with candidates that change one choice at a time (RK4: workgroup size,
+ * binning, tile size, FP64-pass overlap; Tsit5: refill threshold, queue claim,
+ * binning, tile size), each after a warm-up run, keeps the fastest set in the
+ * ctx and returns it with its time (wall time of one job, ms) and the time of
+ * the choices it started from. About 10-14 job runs. */
+int etraj_autotune(etraj_ctx* ctx, const etraj_laser_desc* laser, const etraj_target_desc* target,
+                   const etraj_sampler_desc* sampler, const etraj_run_desc* run, etraj_tuning* best,
+                   double* best_ms, double* start_ms);
 
 /* No reference counterpart: for callers that cannot allocate device memory
  * (tight serving loops, harnesses that forbid hipMalloc). Same arguments,

@@ -39,7 +39,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     import subprocess
     structs = {"etraj_mono_laser": _abi.MonoLaser, "etraj_laser_desc": _abi.LaserDesc,
                "etraj_target_desc": _abi.TargetDesc, "etraj_sampler_desc": _abi.SamplerDesc,
-               "etraj_run_desc": _abi.RunDesc, "etraj_stats": _abi.Stats}
+               "etraj_run_desc": _abi.RunDesc, "etraj_stats": _abi.Stats, "etraj_tuning": _abi.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "etraj.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -135,7 +135,7 @@ int main(int argc, char** argv) {
         laser_field(L, t0 + (2 * s + 2) * (h / 2), F1x, F1y);
         if (form == 0) {
             double SA, SQ;
-            hl_rk4_sums<double, 2, ETRAJ_PHASE_SCTS, true, true>(T.Zd, T.sc, r, p, h, F0x, F0y, Fmx, Fmy, F1x, F1y,
+            hl_rk4_sums<double, 2, ETRAJ_PHASE_SCTS, true, true>(T.Zd, T.sc, r, p, h, 0.5 * h, F0x, F0y, Fmx, Fmy, F1x, F1y,
                                                                    Sr, Sp, SA, SQ, r2);
             ph = fma(pc.cA, SA, fma(pc.cQ, SQ, ph));
         } else {

@@ -64,6 +64,7 @@ struct SlabSlot {
     DevBuf<double2> ft64, ft0;
     DevBuf<uint8_t> flags;
     DevBuf<unsigned long long> queue;   // work-queue counter of the slot's Tsit5 launches
+    DevBuf<double> prep;                // Tsit5 start values (k_ts5_prep), ts5_prep_doubles per column
     std::vector<double> h_t0;
     std::vector<int32_t> h_job, h_b;
     std::vector<int64_t> h_start;
@@ -72,7 +73,7 @@ struct SlabSlot {
     void release() {
         init.release(); seg_t0.release(); counts.release(); counts2.release(); seg_job.release(); seg_b.release();
         seg_start.release(); pre.release(); ft32.release(); ft64.release(); ft0.release(); flags.release();
-        queue.release();
+        queue.release(); prep.release();
         if (done) (void)hipEventDestroy(done);
         done = nullptr;
     }
@@ -1187,6 +1188,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             HIP_TRY(c, Q.ft0.ensure(max_nb));
         }
         if (two_pass || (!rk4 && P.mixed)) HIP_TRY(c, Q.flags.ensure(max_cols));
+        if (!rk4) HIP_TRY(c, Q.prep.ensure((size_t)max_cols * ts5_prep_doubles(P.dim)));
         if (!rk4 && P.mixed) HIP_TRY(c, Q.counts2.ensure(max_nb));
     }
     HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
@@ -1318,13 +1320,14 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                 Om.bin_amp = bbuf.amp;
                 Om.cell0 = j * npix;
             }
+            // start values of every electron, the integration, then the Kepler asymptote and binning of the
+            // final states (launch_tsit5_staged); ev[6k+2] marks the end of the integrator kernel
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5(A, Pj, Om, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
-                                 P.mixed ? Q.flags.p : nullptr, nullptr);
-            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
+                e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
+                                        Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 2]);
             if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
-            if (P.mixed) {   // the flagged electrons again in FP64, on the same stream
+            if (P.mixed) {   // the flagged electrons again in FP64, on the same stream (binned with atomics)
                 IntegP P64 = Pj;
                 P64.mixed = 0;
                 if (e == hipSuccess)
@@ -1332,8 +1335,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                                                Q.counts2.p, nullptr);
                 if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
                 if (e == hipSuccess)
-                    e = launch_tsit5(A, P64, Oj, OUT_PMD, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
-                                     nullptr, nullptr);
+                    e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
+                                            Q.prep.p, nullptr, nullptr);
             }
         }
         if (!two_pass) {

@@ -18,9 +18,6 @@ __device__ inline double readlane_f64(double v, int lane) {
 __device__ inline float readlane_f32(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
-__device__ inline float uniform_f32(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
 __device__ inline double uniform_f64(double v) {
     int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
     int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
@@ -40,6 +37,17 @@ __device__ inline float4 row_load_f4(__amdgpu_buffer_rsrc_t rs, int byte_off) {
 }
 __device__ inline double2 row_load_d2(__amdgpu_buffer_rsrc_t rs, int byte_off) {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, 0, byte_off, 0));
+}
+
+// The segment holding item (0-based over all electrons of all segments):
+// pre[s] <= item < pre[s + 1] (the last such s where segments are empty).
+__device__ inline int64_t item_segment(int64_t item, const int64_t* pre, int64_t n_seg) {
+    int64_t lo = 0, hi = n_seg;
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= item) lo = mid; else hi = mid;
+    }
+    return lo;
 }
 
 __device__ inline int64_t uniform_i64(int64_t v) {

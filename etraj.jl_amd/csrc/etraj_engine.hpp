@@ -77,7 +77,11 @@ struct Contrib {
     int32_t pad;
 };
 
-enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2 };
+// OUT_STATE (Tsit5 in run_job): the kernel leaves each electron's final state
+// (r, p, phase; x = NaN if its integration failed) in its init column for
+// k_ts5_finish, which bins it; the start values of the adaptive step come from
+// k_ts5_prep.
+enum { OUT_PMD = 0, OUT_RECORDS = 1, OUT_CONTRIB = 2, OUT_STATE = 3 };
 // CNT_RESTEP counts the electrons of the mixed pass re-integrated in FP64
 // CNT_LAUNCHED counts the sampled electrons (the sampler kernel adds each batch's survivors)
 // CNT_FIXRANGE counts values too large for the fixed-point accumulator
@@ -271,6 +275,22 @@ hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* 
 hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out, const double* init,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
                         unsigned long long* queue, uint8_t* flags, const int64_t* orig);
+// Doubles per column of the Tsit5 start values (ts5_prep_doubles): the first
+// step size and the first stage derivative (p part, phase part).
+constexpr int ts5_prep_doubles(int dim) { return dim + 2; }
+// The Tsit5 run of run_job in three launches over the items [0, pre[n_seg]) of
+// the segments: k_ts5_prep (the start values of every electron into prep, in
+// parallel), k_tsit5 with OUT_STATE (final states into the init columns,
+// which it writes; P.mixed: flags as launch_tsit5), k_ts5_finish (Kepler
+// asymptote and binning of every item's final state into O, skipping flagged
+// columns when flags is not null). max_items >= pre[n_seg].
+hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
+                               const int64_t* pre, int64_t n_seg, int64_t max_items, unsigned long long* queue,
+                               double* prep, uint8_t* flags, hipEvent_t main_done);
+// k_ts5_finish alone (etraj_kernels.hip): the binning of the final states
+hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                             const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                             const uint8_t* flags);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
 // out[i] = fix_to_double(q[2i], q[2i+1]), i < n

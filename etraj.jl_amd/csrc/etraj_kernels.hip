@@ -475,6 +475,35 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
                                             flags, out_index);
 }
 
+// The Kepler asymptote and binning of the final states that k_tsit5 (OUT_STATE)
+// left in the init columns (x = NaN: the integration failed): one item per
+// lane, 64 consecutive items per wavefront, every lane of a wavefront taking
+// part in emit's wave sums. flags (may be null): columns with flags[col] != 0
+// were stopped for the FP64 re-integration and are not binned here.
+template <int DIM, int PHASE>
+__global__ __launch_bounds__(256) void k_ts5_finish(IntegP P, OutP O, const double* __restrict__ init,
+                                                    const int64_t* __restrict__ seg_start,
+                                                    const int64_t* __restrict__ pre, int64_t n_seg,
+                                                    const uint8_t* __restrict__ flags) {
+    const int64_t total = pre[n_seg];
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (uniform_i64(item - lane_id()) >= total) return;   // the whole wavefront is past the last item
+    bool active = item < total;
+    int64_t col = 0;
+    if (active) {
+        const int64_t sg = item_segment(item, pre, n_seg);
+        col = seg_start[sg] + (item - pre[sg]);
+        if (flags && flags[col]) active = false;
+    }
+    Lane<DIM, PHASE> e;
+    load_column<DIM, PHASE>(init + col * P.rows, e);
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) bad = bad || isnan(e.r[i]) || isnan(e.p[i]);
+    if (PHASE != ETRAJ_PHASE_CTMC) bad = bad || isnan(e.phase0);
+    emit<DIM, PHASE, OUT_PMD>(P, O, col, active, bad, e.r, e.p, e.phase0, e.t0, e.prob);
+}
+
 // In-place, order-preserving compaction of the flagged columns of each
 // segment (one workgroup per segment): count_out[s] receives their number and
 // orig[seg_start[s] + k] the original column (or its out_index entry) of the
@@ -563,6 +592,36 @@ __global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, doubl
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int DIM, int PHASE>
+static hipError_t launch_ts5_finish_p(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                                      const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                      const uint8_t* flags) {
+    hipLaunchKernelGGL((k_ts5_finish<DIM, PHASE>), dim3(grid_for(max_items, 256)), dim3(256), 0, st, P, O, init,
+                       seg_start, pre, n_seg, flags);
+    return hipGetLastError();
+}
+
+template <int DIM>
+static hipError_t launch_ts5_finish_d(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                                      const int64_t* ss, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                      const uint8_t* flags) {
+    switch (P.phase) {
+        case ETRAJ_PHASE_CTMC:
+            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_CTMC>(st, P, O, init, ss, pre, n_seg, max_items, flags);
+        case ETRAJ_PHASE_QTMC:
+            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_QTMC>(st, P, O, init, ss, pre, n_seg, max_items, flags);
+        default: return launch_ts5_finish_p<DIM, ETRAJ_PHASE_SCTS>(st, P, O, init, ss, pre, n_seg, max_items, flags);
+    }
+}
+
+hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
+                             const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                             const uint8_t* flags) {
+    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
+    return P.dim == 3 ? launch_ts5_finish_d<3>(st, P, O, init, seg_start, pre, n_seg, max_items, flags)
+                      : launch_ts5_finish_d<2>(st, P, O, init, seg_start, pre, n_seg, max_items, flags);
+}
 
 hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, int64_t n, double* Fx, double* Fy,
                               double* Ax, double* Ay) {

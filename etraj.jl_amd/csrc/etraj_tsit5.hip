@@ -142,17 +142,6 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
     }
 }
 
-// The segment holding item (0-based over all electrons of all segments):
-// pre[s] <= item < pre[s + 1] (the last such s where segments are empty).
-__device__ inline int64_t item_segment(int64_t item, const int64_t* pre, int64_t n_seg) {
-    int64_t lo = 0, hi = n_seg;
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= item) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // P.ts5_claim: electrons a wave claims from the global queue at a time.
 // P.ts5_refill: idle lanes (finished or empty) a wave collects before it writes
 // their results and refills them: one pass of the divergent start-up (column
@@ -177,11 +166,17 @@ __device__ inline int64_t item_segment(int64_t item, const int64_t* pre, int64_t
 // = 1 (every other column of the launch gets 0), for the FP64 launch over the
 // compacted flagged columns. orig (may be null): the output index of a
 // compacted column (records / contributions of that FP64 launch).
+// OUT_STATE: an electron starts from its column and the step size and first
+// stage derivative k_ts5_prep put in prep, and its final state goes back into
+// its column (state = the init buffer) for k_ts5_finish: the start-up and
+// finish code that runs with only the refilled lanes active is a few loads and
+// stores, so the wave refills often and keeps its lanes busy.
 template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool QUEUE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
     const int64_t* __restrict__ pre, int64_t n_seg, unsigned long long* __restrict__ queue,
-    uint8_t* __restrict__ flags, const int64_t* __restrict__ orig) {
+    uint8_t* __restrict__ flags, const int64_t* __restrict__ orig, double* __restrict__ state,
+    const double* __restrict__ prep) {
     const int lane = lane_id();
     const int64_t total = pre[n_seg];
     const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -213,7 +208,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (idle == ~0ull || (QUEUE && __popcll(idle) >= refill)) {
             if (pending) {
                 pending = false;
-                emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob, acc);
+                if constexpr (OUT == OUT_STATE) {
+                    double* c = state + col * P.rows;
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        c[i] = st.r[i];
+                        c[DIM + i] = st.p[i];
+                    }
+                    if (failed) c[0] = __builtin_nan("");
+                    if (PHASE != ETRAJ_PHASE_CTMC) c[2 * DIM + 2] = st.ph;
+                } else {
+                    emit_lane<DIM, PHASE, OUT>(P, O, orig ? orig[col] : col, failed, st.r, st.p, st.ph, t0, prob,
+                                               acc);
+                }
             }
             if (QUEUE) {
                 uint64_t want = idle;
@@ -261,7 +268,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             prob = c[2 * DIM + 1];
             st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
             close = false;
-            ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+            if constexpr (OUT == OUT_STATE) {
+                const double* q = prep + col * ts5_prep_doubles(DIM);
+                st.t = t0;
+                st.dt = q[0];
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) st.k1p[i] = q[1 + i];
+                st.k1ph = q[1 + DIM];
+                st.lqold = log(1e-4);   // as ts5_init
+                st.q11 = 1.0;
+                st.accepted = 0;
+                st.attempted = 0;
+            } else {
+                ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, st);
+            }
         }
         if (valid) {
             const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters,
@@ -284,7 +304,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
     }
-    if (OUT == OUT_PMD) flush_lane_acc(O, acc, !MIXED);
+    if (OUT == OUT_PMD || OUT == OUT_STATE) flush_lane_acc(O, acc, !MIXED);
+}
+
+// The start values of the adaptive step (ts5_init) of every item, one per
+// thread, into prep (ts5_prep_doubles per column): the first step size and the
+// first stage derivative.
+template <int DIM, int PHASE, int TGT>
+__global__ __launch_bounds__(256) void k_ts5_prep(IntegP P, const double* __restrict__ init,
+                                                  const int64_t* __restrict__ seg_start,
+                                                  const int64_t* __restrict__ pre, int64_t n_seg,
+                                                  double* __restrict__ prep) {
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= pre[n_seg]) return;
+    const int64_t sg = item_segment(item, pre, n_seg);
+    const int64_t col = seg_start[sg] + (item - pre[sg]);
+    const double* c = init + col * P.rows;
+    Ts5Lane<DIM, PHASE> st;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        st.r[i] = c[i];
+        st.p[i] = c[DIM + i];
+    }
+    st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
+    const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
+    ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, c[2 * DIM], P.tf, P.rtol, P.atol, st);
+    double* q = prep + col * ts5_prep_doubles(DIM);
+    q[0] = st.dt;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) q[1 + i] = st.k1p[i];
+    q[1 + DIM] = st.k1ph;
 }
 
 __global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }
@@ -297,6 +346,8 @@ struct Ts5Launch {
     unsigned long long* queue;
     uint8_t* flags;
     const int64_t* orig;
+    double* state;        // OUT_STATE: the init buffer
+    const double* prep;   // OUT_STATE: k_ts5_prep's start values
 };
 
 // Blocks of the resident grid of one k_tsit5 instance: the blocks the chip
@@ -330,7 +381,7 @@ static hipError_t launch_tsit5_k(hipStream_t st, const IntegP& P, const OutP& O,
     if (e != hipSuccess || blocks == 0) return e;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, st, a.queue);
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, st, P, O, a.init, a.seg_start, a.pre, a.n_seg, a.queue,
-                       a.flags, a.orig);
+                       a.flags, a.orig, a.state, a.prep);
     return hipGetLastError();
 }
 
@@ -352,6 +403,7 @@ static hipError_t launch_tsit5_o(hipStream_t st, const IntegP& P, const OutP& O,
     switch (out) {
         case OUT_PMD: return launch_tsit5_t<DIM, PHASE, OUT_PMD, MIXED>(st, P, O, a);
         case OUT_RECORDS: return launch_tsit5_t<DIM, PHASE, OUT_RECORDS, MIXED>(st, P, O, a);
+        case OUT_STATE: return launch_tsit5_t<DIM, PHASE, OUT_STATE, MIXED>(st, P, O, a);
         default: return launch_tsit5_t<DIM, PHASE, OUT_CONTRIB, MIXED>(st, P, O, a);
     }
 }
@@ -369,13 +421,57 @@ hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
                         unsigned long long* queue, uint8_t* flags, const int64_t* orig) {
     if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    if (!queue) return hipErrorInvalidValue;
-    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, orig};
+    if (!queue || out == OUT_STATE) return hipErrorInvalidValue;
+    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, orig, nullptr, nullptr};
     if (P.mixed) {
         if (!flags) return hipErrorInvalidValue;
         return P.dim == 3 ? launch_tsit5_p<3, true>(st, P, O, out, a) : launch_tsit5_p<2, true>(st, P, O, out, a);
     }
     return P.dim == 3 ? launch_tsit5_p<3, false>(st, P, O, out, a) : launch_tsit5_p<2, false>(st, P, O, out, a);
+}
+
+template <int DIM, int PHASE>
+static hipError_t launch_ts5_prep_p(hipStream_t st, const IntegP& P, const double* init, const int64_t* seg_start,
+                                    const int64_t* pre, int64_t n_seg, int64_t max_items, double* prep) {
+    const dim3 grid((unsigned)((max_items + 255) / 256));
+    if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
+        hipLaunchKernelGGL((k_ts5_prep<DIM, PHASE, ETRAJ_TARGET_HYDROGEN_LIKE>), grid, dim3(256), 0, st, P, init,
+                           seg_start, pre, n_seg, prep);
+    else
+        hipLaunchKernelGGL((k_ts5_prep<DIM, PHASE, ETRAJ_TARGET_SAE>), grid, dim3(256), 0, st, P, init, seg_start, pre,
+                           n_seg, prep);
+    return hipGetLastError();
+}
+
+template <int DIM>
+static hipError_t launch_ts5_prep(hipStream_t st, const IntegP& P, const double* init, const int64_t* seg_start,
+                                  const int64_t* pre, int64_t n_seg, int64_t max_items, double* prep) {
+    switch (P.phase) {
+        case ETRAJ_PHASE_CTMC:
+            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_CTMC>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+        case ETRAJ_PHASE_QTMC:
+            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_QTMC>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+        default: return launch_ts5_prep_p<DIM, ETRAJ_PHASE_SCTS>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+    }
+}
+
+hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
+                               const int64_t* pre, int64_t n_seg, int64_t max_items, unsigned long long* queue,
+                               double* prep, uint8_t* flags, hipEvent_t main_done) {
+    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
+    if (!queue || !prep || (P.mixed && !flags)) return hipErrorInvalidValue;
+    hipError_t e = P.dim == 3 ? launch_ts5_prep<3>(st, P, init, seg_start, pre, n_seg, max_items, prep)
+                              : launch_ts5_prep<2>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+    if (e != hipSuccess) return e;
+    const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, nullptr, init, prep};
+    if (P.mixed)
+        e = P.dim == 3 ? launch_tsit5_p<3, true>(st, P, O, OUT_STATE, a) : launch_tsit5_p<2, true>(st, P, O, OUT_STATE, a);
+    else
+        e = P.dim == 3 ? launch_tsit5_p<3, false>(st, P, O, OUT_STATE, a)
+                       : launch_tsit5_p<2, false>(st, P, O, OUT_STATE, a);
+    if (e == hipSuccess && main_done) e = hipEventRecord(main_done, st);
+    if (e != hipSuccess) return e;
+    return launch_ts5_finish(st, P, O, init, seg_start, pre, n_seg, max_items, P.mixed ? flags : nullptr);
 }
 
 }  // namespace etraj

@@ -280,6 +280,23 @@ def traffic_per_launch():
         return None
 
 
+PROBE = os.path.join(ROOT, "etraj.jl_amd", "etraj_hip", "valu_probe")
+
+
+def valu_probe():
+    """The vector-ALU rates this device reaches with non-packed FP32 and FP64 instructions (valu_probe:
+    16 independent FMA / add / rsq chains per lane, vector operands, 8 waves per SIMD), or None if the probe
+    executable is missing. The published FP32 peak counts packed FP32 FMAs, which this build never emits."""
+    import subprocess
+    if not os.path.exists(PROBE):
+        return None
+    r = subprocess.run([PROBE], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log(f"valu_probe failed: {r.stderr.strip()}")
+        return None
+    return json.loads(r.stdout)
+
+
 def copy_calibration(mib):
     """One hipMemcpy device-to-device of `mib` MiB: a copy kernel of known traffic (mib MiB read +
     mib MiB written) whose FETCH_SIZE / WRITE_SIZE counters scale the integrator's."""
@@ -493,19 +510,30 @@ def main():
                     help="launch choices set before the run (etraj_tuning fields, e.g. rk4_block=128,bin_tiles=1)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the engine's autotuner on the workload first and time the chosen launch set")
+    ap.add_argument("--no-valu-probe", action="store_true",
+                    help="skip the vector-ALU throughput probe (the non-packed FP32 / FP64 ceiling of the roofline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trajectory path shards by "
               "t0 strata with one PMD reduction, which this build does not run across GPUs.", file=sys.stderr)
         sys.exit(2)
 
+    probe = None if args.no_valu_probe else valu_probe()
     ctx = _abi.default_context()
     if args.tuning:
         ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
     if args.calibrate_copy > 0:
         copy_calibration(args.calibrate_copy)
     if args.workload != "h-scts-1e8":
-        print(json.dumps(bench_workload(args.workload, args, ctx)), flush=True)
+        res = bench_workload(args.workload, args, ctx)
+        if probe:
+            res["valu_probe"] = probe
+            rf = res.get("roofline")
+            if rf:
+                key = "fma_f64" if rf["peak"] == PEAK_TFLOPS["fp64"] else "fma_f32"
+                rf[f"probe_{key}_tflops"] = probe[key]["tflops"]
+                rf[f"frac_of_probe_{key}"] = rf["achieved"] / probe[key]["tflops"]
+        print(json.dumps(res), flush=True)
         return
     kw = workloads.h_lp_scts_stratified(n_t=args.n_t or 10000, n_k=args.n_k or 34400, precision="mixed")
     if args.restep_radius is not None:
@@ -570,6 +598,11 @@ def main():
                      "rk4_steps_per_job": steps // args.steps, "fp64_steps_per_job": steps64 // args.steps,
                      "both_passes_tflops": flops_all / (np.mean(ms_int) / 1e3) / 1e12},
     }
+    if probe:
+        result["valu_probe"] = probe
+        ceil = probe["fma_f32"]["tflops"]
+        result["roofline"]["probe_fma_f32_tflops"] = ceil
+        result["roofline"]["frac_of_probe_fma_f32"] = achieved / ceil
     if not args.no_fp64:
         d64 = descs(dict(kw, precision="fp64"))
         p64, s64 = run_host(ctx, d64, npix)
@@ -592,6 +625,10 @@ def main():
             f"phase RMS mod 2pi {a['phase_rms_mod_2pi']:.2e}")
     if not args.no_tsit5:
         result["tsit5_entry"] = tsit5_entry(kw, ctx, max(1, args.steps // 2), 1)
+        if probe:
+            rf = result["tsit5_entry"]["roofline"]
+            rf["probe_fma_f64_tflops"] = probe["fma_f64"]["tflops"]
+            rf["frac_of_probe_fma_f64"] = rf["achieved"] / probe["fma_f64"]["tflops"]
         log(f"tsit5 entry: {result['tsit5_entry']['ms_per_step']:.1f} ms per job")
     if args.cpu_batches > 0:
         cpu = host_cpu()

@@ -173,8 +173,10 @@ __global__ __launch_bounds__(256) void k_tile_accum(const uint32_t* __restrict__
         }
         if (B.hits) atomicAdd(&hits[local], 1u);
     }
-    if (FIXED && nr) atomicAdd((unsigned long long*)(B.cnt + (int64_t)(first / B.npix) * CNT_N + CNT_FIXRANGE),
-                               (unsigned long long)nr);
+    if (FIXED && nr)
+        atomicAdd((unsigned long long*)(B.cnt + ((int64_t)(first / B.npix) * kAccSlots + acc_slot()) * CNT_N +
+                                        CNT_FIXRANGE),
+                  (unsigned long long)nr);
     __syncthreads();
     for (int i = threadIdx.x; i < T; i += kBinThreads) {
         const int64_t g = (int64_t)first + i;

@@ -87,11 +87,14 @@ struct etraj_ctx {
     // tile binning (ETRAJ_BIN_TILES): records, their tile-sorted copy and the tile tables
     DevBuf<uint32_t> bin_cell, bin_scell, tile_count, tile_off, chunk_off, cursor;
     DevBuf<double2> bin_amp, bin_samp;
-    bool bin_auto_tiles = false;     // ETRAJ_BIN_AUTO takes the tiles
+    // what ETRAJ_BIN_AUTO means: 1 tiles, 0 atomics, -1 by integrator (tiles for Tsit5, whose binning
+    // runs in its own short kernel, k_ts5_finish; atomics for RK4, whose binning spreads over the
+    // integrator kernel and costs nothing there)
+    int bin_auto_tiles = -1;
     int bin_shift = 0;               // tile of 2^bin_shift cells (0: the largest that fits the LDS)
     int bin_chunk = 16384;           // records per accumulating workgroup
     int rk4_block = 256;             // RK4 threads per workgroup
-    int ts5_refill = 16;             // Tsit5: idle lanes before a wavefront refills
+    int ts5_refill = 4;              // Tsit5: idle lanes before a wavefront refills
     int ts5_claim = 256;             // Tsit5: electrons per queue claim
     SlabSlot slot[kSlabSlots];
     std::string err;
@@ -137,7 +140,7 @@ int hip_fail(etraj_ctx* c, hipError_t e, const char* what) {
 
 etraj_tuning tuning_of(const etraj_ctx* c) {
     etraj_tuning t{};
-    t.bin_tiles = c->bin_auto_tiles ? 1 : 0;
+    t.bin_tiles = c->bin_auto_tiles;
     t.bin_tile_shift = c->bin_shift;
     t.bin_chunk = c->bin_chunk;
     t.overlap_fp64 = c->overlap_fp64 ? 1 : 0;
@@ -148,7 +151,7 @@ etraj_tuning tuning_of(const etraj_ctx* c) {
 }
 
 const char* tuning_invalid(const etraj_tuning& t) {
-    if (t.bin_tiles != 0 && t.bin_tiles != 1) return "tuning.bin_tiles must be 0 or 1";
+    if (t.bin_tiles < -1 || t.bin_tiles > 1) return "tuning.bin_tiles must be -1, 0 or 1";
     if (t.bin_tile_shift != 0 && (t.bin_tile_shift < 6 || t.bin_tile_shift > 16))
         return "tuning.bin_tile_shift must be 0 or 6..16";
     if (t.bin_chunk < 256 || t.bin_chunk > (1 << 20)) return "tuning.bin_chunk must be 256..1048576";
@@ -161,7 +164,7 @@ const char* tuning_invalid(const etraj_tuning& t) {
 }
 
 void apply_tuning(etraj_ctx* c, const etraj_tuning& t) {
-    c->bin_auto_tiles = t.bin_tiles != 0;
+    c->bin_auto_tiles = t.bin_tiles;
     c->bin_shift = t.bin_tile_shift;
     c->bin_chunk = t.bin_chunk;
     c->overlap_fp64 = t.overlap_fp64 != 0;
@@ -1124,7 +1127,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         return fail(c, ETRAJ_EINVAL, "pmd_binning must be ETRAJ_BIN_AUTO, ETRAJ_BIN_ATOMIC or ETRAJ_BIN_TILES");
     BinCfg bcfg{0, c->bin_chunk};
     int ntile = 0;
-    bool tiles = run->pmd_binning == ETRAJ_BIN_TILES || (run->pmd_binning == ETRAJ_BIN_AUTO && c->bin_auto_tiles);
+    const bool auto_tiles = c->bin_auto_tiles < 0 ? run->integrator == ETRAJ_INTEG_TSIT5 : c->bin_auto_tiles == 1;
+    bool tiles = run->pmd_binning == ETRAJ_BIN_TILES || (run->pmd_binning == ETRAJ_BIN_AUTO && auto_tiles);
     if (tiles) {
         const int lds_shift = bin_tile_shift(words, fixed_sums, run->hit_counts != nullptr, 30);
         bcfg.shift = c->bin_shift > 0 ? std::min(c->bin_shift, lds_shift) : lds_shift;
@@ -1159,11 +1163,11 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     }
     const bool fixed = run->pmd_mode == ETRAJ_PMD_FIXED;
     HIP_TRY(c, c->pmd.ensure((size_t)n_jobs * plen));
-    HIP_TRY(c, c->scal.ensure((size_t)2 * n_jobs));
-    HIP_TRY(c, c->cnt.ensure((size_t)CNT_N * n_jobs));
+    HIP_TRY(c, c->scal.ensure((size_t)2 * kAccSlots * n_jobs));
+    HIP_TRY(c, c->cnt.ensure((size_t)CNT_N * kAccSlots * n_jobs));
     if (fixed) {
         HIP_TRY(c, c->pmdq.ensure((size_t)2 * n_jobs * plen));
-        HIP_TRY(c, c->scalq.ensure((size_t)4 * n_jobs));
+        HIP_TRY(c, c->scalq.ensure((size_t)4 * kAccSlots * n_jobs));
     }
     if (run->hit_counts) HIP_TRY(c, c->hits.ensure((size_t)n_jobs * npix));
     HIP_TRY(c, c->sps.ensure(n_jobs));
@@ -1194,11 +1198,11 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->lps.p, hlp.data(), sizeof(LaserP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * n_jobs * plen, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2 * n_jobs, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N * n_jobs, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2 * kAccSlots * n_jobs, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N * kAccSlots * n_jobs, c->stream));
     if (fixed) {
         HIP_TRY(c, hipMemsetAsync(c->pmdq.p, 0, sizeof(uint64_t) * 2 * n_jobs * plen, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->scalq.p, 0, sizeof(uint64_t) * 4 * n_jobs, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->scalq.p, 0, sizeof(uint64_t) * 4 * kAccSlots * n_jobs, c->stream));
     }
     if (run->hit_counts) HIP_TRY(c, hipMemsetAsync(c->hits.p, 0, sizeof(uint64_t) * n_jobs * npix, c->stream));
 
@@ -1307,11 +1311,11 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             Pj.L = jobs[j].L;
             OutP Oj = O;
             Oj.pmd = O.pmd + j * plen;
-            Oj.scal = O.scal + 2 * j;
-            Oj.cnt = O.cnt + (int64_t)j * CNT_N;
+            Oj.scal = O.scal + 2 * kAccSlots * j;
+            Oj.cnt = O.cnt + (int64_t)j * CNT_N * kAccSlots;
             if (fixed) {
                 Oj.pmdq = O.pmdq + 2 * j * plen;
-                Oj.scalq = O.scalq + 4 * j;
+                Oj.scalq = O.scalq + 4 * kAccSlots * j;
             }
             if (O.hits) Oj.hits = O.hits + j * npix;
             OutP Om = Oj;   // the main launch: records for the tiles, or atomics
@@ -1322,10 +1326,11 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             }
             // start values of every electron, the integration, then the Kepler asymptote and binning of the
             // final states (launch_tsit5_staged); ev[6k+2] marks the end of the integrator kernel
+            const int cps = (int)((sl.maxcap + 63) / 64);
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
-                e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
-                                        Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 2]);
+                e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.counts.p, Q.pre.p, nb, cps, sl.cols,
+                                        Q.queue.p, Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 2]);
             if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
             if (P.mixed) {   // the flagged electrons again in FP64, on the same stream (binned with atomics)
                 IntegP P64 = Pj;
@@ -1335,8 +1340,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                                                Q.counts2.p, nullptr);
                 if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
                 if (e == hipSuccess)
-                    e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.pre.p, nb, sl.cols, Q.queue.p,
-                                            Q.prep.p, nullptr, nullptr);
+                    e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.counts2.p, Q.pre.p, nb, cps,
+                                            sl.cols, Q.queue.p, Q.prep.p, nullptr, nullptr);
             }
         }
         if (!two_pass) {
@@ -1372,13 +1377,15 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         HIP_TRY(c, launch_finalize_abs2(c->stream, c->pmd.p, (int64_t)n_jobs * npix, c->pmd_out.p));
         result = c->pmd_out.p;
     }
-    std::vector<double> scal(2 * n_jobs);
-    std::vector<int64_t> cnt((size_t)CNT_N * n_jobs);
-    HIP_TRY(c, hipMemcpyAsync(scal.data(), c->scal.p, sizeof(double) * 2 * n_jobs, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(cnt.data(), c->cnt.p, sizeof(int64_t) * CNT_N * n_jobs, hipMemcpyDeviceToHost, c->stream));
-    std::vector<uint64_t> scalq(fixed ? 4 * n_jobs : 0), hits(run->hit_counts ? n_jobs * npix : 0);
+    // per-slot copies (kAccSlots per job), added on the host below
+    std::vector<double> scal_s((size_t)2 * kAccSlots * n_jobs), scal(2 * n_jobs, 0.0);
+    std::vector<int64_t> cnt_s((size_t)CNT_N * kAccSlots * n_jobs), cnt((size_t)CNT_N * n_jobs, 0);
+    HIP_TRY(c, hipMemcpyAsync(scal_s.data(), c->scal.p, sizeof(double) * scal_s.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipMemcpyAsync(cnt_s.data(), c->cnt.p, sizeof(int64_t) * cnt_s.size(), hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint64_t> scalq_s(fixed ? (size_t)4 * kAccSlots * n_jobs : 0), hits(run->hit_counts ? n_jobs * npix : 0);
     if (fixed)
-        HIP_TRY(c, hipMemcpyAsync(scalq.data(), c->scalq.p, sizeof(uint64_t) * 4 * n_jobs, hipMemcpyDeviceToHost,
+        HIP_TRY(c, hipMemcpyAsync(scalq_s.data(), c->scalq.p, sizeof(uint64_t) * scalq_s.size(), hipMemcpyDeviceToHost,
                                   c->stream));
     if (run->hit_counts)
         HIP_TRY(c, hipMemcpyAsync(hits.data(), c->hits.p, sizeof(uint64_t) * n_jobs * npix, hipMemcpyDeviceToHost,
@@ -1386,12 +1393,28 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     if (host_out)
         HIP_TRY(c, hipMemcpyAsync(host_out, result, sizeof(double) * npix * n_jobs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int32_t j = 0; j < n_jobs; ++j)
+        for (int q = 0; q < kAccSlots; ++q) {
+            const size_t s0 = (size_t)j * kAccSlots + q;
+            scal[2 * j] += scal_s[2 * s0];
+            scal[2 * j + 1] += scal_s[2 * s0 + 1];
+            for (int k = 0; k < CNT_N; ++k) cnt[(size_t)CNT_N * j + k] += cnt_s[CNT_N * s0 + k];
+        }
     if (fixed) {
         // every partial sum of a cell is bounded by the job's sum of |values|: the
         // weights' sum (CTMC) or, for amplitudes sqrt(W), sqrt(n * sum W)
         for (int32_t j = 0; j < n_jobs; ++j) {
-            scal[2 * j] = fix_to_double(scalq[4 * j], scalq[4 * j + 1]);
-            scal[2 * j + 1] = fix_to_double(scalq[4 * j + 2], scalq[4 * j + 3]);
+            uint64_t q4[4] = {0, 0, 0, 0};   // the 128-bit sums of the slots (exact)
+            for (int q = 0; q < kAccSlots; ++q) {
+                const uint64_t* w = scalq_s.data() + 4 * ((size_t)j * kAccSlots + q);
+                for (int h = 0; h < 4; h += 2) {
+                    const uint64_t lo = q4[h] + w[h];
+                    q4[h + 1] += w[h + 1] + (lo < q4[h] ? 1 : 0);
+                    q4[h] = lo;
+                }
+            }
+            scal[2 * j] = fix_to_double(q4[0], q4[1]);
+            scal[2 * j + 1] = fix_to_double(q4[2], q4[3]);
             const int64_t* cj = cnt.data() + (size_t)CNT_N * j;
             const double bound = coh ? std::sqrt((double)cj[CNT_LAUNCHED] * scal[2 * j]) : scal[2 * j];
             if (cj[CNT_FIXRANGE] != 0 || !(bound < 0x1.0p26)) {
@@ -1536,7 +1559,7 @@ int etraj_autotune(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_targ
     }
     if (run->pmd_binning == ETRAJ_BIN_AUTO) {
         try_field(&etraj_tuning::bin_tiles, {0, 1});
-        if (top.bin_tiles) {
+        if (top.bin_tiles == 1 || (top.bin_tiles < 0 && !rk4)) {
             try_field(&etraj_tuning::bin_tile_shift, {0, 10});
             try_field(&etraj_tuning::bin_chunk, {4096, 16384, 65536});
         }

@@ -9,6 +9,8 @@
 namespace etraj {
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+// this workgroup's copy of the per-job scalar sums and counters (kAccSlots)
+__device__ inline int acc_slot() { return (int)(blockIdx.x & (kAccSlots - 1)); }
 
 __device__ inline double readlane_f64(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);

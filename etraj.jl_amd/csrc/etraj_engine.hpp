@@ -91,6 +91,13 @@ enum {
     CNT_FIXRANGE = 6, CNT_ATTEMPTS = 7, CNT_N = 8
 };
 
+// A job's scalar sums (scal, scalq) and counters (cnt) are spread over
+// kAccSlots copies, one per workgroup index mod kAccSlots: a launch's
+// per-wavefront atomic additions then go to kAccSlots addresses instead of
+// one (same-address atomics serialise at the memory side). The host adds the
+// copies. Layout per job: scal [slot][2], scalq [slot][4], cnt [slot][CNT_N].
+constexpr int kAccSlots = 32;
+
 // Order-independent sums (ETRAJ_PMD_FIXED): a value v is held as the 128-bit
 // two's-complement integer round(v * 2^kFixFrac) in two 64-bit words (lo,
 // hi). Integer addition is associative, so the sum of any set of values is
@@ -179,10 +186,10 @@ struct FieldTab {
 // cell0 = job * npix.
 struct OutP {
     double* pmd;       // CTMC: npix; coherent: 2*npix (re, im) per job
-    double* scal;      // [ion_prob, ion_prob_uncollected] per job
-    int64_t* cnt;      // CNT_* per job
+    double* scal;      // [ion_prob, ion_prob_uncollected] per job and slot (kAccSlots)
+    int64_t* cnt;      // CNT_* per job and slot (kAccSlots)
     uint64_t* pmdq;    // fixed-point PMD, 2 words per value
-    uint64_t* scalq;   // fixed-point [ion_prob, ion_prob_uncollected], 4 words per job
+    uint64_t* scalq;   // fixed-point [ion_prob, ion_prob_uncollected], 4 words per job and slot
     uint64_t* hits;    // electrons binned per cell
     int64_t npix;
     double* records;   // 12 per electron
@@ -278,18 +285,21 @@ hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out,
 // Doubles per column of the Tsit5 start values (ts5_prep_doubles): the first
 // step size and the first stage derivative (p part, phase part).
 constexpr int ts5_prep_doubles(int dim) { return dim + 2; }
-// The Tsit5 run of run_job in three launches over the items [0, pre[n_seg]) of
-// the segments: k_ts5_prep (the start values of every electron into prep, in
-// parallel), k_tsit5 with OUT_STATE (final states into the init columns,
+// The Tsit5 run of run_job in three launches over the columns
+// [seg_start[s], seg_start[s] + seg_count[s]) of the segments (pre: the
+// exclusive prefix sums of seg_count, pre[n_seg] <= max_items; cps >=
+// ceil(max seg_count / 64)): k_ts5_prep (the start values of every electron
+// into prep), k_tsit5 with OUT_STATE (final states into the init columns,
 // which it writes; P.mixed: flags as launch_tsit5), k_ts5_finish (Kepler
-// asymptote and binning of every item's final state into O, skipping flagged
-// columns when flags is not null). max_items >= pre[n_seg].
+// asymptote and binning of every final state into O, skipping flagged columns
+// when flags is not null). main_done (may be null) is recorded after k_tsit5.
 hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
-                               const int64_t* pre, int64_t n_seg, int64_t max_items, unsigned long long* queue,
-                               double* prep, uint8_t* flags, hipEvent_t main_done);
+                               const int32_t* seg_count, const int64_t* pre, int64_t n_seg, int cps,
+                               int64_t max_items, unsigned long long* queue, double* prep, uint8_t* flags,
+                               hipEvent_t main_done);
 // k_ts5_finish alone (etraj_kernels.hip): the binning of the final states
 hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                             const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                             const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
                              const uint8_t* flags);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);

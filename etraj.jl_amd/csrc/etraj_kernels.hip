@@ -123,8 +123,8 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
             const Fix128 iu = fix_wave_sum(fix_from_double((active && f.ionized && !inb) ? prob : 0.0, ok4));
             nr = wave_sum_i64((ok1 && ok2 && ok3 && ok4) ? 0 : 1);
             if (lane_id() == 0) {
-                fix_atomic_add(O.scalq + 0, ip);
-                fix_atomic_add(O.scalq + 2, iu);
+                fix_atomic_add(O.scalq + 4 * acc_slot() + 0, ip);
+                fix_atomic_add(O.scalq + 4 * acc_slot() + 2, iu);
             }
         } else {
             if (inb && !rec) {
@@ -138,17 +138,18 @@ __device__ inline void emit(const IntegP& P, const OutP& O, int64_t col_idx, boo
             double ip = wave_sum_f64((active && f.ionized) ? prob : 0.0);
             double iu = wave_sum_f64((active && f.ionized && !inb) ? prob : 0.0);
             if (lane_id() == 0) {
-                if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
-                if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+                if (ip != 0.0) unsafeAtomicAdd(O.scal + 2 * acc_slot() + 0, ip);
+                if (iu != 0.0) unsafeAtomicAdd(O.scal + 2 * acc_slot() + 1, iu);
             }
         }
         if (O.hits && inb && !rec) atomicAdd((unsigned long long*)(O.hits + lin), 1ull);
         int64_t nf = wave_sum_i64((active && failed) ? 1 : 0);
         int64_t nb = wave_sum_i64(bound ? 1 : 0);
         if (lane_id() == 0) {
-            if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
-            if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
-            if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
+            int64_t* cs = O.cnt + CNT_N * acc_slot();
+            if (nf) atomicAdd((unsigned long long*)(cs + CNT_FAILED), (unsigned long long)nf);
+            if (nb) atomicAdd((unsigned long long*)(cs + CNT_BOUND), (unsigned long long)nb);
+            if (nr) atomicAdd((unsigned long long*)(cs + CNT_FIXRANGE), (unsigned long long)nr);
         }
     } else if (OUT == OUT_RECORDS) {
         if (active) {
@@ -394,11 +395,11 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
     if (O.seg_job) {
         const int job = O.seg_job[seg];
         Oj.pmd = O.pmd + job * O.job_stride;
-        Oj.scal = O.scal + 2 * job;
-        Oj.cnt = O.cnt + (int64_t)job * CNT_N;
+        Oj.scal = O.scal + 2 * kAccSlots * job;
+        Oj.cnt = O.cnt + (int64_t)job * CNT_N * kAccSlots;
         if (O.pmdq) {
             Oj.pmdq = O.pmdq + 2 * job * O.job_stride;
-            Oj.scalq = O.scalq + 4 * job;
+            Oj.scalq = O.scalq + 4 * kAccSlots * job;
         }
         if (O.hits) Oj.hits = O.hits + job * O.npix;
         Oj.cell0 = job * O.npix;
@@ -452,10 +453,11 @@ __device__ inline void rk4_chunk(const IntegP& P, const OutP& O, const double* _
         const int64_t nact = wave_sum_i64(active ? 1 : 0);
         const int64_t nclose = wave_sum_i64((active && close) ? 1 : 0);
         if (lane == 0) {
-            atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS), (unsigned long long)(n * nact));
-            atomicAdd((unsigned long long*)(Oj.cnt + CNT_ATTEMPTS), (unsigned long long)(n * nact));
-            if (!MIXED) atomicAdd((unsigned long long*)(Oj.cnt + CNT_STEPS64), (unsigned long long)(n * nact));
-            if (nclose) atomicAdd((unsigned long long*)(Oj.cnt + CNT_RESTEP), (unsigned long long)nclose);
+            int64_t* cs = Oj.cnt + CNT_N * acc_slot();
+            atomicAdd((unsigned long long*)(cs + CNT_STEPS), (unsigned long long)(n * nact));
+            atomicAdd((unsigned long long*)(cs + CNT_ATTEMPTS), (unsigned long long)(n * nact));
+            if (!MIXED) atomicAdd((unsigned long long*)(cs + CNT_STEPS64), (unsigned long long)(n * nact));
+            if (nclose) atomicAdd((unsigned long long*)(cs + CNT_RESTEP), (unsigned long long)nclose);
         }
     }
 }
@@ -476,25 +478,25 @@ __global__ __launch_bounds__(256) void k_rk4(IntegP P, OutP O, const double* __r
 }
 
 // The Kepler asymptote and binning of the final states that k_tsit5 (OUT_STATE)
-// left in the init columns (x = NaN: the integration failed): one item per
-// lane, 64 consecutive items per wavefront, every lane of a wavefront taking
-// part in emit's wave sums. flags (may be null): columns with flags[col] != 0
-// were stopped for the FP64 re-integration and are not binned here.
+// left in the init columns (x = NaN: the integration failed). Wavefront w takes
+// chunk w % cps (64 columns) of segment w / cps, as k_rk4; every lane of a
+// wavefront takes part in emit's wave sums. flags (may be null): columns with
+// flags[col] != 0 were stopped for the FP64 re-integration and are not binned.
 template <int DIM, int PHASE>
 __global__ __launch_bounds__(256) void k_ts5_finish(IntegP P, OutP O, const double* __restrict__ init,
                                                     const int64_t* __restrict__ seg_start,
-                                                    const int64_t* __restrict__ pre, int64_t n_seg,
+                                                    const int32_t* __restrict__ seg_count, int64_t n_seg, int cps,
                                                     const uint8_t* __restrict__ flags) {
-    const int64_t total = pre[n_seg];
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (uniform_i64(item - lane_id()) >= total) return;   // the whole wavefront is past the last item
-    bool active = item < total;
-    int64_t col = 0;
-    if (active) {
-        const int64_t sg = item_segment(item, pre, n_seg);
-        col = seg_start[sg] + (item - pre[sg]);
-        if (flags && flags[col]) active = false;
-    }
+    const int64_t gw = uniform_i64((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int64_t seg = gw / cps;
+    if (seg >= n_seg) return;
+    const int first = (int)(gw - seg * cps) * 64;
+    const int32_t cnt = seg_count[seg];
+    if (first >= cnt) return;   // an empty chunk (wave-uniform)
+    const int lane = lane_id();
+    bool active = first + lane < cnt;
+    const int64_t col = seg_start[seg] + first + (active ? lane : 0);
+    if (active && flags && flags[col]) active = false;
     Lane<DIM, PHASE> e;
     load_column<DIM, PHASE>(init + col * P.rows, e);
     bool bad = false;
@@ -595,32 +597,32 @@ static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs -
 
 template <int DIM, int PHASE>
 static hipError_t launch_ts5_finish_p(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                                      const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                      const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
                                       const uint8_t* flags) {
-    hipLaunchKernelGGL((k_ts5_finish<DIM, PHASE>), dim3(grid_for(max_items, 256)), dim3(256), 0, st, P, O, init,
-                       seg_start, pre, n_seg, flags);
+    hipLaunchKernelGGL((k_ts5_finish<DIM, PHASE>), dim3(grid_for(n_seg * (int64_t)cps, 4)), dim3(256), 0, st, P, O,
+                       init, seg_start, seg_count, n_seg, cps, flags);
     return hipGetLastError();
 }
 
 template <int DIM>
 static hipError_t launch_ts5_finish_d(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                                      const int64_t* ss, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                                      const int64_t* ss, const int32_t* sc, int64_t n_seg, int cps,
                                       const uint8_t* flags) {
     switch (P.phase) {
         case ETRAJ_PHASE_CTMC:
-            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_CTMC>(st, P, O, init, ss, pre, n_seg, max_items, flags);
+            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_CTMC>(st, P, O, init, ss, sc, n_seg, cps, flags);
         case ETRAJ_PHASE_QTMC:
-            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_QTMC>(st, P, O, init, ss, pre, n_seg, max_items, flags);
-        default: return launch_ts5_finish_p<DIM, ETRAJ_PHASE_SCTS>(st, P, O, init, ss, pre, n_seg, max_items, flags);
+            return launch_ts5_finish_p<DIM, ETRAJ_PHASE_QTMC>(st, P, O, init, ss, sc, n_seg, cps, flags);
+        default: return launch_ts5_finish_p<DIM, ETRAJ_PHASE_SCTS>(st, P, O, init, ss, sc, n_seg, cps, flags);
     }
 }
 
 hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
-                             const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
+                             const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,
                              const uint8_t* flags) {
-    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
-    return P.dim == 3 ? launch_ts5_finish_d<3>(st, P, O, init, seg_start, pre, n_seg, max_items, flags)
-                      : launch_ts5_finish_d<2>(st, P, O, init, seg_start, pre, n_seg, max_items, flags);
+    if (n_seg <= 0 || cps <= 0) return hipSuccess;
+    return P.dim == 3 ? launch_ts5_finish_d<3>(st, P, O, init, seg_start, seg_count, n_seg, cps, flags)
+                      : launch_ts5_finish_d<2>(st, P, O, init, seg_start, seg_count, n_seg, cps, flags);
 }
 
 hipError_t launch_laser_field(hipStream_t st, const LaserP& L, const double* t, int64_t n, double* Fx, double* Fy,

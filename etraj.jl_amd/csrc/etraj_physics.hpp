@@ -222,8 +222,10 @@ __host__ __device__ inline void sincos_small(double x, double& s, double& c) {
     c = fma(pc, x2, 1.0);
 }
 
-__host__ __device__ inline double half_tanh_step(double x) {  // 1/2 + tanh(x)/2
-    return x < -19.1 ? 0.0 : 1.0 / (1.0 + exp(-2.0 * x));
+// 1/2 + tanh(x)/2 as 1/(1 + exp(-2x)); beyond |x| = 19.1 the result is exactly 0 or 1 in double
+// (exp(-38.2) < 2^-54), so the exp is skipped there.
+__host__ __device__ inline double half_tanh_step(double x) {
+    return x < -19.1 ? 0.0 : (x > 19.1 ? 1.0 : 1.0 / (1.0 + exp(-2.0 * x)));
 }
 
 __host__ __device__ inline MonoBase mono_base(const MonoP& m, double t) {
@@ -240,6 +242,12 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
     const double t = b.tp + d;
     const double x = m.w * d;
     const bool gauss = m.kind == ETRAJ_LASER_GAUSSIAN;
+    const double win = gauss ? 1.0 : half_tanh_step(m.k * (t + m.halfw)) * half_tanh_step(-m.k * (t - m.halfw));
+    if (win == 0.0) {   // outside the pulse's window the field is exactly zero: no trigonometry
+        Fx = 0.0;
+        Fy = 0.0;
+        return;
+    }
     const double xh = gauss ? 0.0 : x / (2 * m.N);
     double sw, cw, sh = 0.0, ch = 1.0;
     if (fabs(x) <= 0.5 && fabs(xh) <= 0.5) {
@@ -263,7 +271,7 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
         bb = t / m.sigma2 * sw - m.w * cw;
     } else {
         const double env = (m.kind == ETRAJ_LASER_COS2) ? ch : ch * ch * ch;
-        g = m.F0 * env * (half_tanh_step(m.k * (t + m.halfw)) * half_tanh_step(-m.k * (t - m.halfw)));
+        g = m.F0 * env * win;
         a = ch * sw + m.coefN * sh * cw;
         bb = -ch * cw + m.coefN * sh * sw;
     }

@@ -658,8 +658,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     if (tid == 0) {
         counts[g] = base_sh;
-        if (cnt && base_sh) atomicAdd((unsigned long long*)(cnt + (int64_t)job * CNT_N + CNT_LAUNCHED),
-                                      (unsigned long long)base_sh);
+        if (cnt && base_sh)
+            atomicAdd((unsigned long long*)(cnt + ((int64_t)job * kAccSlots + acc_slot()) * CNT_N + CNT_LAUNCHED),
+                      (unsigned long long)base_sh);
     }
 }
 

@@ -121,24 +121,25 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
     if (O.pmdq) {
         const Fix128 ip = fix_wave_sum(a.ipq), iu = fix_wave_sum(a.iuq);
         if (lane_id() == 0) {
-            fix_atomic_add(O.scalq + 0, ip);
-            fix_atomic_add(O.scalq + 2, iu);
+            fix_atomic_add(O.scalq + 4 * acc_slot() + 0, ip);
+            fix_atomic_add(O.scalq + 4 * acc_slot() + 2, iu);
         }
     } else {
         const double ip = wave_sum_f64(a.ip), iu = wave_sum_f64(a.iu);
         if (lane_id() == 0) {
-            if (ip != 0.0) unsafeAtomicAdd(O.scal + 0, ip);
-            if (iu != 0.0) unsafeAtomicAdd(O.scal + 1, iu);
+            if (ip != 0.0) unsafeAtomicAdd(O.scal + 2 * acc_slot() + 0, ip);
+            if (iu != 0.0) unsafeAtomicAdd(O.scal + 2 * acc_slot() + 1, iu);
         }
     }
     if (lane_id() == 0) {
-        if (nf) atomicAdd((unsigned long long*)(O.cnt + CNT_FAILED), (unsigned long long)nf);
-        if (nb) atomicAdd((unsigned long long*)(O.cnt + CNT_BOUND), (unsigned long long)nb);
-        if (ns) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS), (unsigned long long)ns);
-        if (ns && steps64) atomicAdd((unsigned long long*)(O.cnt + CNT_STEPS64), (unsigned long long)ns);
-        if (nr) atomicAdd((unsigned long long*)(O.cnt + CNT_FIXRANGE), (unsigned long long)nr);
-        if (nq) atomicAdd((unsigned long long*)(O.cnt + CNT_RESTEP), (unsigned long long)nq);
-        if (na) atomicAdd((unsigned long long*)(O.cnt + CNT_ATTEMPTS), (unsigned long long)na);
+        int64_t* cs = O.cnt + CNT_N * acc_slot();
+        if (nf) atomicAdd((unsigned long long*)(cs + CNT_FAILED), (unsigned long long)nf);
+        if (nb) atomicAdd((unsigned long long*)(cs + CNT_BOUND), (unsigned long long)nb);
+        if (ns) atomicAdd((unsigned long long*)(cs + CNT_STEPS), (unsigned long long)ns);
+        if (ns && steps64) atomicAdd((unsigned long long*)(cs + CNT_STEPS64), (unsigned long long)ns);
+        if (nr) atomicAdd((unsigned long long*)(cs + CNT_FIXRANGE), (unsigned long long)nr);
+        if (nq) atomicAdd((unsigned long long*)(cs + CNT_RESTEP), (unsigned long long)nq);
+        if (na) atomicAdd((unsigned long long*)(cs + CNT_ATTEMPTS), (unsigned long long)na);
     }
 }
 
@@ -307,18 +308,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (OUT == OUT_PMD || OUT == OUT_STATE) flush_lane_acc(O, acc, !MIXED);
 }
 
-// The start values of the adaptive step (ts5_init) of every item, one per
-// thread, into prep (ts5_prep_doubles per column): the first step size and the
-// first stage derivative.
+// The start values of the adaptive step (ts5_init) of every electron into prep
+// (ts5_prep_doubles per column): the first step size and the first stage
+// derivative. Wavefront w takes chunk w % cps (64 columns) of segment w / cps.
 template <int DIM, int PHASE, int TGT>
 __global__ __launch_bounds__(256) void k_ts5_prep(IntegP P, const double* __restrict__ init,
                                                   const int64_t* __restrict__ seg_start,
-                                                  const int64_t* __restrict__ pre, int64_t n_seg,
+                                                  const int32_t* __restrict__ seg_count, int64_t n_seg, int cps,
                                                   double* __restrict__ prep) {
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= pre[n_seg]) return;
-    const int64_t sg = item_segment(item, pre, n_seg);
-    const int64_t col = seg_start[sg] + (item - pre[sg]);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t seg = gw / cps;
+    if (seg >= n_seg) return;
+    const int k = (int)(gw - seg * cps) * 64 + lane_id();
+    if (k >= seg_count[seg]) return;
+    const int64_t col = seg_start[seg] + k;
     const double* c = init + col * P.rows;
     Ts5Lane<DIM, PHASE> st;
 #pragma unroll
@@ -432,36 +435,37 @@ hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out,
 
 template <int DIM, int PHASE>
 static hipError_t launch_ts5_prep_p(hipStream_t st, const IntegP& P, const double* init, const int64_t* seg_start,
-                                    const int64_t* pre, int64_t n_seg, int64_t max_items, double* prep) {
-    const dim3 grid((unsigned)((max_items + 255) / 256));
+                                    const int32_t* seg_count, int64_t n_seg, int cps, double* prep) {
+    const dim3 grid((unsigned)((n_seg * (int64_t)cps + 3) / 4));
     if (P.T.kind == ETRAJ_TARGET_HYDROGEN_LIKE)
         hipLaunchKernelGGL((k_ts5_prep<DIM, PHASE, ETRAJ_TARGET_HYDROGEN_LIKE>), grid, dim3(256), 0, st, P, init,
-                           seg_start, pre, n_seg, prep);
+                           seg_start, seg_count, n_seg, cps, prep);
     else
-        hipLaunchKernelGGL((k_ts5_prep<DIM, PHASE, ETRAJ_TARGET_SAE>), grid, dim3(256), 0, st, P, init, seg_start, pre,
-                           n_seg, prep);
+        hipLaunchKernelGGL((k_ts5_prep<DIM, PHASE, ETRAJ_TARGET_SAE>), grid, dim3(256), 0, st, P, init, seg_start,
+                           seg_count, n_seg, cps, prep);
     return hipGetLastError();
 }
 
 template <int DIM>
 static hipError_t launch_ts5_prep(hipStream_t st, const IntegP& P, const double* init, const int64_t* seg_start,
-                                  const int64_t* pre, int64_t n_seg, int64_t max_items, double* prep) {
+                                  const int32_t* seg_count, int64_t n_seg, int cps, double* prep) {
     switch (P.phase) {
         case ETRAJ_PHASE_CTMC:
-            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_CTMC>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_CTMC>(st, P, init, seg_start, seg_count, n_seg, cps, prep);
         case ETRAJ_PHASE_QTMC:
-            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_QTMC>(st, P, init, seg_start, pre, n_seg, max_items, prep);
-        default: return launch_ts5_prep_p<DIM, ETRAJ_PHASE_SCTS>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+            return launch_ts5_prep_p<DIM, ETRAJ_PHASE_QTMC>(st, P, init, seg_start, seg_count, n_seg, cps, prep);
+        default: return launch_ts5_prep_p<DIM, ETRAJ_PHASE_SCTS>(st, P, init, seg_start, seg_count, n_seg, cps, prep);
     }
 }
 
 hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
-                               const int64_t* pre, int64_t n_seg, int64_t max_items, unsigned long long* queue,
-                               double* prep, uint8_t* flags, hipEvent_t main_done) {
-    if (n_seg <= 0 || max_items <= 0) return hipSuccess;
+                               const int32_t* seg_count, const int64_t* pre, int64_t n_seg, int cps,
+                               int64_t max_items, unsigned long long* queue, double* prep, uint8_t* flags,
+                               hipEvent_t main_done) {
+    if (n_seg <= 0 || max_items <= 0 || cps <= 0) return hipSuccess;
     if (!queue || !prep || (P.mixed && !flags)) return hipErrorInvalidValue;
-    hipError_t e = P.dim == 3 ? launch_ts5_prep<3>(st, P, init, seg_start, pre, n_seg, max_items, prep)
-                              : launch_ts5_prep<2>(st, P, init, seg_start, pre, n_seg, max_items, prep);
+    hipError_t e = P.dim == 3 ? launch_ts5_prep<3>(st, P, init, seg_start, seg_count, n_seg, cps, prep)
+                              : launch_ts5_prep<2>(st, P, init, seg_start, seg_count, n_seg, cps, prep);
     if (e != hipSuccess) return e;
     const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, nullptr, init, prep};
     if (P.mixed)
@@ -471,7 +475,7 @@ hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, d
                        : launch_tsit5_p<2, false>(st, P, O, OUT_STATE, a);
     if (e == hipSuccess && main_done) e = hipEventRecord(main_done, st);
     if (e != hipSuccess) return e;
-    return launch_ts5_finish(st, P, O, init, seg_start, pre, n_seg, max_items, P.mixed ? flags : nullptr);
+    return launch_ts5_finish(st, P, O, init, seg_start, seg_count, n_seg, cps, P.mixed ? flags : nullptr);
 }
 
 }  // namespace etraj

@@ -198,8 +198,8 @@ enum {
 };
 /* How the device binning of etraj_run_job / etraj_run_scan adds the electrons to the PMD */
 enum {
-    ETRAJ_BIN_AUTO = 0,     /* the context's choice (ETRAJ_BIN_ATOMIC unless etraj_autotune
-                               measured the tiles faster for the workload)                 */
+    ETRAJ_BIN_AUTO = 0,     /* the context's choice (etraj_tuning.bin_tiles): by default the
+                               tiles for Tsit5 and the atomics for RK4                      */
     ETRAJ_BIN_ATOMIC = 1,   /* one global atomic addition per electron and value, from the
                                integrator kernel                                            */
     ETRAJ_BIN_TILES = 2     /* the integrator writes (cell, contribution) records; they are
@@ -355,7 +355,8 @@ int etraj_run_scan_device(etraj_ctx* ctx, int32_t n_jobs, const etraj_laser_desc
  * choices of a ctx. Each choice changes the speed only; the results are the same
  * up to the order of FP64 sums (bit for bit with ETRAJ_PMD_FIXED). */
 typedef struct etraj_tuning {
-    int32_t bin_tiles;       /* what ETRAJ_BIN_AUTO means: 0 atomics, 1 LDS tiles              */
+    int32_t bin_tiles;       /* what ETRAJ_BIN_AUTO means: 0 atomics, 1 LDS tiles, -1 (the
+                                default) tiles for Tsit5 and atomics for RK4                  */
     int32_t bin_tile_shift;  /* tiles of 2^shift PMD cells, 6..16; 0: the largest that fits the
                                 64 KiB LDS accumulator                                        */
     int32_t bin_chunk;       /* records per accumulating workgroup, 256..1048576               */

@@ -1217,7 +1217,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     O.hits = run->hit_counts ? c->hits.p : nullptr;
     const BinOut bout{O.pmd, O.pmdq, O.hits, O.cnt, npix};
     // per slab, first stream: start, sampled, main integrator kernel done, first-stream work done;
-    // second stream: FP64 pass start and end
+    // RK4 two-pass: FP64 pass start and end (second stream); Tsit5: [4] main integrator kernel start
     std::vector<hipEvent_t> ev(6 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
     // The FP64 pass of a slab runs on the second stream (c->overlap_fp64, the
@@ -1330,7 +1330,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
             if (e == hipSuccess)
                 e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.counts.p, Q.pre.p, nb, cps, sl.cols,
-                                        Q.queue.p, Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 2]);
+                                        Q.queue.p, Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 4],
+                                        ev[6 * k + 2]);
             if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
             if (P.mixed) {   // the flagged electrons again in FP64, on the same stream (binned with atomics)
                 IntegP P64 = Pj;
@@ -1341,7 +1342,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
                 if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
                 if (e == hipSuccess)
                     e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.counts2.p, Q.pre.p, nb, cps,
-                                            sl.cols, Q.queue.p, Q.prep.p, nullptr, nullptr);
+                                            sl.cols, Q.queue.p, Q.prep.p, nullptr, nullptr, nullptr);
             }
         }
         if (!two_pass) {
@@ -1433,7 +1434,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         float a = 0, b = 0, m = 0, f = 0;
         (void)hipEventElapsedTime(&a, ev[6 * k], ev[6 * k + 1]);
         (void)hipEventElapsedTime(&b, ev[6 * k + 1], ev[6 * k + 3]);
-        (void)hipEventElapsedTime(&m, ev[6 * k + 1], ev[6 * k + 2]);
+        // the main integrator kernel: RK4 right after sampling; Tsit5 after its start values
+        (void)hipEventElapsedTime(&m, ev[6 * k + (rk4 ? 1 : 4)], ev[6 * k + 2]);
         if (two_pass) (void)hipEventElapsedTime(&f, ev[6 * k + 4], ev[6 * k + 5]);
         ms_s += a;
         ms_i += b + f;

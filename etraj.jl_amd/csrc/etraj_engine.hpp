@@ -292,11 +292,12 @@ constexpr int ts5_prep_doubles(int dim) { return dim + 2; }
 // into prep), k_tsit5 with OUT_STATE (final states into the init columns,
 // which it writes; P.mixed: flags as launch_tsit5), k_ts5_finish (Kepler
 // asymptote and binning of every final state into O, skipping flagged columns
-// when flags is not null). main_done (may be null) is recorded after k_tsit5.
+// when flags is not null). main_start / main_done (may be null) are recorded
+// right before and after k_tsit5.
 hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
                                const int32_t* seg_count, const int64_t* pre, int64_t n_seg, int cps,
                                int64_t max_items, unsigned long long* queue, double* prep, uint8_t* flags,
-                               hipEvent_t main_done);
+                               hipEvent_t main_start, hipEvent_t main_done);
 // k_ts5_finish alone (etraj_kernels.hip): the binning of the final states
 hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, const double* init,
                              const int64_t* seg_start, const int32_t* seg_count, int64_t n_seg, int cps,

@@ -461,11 +461,12 @@ static hipError_t launch_ts5_prep(hipStream_t st, const IntegP& P, const double*
 hipError_t launch_tsit5_staged(hipStream_t st, const IntegP& P, const OutP& O, double* init, const int64_t* seg_start,
                                const int32_t* seg_count, const int64_t* pre, int64_t n_seg, int cps,
                                int64_t max_items, unsigned long long* queue, double* prep, uint8_t* flags,
-                               hipEvent_t main_done) {
+                               hipEvent_t main_start, hipEvent_t main_done) {
     if (n_seg <= 0 || max_items <= 0 || cps <= 0) return hipSuccess;
     if (!queue || !prep || (P.mixed && !flags)) return hipErrorInvalidValue;
     hipError_t e = P.dim == 3 ? launch_ts5_prep<3>(st, P, init, seg_start, seg_count, n_seg, cps, prep)
                               : launch_ts5_prep<2>(st, P, init, seg_start, seg_count, n_seg, cps, prep);
+    if (e == hipSuccess && main_start) e = hipEventRecord(main_start, st);
     if (e != hipSuccess) return e;
     const Ts5Launch a{init, seg_start, pre, n_seg, max_items, queue, flags, nullptr, init, prep};
     if (P.mixed)

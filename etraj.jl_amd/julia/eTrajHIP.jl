@@ -59,6 +59,11 @@ struct RunDesc
     final_p_max::NTuple{3,Float64}; final_p_num::NTuple{3,Int32}; quirks::Int32; maxiters::Int64
     pmd_mode::Int32; pmd_binning::Int32; hit_counts::Ptr{UInt64}
 end
+mutable struct TuningDesc   # etraj_tuning: the launch choices of a ctx
+    bin_tiles::Int32; bin_tile_shift::Int32; bin_chunk::Int32; overlap_fp64::Int32
+    rk4_block::Int32; tsit5_refill::Int32; tsit5_claim::Int32; reserved::Int32
+    TuningDesc() = new(-1, 0, 16384, 1, 256, 4, 256, 0)
+end
 mutable struct Stats
     ion_prob::Float64; ion_prob_uncollected::Float64
     num_effective_traj::Int64; n_failed::Int64; n_bound::Int64; n_restep::Int64; n_steps::Int64; n_steps_fp64::Int64
@@ -225,6 +230,27 @@ function run_job_hip(; laser, target, dimension, sample_t_intv, sample_t_num, tr
             num_effective_traj = st.num_effective_traj)
 end
 
+# ---- launch tuning: etraj_autotune times the ctx's launch choices on a job ----------
+# (the keywords of run_job_hip); the fastest set stays in the ctx `c` for its later calls.
+function autotune_hip(c::Ctx; laser, target, dimension, sample_t_intv, sample_t_num, traj_t_final, final_p_max,
+                      final_p_num, traj_phase_method=:CTMC, dt=0.1, precision=:mixed, restep_radius=2.0,
+                      integrator=:rk4, traj_rtol=1e-6, pmd_mode=:float, pmd_binning=:auto, kwargs...)
+    sdv, keep = sampler_desc(; dimension, sample_t_intv, sample_t_num, traj_phase_method, target, kwargs...)
+    sd = Ref(sdv)
+    rd = Ref(run_desc(dimension, traj_phase_method, traj_t_final, final_p_max, final_p_num; dt, precision,
+                      restep_radius, integrator, rtol=traj_rtol, pmd_mode, pmd_binning))
+    best = TuningDesc()
+    best_ms, start_ms = Ref{Float64}(0), Ref{Float64}(0)
+    GC.@preserve keep best begin
+        rc = ccall((:etraj_autotune, libetraj), Cint,
+                   (Ptr{Cvoid}, Ref{LaserDesc}, Ref{TargetDesc}, Ref{SamplerDesc}, Ref{RunDesc}, Ref{TuningDesc},
+                    Ref{Float64}, Ref{Float64}),
+                   c.h, Ref(laser_desc(laser)), Ref(target_desc(target)), sd, rd, best, best_ms, start_ms)
+        check(c, rc)
+    end
+    return (best = best, best_ms = best_ms[], start_ms = start_ms[])
+end
+
 # ---- parameter scans: the reference's loops of perform_traj_simulation calls --------
 # (examples/test_Bichromatic_CCP.jl:4-27) as one etraj_run_scan; `lasers` and the
 # sampler keyword sets differ per point, target and run settings are shared.
@@ -323,6 +349,6 @@ function write_output_hip!(job, res)
     return job.output_path
 end
 
-export launch_and_collect_hip!, run_job_hip, run_scan_hip, perform_traj_simulation_hip
+export launch_and_collect_hip!, run_job_hip, run_scan_hip, perform_traj_simulation_hip, autotune_hip
 
 end # module

@@ -358,6 +358,8 @@ def _timed(ctx, fn, steps, warmup):
 def _roofline_rk4(st_list, steps, flops32, flops64):
     """Roofline of the FP32 pass from summed per-call stats (RK4 mixed runs)."""
     s0 = st_list[0]
+    if not s0.ms_integrate_main > 0:   # graph replay: no per-kernel timing
+        return None
     n_steps = sum(s.n_steps for s in st_list) / steps
     n64 = sum(s.n_steps_fp64 for s in st_list) / steps
     launch_ms = s0.ms_integrate_main / max(s0.n_slabs, 1)

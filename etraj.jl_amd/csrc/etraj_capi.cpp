@@ -96,6 +96,11 @@ struct etraj_ctx {
     int rk4_block = 256;             // RK4 threads per workgroup
     int ts5_refill = 4;              // Tsit5: idle lanes before a wavefront refills
     int ts5_claim = 256;             // Tsit5: electrons per queue claim
+    // run_job of a job whose batches fit one slab: its launches captured once into a HIP graph
+    // (gexec), replayed while the launch arguments (gkey) repeat
+    bool graph = false;
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
     SlabSlot slot[kSlabSlots];
     std::string err;
     DevBuf<double> init, pmd, pmd_out, io1, io2, io3, io4, scal, t0s, seg_t0, wfat, prec;
@@ -147,6 +152,7 @@ etraj_tuning tuning_of(const etraj_ctx* c) {
     t.rk4_block = c->rk4_block;
     t.tsit5_refill = c->ts5_refill;
     t.tsit5_claim = c->ts5_claim;
+    t.graph = c->graph ? 1 : 0;
     return t;
 }
 
@@ -159,7 +165,7 @@ const char* tuning_invalid(const etraj_tuning& t) {
     if (t.rk4_block != 64 && t.rk4_block != 128 && t.rk4_block != 256) return "tuning.rk4_block must be 64, 128 or 256";
     if (t.tsit5_refill < 1 || t.tsit5_refill > 64) return "tuning.tsit5_refill must be 1..64";
     if (t.tsit5_claim < 64 || t.tsit5_claim > 65536) return "tuning.tsit5_claim must be 64..65536";
-    if (t.reserved != 0) return "tuning.reserved must be 0";
+    if (t.graph != 0 && t.graph != 1) return "tuning.graph must be 0 or 1";
     return nullptr;
 }
 
@@ -171,6 +177,7 @@ void apply_tuning(etraj_ctx* c, const etraj_tuning& t) {
     c->rk4_block = t.rk4_block;
     c->ts5_refill = t.tsit5_refill;
     c->ts5_claim = t.tsit5_claim;
+    c->graph = t.graph != 0;
 }
 
 int rows_for(int dim, int phase) {
@@ -719,6 +726,7 @@ void etraj_ctx_destroy(etraj_ctx* c) {
     c->bin_cell.release(); c->bin_scell.release(); c->tile_count.release(); c->tile_off.release();
     c->chunk_off.release(); c->cursor.release(); c->bin_amp.release(); c->bin_samp.release();
     for (SlabSlot& Q : c->slot) Q.release();
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1071,8 +1079,8 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     if (!c) return ETRAJ_EINVAL;
     c->err.clear();
     auto wall0 = std::chrono::steady_clock::now();
-    TargetP T;
-    IntegP P;
+    TargetP T{};   // zero-filled, padding included: their bytes key the graph replay
+    IntegP P{};
     if (int rc = check_target(c, target, T)) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     std::vector<JobPlan> jobs;
@@ -1197,63 +1205,25 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     }
     HIP_TRY(c, hipMemcpyAsync(c->sps.p, hsp.data(), sizeof(SamplerP) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->lps.p, hlp.data(), sizeof(LaserP) * n_jobs, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->pmd.p, 0, sizeof(double) * n_jobs * plen, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->scal.p, 0, sizeof(double) * 2 * kAccSlots * n_jobs, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->cnt.p, 0, sizeof(int64_t) * CNT_N * kAccSlots * n_jobs, c->stream));
-    if (fixed) {
-        HIP_TRY(c, hipMemsetAsync(c->pmdq.p, 0, sizeof(uint64_t) * 2 * n_jobs * plen, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->scalq.p, 0, sizeof(uint64_t) * 4 * kAccSlots * n_jobs, c->stream));
-    }
-    if (run->hit_counts) HIP_TRY(c, hipMemsetAsync(c->hits.p, 0, sizeof(uint64_t) * n_jobs * npix, c->stream));
+    if (coh) HIP_TRY(c, c->pmd_out.ensure((size_t)n_jobs * npix));
+    const double* result = coh ? c->pmd_out.p : c->pmd.p;
 
-    OutP O{};
-    O.pmd = c->pmd.p;
-    O.scal = c->scal.p;
-    O.cnt = c->cnt.p;
-    O.job_stride = plen;
-    O.npix = npix;
-    O.pmdq = fixed ? c->pmdq.p : nullptr;
-    O.scalq = fixed ? c->scalq.p : nullptr;
-    O.hits = run->hit_counts ? c->hits.p : nullptr;
-    const BinOut bout{O.pmd, O.pmdq, O.hits, O.cnt, npix};
+    // A job of one slab with ctx->graph set runs as a HIP graph: its launches are captured once and
+    // replayed while their arguments repeat (the key below: the device buffers, the parameter
+    // blocks and the segment table). The host-to-device copies (parameter blocks, segment table)
+    // stay outside the graph, made before each launch; the graph holds kernels only (the clearing
+    // included) and runs on the first stream alone (no FP64-pass overlap); the per-stage event
+    // timings give way to the graph's total.
+    const bool use_graph = c->graph && nslab == 1;
     // per slab, first stream: start, sampled, main integrator kernel done, first-stream work done;
     // RK4 two-pass: FP64 pass start and end (second stream); Tsit5: [4] main integrator kernel start
     std::vector<hipEvent_t> ev(6 * nslab, nullptr);
     for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
-    // The FP64 pass of a slab runs on the second stream (c->overlap_fp64, the
-    // default), put there once the next slab has been sampled so that it
-    // overlaps the next slab's FP32 pass rather than that slab's short
-    // sampling kernels, which it would starve of dispatch slots; or on the
-    // first stream right after the slab's FP32 pass and compaction. On the
-    // 1e8-electron job the overlap saves about 2 % (A/B on one box).
-    struct Fp64Pass { int k = -1, nb = 0, cps = 0; SlabSlot* Q = nullptr; FieldTab FT{}; };
-    Fp64Pass prev;
-    const bool overlap = c->overlap_fp64;
-    auto launch_fp64 = [&](const Fp64Pass& f, hipEvent_t after) -> hipError_t {
-        hipStream_t B = overlap ? c->stream2 : c->stream;
-        hipError_t e = hipStreamWaitEvent(B, after, 0);
-        if (e == hipSuccess) e = hipEventRecord(ev[6 * f.k + 4], B);
-        OutP Os = O;
-        Os.seg_job = f.Q->seg_job.p;
-        IntegP P64 = P;
-        P64.mixed = 0;
-        if (e == hipSuccess)
-            e = launch_rk4(B, P64, Os, OUT_PMD, f.Q->init.p, f.Q->seg_start.p, f.Q->counts2.p, f.nb, f.cps, f.FT,
-                           nullptr, nullptr);
-        if (e == hipSuccess) e = hipEventRecord(ev[6 * f.k + 5], B);
-        if (e == hipSuccess) e = hipEventRecord(f.Q->done, B);
-        return e;
-    };
-    int rc = ETRAJ_OK;
-    for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
+    auto rec = [&](int i, hipStream_t s) -> hipError_t { return use_graph ? hipSuccess : hipEventRecord(ev[i], s); };
+    // the segment table of slab k (host staging in its slot, then copied to the slot's device buffers)
+    auto stage_slab = [&](int k, SlabSlot& Q) -> hipError_t {
         const Slab& sl = slabs[k];
         const int nb = (int)(sl.i1 - sl.i0);
-        SlabSlot& Q = c->slot[k % kSlabSlots];
-        hipError_t e = hipSuccess;
-        if (Q.pending) {   // the slot's previous slab (device buffers and host staging) must be finished
-            e = hipEventSynchronize(Q.done);
-            Q.pending = false;
-        }
         Q.h_t0.resize(nb); Q.h_job.resize(nb); Q.h_b.resize(nb); Q.h_start.resize(nb);
         int64_t col = 0;
         for (int i = 0; i < nb; ++i) {
@@ -1265,103 +1235,235 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
             col += it.cap;
         }
         hipStream_t A = c->stream;
-        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_t0.p, Q.h_t0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, A);
+        hipError_t e = hipMemcpyAsync(Q.seg_t0.p, Q.h_t0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, A);
         if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_job.p, Q.h_job.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, A);
         if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_b.p, Q.h_b.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, A);
-        if (e == hipSuccess) e = hipMemcpyAsync(Q.seg_start.p, Q.h_start.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, A);
-        if (e == hipSuccess) e = hipEventRecord(ev[6 * k], A);
-        FieldTab FT{Q.ft32.p, Q.ft64.p, Q.ft0.p, stride};
-        if (e == hipSuccess && rk4) e = launch_field_table(A, c->lps.p, Q.seg_job.p, Q.seg_t0.p, nb, P.tf, P.dt, FT);
-        const SegTab G{Q.seg_t0.p, Q.seg_job.p, Q.seg_b.p, Q.seg_start.p};
         if (e == hipSuccess)
-            e = launch_adk_sample(A, jobs[0].S.method, jobs[0].S.pre_L >= 0, c->sps.p, c->lps.p, T, G, nb, Q.init.p, Q.counts.p, c->cnt.p);
-        if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 1], A);
-        if (e == hipSuccess && prev.k >= 0) {
-            e = launch_fp64(prev, ev[6 * k + 1]);
-            prev.k = -1;
+            e = hipMemcpyAsync(Q.seg_start.p, Q.h_start.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, A);
+        return e;
+    };
+
+    const BinOut bout{c->pmd.p, fixed ? c->pmdq.p : nullptr, run->hit_counts ? c->hits.p : nullptr, c->cnt.p, npix};
+    // every launch of the job, on the first stream (and the second for overlapped FP64 passes)
+    auto enqueue = [&]() -> int {
+        hipStream_t A = c->stream;
+        HIP_TRY(c, launch_zero(A, c->pmd.p, sizeof(double) * n_jobs * plen));
+        HIP_TRY(c, launch_zero(A, c->scal.p, sizeof(double) * 2 * kAccSlots * n_jobs));
+        HIP_TRY(c, launch_zero(A, c->cnt.p, sizeof(int64_t) * CNT_N * kAccSlots * n_jobs));
+        if (fixed) {
+            HIP_TRY(c, launch_zero(A, c->pmdq.p, sizeof(uint64_t) * 2 * n_jobs * plen));
+            HIP_TRY(c, launch_zero(A, c->scalq.p, sizeof(uint64_t) * 4 * kAccSlots * n_jobs));
         }
-        if (tiles && e == hipSuccess) e = launch_bin_reset(A, bbuf.cell, sl.cols, bbuf.tile_count, ntile);
-        if (rk4) {
+        if (run->hit_counts) HIP_TRY(c, launch_zero(A, c->hits.p, sizeof(uint64_t) * n_jobs * npix));
+        OutP O{};
+        O.pmd = c->pmd.p;
+        O.scal = c->scal.p;
+        O.cnt = c->cnt.p;
+        O.job_stride = plen;
+        O.npix = npix;
+        O.pmdq = fixed ? c->pmdq.p : nullptr;
+        O.scalq = fixed ? c->scalq.p : nullptr;
+        O.hits = run->hit_counts ? c->hits.p : nullptr;
+        // The FP64 pass of a slab runs on the second stream (c->overlap_fp64, the
+        // default), put there once the next slab has been sampled so that it
+        // overlaps the next slab's FP32 pass rather than that slab's short
+        // sampling kernels, which it would starve of dispatch slots; or on the
+        // first stream right after the slab's FP32 pass and compaction. On the
+        // 1e8-electron job the overlap saves about 2 % (A/B on one box).
+        struct Fp64Pass { int k = -1, nb = 0, cps = 0; SlabSlot* Q = nullptr; FieldTab FT{}; };
+        Fp64Pass prev;
+        const bool overlap = c->overlap_fp64 && !use_graph;
+        auto launch_fp64 = [&](const Fp64Pass& f, hipEvent_t after) -> hipError_t {
+            hipStream_t B = overlap ? c->stream2 : A;
+            hipError_t e = overlap ? hipStreamWaitEvent(B, after, 0) : hipSuccess;
+            if (e == hipSuccess) e = rec(6 * f.k + 4, B);
             OutP Os = O;
-            Os.seg_job = Q.seg_job.p;
-            if (tiles) {
-                Os.bin_cell = bbuf.cell;
-                Os.bin_amp = bbuf.amp;
-            }
-            const int cps = (int)((sl.maxcap + 63) / 64);
+            Os.seg_job = f.Q->seg_job.p;
+            IntegP P64 = P;
+            P64.mixed = 0;
             if (e == hipSuccess)
-                e = launch_rk4(A, P, Os, OUT_PMD, Q.init.p, Q.seg_start.p, Q.counts.p, nb, cps, FT,
-                               two_pass ? Q.flags.p : nullptr, nullptr);
-            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 2], A);
-            if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
-            if (two_pass) {
-                // the electrons flagged by the FP32 pass, integrated again in FP64: now, or (overlap)
-                // on the second stream after the next slab's sampling or after the loop
-                if (e == hipSuccess)
-                    e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p, nullptr,
-                                               Q.counts2.p, nullptr);
-                if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);
-                const Fp64Pass cur{k, nb, cps, &Q, FT};
-                if (overlap) prev = cur;
-                else if (e == hipSuccess) e = launch_fp64(cur, ev[6 * k + 3]);
+                e = launch_rk4(B, P64, Os, OUT_PMD, f.Q->init.p, f.Q->seg_start.p, f.Q->counts2.p, f.nb, f.cps, f.FT,
+                               nullptr, nullptr);
+            if (e == hipSuccess) e = rec(6 * f.k + 5, B);
+            if (e == hipSuccess && !use_graph) e = hipEventRecord(f.Q->done, B);
+            return e;
+        };
+        int rc = ETRAJ_OK;
+        for (int k = 0; k < nslab && rc == ETRAJ_OK; ++k) {
+            const Slab& sl = slabs[k];
+            const int nb = (int)(sl.i1 - sl.i0);
+            SlabSlot& Q = c->slot[k % kSlabSlots];
+            hipError_t e = hipSuccess;
+            if (Q.pending) {   // the slot's previous slab (device buffers and host staging) must be finished
+                e = hipEventSynchronize(Q.done);
+                Q.pending = false;
             }
-        } else {
-            const int32_t j = items[sl.i0].job;   // a Tsit5 slab holds one job
-            IntegP Pj = P;
-            Pj.L = jobs[j].L;
-            OutP Oj = O;
-            Oj.pmd = O.pmd + j * plen;
-            Oj.scal = O.scal + 2 * kAccSlots * j;
-            Oj.cnt = O.cnt + (int64_t)j * CNT_N * kAccSlots;
-            if (fixed) {
-                Oj.pmdq = O.pmdq + 2 * j * plen;
-                Oj.scalq = O.scalq + 4 * kAccSlots * j;
-            }
-            if (O.hits) Oj.hits = O.hits + j * npix;
-            OutP Om = Oj;   // the main launch: records for the tiles, or atomics
-            if (tiles) {
-                Om.bin_cell = bbuf.cell;
-                Om.bin_amp = bbuf.amp;
-                Om.cell0 = j * npix;
-            }
-            // start values of every electron, the integration, then the Kepler asymptote and binning of the
-            // final states (launch_tsit5_staged); ev[6k+2] marks the end of the integrator kernel
-            const int cps = (int)((sl.maxcap + 63) / 64);
-            if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
+            if (e == hipSuccess && !use_graph) e = stage_slab(k, Q);   // graph: staged before the launch
+            if (e == hipSuccess) e = rec(6 * k, A);
+            FieldTab FT{Q.ft32.p, Q.ft64.p, Q.ft0.p, stride};
+            if (e == hipSuccess && rk4) e = launch_field_table(A, c->lps.p, Q.seg_job.p, Q.seg_t0.p, nb, P.tf, P.dt, FT);
+            const SegTab G{Q.seg_t0.p, Q.seg_job.p, Q.seg_b.p, Q.seg_start.p};
             if (e == hipSuccess)
-                e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.counts.p, Q.pre.p, nb, cps, sl.cols,
-                                        Q.queue.p, Q.prep.p, P.mixed ? Q.flags.p : nullptr, ev[6 * k + 4],
-                                        ev[6 * k + 2]);
-            if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
-            if (P.mixed) {   // the flagged electrons again in FP64, on the same stream (binned with atomics)
-                IntegP P64 = Pj;
-                P64.mixed = 0;
+                e = launch_adk_sample(A, jobs[0].S.method, jobs[0].S.pre_L >= 0, c->sps.p, c->lps.p, T, G, nb, Q.init.p,
+                                      Q.counts.p, c->cnt.p);
+            if (e == hipSuccess) e = rec(6 * k + 1, A);
+            if (e == hipSuccess && prev.k >= 0) {
+                e = launch_fp64(prev, ev[6 * k + 1]);
+                prev.k = -1;
+            }
+            if (tiles && e == hipSuccess) e = launch_bin_reset(A, bbuf.cell, sl.cols, bbuf.tile_count, ntile);
+            if (rk4) {
+                OutP Os = O;
+                Os.seg_job = Q.seg_job.p;
+                if (tiles) {
+                    Os.bin_cell = bbuf.cell;
+                    Os.bin_amp = bbuf.amp;
+                }
+                const int cps = (int)((sl.maxcap + 63) / 64);
                 if (e == hipSuccess)
-                    e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p, nullptr,
-                                               Q.counts2.p, nullptr);
-                if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
+                    e = launch_rk4(A, P, Os, OUT_PMD, Q.init.p, Q.seg_start.p, Q.counts.p, nb, cps, FT,
+                                   two_pass ? Q.flags.p : nullptr, nullptr);
+                if (e == hipSuccess) e = rec(6 * k + 2, A);
+                if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
+                if (two_pass) {
+                    // the electrons flagged by the FP32 pass, integrated again in FP64: now, or (overlap)
+                    // on the second stream after the next slab's sampling or after the loop
+                    if (e == hipSuccess)
+                        e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p,
+                                                   nullptr, Q.counts2.p, nullptr);
+                    if (e == hipSuccess) e = rec(6 * k + 3, A);
+                    const Fp64Pass cur{k, nb, cps, &Q, FT};
+                    if (overlap) prev = cur;
+                    else if (e == hipSuccess) e = launch_fp64(cur, ev[6 * k + 3]);
+                }
+            } else {
+                const int32_t j = items[sl.i0].job;   // a Tsit5 slab holds one job
+                IntegP Pj = P;
+                Pj.L = jobs[j].L;
+                OutP Oj = O;
+                Oj.pmd = O.pmd + j * plen;
+                Oj.scal = O.scal + 2 * kAccSlots * j;
+                Oj.cnt = O.cnt + (int64_t)j * CNT_N * kAccSlots;
+                if (fixed) {
+                    Oj.pmdq = O.pmdq + 2 * j * plen;
+                    Oj.scalq = O.scalq + 4 * kAccSlots * j;
+                }
+                if (O.hits) Oj.hits = O.hits + j * npix;
+                OutP Om = Oj;   // the main launch: records for the tiles, or atomics
+                if (tiles) {
+                    Om.bin_cell = bbuf.cell;
+                    Om.bin_amp = bbuf.amp;
+                    Om.cell0 = j * npix;
+                }
+                // start values of every electron, the integration, then the Kepler asymptote and binning of the
+                // final states (launch_tsit5_staged); ev[6k+2] marks the end of the integrator kernel
+                const int cps = (int)((sl.maxcap + 63) / 64);
+                if (e == hipSuccess) e = launch_scan_counts(A, Q.counts.p, nb, Q.pre.p);
                 if (e == hipSuccess)
-                    e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.counts2.p, Q.pre.p, nb, cps,
-                                            sl.cols, Q.queue.p, Q.prep.p, nullptr, nullptr, nullptr);
+                    e = launch_tsit5_staged(A, Pj, Om, Q.init.p, Q.seg_start.p, Q.counts.p, Q.pre.p, nb, cps, sl.cols,
+                                            Q.queue.p, Q.prep.p, P.mixed ? Q.flags.p : nullptr,
+                                            use_graph ? nullptr : ev[6 * k + 4], use_graph ? nullptr : ev[6 * k + 2]);
+                if (tiles && e == hipSuccess) e = launch_bin_tiles(A, bcfg, bbuf, sl.cols, total_cells, words, fixed, bout);
+                if (P.mixed) {   // the flagged electrons again in FP64, on the same stream (binned with atomics)
+                    IntegP P64 = Pj;
+                    P64.mixed = 0;
+                    if (e == hipSuccess)
+                        e = launch_compact_flagged(A, Q.init.p, P.rows, Q.seg_start.p, Q.counts.p, nb, Q.flags.p,
+                                                   nullptr, Q.counts2.p, nullptr);
+                    if (e == hipSuccess) e = launch_scan_counts(A, Q.counts2.p, nb, Q.pre.p);
+                    if (e == hipSuccess)
+                        e = launch_tsit5_staged(A, P64, Oj, Q.init.p, Q.seg_start.p, Q.counts2.p, Q.pre.p, nb, cps,
+                                                sl.cols, Q.queue.p, Q.prep.p, nullptr, nullptr, nullptr);
+                }
+            }
+            if (!two_pass) {
+                if (e == hipSuccess) e = rec(6 * k + 3, A);
+                if (e == hipSuccess && !use_graph) e = hipEventRecord(Q.done, A);
+            }
+            Q.pending = !use_graph;
+            if (e != hipSuccess) rc = hip_fail(c, e, "run_job slab");
+        }
+        if (rc == ETRAJ_OK && prev.k >= 0) {
+            hipError_t e = launch_fp64(prev, ev[6 * prev.k + 3]);
+            if (e != hipSuccess) rc = hip_fail(c, e, "run_job FP64 pass");
+        }
+        // the first stream finishes the job after every FP64 pass
+        for (int q = 0; q < kSlabSlots; ++q)
+            if (c->slot[q].pending && rc == ETRAJ_OK) {
+                hipError_t e = hipStreamWaitEvent(A, c->slot[q].done, 0);
+                if (e != hipSuccess) rc = hip_fail(c, e, "run_job join");
+            }
+        if (rc) return rc;
+        if (fixed) HIP_TRY(c, launch_fix_to_f64(A, c->pmdq.p, (int64_t)n_jobs * plen, c->pmd.p));
+        if (coh) HIP_TRY(c, launch_finalize_abs2(A, c->pmd.p, (int64_t)n_jobs * npix, c->pmd_out.p));
+        return ETRAJ_OK;
+    };
+
+    int rc = ETRAJ_OK;
+    if (!use_graph) {
+        rc = enqueue();
+    } else {
+        hipError_t e = stage_slab(0, c->slot[0]);
+        // the graph's launches are fixed by these: a replay needs them all unchanged
+        std::string key;
+        auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+        auto put_v = [&put](auto v) { put(&v, sizeof v); };
+        put(&P, sizeof P);
+        put(&T, sizeof T);
+        put(hsp.data(), sizeof(SamplerP) * hsp.size());
+        put(hlp.data(), sizeof(LaserP) * hlp.size());
+        for (const Item& it : items) {
+            put_v(it.job); put_v(it.b); put_v(it.t0); put_v(it.cap);
+        }
+        put_v(n_jobs); put_v(npix); put_v(plen); put_v(stride); put_v(total_cells); put_v(ntile); put_v(bcfg.shift);
+        put_v(bcfg.chunk); put_v(words); put_v(fixed); put_v(coh); put_v(tiles); put_v(two_pass); put_v(rk4);
+        put_v(run->hit_counts != nullptr);
+        const SlabSlot& Q0 = c->slot[0];
+        for (const void* ptr : {(const void*)c->pmd.p, (const void*)c->pmd_out.p, (const void*)c->scal.p,
+                                (const void*)c->cnt.p, (const void*)c->pmdq.p, (const void*)c->scalq.p,
+                                (const void*)c->hits.p, (const void*)c->sps.p, (const void*)c->lps.p,
+                                (const void*)Q0.init.p, (const void*)Q0.counts.p, (const void*)Q0.counts2.p,
+                                (const void*)Q0.seg_start.p, (const void*)Q0.seg_t0.p, (const void*)Q0.seg_job.p,
+                                (const void*)Q0.seg_b.p, (const void*)Q0.pre.p, (const void*)Q0.queue.p,
+                                (const void*)Q0.ft32.p, (const void*)Q0.ft64.p, (const void*)Q0.ft0.p,
+                                (const void*)Q0.flags.p, (const void*)Q0.prep.p, (const void*)bbuf.cell,
+                                (const void*)bbuf.amp, (const void*)bbuf.tile_count, (const void*)bbuf.tile_off,
+                                (const void*)bbuf.chunk_off, (const void*)bbuf.cursor, (const void*)bbuf.scell,
+                                (const void*)bbuf.samp})
+            put_v(ptr);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "run_job segment table");
+        } else if (!c->gexec || key != c->gkey) {
+            if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+            c->gkey.clear();
+            e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) {
+                rc = hip_fail(c, e, "run_job graph capture");
+            } else {
+                rc = enqueue();
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(c->stream, &g);
+                if (!rc && e != hipSuccess) rc = hip_fail(c, e, "run_job graph capture");
+                if (!rc) {
+                    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+                    if (e != hipSuccess) {
+                        c->gexec = nullptr;
+                        rc = hip_fail(c, e, "run_job graph instantiate");
+                    } else {
+                        c->gkey = std::move(key);
+                    }
+                }
+                if (g) (void)hipGraphDestroy(g);
             }
         }
-        if (!two_pass) {
-            if (e == hipSuccess) e = hipEventRecord(ev[6 * k + 3], A);
-            if (e == hipSuccess) e = hipEventRecord(Q.done, A);
+        if (!rc) {
+            e = hipEventRecord(ev[0], c->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(c->gexec, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[3], c->stream);
+            if (e != hipSuccess) rc = hip_fail(c, e, "run_job graph launch");
         }
-        Q.pending = true;
-        if (e != hipSuccess) rc = hip_fail(c, e, "run_job slab");
     }
-    if (rc == ETRAJ_OK && prev.k >= 0) {
-        hipError_t e = launch_fp64(prev, ev[6 * prev.k + 3]);
-        if (e != hipSuccess) rc = hip_fail(c, e, "run_job FP64 pass");
-    }
-    // the first stream finishes the job after every FP64 pass
-    for (int q = 0; q < kSlabSlots; ++q)
-        if (c->slot[q].pending && rc == ETRAJ_OK) {
-            hipError_t e = hipStreamWaitEvent(c->stream, c->slot[q].done, 0);
-            if (e != hipSuccess) rc = hip_fail(c, e, "run_job join");
-        }
     if (rc) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamSynchronize(c->stream);
@@ -1369,14 +1471,6 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         return rc;
-    }
-
-    if (fixed) HIP_TRY(c, launch_fix_to_f64(c->stream, c->pmdq.p, (int64_t)n_jobs * plen, c->pmd.p));
-    const double* result = c->pmd.p;
-    if (coh) {
-        HIP_TRY(c, c->pmd_out.ensure((size_t)n_jobs * npix));
-        HIP_TRY(c, launch_finalize_abs2(c->stream, c->pmd.p, (int64_t)n_jobs * npix, c->pmd_out.p));
-        result = c->pmd_out.p;
     }
     // per-slot copies (kAccSlots per job), added on the host below
     std::vector<double> scal_s((size_t)2 * kAccSlots * n_jobs), scal(2 * n_jobs, 0.0);
@@ -1430,7 +1524,12 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         for (int64_t i = 0; i < n_jobs * npix; ++i) run->hit_counts[i] += hits[i];
     for (SlabSlot& Q : c->slot) Q.pending = false;
     double ms_s = 0, ms_i = 0, ms_m = 0;
-    for (int k = 0; k < nslab; ++k) {
+    if (use_graph) {   // the graph's device time, all of it as integration
+        float g = 0;
+        (void)hipEventElapsedTime(&g, ev[0], ev[3]);
+        ms_i = g;
+    }
+    for (int k = 0; k < nslab && !use_graph; ++k) {
         float a = 0, b = 0, m = 0, f = 0;
         (void)hipEventElapsedTime(&a, ev[6 * k], ev[6 * k + 1]);
         (void)hipEventElapsedTime(&b, ev[6 * k + 1], ev[6 * k + 3]);
@@ -1566,6 +1665,7 @@ int etraj_autotune(etraj_ctx* c, const etraj_laser_desc* laser, const etraj_targ
             try_field(&etraj_tuning::bin_chunk, {4096, 16384, 65536});
         }
     }
+    try_field(&etraj_tuning::graph, {0, 1});   // no effect on a job of several slabs: never kept there
     if (rc) {
         apply_tuning(c, start);
         return rc;

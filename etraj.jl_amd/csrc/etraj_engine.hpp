@@ -304,6 +304,8 @@ hipError_t launch_ts5_finish(hipStream_t st, const IntegP& P, const OutP& O, con
                              const uint8_t* flags);
 hipError_t launch_scan_counts(hipStream_t st, const int32_t* count, int64_t n, int64_t* pre);
 hipError_t launch_finalize_abs2(hipStream_t st, const double* amp, int64_t n, double* out);
+// zeroes `bytes` (a multiple of 8) at p with a kernel (capturable, unlike hipMemsetAsync)
+hipError_t launch_zero(hipStream_t st, void* p, int64_t bytes);
 // out[i] = fix_to_double(q[2i], q[2i+1]), i < n
 hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out);
 

@@ -583,6 +583,14 @@ __global__ void k_fix_to_f64(const uint64_t* __restrict__ q, int64_t n, double* 
     out[i] = fix_to_double(q[2 * i], q[2 * i + 1]);
 }
 
+// Zeroes n 8-byte words (grid-stride): the clearing of the job's sums and PMD, a
+// kernel rather than hipMemsetAsync so that the job's launches can be captured
+// into a HIP graph.
+__global__ void k_zero_words(uint64_t* __restrict__ p, int64_t n) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = 0ull;
+}
+
 __global__ void k_finalize_abs2(const double* __restrict__ amp, int64_t n, double* __restrict__ out) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -721,6 +729,15 @@ hipError_t launch_field_table(hipStream_t st, const LaserP* lps, const int32_t* 
 hipError_t launch_fix_to_f64(hipStream_t st, const uint64_t* q, int64_t n, double* out) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fix_to_f64, dim3(grid_for(n, 256)), dim3(256), 0, st, q, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(hipStream_t st, void* p, int64_t bytes) {
+    if (bytes <= 0) return hipSuccess;
+    if (bytes % 8 != 0 || reinterpret_cast<uintptr_t>(p) % 8 != 0) return hipErrorInvalidValue;
+    const int64_t n = bytes / 8;
+    hipLaunchKernelGGL(k_zero_words, dim3(std::min<unsigned>(grid_for(n, 256), 8192u)), dim3(256), 0, st,
+                       static_cast<uint64_t*>(p), n);
     return hipGetLastError();
 }
 

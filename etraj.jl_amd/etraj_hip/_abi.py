@@ -95,10 +95,10 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("bin_tiles", C.c_int32), ("bin_tile_shift", C.c_int32), ("bin_chunk", C.c_int32),
                 ("overlap_fp64", C.c_int32), ("rk4_block", C.c_int32), ("tsit5_refill", C.c_int32),
-                ("tsit5_claim", C.c_int32), ("reserved", C.c_int32)]
+                ("tsit5_claim", C.c_int32), ("graph", C.c_int32)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 P = C.POINTER

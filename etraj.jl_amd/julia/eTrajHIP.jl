@@ -61,7 +61,7 @@ struct RunDesc
 end
 mutable struct TuningDesc   # etraj_tuning: the launch choices of a ctx
     bin_tiles::Int32; bin_tile_shift::Int32; bin_chunk::Int32; overlap_fp64::Int32
-    rk4_block::Int32; tsit5_refill::Int32; tsit5_claim::Int32; reserved::Int32
+    rk4_block::Int32; tsit5_refill::Int32; tsit5_claim::Int32; graph::Int32
     TuningDesc() = new(-1, 0, 16384, 1, 256, 4, 256, 0)
 end
 mutable struct Stats

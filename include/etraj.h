@@ -365,7 +365,11 @@ typedef struct etraj_tuning {
     int32_t rk4_block;       /* RK4 threads per workgroup: 64, 128 or 256                      */
     int32_t tsit5_refill;    /* Tsit5: idle lanes of a wavefront before it refills them, 1..64 */
     int32_t tsit5_claim;     /* Tsit5: electrons a wavefront claims from the queue, 64..65536  */
-    int32_t reserved;        /* 0 */
+    int32_t graph;           /* 1: a job whose batches fit one slab (at most 2^26 init columns) runs
+                                as a HIP graph, captured at its first call and replayed while the
+                                call's launch arguments repeat (its stats then give the graph's
+                                device time as ms_integrate, with ms_sample and ms_integrate_main
+                                0); 0 (the default) launches every kernel on each call          */
 } etraj_tuning;
 
 /* The ctx's current choices. */
@@ -375,9 +379,9 @@ int etraj_set_tuning(etraj_ctx* ctx, const etraj_tuning* tuning);
 /* The autotuner: runs the job (as etraj_run_job_device) with the ctx's choices
  * and with candidates that change one choice at a time (RK4: workgroup size,
  * binning, tile size, FP64-pass overlap; Tsit5: refill threshold, queue claim,
- * binning, tile size), each after a warm-up run, keeps the fastest set in the
- * ctx and returns it with its time (wall time of one job, ms) and the time of
- * the choices it started from. About 10-14 job runs. */
+ * binning, tile size; both: graph replay), each after a warm-up run, keeps the
+ * fastest set in the ctx and returns it with its time (wall time of one job, ms)
+ * and the time of the choices it started from. About 12-16 job runs. */
 int etraj_autotune(etraj_ctx* ctx, const etraj_laser_desc* laser, const etraj_target_desc* target,
                    const etraj_sampler_desc* sampler, const etraj_run_desc* run, etraj_tuning* best,
                    double* best_ms, double* start_ms);

@@ -10,6 +10,7 @@
 // SIMD instead of two).
 #include "etraj_engine.hpp"
 #include "etraj_device.hpp"
+#include "etraj_ts5.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -172,7 +173,8 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
 // its column (state = the init buffer) for k_ts5_finish: the start-up and
 // finish code that runs with only the refilled lanes active is a few loads and
 // stores, so the wave refills often and keeps its lanes busy.
-template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool QUEUE>
+// SL: the laser is simple_laser (one cos^n colour, azimuth 0; the launcher checks).
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool QUEUE, bool SL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,
     const int64_t* __restrict__ pre, int64_t n_seg, unsigned long long* __restrict__ queue,
@@ -276,8 +278,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) st.k1p[i] = q[1 + i];
                 st.k1ph = q[1 + DIM];
-                st.lqold = log(1e-4);   // as ts5_init
-                st.q11 = 1.0;
+                st.lqold = kTs5LogQmin;   // as ts5_init
                 st.accepted = 0;
                 st.attempted = 0;
             } else {
@@ -285,8 +286,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         if (valid) {
-            const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol, P.maxiters,
-                                                               st, rc2, &close);
+            const int rc = ts5_attempt<DIM, PHASE, TGT, MIXED, SL>(P.L, P.T, saeq, t0, P.tf, P.rtol, atol,
+                                                                   P.maxiters, st, rc2, &close);
             if (MIXED && close) {   // re-integrated in FP64 by the next launch
                 valid = false;
                 acc.steps += st.accepted;
@@ -376,9 +377,9 @@ static hipError_t ts5_grid(K kernel, int64_t max_items, unsigned& blocks) {
     return hipSuccess;
 }
 
-template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
-static hipError_t launch_tsit5_k(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
-    auto kernel = k_tsit5<DIM, PHASE, OUT, TGT, MIXED, true>;
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool SL>
+static hipError_t launch_tsit5_s(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
+    auto kernel = k_tsit5<DIM, PHASE, OUT, TGT, MIXED, true, SL>;
     unsigned blocks = 0;
     hipError_t e = ts5_grid(kernel, a.max_items, blocks);
     if (e != hipSuccess || blocks == 0) return e;
@@ -386,6 +387,15 @@ static hipError_t launch_tsit5_k(hipStream_t st, const IntegP& P, const OutP& O,
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, st, P, O, a.init, a.seg_start, a.pre, a.n_seg, a.queue,
                        a.flags, a.orig, a.state, a.prep);
     return hipGetLastError();
+}
+
+// The run_job launches (OUT_STATE) of a simple laser take the kernel built for it.
+template <int DIM, int PHASE, int OUT, int TGT, bool MIXED>
+static hipError_t launch_tsit5_k(hipStream_t st, const IntegP& P, const OutP& O, const Ts5Launch& a) {
+    if constexpr (OUT == OUT_STATE) {
+        if (simple_laser(P.L)) return launch_tsit5_s<DIM, PHASE, OUT, TGT, MIXED, true>(st, P, O, a);
+    }
+    return launch_tsit5_s<DIM, PHASE, OUT, TGT, MIXED, false>(st, P, O, a);
 }
 
 template <int DIM, int PHASE, int OUT, bool MIXED>

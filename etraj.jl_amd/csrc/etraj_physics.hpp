@@ -241,9 +241,11 @@ __host__ __device__ inline MonoBase mono_base(const MonoP& m, double t) {
 }
 
 // SIMPLE: the caller guarantees a cos^2 / cos^4 pulse with azimuth 0 (simple_laser), so that the
-// Gaussian and rotated forms, and their parameters, drop out of the code.
+// Gaussian and rotated forms, and their parameters, drop out of the code. at (may be null): the
+// carrier and envelope sin/cos at t + d, with at->tp = 0 where the window skipped them.
 template <bool SIMPLE = false>
-__host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b, double d, double& Fx, double& Fy) {
+__host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b, double d, double& Fx, double& Fy,
+                                              MonoBase* at = nullptr) {
     const double t = b.tp + d;
     const double x = m.w * d;
     const bool gauss = !SIMPLE && m.kind == ETRAJ_LASER_GAUSSIAN;
@@ -251,6 +253,7 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
     if (win == 0.0) {   // outside the pulse's window the field is exactly zero: no trigonometry
         Fx = 0.0;
         Fy = 0.0;
+        if (at) at->tp = 0.0;
         return;
     }
     const double xh = gauss ? 0.0 : x * m.ihalfN;
@@ -268,6 +271,13 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
     } else {
         sincos(m.w * t + m.cep, &sw, &cw);
         if (!gauss) sincos(m.w * t * m.ihalfN, &sh, &ch);
+    }
+    if (at) {
+        at->tp = 1.0;
+        at->sw = sw;
+        at->cw = cw;
+        at->sh = sh;
+        at->ch = ch;
     }
     double a, bb, g;
     if (gauss) {

@@ -58,7 +58,10 @@ struct Ts5Lane {
     double r[DIM], p[DIM], ph;
     double k1p[DIM], k1ph;
     double t, dt, lqold;   // lqold = log(qold) of the PI controller
-    int32_t accepted, attempted;
+    // the simple laser's carrier and envelope sin/cos at t, carried from the last accepted step
+    // (valid while base_ok != 0)
+    double bsw, bcw, bsh, bch;
+    int32_t accepted, attempted, base_ok;
 };
 
 template <int DIM, int PHASE, int TGT>
@@ -119,6 +122,7 @@ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool
     constexpr int N = 2 * DIM + (PH ? 1 : 0);
     s.t = t0;
     s.lqold = kTs5LogQmin;
+    s.base_ok = 0;
     s.accepted = 0;
     s.attempted = 0;
     const double dtmax = tf - t0;
@@ -259,7 +263,20 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     kh[0] = s.k1ph;
     double yr[DIM], yp[DIM];
     ts5_inputs<2, DIM>(s.r, s.p, kp, dt, yr, yp);
-    const LaserBase lb = laser_base<SL>(L, t);
+    // SL: the laser's phases at t from the last accepted step's stage-7 angles (angle addition
+    // from the previous base, so their rounding errors add up: every 16th accepted step, and
+    // after a step whose stage 7 lay outside the pulse window, they are taken afresh)
+    LaserBase lb;
+    if (SL && s.base_ok) {
+        lb.c[0].tp = t - L.c[0].tshift;
+        lb.c[0].sw = s.bsw;
+        lb.c[0].cw = s.bcw;
+        lb.c[0].sh = s.bsh;
+        lb.c[0].ch = s.bch;
+        lb.c[1] = lb.c[0];
+    } else {
+        lb = laser_base<SL>(L, t);
+    }
     double Fx, Fy;
     laser_field_at<SL>(L, lb, ts5::c2 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[1], kh[1], rc2, close);
@@ -274,7 +291,9 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[4], kh[4], rc2, close);
     ts5_inputs<6, DIM>(s.r, s.p, kp, dt, yr, yp);
     double F1x, F1y;                     // stages 6 and 7 share t + dt (c6 = c7 = 1)
-    laser_field_at<SL>(L, lb, dt, F1x, F1y);
+    MonoBase at;                         // SL: the phases at t + dt
+    if (SL) mono_field_at<true>(L.c[0], lb.c[0], dt, F1x, F1y, &at);
+    else laser_field_at<SL>(L, lb, dt, F1x, F1y);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, F1x, F1y, yr, yp, kp[5], kh[5], rc2, close);
     double unr[DIM], unp[DIM], er[DIM], ep[DIM], unh = 0, eh = 0;
     ts5_inputs<7, DIM>(s.r, s.p, kp, dt, unr, unp);
@@ -340,6 +359,13 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
         s.ph = unh;
         s.k1ph = k7h;
         ++s.accepted;
+        if (SL) {
+            s.base_ok = at.tp != 0.0 && (s.accepted & 15) != 0;
+            s.bsw = at.sw;
+            s.bcw = at.cw;
+            s.bsh = at.sh;
+            s.bch = at.ch;
+        }
         s.lqold = fmax(lE, kTs5LogQmin);   // log(max(EEst, qoldinit))
         s.dt = fmin(dt * iq, tf - t0);
     } else {

@@ -279,6 +279,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 for (int i = 0; i < DIM; ++i) st.k1p[i] = q[1 + i];
                 st.k1ph = q[1 + DIM];
                 st.lqold = kTs5LogQmin;   // as ts5_init
+                st.base_ok = 0;
                 st.accepted = 0;
                 st.attempted = 0;
             } else {

@@ -283,8 +283,9 @@ hipError_t launch_tsit5(hipStream_t st, const IntegP& P, const OutP& O, int out,
                         const int64_t* seg_start, const int64_t* pre, int64_t n_seg, int64_t max_items,
                         unsigned long long* queue, uint8_t* flags, const int64_t* orig);
 // Doubles per column of the Tsit5 start values (ts5_prep_doubles): the first
-// step size and the first stage derivative (p part, phase part).
-constexpr int ts5_prep_doubles(int dim) { return dim + 2; }
+// step size, the first stage derivative (p part, phase part) and, for a simple
+// laser (simple_laser), the carrier's and the envelope's sin and cos at t0.
+constexpr int ts5_prep_doubles(int dim) { return dim + 6; }
 // The Tsit5 run of run_job in three launches over the columns
 // [seg_start[s], seg_start[s] + seg_count[s]) of the segments (pre: the
 // exclusive prefix sums of seg_count, pre[n_seg] <= max_items; cps >=

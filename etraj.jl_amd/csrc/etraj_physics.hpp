@@ -242,7 +242,8 @@ __host__ __device__ inline MonoBase mono_base(const MonoP& m, double t) {
 
 // SIMPLE: the caller guarantees a cos^2 / cos^4 pulse with azimuth 0 (simple_laser), so that the
 // Gaussian and rotated forms, and their parameters, drop out of the code. at (may be null): the
-// carrier and envelope sin/cos at t + d, with at->tp = 0 where the window skipped them.
+// carrier and envelope sin/cos at t + d (at->tp = 1), or at->tp = 2 (t + d past the window) or 0
+// (before it) where the window skipped them.
 template <bool SIMPLE = false>
 __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b, double d, double& Fx, double& Fy,
                                               MonoBase* at = nullptr) {
@@ -253,7 +254,7 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
     if (win == 0.0) {   // outside the pulse's window the field is exactly zero: no trigonometry
         Fx = 0.0;
         Fy = 0.0;
-        if (at) at->tp = 0.0;
+        if (at) at->tp = t > 0.0 ? 2.0 : 0.0;
         return;
     }
     const double xh = gauss ? 0.0 : x * m.ihalfN;

@@ -263,9 +263,10 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     kh[0] = s.k1ph;
     double yr[DIM], yp[DIM];
     ts5_inputs<2, DIM>(s.r, s.p, kp, dt, yr, yp);
-    // SL: the laser's phases at t from the last accepted step's stage-7 angles (angle addition
-    // from the previous base, so their rounding errors add up: every 16th accepted step, and
-    // after a step whose stage 7 lay outside the pulse window, they are taken afresh)
+    // SL: the laser's phases at t carried from the start values (k_ts5_prep) or the last accepted
+    // step's stage 7, by angle addition; their rounding errors add up by about an ulp per
+    // accepted step (1e-13 after a thousand steps), the same whatever the launch's scheduling.
+    // A step after one whose stage 7 lay before the pulse window takes them afresh.
     LaserBase lb;
     if (SL && s.base_ok) {
         lb.c[0].tp = t - L.c[0].tshift;
@@ -360,11 +361,15 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
         s.k1ph = k7h;
         ++s.accepted;
         if (SL) {
-            s.base_ok = at.tp != 0.0 && (s.accepted & 15) != 0;
-            s.bsw = at.sw;
-            s.bcw = at.cw;
-            s.bsh = at.sh;
-            s.bch = at.ch;
+            // at.tp: 1 phases at t + dt; 2 past the pulse window, where no later stage needs them
+            // (the window is exactly zero there and stays so); 0 before the window
+            s.base_ok = at.tp != 0.0;
+            if (at.tp == 1.0) {
+                s.bsw = at.sw;
+                s.bcw = at.cw;
+                s.bsh = at.sh;
+                s.bch = at.ch;
+            }
         }
         s.lqold = fmax(lE, kTs5LogQmin);   // log(max(EEst, qoldinit))
         s.dt = fmin(dt * iq, tf - t0);

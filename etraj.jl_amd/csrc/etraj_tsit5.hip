@@ -279,7 +279,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 for (int i = 0; i < DIM; ++i) st.k1p[i] = q[1 + i];
                 st.k1ph = q[1 + DIM];
                 st.lqold = kTs5LogQmin;   // as ts5_init
-                st.base_ok = 0;
+                if constexpr (SL) {   // the laser's phases at t0
+                    st.bsw = q[2 + DIM];
+                    st.bcw = q[3 + DIM];
+                    st.bsh = q[4 + DIM];
+                    st.bch = q[5 + DIM];
+                    st.base_ok = 1;
+                } else {
+                    st.base_ok = 0;
+                }
                 st.accepted = 0;
                 st.attempted = 0;
             } else {
@@ -339,6 +347,13 @@ __global__ __launch_bounds__(256) void k_ts5_prep(IntegP P, const double* __rest
 #pragma unroll
     for (int i = 0; i < DIM; ++i) q[1 + i] = st.k1p[i];
     q[1 + DIM] = st.k1ph;
+    if (simple_laser(P.L)) {   // the laser's phases at t0 for the integrator's simple-laser kernel
+        const MonoBase b = mono_base(P.L.c[0], c[2 * DIM]);
+        q[2 + DIM] = b.sw;
+        q[3 + DIM] = b.cw;
+        q[4 + DIM] = b.sh;
+        q[5 + DIM] = b.ch;
+    }
 }
 
 __global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }

@@ -507,18 +507,22 @@ __global__ __launch_bounds__(256) void k_ts5_finish(IntegP P, OutP O, const doub
 }
 
 // In-place, order-preserving compaction of the flagged columns of each
-// segment (one workgroup per segment): count_out[s] receives their number and
-// orig[seg_start[s] + k] the original column (or its out_index entry) of the
-// k-th. Every column of a chunk is read before any is written, and writes
-// never pass the read position, so the compaction can overwrite in place.
-__global__ __launch_bounds__(256) void k_compact_flagged(double* __restrict__ init, int rows,
-                                                         const int64_t* __restrict__ seg_start,
-                                                         const int32_t* __restrict__ seg_count,
-                                                         const uint8_t* __restrict__ flags,
-                                                         const int64_t* __restrict__ out_index,
-                                                         int32_t* __restrict__ count_out,
-                                                         int64_t* __restrict__ orig) {
-    __shared__ int wave_tot[4];
+// segment (one workgroup of kCompactWaves wavefronts per segment): count_out[s]
+// receives their number and orig[seg_start[s] + k] the original column (or its
+// out_index entry) of the k-th. Every column of a chunk is read before any is
+// written, and writes never pass the read position, so the compaction can
+// overwrite in place. The flags of the next chunk are loaded while this one is
+// compacted (they are never written here).
+constexpr int kCompactWaves = 16;
+__global__ __launch_bounds__(64 * kCompactWaves) void k_compact_flagged(double* __restrict__ init, int rows,
+                                                                       const int64_t* __restrict__ seg_start,
+                                                                       const int32_t* __restrict__ seg_count,
+                                                                       const uint8_t* __restrict__ flags,
+                                                                       const int64_t* __restrict__ out_index,
+                                                                       int32_t* __restrict__ count_out,
+                                                                       int64_t* __restrict__ orig) {
+    constexpr int kChunk = 64 * kCompactWaves;
+    __shared__ int wave_tot[kCompactWaves];
     __shared__ int base_sh;
     const int64_t seg = blockIdx.x;
     const int64_t base = seg_start[seg];
@@ -527,18 +531,24 @@ __global__ __launch_bounds__(256) void k_compact_flagged(double* __restrict__ in
     if (tid == 0) base_sh = 0;
     __syncthreads();
     double col[9];
-    for (int j0 = 0; j0 < cnt; j0 += 256) {
+    bool fnext = tid < cnt && flags[base + tid] != 0;
+    for (int j0 = 0; j0 < cnt; j0 += kChunk) {
         const int j = j0 + tid;
-        const bool keep = j < cnt && flags[base + j] != 0;
+        const bool keep = fnext;
+        const int jn = j + kChunk;
+        fnext = jn < cnt && flags[base + jn] != 0;
         if (keep)
             for (int r = 0; r < rows; ++r) col[r] = init[(base + j) * rows + r];
         const uint64_t m = __ballot(keep);
         const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
         if (lane == 0) wave_tot[wv] = __popcll(m);
         __syncthreads();
-        int off = base_sh;
-        for (int w = 0; w < wv; ++w) off += wave_tot[w];
-        const int tot = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        int off = base_sh, tot = 0;
+        for (int w = 0; w < kCompactWaves; ++w) {
+            const int c = wave_tot[w];
+            off += w < wv ? c : 0;
+            tot += c;
+        }
         if (keep) {
             const int64_t dst = base + off + pre;
             for (int r = 0; r < rows; ++r) init[dst * rows + r] = col[r];
@@ -707,8 +717,8 @@ hipError_t launch_compact_flagged(hipStream_t st, double* init, int rows, const 
                                   const int32_t* seg_count, int64_t n_seg, const uint8_t* flags,
                                   const int64_t* out_index, int32_t* count_out, int64_t* orig) {
     if (n_seg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_compact_flagged, dim3((unsigned)n_seg), dim3(256), 0, st, init, rows, seg_start, seg_count,
-                       flags, out_index, count_out, orig);
+    hipLaunchKernelGGL(k_compact_flagged, dim3((unsigned)n_seg), dim3(64 * kCompactWaves), 0, st, init, rows, seg_start,
+                       seg_count, flags, out_index, count_out, orig);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ struct AdkBatch {
     const LaserTr* lt;  // SPA: the laser at tr, tr + 0.01, tr - 0.01 (LDS; complex-time evaluations)
     double dFx, dFy;    // SPANE: dF/dt at tr
     double cross;       // SPANE: Fx dFy - dFx Fy (SPANESampler.jl:238)
+    double iF, i3F, sqF;   // 1/F, 1/(3F) and sqrt(F) at tr
     bool skip;          // SPA: F(tr) <= 1e-3 empties the batch (SPASampler.jl:167-169)
 };
 
@@ -149,6 +150,9 @@ __device__ inline AdkBatch adk_batch_setup(const SamplerP& S, const LaserP& L, c
         laser_field(L, tr, Fx, Fy);
     }
     B.F = hypot(Fx, Fy);                                            // :172
+    B.iF = 1.0 / B.F;
+    B.i3F = 1.0 / (3 * B.F);
+    B.sqF = sqrt(B.F);
     double phi = atan2(-Fy, -Fx);                                   // :173
     sincos(phi, &B.sphi, &B.cphi);
     double Ip = S.Ip;
@@ -406,6 +410,17 @@ __device__ inline void spa_amp(const SamplerP& S, const LaserP& L, const TargetP
 // The sample point (kd0, kz0) of sample j of batch b and its weight w (the
 // step grid of :316-320, shared_methods.jl:4-12 Monte-Carlo, or the stratified
 // Gaussian extension).
+// x^1.5 as x sqrt(x), within two ulp of pow(x, 1.5)
+__device__ inline double pow15(double x) { return x * sqrt(x); }
+
+// x^e for the prefactor's exponent e = (n* + 1)/4 of a job: a square root for e = 1/2 (n* = 1,
+// the hydrogen-like ground states), x for e = 1, else exp(e log x); within a few ulp of pow(x, e)
+__device__ inline double pow_nstar(double x, double e) {
+    if (e == 0.5) return sqrt(x);
+    if (e == 1.0) return x;
+    return exp(e * log(x));
+}
+
 __device__ inline void sample_point(const SamplerP& S, const AdkBatch& B, int b, int64_t j, double& kd0, double& kz0,
                                     double& w) {
     kz0 = 0.0;
@@ -424,15 +439,20 @@ __device__ inline void sample_point(const SamplerP& S, const AdkBatch& B, int b,
                 if (S.mc_kz_max == 0) kz0 = 0.0;
             }
         } else {                                                  // stratified Gaussian (extension)
-            double r = sqrt(-2.0 * log(unit_oc(o.v[0])));
-            double a = 2 * kPi * unit_co(o.v[1]);
+            // Box-Muller; w is divided by the normal densities of kd0 (and kz0), i.e. multiplied
+            // by sigma sqrt(2 pi) exp(z^2 / 2) with z = kd0 / sigma = r cos a (r sin a)
+            const double r = sqrt(-2.0 * log(unit_oc(o.v[0])));
             double sa, ca;
-            sincos(a, &sa, &ca);
-            kd0 = S.kd_sigma * (r * ca);
-            w = w / (exp(-0.5 * (kd0 / S.kd_sigma) * (kd0 / S.kd_sigma)) / (S.kd_sigma * sqrt(2 * kPi)));
+            sincos(2 * kPi * unit_co(o.v[1]), &sa, &ca);         // a = 2 pi u
+            const double zd = r * ca;
+            kd0 = S.kd_sigma * zd;
+            constexpr double kSqrt2Pi = 2.5066282746310002;
             if (S.dim == 3) {
-                kz0 = S.kz_sigma * (r * sa);
-                w = w / (exp(-0.5 * (kz0 / S.kz_sigma) * (kz0 / S.kz_sigma)) / (S.kz_sigma * sqrt(2 * kPi)));
+                const double zz = r * sa;
+                kz0 = S.kz_sigma * zz;
+                w = w * ((S.kd_sigma * kSqrt2Pi) * (S.kz_sigma * kSqrt2Pi)) * exp(0.5 * (zd * zd + zz * zz));
+            } else {
+                w = w * (S.kd_sigma * kSqrt2Pi) * exp(0.5 * (zd * zd));
             }
         }
     }
@@ -474,11 +494,11 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         x0 = r0 * B.cphi;
         y0 = r0 * B.sphi;
         if (f2 <= B.F * B.F / (S.sampling == ETRAJ_SAMPLING_STEP ? 9.0 : 25.0)) return false;   // :332, :378
-        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * sqrt(f2)));
+        double amp = sqrt(w) * exp(-pow15(kd0 * kd0 + kz0 * kz0 + 2 * Ip) / (3 * sqrt(f2)));
         double aim = 0.0;
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {
             const double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
-            const double den = pow((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
+            const double den = pow_nstar((k2 + 2 * Ip) * f2, (S.nstar + 1) / 4);   // :262-269
             if (!PRE) {
                 amp *= cpre * S.numer / den;
             } else {   // ti = sqrt((kappa^2+kd^2+kz^2)/F2eff), k_ts of :241-242
@@ -505,15 +525,15 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
         const double k2 = 2 * Ip, q2 = kd0 * kd0 + kz0 * kz0;
         double poly = 0.0;
         for (int am = B.wM; am >= 0; --am) poly = poly * q2 + B.wB[am];
-        rate = poly * exp(-2 * pow(k2 + q2, 1.5) / (3 * B.F)) * w;
+        rate = poly * exp(-2 * pow15(k2 + q2) * B.i3F) * w;
     } else {
         if (fabs(kd0) < 1e-4) return false;                           // ADKSampler.jl:312,:321
         const double kx = kd0 * -B.sphi, ky = kd0 * B.cphi;
-        double amp = sqrt(w) * exp(-pow(kd0 * kd0 + kz0 * kz0 + 2 * Ip, 1.5) / (3 * B.F));  // :198
+        double amp = sqrt(w) * exp(-pow15(kd0 * kd0 + kz0 * kz0 + 2 * Ip) * B.i3F);  // :198
         double aim = 0.0;                                             // Im amp (prefactor beyond l = 0)
         if (S.prefix & (ETRAJ_PREFIX_PRE | ETRAJ_PREFIX_PRECC)) {    // :258-265
             double cpre = (S.prefix & ETRAJ_PREFIX_PRE) ? S.c_pre : B.c_cc;
-            const double den = pow((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
+            const double den = pow_nstar((kx * kx + ky * ky + kz0 * kz0 + 2 * Ip) * (B.F * B.F), (S.nstar + 1) / 4);
             if (!PRE) {
                 amp *= cpre * S.numer / den;
             } else {   // k_ts at ts = tr + i ti, ti = sqrt(kappa^2+kd^2+kz^2)/F (:237-238)
@@ -525,8 +545,8 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
             }
         }
         if (S.prefix & ETRAJ_PREFIX_JAC) {                            // :266,:283
-            amp *= sqrt(B.F);
-            aim *= sqrt(B.F);
+            amp *= B.sqF;
+            aim *= B.sqF;
         }
         rate = amp * amp + aim * aim;
         if (PRE) phase = atan2(aim, amp);                    // angle(amp)
@@ -534,7 +554,7 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
     if (M == ETRAJ_INIT_ADK || M == ETRAJ_INIT_WFAT) {
         kx0 = kd0 * -B.sphi;                                          // :324-325
         ky0 = kd0 * B.cphi;
-        const double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) / B.F;  // :326
+        const double r0 = (Ip + (kd0 * kd0 + kz0 * kz0) / 2) * B.iF;  // :326
         x0 = r0 * B.cphi;
         y0 = r0 * B.sphi;
     }

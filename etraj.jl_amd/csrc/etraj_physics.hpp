@@ -201,26 +201,25 @@ struct LaserBase {
     MonoBase c[2];
 };
 
-// sin and cos of |x| <= 1/2 from their Taylor series through x^15 and x^16
-// (the first omitted terms are below 1e-19).
-__host__ __device__ inline void sincos_small(double x, double& s, double& c) {
+// sin and cos of |x| <= sincos_taylor_bound(KS) from their Taylor series through
+// x^(2 KS + 1) and x^(2 KS + 2): KS = 7 for |x| <= 1/2, 5 for |x| <= 1/4, 4 for
+// |x| <= 1/8 (in each case the first omitted term is below 1e-17 of the value).
+constexpr double sincos_taylor_bound(int KS) { return KS >= 7 ? 0.5 : KS >= 5 ? 0.25 : 0.125; }
+constexpr double inv_factorial(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= i;   // exact in double up to 18!
+    return 1.0 / f;
+}
+template <int KS>
+__host__ __device__ inline void sincos_taylor(double x, double& s, double& c) {
     const double x2 = x * x;
-    double ps = -1.0 / 1307674368000.0;
-    ps = fma(ps, x2, 1.0 / 6227020800.0);
-    ps = fma(ps, x2, -1.0 / 39916800.0);
-    ps = fma(ps, x2, 1.0 / 362880.0);
-    ps = fma(ps, x2, -1.0 / 5040.0);
-    ps = fma(ps, x2, 1.0 / 120.0);
-    ps = fma(ps, x2, -1.0 / 6.0);
+    double ps = ((KS & 1) ? -1.0 : 1.0) * inv_factorial(2 * KS + 1);
+#pragma unroll
+    for (int k = KS - 1; k >= 1; --k) ps = fma(ps, x2, ((k & 1) ? -1.0 : 1.0) * inv_factorial(2 * k + 1));
     s = fma(ps * x2, x, x);
-    double pc = 1.0 / 20922789888000.0;
-    pc = fma(pc, x2, -1.0 / 87178291200.0);
-    pc = fma(pc, x2, 1.0 / 479001600.0);
-    pc = fma(pc, x2, -1.0 / 3628800.0);
-    pc = fma(pc, x2, 1.0 / 40320.0);
-    pc = fma(pc, x2, -1.0 / 720.0);
-    pc = fma(pc, x2, 1.0 / 24.0);
-    pc = fma(pc, x2, -0.5);
+    double pc = (((KS + 1) & 1) ? -1.0 : 1.0) * inv_factorial(2 * KS + 2);
+#pragma unroll
+    for (int k = KS; k >= 1; --k) pc = fma(pc, x2, ((k & 1) ? -1.0 : 1.0) * inv_factorial(2 * k));
     c = fma(pc, x2, 1.0);
 }
 
@@ -243,8 +242,12 @@ __host__ __device__ inline MonoBase mono_base(const MonoP& m, double t) {
 // SIMPLE: the caller guarantees a cos^2 / cos^4 pulse with azimuth 0 (simple_laser), so that the
 // Gaussian and rotated forms, and their parameters, drop out of the code. at (may be null): the
 // carrier and envelope sin/cos at t + d (at->tp = 1), or at->tp = 2 (t + d past the window) or 0
-// (before it) where the window skipped them.
-template <bool SIMPLE = false>
+// (before it) where the window skipped them. KX: the Taylor order (sincos_taylor) of the carrier
+// increment, for callers whose increments are known to be small (a Tsit5 stage at c dt, c < 1,
+// whose step passes the |w dt| <= 1/2 test of the last stage); beyond its range the full sin/cos
+// is taken. SIMPLE (N >= 2 cycles): the envelope increment x/(2N) is at most 1/8 where the
+// carrier's is at most 1/2 and takes the order-4 series.
+template <bool SIMPLE = false, int KX = 7>
 __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b, double d, double& Fx, double& Fy,
                                               MonoBase* at = nullptr) {
     const double t = b.tp + d;
@@ -258,14 +261,16 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
         return;
     }
     const double xh = gauss ? 0.0 : x * m.ihalfN;
+    constexpr bool env_short = SIMPLE;   // N >= 2 (simple_laser)
     double sw, cw, sh = 0.0, ch = 1.0;
-    if (fabs(x) <= 0.5 && fabs(xh) <= 0.5) {
+    if (fabs(x) <= sincos_taylor_bound(KX) && fabs(xh) <= (env_short ? 0.125 : 0.5)) {
         double sd, cd;
-        sincos_small(x, sd, cd);
+        sincos_taylor<KX>(x, sd, cd);
         sw = b.sw * cd + b.cw * sd;
         cw = b.cw * cd - b.sw * sd;
         if (!gauss) {
-            sincos_small(xh, sd, cd);
+            if (env_short) sincos_taylor<4>(xh, sd, cd);
+            else sincos_taylor<7>(xh, sd, cd);
             sh = b.sh * cd + b.ch * sd;
             ch = b.ch * cd - b.sh * sd;
         }
@@ -300,9 +305,10 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
     }
 }
 
-// One colour, cos^2 or cos^4 envelope, azimuth 0: the laser the SIMPLE forms below take.
+// One colour, cos^2 or cos^4 envelope of at least two cycles, azimuth 0: the laser the SIMPLE
+// forms below take.
 __host__ __device__ inline bool simple_laser(const LaserP& L) {
-    return L.n == 1 && L.c[0].kind != ETRAJ_LASER_GAUSSIAN && !L.c[0].azi_nz;
+    return L.n == 1 && L.c[0].kind != ETRAJ_LASER_GAUSSIAN && !L.c[0].azi_nz && L.c[0].ihalfN <= 0.25;
 }
 
 // SIMPLE: simple_laser(L) holds, so the second colour's parameters drop out of the code.

@@ -243,7 +243,7 @@ __device__ __forceinline__ void ts5_inputs(const double* r, const double* p, con
 
 // One attempted step. MIXED: stage derivatives in FP32 (ts5_deriv32); `close`
 // is set when a stage position comes within the re-step radius (r2 < rc2).
-// SL: simple_laser(L) holds (one cos^n colour, azimuth 0), which keeps the
+// SL: simple_laser(L) holds (one cos^n colour of N >= 2 cycles, azimuth 0), which keeps the
 // laser parameters the stage fields read few enough for the scalar registers.
 template <int DIM, int PHASE, int TGT, bool MIXED = false, bool SL = false>
 __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bool saeq, double t0,
@@ -278,11 +278,15 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     } else {
         lb = laser_base<SL>(L, t);
     }
+    // SL: the carrier increments of stages 2 and 3, 0.161 and 0.327 of w dt, take the order-4
+    // and order-5 Taylor series wherever the last stage's |w dt| <= 1/2 takes the order-7 one
     double Fx, Fy;
-    laser_field_at<SL>(L, lb, ts5::c2 * dt, Fx, Fy);
+    if (SL) mono_field_at<true, 4>(L.c[0], lb.c[0], ts5::c2 * dt, Fx, Fy);
+    else laser_field_at<SL>(L, lb, ts5::c2 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[1], kh[1], rc2, close);
     ts5_inputs<3, DIM>(s.r, s.p, kp, dt, yr, yp);
-    laser_field_at<SL>(L, lb, ts5::c3 * dt, Fx, Fy);
+    if (SL) mono_field_at<true, 5>(L.c[0], lb.c[0], ts5::c3 * dt, Fx, Fy);
+    else laser_field_at<SL>(L, lb, ts5::c3 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[2], kh[2], rc2, close);
     ts5_inputs<4, DIM>(s.r, s.p, kp, dt, yr, yp);
     laser_field_at<SL>(L, lb, ts5::c4 * dt, Fx, Fy);

@@ -173,7 +173,7 @@ __device__ inline void flush_lane_acc(const OutP& O, const LaneAcc& a, bool step
 // its column (state = the init buffer) for k_ts5_finish: the start-up and
 // finish code that runs with only the refilled lanes active is a few loads and
 // stores, so the wave refills often and keeps its lanes busy.
-// SL: the laser is simple_laser (one cos^n colour, azimuth 0; the launcher checks).
+// SL: the laser is simple_laser (one cos^n colour of N >= 2 cycles, azimuth 0; the launcher checks).
 template <int DIM, int PHASE, int OUT, int TGT, bool MIXED, bool QUEUE, bool SL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tsit5(
     IntegP P, OutP O, const double* __restrict__ init, const int64_t* __restrict__ seg_start,

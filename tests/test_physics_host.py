@@ -8,6 +8,10 @@ form evaluated with the C library:
     envelope, ellipticity, azimuth, CEP, time shift, two colours, increments up
     to 20 a.u. (past the Taylor range of the rotation) and times over the pulse
     and its edges;
+  * the one-colour cos^n stage fields (mono_field_at<true, KX>, shorter Taylor
+    series for Tsit5's stages 2 and 3 and for the envelope of N >= 2 cycles)
+    equal laser_field(t + d) to 1e-14 of the peak field, and each Taylor order
+    agrees with the C library's sin/cos to 2 ulp over its range;
   * cosh_sinh (both from one exp, sinh by its Taylor series below 1/2) equals
     cosh / sinh to 3 ulp over |x| up to 30.
 """
@@ -62,6 +66,51 @@ int main() {
             }
         }
     }
+    // the simple-laser stage fields (mono_field_at<true, KX>, the orders the Tsit5 stages use)
+    double worst_simple = 0;
+    for (int kind = 1; kind <= 2; ++kind)
+    for (int cyc = 2; cyc <= 4; ++cyc) {
+        etraj_laser_desc d{};
+        d.n_colors = 1;
+        d.color[0].kind = kind;
+        d.color[0].peak_int = 2e14;
+        d.color[0].wave_len = 800;
+        d.color[0].cyc_num = cyc == 4 ? 6 : cyc == 3 ? 2.5 : cyc;
+        d.color[0].ellip = 0.7;
+        d.color[0].cep = 0.2;
+        d.color[0].t_shift = -3;
+        LaserP L;
+        if (make_laser(d, L) || !simple_laser(L)) return 3;
+        for (int i = 0; i < 100000; ++i) {
+            const double t = -500 + 1000 * U(g);
+            const double dt = (i % 3 == 0) ? 20 * U(g) : (i % 3 == 1) ? 14 * U(g) : 0.5 * U(g);
+            const MonoBase b = mono_base(L.c[0], t);
+            const double cs[5] = {0.161, 0.327, 0.9, 0.9800255409045097, 1.0};
+            for (int j = 0; j < 5; ++j) {
+                double fx, fy, rx, ry;
+                if (j == 0) mono_field_at<true, 4>(L.c[0], b, cs[j] * dt, fx, fy);
+                else if (j == 1) mono_field_at<true, 5>(L.c[0], b, cs[j] * dt, fx, fy);
+                else mono_field_at<true, 7>(L.c[0], b, cs[j] * dt, fx, fy);
+                laser_field(L, t + cs[j] * dt, rx, ry);
+                const double e = fmax(fabs(fx - rx), fabs(fy - ry)) / L.c[0].F0;
+                if (e > worst_simple) worst_simple = e;
+            }
+        }
+    }
+    // the Taylor series against the C library over their ranges, in ulp of the value
+    double worst_taylor = 0;
+    for (int i = 0; i < 3000000; ++i) {
+        const int ks = i % 3;
+        const double bound = ks == 0 ? 0.5 : ks == 1 ? 0.25 : 0.125;
+        const double x = (2 * U(g) - 1) * bound;
+        double s, c;
+        if (ks == 0) sincos_taylor<7>(x, s, c);
+        else if (ks == 1) sincos_taylor<5>(x, s, c);
+        else sincos_taylor<4>(x, s, c);
+        const double es = fabs(s - sin(x)) / (fabs(sin(x)) * 0x1.0p-52 + 1e-300), ec = fabs(c - cos(x)) / (cos(x) * 0x1.0p-52);
+        if (es > worst_taylor) worst_taylor = es;
+        if (ec > worst_taylor) worst_taylor = ec;
+    }
     double worst_ch = 0, worst_sh = 0;
     for (int i = 0; i < 2000000; ++i) {
         const double s = (i & 3) == 0 ? 0.6 : (i & 3) == 1 ? 3.0 : (i & 3) == 2 ? 30.0 : 1e-3;
@@ -73,7 +122,7 @@ int main() {
         if (ec > worst_ch) worst_ch = ec;
         if (es > worst_sh) worst_sh = es;
     }
-    printf("%.17g %.17g %.17g\n", worst_field, worst_ch, worst_sh);
+    printf("%.17g %.17g %.17g %.17g %.17g\n", worst_field, worst_ch, worst_sh, worst_simple, worst_taylor);
     return 0;
 }
 """
@@ -101,6 +150,17 @@ def test_stage_field_equals_the_direct_field(results):
 def test_cosh_sinh_from_one_exp(results):
     ulp = 2.0 ** -52
     assert results[1] <= 3 * ulp and results[2] <= 3 * ulp
+
+
+def test_simple_laser_stage_fields_equal_the_direct_field(results):
+    # the one-colour cos^n stage fields with the shorter series of Tsit5's stages 2 and 3 and
+    # of the envelope (N >= 2), and the full sin/cos beyond their ranges
+    assert results[3] <= 1e-14
+
+
+def test_taylor_sin_cos_orders_within_their_ranges(results):
+    # orders 7, 5, 4 over |x| <= 1/2, 1/4, 1/8 against the C library, in ulp of the value
+    assert results[4] <= 2.0
 
 
 # ---- the device's RK4 step, host-compiled, against the oracle's RK4 ------------------------

@@ -203,8 +203,11 @@ struct LaserBase {
 
 // sin and cos of |x| <= sincos_taylor_bound(KS) from their Taylor series through
 // x^(2 KS + 1) and x^(2 KS + 2): KS = 7 for |x| <= 1/2, 5 for |x| <= 1/4, 4 for
-// |x| <= 1/8 (in each case the first omitted term is below 1e-17 of the value).
-constexpr double sincos_taylor_bound(int KS) { return KS >= 7 ? 0.5 : KS >= 5 ? 0.25 : 0.125; }
+// |x| <= 1/8, 3 for |x| <= 1/20, 2 for |x| <= 1/100 (in each case the first omitted
+// term is below 1e-17, and below an ulp of the value).
+constexpr double sincos_taylor_bound(int KS) {
+    return KS >= 7 ? 0.5 : KS >= 5 ? 0.25 : KS == 4 ? 0.125 : KS == 3 ? 0.05 : 0.01;
+}
 constexpr double inv_factorial(int n) {
     double f = 1.0;
     for (int i = 2; i <= n; ++i) f *= i;   // exact in double up to 18!
@@ -303,6 +306,36 @@ __host__ __device__ inline void mono_field_at(const MonoP& m, const MonoBase& b,
         Fx = g * (a * m.cazi + bb * m.ellip * m.sazi);
         Fy = g * (a * -m.sazi + bb * m.ellip * m.cazi);
     }
+}
+
+// The simple laser's field at b.tp + d from the phases `a6` at the later time b.tp + d + back
+// (a6.tp == 1, as mono_field_at<true> gives them), turned back by w back and w back/(2N) with
+// the order-KB and order-KH Taylor series (|w back| <= sincos_taylor_bound(KB), and its
+// envelope part within KH's bound); otherwise mono_field_at<true> from the base b.
+template <int KB, int KH>
+__host__ __device__ inline void simple_field_back(const MonoP& m, const MonoBase& b, double d, const MonoBase& a6,
+                                                  double back, double& Fx, double& Fy) {
+    const double xb = m.w * back;
+    if (!(a6.tp == 1.0 && fabs(xb) <= sincos_taylor_bound(KB))) {
+        mono_field_at<true>(m, b, d, Fx, Fy);
+        return;
+    }
+    const double t = b.tp + d;
+    const double win = half_tanh_step(m.k * (t + m.halfw)) * half_tanh_step(-m.k * (t - m.halfw));
+    if (win == 0.0) {
+        Fx = 0.0;
+        Fy = 0.0;
+        return;
+    }
+    double sd, cd;
+    sincos_taylor<KB>(xb, sd, cd);
+    const double sw = a6.sw * cd - a6.cw * sd, cw = a6.cw * cd + a6.sw * sd;
+    sincos_taylor<KH>(xb * m.ihalfN, sd, cd);
+    const double sh = a6.sh * cd - a6.ch * sd, ch = a6.ch * cd + a6.sh * sd;
+    const double env = (m.kind == ETRAJ_LASER_COS2) ? ch : ch * ch * ch;
+    const double g = m.F0 * env * win;
+    Fx = g * (ch * sw + m.coefN * sh * cw);
+    Fy = g * (-ch * cw + m.coefN * sh * sw) * m.ellip;
 }
 
 // One colour, cos^2 or cos^4 envelope of at least two cycles, azimuth 0: the laser the SIMPLE

@@ -278,8 +278,13 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     } else {
         lb = laser_base<SL>(L, t);
     }
-    // SL: the carrier increments of stages 2 and 3, 0.161 and 0.327 of w dt, take the order-4
-    // and order-5 Taylor series wherever the last stage's |w dt| <= 1/2 takes the order-7 one
+    // SL: the field of stages 6 and 7 (t + dt) first; stages 4 and 5 turn its phases back by
+    // 0.1 and 0.02 of w dt (order-3 and order-2 Taylor series), and the carrier increments of
+    // stages 2 and 3, 0.161 and 0.327 of w dt from t, take the order-4 and order-5 series,
+    // wherever the last stage's |w dt| <= 1/2 takes the order-7 one
+    double F1x, F1y;                     // stages 6 and 7 share t + dt (c6 = c7 = 1)
+    MonoBase at;                         // SL: the phases at t + dt
+    if (SL) mono_field_at<true>(L.c[0], lb.c[0], dt, F1x, F1y, &at);
     double Fx, Fy;
     if (SL) mono_field_at<true, 4>(L.c[0], lb.c[0], ts5::c2 * dt, Fx, Fy);
     else laser_field_at<SL>(L, lb, ts5::c2 * dt, Fx, Fy);
@@ -289,16 +294,15 @@ __device__ __forceinline__ int ts5_attempt(const LaserP& L, const TargetP& T, bo
     else laser_field_at<SL>(L, lb, ts5::c3 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[2], kh[2], rc2, close);
     ts5_inputs<4, DIM>(s.r, s.p, kp, dt, yr, yp);
-    laser_field_at<SL>(L, lb, ts5::c4 * dt, Fx, Fy);
+    if (SL) simple_field_back<3, 3>(L.c[0], lb.c[0], ts5::c4 * dt, at, (1.0 - ts5::c4) * dt, Fx, Fy);
+    else laser_field_at<SL>(L, lb, ts5::c4 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[3], kh[3], rc2, close);
     ts5_inputs<5, DIM>(s.r, s.p, kp, dt, yr, yp);
-    laser_field_at<SL>(L, lb, ts5::c5 * dt, Fx, Fy);
+    if (SL) simple_field_back<2, 2>(L.c[0], lb.c[0], ts5::c5 * dt, at, (1.0 - ts5::c5) * dt, Fx, Fy);
+    else laser_field_at<SL>(L, lb, ts5::c5 * dt, Fx, Fy);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, Fx, Fy, yr, yp, kp[4], kh[4], rc2, close);
     ts5_inputs<6, DIM>(s.r, s.p, kp, dt, yr, yp);
-    double F1x, F1y;                     // stages 6 and 7 share t + dt (c6 = c7 = 1)
-    MonoBase at;                         // SL: the phases at t + dt
-    if (SL) mono_field_at<true>(L.c[0], lb.c[0], dt, F1x, F1y, &at);
-    else laser_field_at<SL>(L, lb, dt, F1x, F1y);
+    if (!SL) laser_field_at<SL>(L, lb, dt, F1x, F1y);
     ts5_stage_deriv<DIM, PHASE, TGT, MIXED>(T, saeq, F1x, F1y, yr, yp, kp[5], kh[5], rc2, close);
     double unr[DIM], unp[DIM], er[DIM], ep[DIM], unh = 0, eh = 0;
     ts5_inputs<7, DIM>(s.r, s.p, kp, dt, unr, unp);

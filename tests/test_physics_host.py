@@ -9,7 +9,8 @@ form evaluated with the C library:
     to 20 a.u. (past the Taylor range of the rotation) and times over the pulse
     and its edges;
   * the one-colour cos^n stage fields (mono_field_at<true, KX>, shorter Taylor
-    series for Tsit5's stages 2 and 3 and for the envelope of N >= 2 cycles)
+    series for Tsit5's stages 2 and 3 and for the envelope of N >= 2 cycles;
+    simple_field_back for stages 4 and 5, turned back from stage 6's phases)
     equal laser_field(t + d) to 1e-14 of the peak field, and each Taylor order
     agrees with the C library's sin/cos to 2 ulp over its range;
   * cosh_sinh (both from one exp, sinh by its Taylor series below 1/2) equals
@@ -88,9 +89,14 @@ int main() {
             const double cs[5] = {0.161, 0.327, 0.9, 0.9800255409045097, 1.0};
             for (int j = 0; j < 5; ++j) {
                 double fx, fy, rx, ry;
+                MonoBase a6;
+                double f6x, f6y;
+                mono_field_at<true>(L.c[0], b, dt, f6x, f6y, &a6);
                 if (j == 0) mono_field_at<true, 4>(L.c[0], b, cs[j] * dt, fx, fy);
                 else if (j == 1) mono_field_at<true, 5>(L.c[0], b, cs[j] * dt, fx, fy);
-                else mono_field_at<true, 7>(L.c[0], b, cs[j] * dt, fx, fy);
+                else if (j == 2) simple_field_back<3, 3>(L.c[0], b, cs[j] * dt, a6, (1.0 - cs[j]) * dt, fx, fy);
+                else if (j == 3) simple_field_back<2, 2>(L.c[0], b, cs[j] * dt, a6, (1.0 - cs[j]) * dt, fx, fy);
+                else { fx = f6x; fy = f6y; }
                 laser_field(L, t + cs[j] * dt, rx, ry);
                 const double e = fmax(fabs(fx - rx), fabs(fy - ry)) / L.c[0].F0;
                 if (e > worst_simple) worst_simple = e;
@@ -99,14 +105,16 @@ int main() {
     }
     // the Taylor series against the C library over their ranges, in ulp of the value
     double worst_taylor = 0;
-    for (int i = 0; i < 3000000; ++i) {
-        const int ks = i % 3;
-        const double bound = ks == 0 ? 0.5 : ks == 1 ? 0.25 : 0.125;
+    for (int i = 0; i < 5000000; ++i) {
+        const int ks = i % 5;
+        const double bound = ks == 0 ? 0.5 : ks == 1 ? 0.25 : ks == 2 ? 0.125 : ks == 3 ? 0.05 : 0.01;
         const double x = (2 * U(g) - 1) * bound;
         double s, c;
         if (ks == 0) sincos_taylor<7>(x, s, c);
         else if (ks == 1) sincos_taylor<5>(x, s, c);
-        else sincos_taylor<4>(x, s, c);
+        else if (ks == 2) sincos_taylor<4>(x, s, c);
+        else if (ks == 3) sincos_taylor<3>(x, s, c);
+        else sincos_taylor<2>(x, s, c);
         const double es = fabs(s - sin(x)) / (fabs(sin(x)) * 0x1.0p-52 + 1e-300), ec = fabs(c - cos(x)) / (cos(x) * 0x1.0p-52);
         if (es > worst_taylor) worst_taylor = es;
         if (ec > worst_taylor) worst_taylor = ec;
@@ -159,7 +167,8 @@ def test_simple_laser_stage_fields_equal_the_direct_field(results):
 
 
 def test_taylor_sin_cos_orders_within_their_ranges(results):
-    # orders 7, 5, 4 over |x| <= 1/2, 1/4, 1/8 against the C library, in ulp of the value
+    # orders 7, 5, 4, 3, 2 over |x| <= 1/2, 1/4, 1/8, 1/20, 1/100 against the C library, in ulp of
+    # the value
     assert results[4] <= 2.0
 
 

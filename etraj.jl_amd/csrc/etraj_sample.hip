@@ -268,12 +268,14 @@ __device__ inline Cx pre_eval(const SamplerP& S, const AdkBatch& B, Cx kx, Cx ky
 // bracket (px + Ax)^2 + (py + Ay)^2 is real and equals kd^2 - |Im A(ts)|^2,
 // so the equation of :233-238 is f(ti) = (kd^2 + kz^2 - |Im A|^2)/2 + Ip = 0
 // with f'(ti) = Im Ax Re Fx + Im Ay Re Fy (dA/dti = -i F). Newton from the
-// reference's initial guess (:240); accepted if |f| <= 1e-8 (NLsolve's ftol)
-// and ti > 0 (:241-246), else 0.
+// reference's initial guess (:240), or from ti0 > 0 (the root of a nearby
+// equation: the Jacobian's solves start at the sample's own root, which they
+// reach in fewer iterations, to the same convergence test); accepted if
+// |f| <= 1e-8 (NLsolve's ftol) and ti > 0 (:241-246), else 0.
 __device__ inline double spa_solve(const LaserP& L, const LaserTr& LT, double tr, double Ftr, double omega, double Ip,
-                                   double kd, double kz) {
+                                   double kd, double kz, double ti0 = 0.0) {
     const double k2 = kd * kd + kz * kz;
-    double ti = asinh(omega / Ftr * sqrt(k2 + 2 * Ip)) / omega;
+    double ti = ti0 > 0.0 ? ti0 : asinh(omega / Ftr * sqrt(k2 + 2 * Ip)) / omega;
     Cx Fx, Fy, Ax, Ay;
 #pragma unroll 1
     for (int it = 0; it < 100; ++it) {
@@ -336,7 +338,7 @@ __device__ inline void spa_saddles(const SamplerP& S, const LaserP& L, const Adk
         const double trq = tr + (q == 0 ? dtr : q == 1 ? -dtr : 0.0);
         const double kdq = kd + (q == 2 ? dkd : q == 3 ? -dkd : 0.0);
         const LaserTr& LTq = B.lt[q == 0 ? 1 : q == 1 ? 2 : 0];   // laser at trq
-        const double tq = spa_solve(L, LTq, trq, B.F, S.omega, Ip, kdq, kz);
+        const double tq = spa_solve(L, LTq, trq, B.F, S.omega, Ip, kdq, kz, ti);
         Cx f1, f2, a1, b1;
         laser_field_pot_tau(L, LTq, trq, tq, f1, f2, a1, b1);
         double qx, qy;

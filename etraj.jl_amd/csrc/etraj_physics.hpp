@@ -471,13 +471,25 @@ __host__ __device__ inline LaserTr laser_tr(const LaserP& L, double tr) {
     r.m[1] = L.n == 2 ? mono_tr(L.c[1], tr - L.delay) : r.m[0];
     return r;
 }
+// 1/x for x >= 1: on the device the hardware reciprocal refined by two Newton
+// steps (within an ulp of the quotient), on the host the division.
+__host__ __device__ inline double recip_ge1(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // cosh and sinh of x from one exp: cosh = (e + 1/e)/2, and sinh = (e - 1/e)/2
 // for |x| >= 1/2, where the difference loses at most a bit and a half; below
 // that the Taylor series of sinh through x^15 (first omitted term < 1e-19).
 __host__ __device__ inline void cosh_sinh(double x, double& ch, double& sh) {
     const double ax = fabs(x);
     const double e = exp(ax);
-    const double ie = 1.0 / e;
+    const double ie = recip_ge1(e);
     ch = 0.5 * (e + ie);
     const double x2 = x * x;
     double ps = 1.0 / 1307674368000.0;
@@ -490,6 +502,20 @@ __host__ __device__ inline void cosh_sinh(double x, double& ch, double& sh) {
     const double small = fma(ps * x2, x, x);
     const double big = copysign(0.5 * (e - ie), x);
     sh = ax < 0.5 ? small : big;
+}
+
+// cosh and sinh of |x| <= 1/2 from their Taylor series through x^16 and x^15 (the
+// first omitted terms are below 1e-19), no exp.
+__host__ __device__ inline void cosh_sinh_small(double x, double& ch, double& sh) {
+    const double x2 = x * x;
+    double ps = inv_factorial(15);
+#pragma unroll
+    for (int k = 6; k >= 1; --k) ps = fma(ps, x2, inv_factorial(2 * k + 1));
+    sh = fma(ps * x2, x, x);
+    double pc = inv_factorial(16);
+#pragma unroll
+    for (int k = 7; k >= 1; --k) pc = fma(pc, x2, inv_factorial(2 * k));
+    ch = fma(pc, x2, 1.0);
 }
 
 __host__ __device__ inline void mono_field_pot_tau(const MonoP& m, const MonoTr& r, double trm, double tau, Cx& Fx,
@@ -508,9 +534,10 @@ __host__ __device__ inline void mono_field_pot_tau(const MonoP& m, const MonoTr&
         a = ts * cw + m.w * sw;
         b = ts * sw - m.w * cw;
     } else {
-        const double ei = m.w * tau / (2 * m.N);
+        const double ei = m.w * tau * m.ihalfN;
         double che, she;
-        cosh_sinh(ei, che, she);
+        if (fabs(ei) <= 0.5) cosh_sinh_small(ei, che, she);   // the envelope's small argument
+        else cosh_sinh(ei, che, she);
         const Cx sh{r.sae * che, r.cae * she}, ch{r.cae * che, -r.sae * she};
         const Cx c2 = ch * ch;
         const bool cos2 = m.kind == ETRAJ_LASER_COS2;

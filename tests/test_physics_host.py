@@ -14,7 +14,8 @@ form evaluated with the C library:
     equal laser_field(t + d) to 1e-14 of the peak field, and each Taylor order
     agrees with the C library's sin/cos to 2 ulp over its range;
   * cosh_sinh (both from one exp, sinh by its Taylor series below 1/2) equals
-    cosh / sinh to 3 ulp over |x| up to 30.
+    cosh / sinh to 3 ulp over |x| up to 30, and cosh_sinh_small (both by their
+    Taylor series) to 3 ulp over |x| <= 1/2.
 """
 import os
 import shutil
@@ -124,7 +125,8 @@ int main() {
         const double s = (i & 3) == 0 ? 0.6 : (i & 3) == 1 ? 3.0 : (i & 3) == 2 ? 30.0 : 1e-3;
         const double x = (2 * U(g) - 1) * s;
         double c, sh;
-        cosh_sinh(x, c, sh);
+        if (i & 4 && fabs(x) <= 0.5) cosh_sinh_small(x, c, sh);
+        else cosh_sinh(x, c, sh);
         const double rc = cosh(x), rs = sinh(x);
         const double ec = fabs(c - rc) / rc, es = rs != 0 ? fabs(sh - rs) / fabs(rs) : fabs(sh);
         if (ec > worst_ch) worst_ch = ec;

@@ -577,11 +577,11 @@ __device__ inline bool adk_sample(const SamplerP& S, const LaserP& L, const Targ
 // One workgroup per batch; survivors are compacted in sample order with a
 // wave ballot + LDS scan, so column order equals the reference's loop order.
 // Workgroup g samples batch G.batch[g] of job G.job[g] (per-job parameters
-// in sps / lps) into the columns from G.start[g]. At least two waves per SIMD:
-// the SFA-SPA form would otherwise take 256 VGPRs + 130 AGPRs (one wave); with
-// ~140 values spilled to scratch it samples the Ar 3D grid 7 % faster.
+// in sps / lps) into the columns from G.start[g]. At least three waves per SIMD:
+// the SFA-SPA form then fits 168 VGPRs with two values spilled to scratch and
+// samples the Ar 3D grid 6 % faster than with two waves (214 VGPRs, no spill).
 template <int M, bool PRE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_adk_sample(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_adk_sample(
     const SamplerP* __restrict__ sps, const LaserP* __restrict__ lps, TargetP T, SegTab G, double* __restrict__ out,
     int32_t* __restrict__ counts, int64_t* __restrict__ cnt) {
     __shared__ int wave_tot[4];

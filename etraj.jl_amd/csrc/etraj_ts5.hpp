@@ -115,9 +115,16 @@ __device__ __forceinline__ void ts5_stage_deriv(const TargetP& T, bool saeq, dou
     }
 }
 
-template <int DIM, int PHASE, int TGT>
+// Hairer's initial step (OrdinaryDiffEq's ode_determine_initdt) and the first
+// stage derivative. SL: the laser is simple_laser and b0 its phases at t0
+// (mono_base), from which the fields at t0 and t0 + dt0 follow by angle
+// addition. The norms multiply by reciprocals of their scales, and
+// 10^(-(2 + log10 mx)/5) is taken as exp(-(2 ln 10 + ln mx)/5): all within ulps
+// of the quotients and the power.
+template <int DIM, int PHASE, int TGT, bool SL = false>
 __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool saeq, double t0, double tf,
-                                                  double rtol, double atol, Ts5Lane<DIM, PHASE>& s) {
+                                         double rtol, double atol, Ts5Lane<DIM, PHASE>& s,
+                                         const MonoBase* b0 = nullptr) {
     constexpr bool PH = PHASE != ETRAJ_PHASE_CTMC;
     constexpr int N = 2 * DIM + (PH ? 1 : 0);
     s.t = t0;
@@ -128,7 +135,7 @@ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool
     const double dtmax = tf - t0;
     const double dtmin0 = fmax(0.0, fabs(t0) * 0x1.0p-52);
     const double smalldt = fmax(dtmin0, 1e-6);
-    double u[N], sk[N], d0s = 0, d1s = 0;
+    double u[N], isk[N], d0s = 0, d1s = 0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         u[i] = s.r[i];
@@ -136,7 +143,8 @@ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool
     }
     if (PH) u[N - 1] = s.ph;
     double Fx, Fy;
-    laser_field(L, t0, Fx, Fy);
+    if (SL) mono_field_at<true>(L.c[0], *b0, 0.0, Fx, Fy);
+    else laser_field(L, t0, Fx, Fy);
     ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, s.r, s.p, s.k1p, s.k1ph);
     double f0[N];
 #pragma unroll
@@ -147,12 +155,12 @@ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool
     if (PH) f0[N - 1] = s.k1ph;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        sk[i] = atol + fabs(u[i]) * rtol;
-        const double a = u[i] / sk[i], b = f0[i] / sk[i];
+        isk[i] = recip(atol + fabs(u[i]) * rtol);
+        const double a = u[i] * isk[i], b = f0[i] * isk[i];
         d0s += a * a;
         d1s += b * b;
     }
-    const double d0 = sqrt(d0s / N), d1 = sqrt(d1s / N);
+    const double d0 = sqrt(d0s * (1.0 / N)), d1 = sqrt(d1s * (1.0 / N));
     double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : 0.01 * (d0 / d1);
     dt0 = fmin(dt0, dtmax);
     double yr[DIM], yp[DIM], k2p[DIM], k2ph;
@@ -161,18 +169,20 @@ __device__ __forceinline__ void ts5_init(const LaserP& L, const TargetP& T, bool
         yr[i] = u[i] + dt0 * f0[i];
         yp[i] = u[DIM + i] + dt0 * f0[DIM + i];
     }
-    laser_field(L, t0 + dt0, Fx, Fy);
+    if (SL) mono_field_at<true>(L.c[0], *b0, dt0, Fx, Fy);
+    else laser_field(L, t0 + dt0, Fx, Fy);
     ts5_deriv<DIM, PHASE, TGT>(T, saeq, Fx, Fy, yr, yp, k2p, k2ph);
     double d2s = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double f1 = i < DIM ? yp[i] : (i < 2 * DIM ? k2p[i - DIM] : k2ph);
-        const double a = (f1 - f0[i]) / sk[i];
+        const double a = (f1 - f0[i]) * isk[i];
         d2s += a * a;
     }
-    const double d2 = sqrt(d2s / N) / dt0;
+    const double d2 = sqrt(d2s * (1.0 / N)) / dt0;
     const double mx = fmax(d1, d2);
-    const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(10.0, -(2 + log10(mx)) / 5.0);
+    constexpr double kLn10 = 2.302585092994045684;
+    const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : exp(-0.2 * (2 * kLn10 + log(mx)));
     s.dt = fmax(dtmin0, fmin(fmin(100 * dt0, dt1), dtmax));
 }
 

@@ -341,19 +341,21 @@ __global__ __launch_bounds__(256) void k_ts5_prep(IntegP P, const double* __rest
     }
     st.ph = (PHASE != ETRAJ_PHASE_CTMC) ? c[2 * DIM + 2] : 0.0;
     const bool saeq = (DIM == 2 && PHASE == ETRAJ_PHASE_SCTS && (P.quirks & ETRAJ_QUIRK_SAE2D_SCTS) != 0);
-    ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, c[2 * DIM], P.tf, P.rtol, P.atol, st);
     double* q = prep + col * ts5_prep_doubles(DIM);
-    q[0] = st.dt;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) q[1 + i] = st.k1p[i];
-    q[1 + DIM] = st.k1ph;
-    if (simple_laser(P.L)) {   // the laser's phases at t0 for the integrator's simple-laser kernel
+    if (simple_laser(P.L)) {   // the laser's phases at t0, for the start values and the simple-laser kernel
         const MonoBase b = mono_base(P.L.c[0], c[2 * DIM]);
+        ts5_init<DIM, PHASE, TGT, true>(P.L, P.T, saeq, c[2 * DIM], P.tf, P.rtol, P.atol, st, &b);
         q[2 + DIM] = b.sw;
         q[3 + DIM] = b.cw;
         q[4 + DIM] = b.sh;
         q[5 + DIM] = b.ch;
+    } else {
+        ts5_init<DIM, PHASE, TGT>(P.L, P.T, saeq, c[2 * DIM], P.tf, P.rtol, P.atol, st);
     }
+    q[0] = st.dt;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) q[1 + i] = st.k1p[i];
+    q[1 + DIM] = st.k1ph;
 }
 
 __global__ void k_zero_queue(unsigned long long* q) { *q = 0ull; }

@@ -473,7 +473,8 @@ def bench_workload(name, args, ctx):
     if hits is not None:
         result["hit_counts_per_run"] = int(hits.sum()) // (steps + warmup)
         ds[0][3].hit_counts = None
-    if mixed:
+    ds64 = None
+    if mixed and not args.no_fp64:
         ds64, keep64 = _scan_descs([dict(kw, precision="fp64") for kw in jobs])
         sec64, _ = _timed(ctx, lambda: _run_scan(ctx, ds64)[1], steps, warmup)
         p64, s64 = _run_scan(ctx, ds64, device=False)
@@ -482,6 +483,8 @@ def bench_workload(name, args, ctx):
         result["mixed_over_fp64"] = sec64 / sec
         result["pmd_l1_mixed_vs_fp64_gpu"] = max(l1(pmx[j], p64[j]) for j in range(len(ds)))
     if args.cpu_batches > 0:
+        if mixed and ds64 is None:
+            ds64, keep64 = _scan_descs([dict(kw, precision="fp64") for kw in jobs])
         cds = [(ld, td, sd, rd) for (ld, td, sd, rd) in (ds64 if mixed else ds)]
         n, dt = _cpu_sample(cds, threads, n_jobs_sample=min(2, len(cds)),
                             stride=max(1, cds[0][2].sample_t_num // args.cpu_batches))

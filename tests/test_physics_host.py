@@ -13,6 +13,8 @@ form evaluated with the C library:
     simple_field_back for stages 4 and 5, turned back from stage 6's phases)
     equal laser_field(t + d) to 1e-14 of the peak field, and each Taylor order
     agrees with the C library's sin/cos to 2 ulp over its range;
+  * laser_field_pot_tau (the samplers' complex-time laser from its real-time
+    parts) equals laser_field_pot_c to 1e-13 of the largest component;
   * cosh_sinh (both from one exp, sinh by its Taylor series below 1/2) equals
     cosh / sinh to 3 ulp over |x| up to 30, and cosh_sinh_small (both by their
     Taylor series) to 3 ulp over |x| <= 1/2.
@@ -120,6 +122,35 @@ int main() {
         if (es > worst_taylor) worst_taylor = es;
         if (ec > worst_taylor) worst_taylor = ec;
     }
+    // the complex-time laser from its real-time parts (the SPA samplers' form, the carrier's exp
+    // from the envelope's Taylor series when 2N is an integer) against the direct complex form
+    double worst_tau = 0;
+    for (int kind = 1; kind <= 2; ++kind)
+    for (int cyc = 0; cyc < 4; ++cyc) {
+        const double cycs[4] = {2.0, 1.5, 2.5, 3.7};
+        etraj_laser_desc d{};
+        d.n_colors = 1;
+        d.color[0].kind = kind;
+        d.color[0].peak_int = 2e14;
+        d.color[0].wave_len = 800;
+        d.color[0].cyc_num = cycs[cyc];
+        d.color[0].ellip = 1.0;
+        d.color[0].cep = 0.3;
+        LaserP L;
+        if (make_laser(d, L)) return 4;
+        for (int i = 0; i < 20000; ++i) {
+            const double tr = (2 * U(g) - 1) * L.c[0].halfw, tau = 45 * U(g);
+            const LaserTr r = laser_tr(L, tr);
+            Cx fx, fy, ax, ay, gx, gy, bx, by;
+            laser_field_pot_tau(L, r, tr, tau, fx, fy, ax, ay);
+            laser_field_pot_c(L, Cx{tr, tau}, gx, gy, bx, by);
+            const double sF = fmax(hypot(gx.re, gx.im), hypot(gy.re, gy.im)) + 1e-300;
+            const double sA = fmax(hypot(bx.re, bx.im), hypot(by.re, by.im)) + 1e-300;
+            const double e = fmax(fmax(hypot(fx.re - gx.re, fx.im - gx.im), hypot(fy.re - gy.re, fy.im - gy.im)) / sF,
+                                  fmax(hypot(ax.re - bx.re, ax.im - bx.im), hypot(ay.re - by.re, ay.im - by.im)) / sA);
+            if (e > worst_tau) worst_tau = e;
+        }
+    }
     double worst_ch = 0, worst_sh = 0;
     for (int i = 0; i < 2000000; ++i) {
         const double s = (i & 3) == 0 ? 0.6 : (i & 3) == 1 ? 3.0 : (i & 3) == 2 ? 30.0 : 1e-3;
@@ -132,7 +163,8 @@ int main() {
         if (ec > worst_ch) worst_ch = ec;
         if (es > worst_sh) worst_sh = es;
     }
-    printf("%.17g %.17g %.17g %.17g %.17g\n", worst_field, worst_ch, worst_sh, worst_simple, worst_taylor);
+    printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", worst_field, worst_ch, worst_sh, worst_simple, worst_taylor,
+           worst_tau);
     return 0;
 }
 """
@@ -166,6 +198,12 @@ def test_simple_laser_stage_fields_equal_the_direct_field(results):
     # the one-colour cos^n stage fields with the shorter series of Tsit5's stages 2 and 3 and
     # of the envelope (N >= 2), and the full sin/cos beyond their ranges
     assert results[3] <= 1e-14
+
+
+def test_complex_time_laser_from_real_time_parts(results):
+    # laser_field_pot_tau (envelope cosh/sinh by Taylor, the carrier's exp as a power of the
+    # envelope's for 2N = 4, 3, 5) against laser_field_pot_c, relative to the largest component
+    assert results[5] <= 1e-13
 
 
 def test_taylor_sin_cos_orders_within_their_ranges(results):

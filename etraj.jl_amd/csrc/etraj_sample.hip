@@ -268,27 +268,39 @@ __device__ inline Cx pre_eval(const SamplerP& S, const AdkBatch& B, Cx kx, Cx ky
 // bracket (px + Ax)^2 + (py + Ay)^2 is real and equals kd^2 - |Im A(ts)|^2,
 // so the equation of :233-238 is f(ti) = (kd^2 + kz^2 - |Im A|^2)/2 + Ip = 0
 // with f'(ti) = Im Ax Re Fx + Im Ay Re Fy (dA/dti = -i F). Newton from the
-// reference's initial guess (:240), or from ti0 > 0 (the root of a nearby
-// equation: the Jacobian's solves start at the sample's own root, which they
-// reach in fewer iterations, to the same convergence test); accepted if
-// |f| <= 1e-8 (NLsolve's ftol) and ti > 0 (:241-246), else 0.
+// reference's initial guess (:240), or from ti0 > 0 (a start near the root: the
+// Jacobian's solves start at the sample's own root moved to first order in the
+// change of tr or kd, and reach their roots in fewer iterations, to the same
+// convergence test); accepted if
+// |f| <= 1e-8 (NLsolve's ftol) and ti > 0 (:241-246), else 0. Fx, Fy, Ax, Ay
+// receive the laser at tr + i t for the returned t (the acceptance test's
+// evaluation, which the callers would otherwise repeat); for a failed solve
+// only with AT_FAIL.
+template <bool AT_FAIL>
 __device__ inline double spa_solve(const LaserP& L, const LaserTr& LT, double tr, double Ftr, double omega, double Ip,
-                                   double kd, double kz, double ti0 = 0.0) {
+                                   double kd, double kz, Cx& Fx, Cx& Fy, Cx& Ax, Cx& Ay, double ti0 = 0.0) {
     const double k2 = kd * kd + kz * kz;
     double ti = ti0 > 0.0 ? ti0 : asinh(omega / Ftr * sqrt(k2 + 2 * Ip)) / omega;
-    Cx Fx, Fy, Ax, Ay;
+    bool ok = true;
 #pragma unroll 1
     for (int it = 0; it < 100; ++it) {
         laser_field_pot_tau(L, LT, tr, ti, Fx, Fy, Ax, Ay);
         const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
         const double step = f / (Ax.im * Fx.re + Ay.im * Fy.re);
-        if (!isfinite(step)) return 0.0;
+        if (!isfinite(step)) {
+            ok = false;
+            break;
+        }
         ti -= step;
         if (fabs(step) <= 1e-15 * fabs(ti)) break;
     }
-    laser_field_pot_tau(L, LT, tr, ti, Fx, Fy, Ax, Ay);
-    const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
-    return (fabs(f) <= 1e-8 && ti > 0) ? ti : 0.0;
+    if (ok) {
+        laser_field_pot_tau(L, LT, tr, ti, Fx, Fy, Ax, Ay);
+        const double f = (k2 - (Ax.im * Ax.im + Ay.im * Ay.im)) / 2 + Ip;
+        if (fabs(f) <= 1e-8 && ti > 0) return ti;
+    }
+    if (AT_FAIL) laser_field_pot_tau(L, LT, tr, 0.0, Fx, Fy, Ax, Ay);   // a failed solve: the laser at t = 0
+    return 0.0;
 }
 
 // px_/py_ (SPASampler.jl:231-232)
@@ -320,11 +332,10 @@ template <bool JAC>
 __device__ inline void spa_saddles(const SamplerP& S, const LaserP& L, const AdkBatch& B, double kd, double kz,
                                    SpaSlots o) {
     const double Ip = S.Ip, tr = B.tr;
-    const double ti = spa_solve(L, B.lt[0], tr, B.F, S.omega, Ip, kd, kz);
+    Cx Fx, Fy, Ax, Ay;
+    const double ti = spa_solve<false>(L, B.lt[0], tr, B.F, S.omega, Ip, kd, kz, Fx, Fy, Ax, Ay);
     o[0] = ti;
     if (ti == 0.0) return;
-    Cx Fx, Fy, Ax, Ay;
-    laser_field_pot_tau(L, B.lt[0], tr, ti, Fx, Fy, Ax, Ay);
     double px, py;
     spa_p(kd, Ax, Ay, px, py);
     o[1] = px;
@@ -333,14 +344,21 @@ __device__ inline void spa_saddles(const SamplerP& S, const LaserP& L, const Adk
     o[7] = Ax.re; o[8] = Ax.im; o[9] = Ay.re; o[10] = Ay.im;
     if (!JAC) return;
     const double dtr = 0.01, dkd = 0.01;
+    // start values of the four solves: the root moved to first order in dtr (df/dtr =
+    // Im Ax Im Fx + Im Ay Im Fy, as dA/dtr = -F) and exactly in the kd-dependence of f
+    // (kd dkd + dkd^2 / 2); the solves converge from there to the same test
+    const double dfdti = Ax.im * Fx.re + Ay.im * Fy.re;
+    const double dfdtr = Ax.im * Fx.im + Ay.im * Fy.im;
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const double trq = tr + (q == 0 ? dtr : q == 1 ? -dtr : 0.0);
         const double kdq = kd + (q == 2 ? dkd : q == 3 ? -dkd : 0.0);
         const LaserTr& LTq = B.lt[q == 0 ? 1 : q == 1 ? 2 : 0];   // laser at trq
-        const double tq = spa_solve(L, LTq, trq, B.F, S.omega, Ip, kdq, kz, ti);
+        const double df = q < 2 ? dfdtr * (trq - tr) : (kdq * kdq - kd * kd) / 2;
+        double t0 = ti - df / dfdti;
+        if (!(t0 > 0.0) || !isfinite(t0)) t0 = ti;
         Cx f1, f2, a1, b1;
-        laser_field_pot_tau(L, LTq, trq, tq, f1, f2, a1, b1);
+        (void)spa_solve<true>(L, LTq, trq, B.F, S.omega, Ip, kdq, kz, f1, f2, a1, b1, t0);
         double qx, qy;
         spa_p(kdq, a1, b1, qx, qy);
         o[11 + 2 * q] = qx;

@@ -466,9 +466,10 @@ constexpr int64_t kFieldTabBytesPerStep = sizeof(float4) + 2 * sizeof(double2);
 // device even when only a few per cent of the electrons are flagged.
 constexpr int64_t kFieldTabBudget = (int64_t)8 << 30;
 constexpr int64_t kMaxSlabCols = (int64_t)1 << 26;
-// Tsit5 slabs (one job each) take up to 2^28 columns, split evenly: the work
-// queue of each integrator launch ends in a tail of about 1.5 ms in which the
-// slowest electrons run on a few lanes, so fewer, larger slabs save their
+// Tsit5 slabs (one job each) take up to 2^28 columns, split evenly, or the
+// whole job up to 2^29 (two slots of half its size would hold as much): the
+// work queue of each integrator launch ends in a tail of about 1.5 ms in which
+// the slowest electrons run on a few lanes, so fewer, larger slabs save their
 // tails (1e8-electron job: 6 slabs 159.3 ms, 2 slabs 152.6 ms). 2^28 columns
 // of a 3D job are 39 GB of columns and start values per slot.
 constexpr int64_t kMaxTs5SlabCols = (int64_t)1 << 28;
@@ -1110,13 +1111,14 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
         }
     }
     const int64_t seg_max = std::max<int64_t>(1, kFieldTabBudget / (stride * kFieldTabBytesPerStep));
-    // Tsit5: a job's batches in the fewest slabs of at most kMaxTs5SlabCols columns, each
-    // filled to the job's columns / number of slabs (the last takes the rest)
+    // Tsit5: a job's batches in one slab up to 2 kMaxTs5SlabCols columns, else in the fewest
+    // slabs of at most kMaxTs5SlabCols, each filled to the job's columns / number of slabs
+    // (the last takes the rest)
     std::vector<int64_t> ts5_fill(n_jobs, 0);
     if (!rk4) {
         for (const Item& it : items) ts5_fill[it.job] += it.cap;
         for (int64_t& v : ts5_fill) {
-            const int64_t n = std::max<int64_t>(1, (v + kMaxTs5SlabCols - 1) / kMaxTs5SlabCols);
+            const int64_t n = v <= 2 * kMaxTs5SlabCols ? 1 : (v + kMaxTs5SlabCols - 1) / kMaxTs5SlabCols;
             v = std::max<int64_t>(1, (v + n - 1) / n);
         }
     }
@@ -1124,7 +1126,7 @@ static int run_scan_impl(etraj_ctx* c, int32_t n_jobs, const etraj_laser_desc* l
     std::vector<Slab> slabs;
     for (size_t i = 0; i < items.size();) {
         Slab sl{i, i, 0, 0};
-        const int64_t max_cols_slab = rk4 ? kMaxSlabCols : kMaxTs5SlabCols;
+        const int64_t max_cols_slab = rk4 ? kMaxSlabCols : 2 * kMaxTs5SlabCols;
         while (sl.i1 < items.size()) {
             const Item& it = items[sl.i1];
             const bool empty = sl.i1 == sl.i0;

@@ -6,12 +6,12 @@ need the oracle to run at that size:
     >= 1e8 electrons; with fixed-point sums two mixed runs are the same bits;
     the mixed PMD is within relative L1 1e-3 of the FP64 PMD (the stated bound),
     and both runs give the same ion probability;
-  * the same 1e8-electron job through Tsit5 (CTMC, fixed-point sums, hit counts),
-    whose columns fill two slabs: every electron is integrated once (the CTMC
+  * that job with 16000 strata (1.6e8 electrons) through Tsit5 (CTMC, fixed-point
+    sums, hit counts), whose columns fill three slabs: every electron is integrated once (the CTMC
     identity sum(PMD) = ion_prob - ion_prob_uncollected, hit counts within the
     collected electrons), with no failures, and its ionization and PMD are those
     of the FP64 RK4 run of the same electrons up to the two integrators'
-    truncation difference (ion_prob 1.2 % apart, PMD L1 7.8e-2);
+    truncation difference (1e8 electrons: ion_prob 1.2 % apart, PMD L1 7.8e-2);
   * the reference's CPU-runnable case (2000 t0 x 50 kd CTMC grid, 500^2 PMD) in
     FP64 against the oracle at full size (relative L1 <= 1e-6), with the
     CTMC identity sum(PMD) = ion_prob - ion_prob_uncollected and the hit counts
@@ -113,20 +113,21 @@ def test_ar_spa_3d_at_full_size(ctx):
     assert mix <= 1.5 * trunc
 
 
-def test_tsit5_job_in_two_slabs_at_full_size(ctx):
-    kw = dict(workloads.h_lp_scts_stratified(precision="fp64", p_num=500), traj_phase_method="CTMC",
+def test_tsit5_job_in_three_slabs_at_full_size(ctx):
+    kw = dict(workloads.h_lp_scts_stratified(n_t=16000, precision="fp64", p_num=500), traj_phase_method="CTMC",
               pmd_accumulator="fixed")
     ts5, st, hits, _ = _job(ctx, dict(kw, traj_integrator="tsit5", traj_rtol=1e-6), hits=True)
     rk4, sr, _, _ = _job(ctx, kw)
-    print("Tsit5 1e8 CTMC:", st["n_slabs"], "slabs,", st["num_effective_traj"], "electrons, ion_prob",
+    print("Tsit5 1.6e8 CTMC:", st["n_slabs"], "slabs,", st["num_effective_traj"], "electrons, ion_prob",
           st["ion_prob"], "(RK4", sr["ion_prob"], "), PMD L1 vs RK4", l1(ts5, rk4), flush=True)
-    assert st["n_slabs"] == 2
-    assert st["num_effective_traj"] == sr["num_effective_traj"] >= 1e8
+    assert st["n_slabs"] == 3
+    assert st["num_effective_traj"] == sr["num_effective_traj"] >= 1.5e8
     assert st["n_failed"] == 0 and st["n_attempts"] >= st["n_steps"] > st["num_effective_traj"]
     assert ts5.sum() == pytest.approx(st["ion_prob"] - st["ion_prob_uncollected"], rel=1e-12)
     assert 0 < hits.sum() <= st["num_effective_traj"] - st["n_failed"] - st["n_bound"]
-    # RK4 at dt 0.1 against Tsit5 at rtol 1e-6 on these electrons: ion_prob 1.2 % apart, PMD L1 7.8e-2
-    # (measured); a slab lost or counted twice would move ion_prob by tens of per cent
+    # RK4 at dt 0.1 against Tsit5 at rtol 1e-6 on the 1e8 electrons of 10000 strata: ion_prob 1.2 %
+    # apart, PMD L1 7.8e-2 (measured); a slab lost or counted twice would move ion_prob by tens of
+    # per cent
     assert st["ion_prob"] == pytest.approx(sr["ion_prob"], rel=3e-2)
     assert l1(ts5, rk4) <= 0.15
 

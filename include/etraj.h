@@ -366,7 +366,7 @@ typedef struct etraj_tuning {
     int32_t tsit5_refill;    /* Tsit5: idle lanes of a wavefront before it refills them, 1..64 */
     int32_t tsit5_claim;     /* Tsit5: electrons a wavefront claims from the queue, 64..65536  */
     int32_t graph;           /* 1: a job whose batches fit one slab (at most 2^26 init columns for
-                                RK4, 2^28 for Tsit5) runs as a HIP graph, captured at its first
+                                RK4, 2^29 for Tsit5) runs as a HIP graph, captured at its first
                                 call and replayed while the call's launch arguments repeat (its stats then give the graph's
                                 device time as ms_integrate, with ms_sample and ms_integrate_main
                                 0); 0 (the default) launches every kernel on each call          */

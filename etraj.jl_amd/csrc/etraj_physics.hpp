@@ -226,10 +226,24 @@ __host__ __device__ inline void sincos_taylor(double x, double& s, double& c) {
     c = fma(pc, x2, 1.0);
 }
 
+// 1/x for x >= 1: on the device the hardware reciprocal refined by two Newton
+// steps (within an ulp of the quotient), on the host the division.
+__host__ __device__ inline double recip_ge1(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // 1/2 + tanh(x)/2 as 1/(1 + exp(-2x)); beyond |x| = 19.1 the result is exactly 0 or 1 in double
-// (exp(-38.2) < 2^-54), so the exp is skipped there.
+// (exp(-38.2) < 2^-54), so the exp is skipped there. (On the device the lanes near a pulse edge
+// evaluate it while the rest of their wavefront waits: the reciprocal takes 5 instructions where
+// the division takes 10.)
 __host__ __device__ inline double half_tanh_step(double x) {
-    return x < -19.1 ? 0.0 : (x > 19.1 ? 1.0 : 1.0 / (1.0 + exp(-2.0 * x)));
+    return x < -19.1 ? 0.0 : (x > 19.1 ? 1.0 : recip_ge1(1.0 + exp(-2.0 * x)));
 }
 
 __host__ __device__ inline MonoBase mono_base(const MonoP& m, double t) {
@@ -471,18 +485,6 @@ __host__ __device__ inline LaserTr laser_tr(const LaserP& L, double tr) {
     r.m[1] = L.n == 2 ? mono_tr(L.c[1], tr - L.delay) : r.m[0];
     return r;
 }
-// 1/x for x >= 1: on the device the hardware reciprocal refined by two Newton
-// steps (within an ulp of the quotient), on the host the division.
-__host__ __device__ inline double recip_ge1(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return fma(r, fma(-x, r, 1.0), r);
-#else
-    return 1.0 / x;
-#endif
-}
-
 // cosh and sinh of x from one exp: cosh = (e + 1/e)/2, and sinh = (e - 1/e)/2
 // for |x| >= 1/2, where the difference loses at most a bit and a half; below
 // that the Taylor series of sinh through x^15 (first omitted term < 1e-19).
